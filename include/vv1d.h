@@ -1,0 +1,217 @@
+/*
+ * vv1d.h -- C ABI of libvv1d.so, the MI355X (gfx950) implicit timestep of the
+ * 1D1V Vlasov-Poisson variational integrator (split mode, matrix-free,
+ * nonlinear implicit midpoint; Arakawa J1/J2/J4 brackets; kinetic
+ * FFT(x) x tridiagonal(v) preconditioner; GMRES; periodic Poisson).
+ *
+ * Conventions
+ *  - Every field argument is a DEVICE pointer to float64 in the reference's
+ *    PETSc natural order: element (i, j) (x index i, v index j) lives at
+ *    i + nx*j, i.e. a C-contiguous (nv, nx) array.  A petsc4py Vec array of
+ *    the reference's da1 maps onto it without a copy.
+ *  - 1D x-arrays have nx entries, 1D v-arrays nv entries.
+ *  - All work is enqueued on the context's stream; functions that return a
+ *    host scalar synchronise that stream.
+ *  - Every function returns 0 on success or a negative VV_ERR_* code;
+ *    vv_last_error(ctx) (or vv_last_error(NULL) after a failed create)
+ *    returns a message.
+ *  - One context serves one stream; calls on one context must be serialised
+ *    by the caller.  Separate contexts are independent.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (paths relative to the viVlasov1D source tree).
+ */
+#ifndef VV1D_H
+#define VV1D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vv_ctx vv_ctx;
+
+/* error codes */
+#define VV_OK               0
+#define VV_ERR_INVALID     -1   /* bad argument / unsupported size          */
+#define VV_ERR_HIP         -2   /* HIP runtime error (launch, alloc, copy)  */
+#define VV_ERR_UNSUPPORTED -3   /* option the library does not provide       */
+#define VV_ERR_NONFINITE   -4   /* non-finite value met where one is fatal   */
+
+/* Poisson bracket: PoissonBracket.create('arakawaj1'|'arakawaj2'|'arakawaj4')
+ * (vlasov/solvers/components/PoissonBracket.pyx:37-50) */
+#define VV_BRACKET_J1 1
+#define VV_BRACKET_J2 2
+#define VV_BRACKET_J4 4
+
+/* preconditioner: TensorProductPreconditioner.create(None|'tensor_kinetic')
+ * (vlasov/solvers/preconditioner/TensorProduct.pyx:66-86) */
+#define VV_PRECOND_NONE           0
+#define VV_PRECOND_TENSOR_KINETIC 1
+
+/* Laplacian: poisson module PETScPoissonCFD2 | PETScPoissonCFD4
+ * (vlasov/run/run_base.py:93-94, 115) */
+#define VV_LAPLACE_CFD2 2
+#define VV_LAPLACE_CFD4 4
+
+/* KSP converged reasons (PETSc numbering, as KSPGetConvergedReason) */
+#define VV_KSP_CONVERGED_RTOL      2
+#define VV_KSP_CONVERGED_ATOL      3
+#define VV_KSP_DIVERGED_ITS       -3
+#define VV_KSP_DIVERGED_DTOL      -4
+#define VV_KSP_DIVERGED_BREAKDOWN -5
+#define VV_KSP_DIVERGED_NANORINF  -9
+
+/* Grid and operator selection.  Mirrors Grid.create
+ * (vlasov/core/Grid.pyx:14-51) and the solver options of run_base.py:35-196. */
+typedef struct {
+    int32_t nx;            /* points in x (periodic); power of two, 8..4096   */
+    int32_t nv;            /* points in v; 8..8192                            */
+    double  hx;            /* L / nx                                          */
+    double  hv;            /* (vmax - vmin) / (nv - 1)                        */
+    double  ht;            /* time step                                       */
+    double  charge;        /* particle charge q (Poisson right-hand side)     */
+    int32_t bracket;       /* VV_BRACKET_*                                    */
+    int32_t precond;       /* VV_PRECOND_*                                    */
+    int32_t laplace;       /* VV_LAPLACE_*                                    */
+    int32_t restart;       /* GMRES restart length (PETSc default 30), <= 64  */
+    const double *v;       /* HOST, nv: velocity grid v_j                     */
+    const double *eigen;   /* HOST, 2*nx (re, im interleaved): the kinetic
+                              eigenvalues of TensorProduct.compute_eigenvalues
+                              (TensorProduct.pyx:109-116); required when
+                              precond = VV_PRECOND_TENSOR_KINETIC             */
+} vv_params;
+
+/* Solver construction: PETScVlasovSolverBase.__init__
+ * (vlasov/solvers/vlasov/PETScVlasovSolver.pyx:27-104) together with the
+ * factored preconditioner (TensorProductKineticFast.pyx:25-95) and the
+ * Poisson solver (PETScPoissonSolver.pyx:23-45).  stream: a hipStream_t, or
+ * NULL for the null stream. */
+int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out);
+int vv_ctx_destroy(vv_ctx *ctx);
+const char *vv_last_error(const vv_ctx *ctx);
+int vv_set_stream(vv_ctx *ctx, void *stream);
+int vv_synchronize(vv_ctx *ctx);
+/* bytes of device scratch the context owns (Krylov basis, spectra, ...) */
+int64_t vv_scratch_bytes(const vv_ctx *ctx);
+
+/* PETScVlasovSolverBase.update_previous (PETScVlasovSolver.pyx:142-157):
+ * H_ave = ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i), stored as
+ * its 1D factors.  h0: nv entries; h1p, h1h, h2p, h2h: nx entries (device).
+ * The context keeps a copy, as the reference keeps H_ave. */
+int vv_set_hamiltonian(vv_ctx *ctx, const double *h0, const double *h1p,
+                       const double *h1h, const double *h2p, const double *h2h);
+
+/* PETScNLVlasovMP.function_solver (PETScNLVlasovMP.pyx:43-58):
+ * y = (f - fh)/dt + B(.5 fh + .5 f, H_ave) on rows 2..nv-3, (f - fh)/dt on
+ * the four v-boundary rows.  Overwrites y. */
+int vv_residual(vv_ctx *ctx, const double *f, const double *fh, double *y);
+
+/* PETScNLVlasovMP.jacobian_solver (PETScNLVlasovMP.pyx:33-40):
+ * y = u/dt + .5 B(u, H_ave).  Overwrites y. */
+int vv_jacobian_apply(vv_ctx *ctx, const double *u, double *y);
+
+/* PoissonBracket.function (PoissonBracket.pyx:70-75, 325-330) with H = H_ave:
+ * y += factor * {x, H_ave}.  Accumulates into y. */
+int vv_bracket_add(vv_ctx *ctx, const double *x, double *y, double factor);
+
+/* TensorProductPreconditionerKinetic.tensorProduct
+ * (TensorProductKinetic.pyx:146-160, KineticFast.pyx:102-224):
+ * y = irfft_x(T_k^-1 rfft_x(x)).  x and y may alias. */
+int vv_precond_apply(vv_ctx *ctx, const double *x, double *y);
+
+/* PETScVlasovSolverBase.function / function_snes_mult / function_mult
+ * (PETScVlasovSolver.pyx:168-173, 188-193): y = P^-1 F(f). */
+int vv_function(vv_ctx *ctx, const double *f, const double *fh, double *y);
+
+/* PETScVlasovSolverBase.jacobian / mult / snes_mult / jacobian_mult
+ * (PETScVlasovSolver.pyx:160-165, 176-186): y = P^-1 J u. */
+int vv_jacobian(vv_ctx *ctx, const double *u, double *y);
+
+/* PETScVlasovSolverBase.function_norm (PETScVlasovSolver.pyx:196-198):
+ * y = F(f) (unpreconditioned), *out = ||y||_2 (host). */
+int vv_function_norm(vv_ctx *ctx, const double *f, const double *fh, double *y,
+                     double *out);
+
+/* Toolbox.compute_density (vlasov/toolbox/Toolbox.pyx:79-98):
+ * n_i = hv * sum_j f_ij. */
+int vv_density(vv_ctx *ctx, const double *f, double *n);
+
+/* PETScPoissonSolver.formRHS + the Poisson KSPSolve
+ * (PETScPoissonSolver.pyx:58-74; run_base.py:689-690): solves A phi = b,
+ * b_i = -(n_i - mean n) q, exactly on the periodic grid; phi is returned
+ * mean-free (the reference's null-space constant never reaches f). */
+int vv_poisson_solve(vv_ctx *ctx, const double *n, double *phi);
+
+/* PETScPoisson{CFD4,CFD2}.matrix_mult (PETScPoissonCFD4.pyx:64-78) */
+int vv_poisson_matrix_mult(vv_ctx *ctx, const double *x, double *y);
+
+/* PETScPoisson{CFD4,CFD2}.function_mult (PETScPoissonCFD4.pyx:83-100):
+ * y = A x + (n - mean n) q. */
+int vv_poisson_function_mult(vv_ctx *ctx, const double *x, const double *n,
+                             double *y);
+
+/* Toolbox.potential_to_hamiltonian (Toolbox.pyx:38-74), kept 1D:
+ * h1_i = phi_i - sum(phi)/nx. */
+int vv_potential_to_hamiltonian(vv_ctx *ctx, const double *phi, double *h1);
+
+/* run_base.calculate_moments(potential=True) + copy_pint_to_h
+ * (run_base.py:640-704, 741-742) fused: n from f, phi from n, h1 from phi,
+ * and *pnorm = ||A phi + (n - mean n) q||_2 (host, may be NULL). */
+int vv_moments(vv_ctx *ctx, const double *f, double *n, double *phi, double *h1,
+               double *pnorm);
+
+/* VecNorm(NORM_2), VecDot, VecAXPY over len doubles. */
+int vv_norm2(vv_ctx *ctx, const double *x, int64_t len, double *out);
+int vv_dot(vv_ctx *ctx, const double *x, const double *y, int64_t len, double *out);
+int vv_axpy(vv_ctx *ctx, double alpha, const double *x, double *y, int64_t len);
+
+/* PETSc KSPGMRES on the operator P^-1 J of this context (the MatPython
+ * context set at run_nonlinear_matrixfree_split_mp.py:42-53): restart
+ * `restart`, classical Gram-Schmidt, Givens rotations, x0 = 0, stop when
+ * ||r_k|| <= max(rtol ||b||, atol) (preconditioned residual), or
+ * ||r_k|| >= dtol ||b||, or max_it iterations. */
+typedef struct {
+    double  rtol;
+    double  atol;
+    double  dtol;
+    int32_t max_it;
+} vv_ksp_params;
+
+typedef struct {
+    int32_t its;       /* KSPGetIterationNumber        */
+    int32_t reason;    /* KSPGetConvergedReason        */
+    double  rnorm;     /* last residual estimate       */
+    double  rnorm0;    /* ||b||                        */
+} vv_ksp_result;
+
+int vv_ksp_solve(vv_ctx *ctx, const double *b, double *x, const vv_ksp_params *kp,
+                 vv_ksp_result *res);
+
+/* SNESSolve_KSPONLY as set up at run_nonlinear_matrixfree_split_mp.py:47-53:
+ * b = P^-1 F(f); KSPSolve(P^-1 J, b) -> delta; f -= delta unless the linear
+ * solve diverged.  f is updated in place. */
+int vv_snes_ksponly(vv_ctx *ctx, double *f, const double *fh,
+                    const vv_ksp_params *kp, vv_ksp_result *res);
+
+/* Invariants of core/distribution_function.pyx:130-199,
+ * core/hamiltonian.pyx:67-100 and core/potential.pyx:48-59:
+ * out[0] mass, out[1] L2, out[2] total energy, out[3] kinetic energy,
+ * out[4] potential energy, out[5] field energy (host array of 6). */
+int vv_invariants(vv_ctx *ctx, const double *f, const double *h0,
+                  const double *h1, const double *h2, const double *phi,
+                  double *out);
+
+/* Per-kernel timing with HIP events on the context stream (for the
+ * roofline report).  kernel: "jacobian", "residual", "rfft", "tridiag",
+ * "irfft_mdot", "maxpy_norm".  Enable, run, then read the summed time in ms
+ * and the launch count. */
+int vv_profile_enable(vv_ctx *ctx, int enable);
+int vv_profile_read(vv_ctx *ctx, const char *kernel, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VV1D_H */

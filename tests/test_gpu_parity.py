@@ -1,0 +1,233 @@
+"""GPU parity of libvv1d.so against the CPU oracle (same seeded inputs).
+
+Bars (each written in the test):
+  * Arakawa residual / Jacobian / bracket accumulation: bit-identical to the
+    oracle (both evaluate the reference's unfused expression order).
+  * FFT-based operators (P^-1, P^-1 F, P^-1 J u): max |gpu - oracle| <=
+    1e-12 * max |oracle|  (the reference's FFTW/LAPACK result is defined only
+    to rounding; the oracle uses numpy.fft + LAPACK zgbtrs).
+  * Poisson, density, reductions: <= 1e-12 relative.
+  * GMRES and Picard steps: tolerance-limited (see each test).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import vlasov_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(8, 16), (16, 32), (32, 50), (64, 128), (128, 64), (256, 512)]
+KINDS = ['arakawaj1', 'arakawaj2', 'arakawaj4']
+
+
+def _nat(a):
+    """oracle (nx, nv) -> natural-order (nv, nx) C array"""
+    return np.ascontiguousarray(a.T)
+
+
+def _setup(nx, nv, kind, seed, cuda, precond='tensor_kinetic', restart=30):
+    import torch
+    from vivlasov1d_amd.solvers import Context
+    rng = np.random.default_rng(seed)
+    g = O.Grid(nx, nv, 4 * np.pi, 8.0, ht=0.05)
+    from vivlasov1d_amd.core import Grid as PGrid
+    pg = PGrid(nx, nv, 4 * np.pi, 8.0, ht=0.05)
+    assert np.array_equal(pg.v, g.v) and pg.hx == g.hx and pg.hv == g.hv
+    ctx = Context(pg, bracket=kind, preconditioner=precond, laplace='cfd4', charge=-1.0, restart=restart)
+    h0 = O.kinetic_hamiltonian(g)
+    h1p, h1h, h2p, h2h = (0.1 * rng.standard_normal(nx) for _ in range(4))
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    ctx.set_hamiltonian(T(h0), T(h1p), T(h1h), T(h2p), T(h2h))
+    H = O.h_average(h0, h1p, h1h, h2p, h2h)
+    return g, ctx, H, rng, T
+
+
+@pytest.mark.parametrize("nx,nv", GRIDS)
+@pytest.mark.parametrize("kind", KINDS)
+def test_residual_and_jacobian_bitwise(nx, nv, kind, cuda):
+    import torch
+    print("case", nx, nv, kind, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, kind, 1 + nx + nv, cuda)
+    f = rng.random((nx, nv))
+    fh = f + 0.01 * rng.standard_normal((nx, nv))
+    u = rng.standard_normal((nx, nv))
+    ref_F = O.residual(kind, f, fh, H, g)
+    ref_J = O.jacobian(kind, u, H, g)
+    y = ctx.new_field()
+    ctx.residual(T(_nat(f)), T(_nat(fh)), y)
+    got_F = y.cpu().numpy().T
+    assert np.array_equal(got_F, ref_F), np.abs(got_F - ref_F).max()
+    ctx.jacobian_apply(T(_nat(u)), y)
+    got_J = y.cpu().numpy().T
+    assert np.array_equal(got_J, ref_J), np.abs(got_J - ref_J).max()
+    # accumulate form y += factor {x, H}
+    y0 = rng.standard_normal((nx, nv))
+    yd = T(_nat(y0))
+    ctx.bracket_add(T(_nat(u)), yd, 0.75)
+    ref = O.bracket_add(kind, u, H, y0.copy(), 0.75, g)
+    assert np.array_equal(yd.cpu().numpy().T, ref)
+
+
+@pytest.mark.parametrize("nx,nv", GRIDS + [(1024, 2048)])
+def test_preconditioner(nx, nv, cuda):
+    print("case", nx, nv, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, 'arakawaj4', 7 + nx, cuda)
+    pc = O.Preconditioner(g, 'lapack') if nx * nv <= 256 * 512 else O.Preconditioner(g, 'scipy')
+    x = rng.standard_normal((nx, nv))
+    ref = pc.apply(x)
+    y = ctx.new_field()
+    xd = T(_nat(x))
+    ctx.precond(xd, y)
+    got = y.cpu().numpy().T
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= 1e-12, err
+    # in place (x and y alias)
+    ctx.precond(xd, xd)
+    assert np.array_equal(xd.cpu().numpy().T, got)
+
+
+@pytest.mark.parametrize("nx,nv", [(16, 32), (64, 128), (256, 512)])
+@pytest.mark.parametrize("kind", KINDS)
+def test_preconditioned_operators(nx, nv, kind, cuda):
+    g, ctx, H, rng, T = _setup(nx, nv, kind, 3 + nv, cuda)
+    pc = O.Preconditioner(g, 'lapack')
+    f = rng.random((nx, nv))
+    fh = f + 0.01 * rng.standard_normal((nx, nv))
+    u = rng.standard_normal((nx, nv))
+    y = ctx.new_field()
+    ctx.function(T(_nat(f)), T(_nat(fh)), y)
+    ref = pc.apply(O.residual(kind, f, fh, H, g))
+    assert np.abs(y.cpu().numpy().T - ref).max() <= 1e-12 * np.abs(ref).max()
+    ctx.jacobian(T(_nat(u)), y)
+    ref = pc.apply(O.jacobian(kind, u, H, g))
+    assert np.abs(y.cpu().numpy().T - ref).max() <= 1e-12 * np.abs(ref).max()
+    nrm = ctx.function_norm(T(_nat(f)), T(_nat(fh)), y)
+    refn = np.linalg.norm(O.residual(kind, f, fh, H, g))
+    assert abs(nrm - refn) <= 1e-13 * refn
+
+
+@pytest.mark.parametrize("nx,nv", [(8, 16), (64, 128), (1024, 2048)])
+@pytest.mark.parametrize("laplace", ['cfd4', 'cfd2'])
+def test_moments_and_poisson(nx, nv, laplace, cuda):
+    import torch
+    from vivlasov1d_amd.core import Grid as PGrid
+    from vivlasov1d_amd.solvers import Context
+    rng = np.random.default_rng(nx + nv)
+    g = O.Grid(nx, nv, 10 * np.pi, 10.0, ht=0.05)
+    ctx = Context(PGrid(nx, nv, 10 * np.pi, 10.0, ht=0.05), laplace=laplace, charge=-1.0)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    f = rng.random((nx, nv))
+    n_ref = O.density(f, g)
+    n = ctx.new_x()
+    ctx.density(T(_nat(f)), n)
+    assert np.abs(n.cpu().numpy() - n_ref).max() <= 1e-13 * np.abs(n_ref).max()
+    phi_ref = O.poisson_solve(O.poisson_rhs(n_ref, -1.0, nx), g.hx, laplace)
+    phi = ctx.new_x()
+    ctx.poisson_solve(T(n_ref), phi)
+    p = phi.cpu().numpy()
+    scale = np.abs(phi_ref).max()
+    assert np.abs((p - p.mean()) - (phi_ref - phi_ref.mean())).max() <= 1e-11 * scale
+    # residual of the GPU solve through the reference's function_mult
+    r = ctx.new_x()
+    ctx.poisson_function_mult(phi, T(n_ref), r)
+    r_ref = O.poisson_function_mult(p, n_ref, -1.0, g.hx2_inv, laplace)
+    assert np.abs(r.cpu().numpy() - r_ref).max() <= 1e-9 * max(1.0, np.abs(n_ref).max())
+    assert np.linalg.norm(r_ref) <= 1e-9 * np.linalg.norm(n_ref - n_ref.mean())
+    # fused moments: n, phi, h1 and ||A phi + (n - mean n) q||
+    n2, phi2, h1 = ctx.new_x(), ctx.new_x(), ctx.new_x()
+    pn = ctx.moments(T(_nat(f)), n2, phi2, h1)
+    assert np.abs(n2.cpu().numpy() - n_ref).max() <= 1e-13 * np.abs(n_ref).max()
+    h1_ref = O.potential_to_hamiltonian(phi_ref, nx)
+    assert np.abs(h1.cpu().numpy() - h1_ref).max() <= 1e-11 * scale
+    pr = np.linalg.norm(O.poisson_function_mult(phi2.cpu().numpy(), n_ref, -1.0, g.hx2_inv, laplace))
+    assert abs(pn - pr) <= 1e-6 * max(pr, 1e-300) + 1e-14
+    h1b = ctx.new_x()
+    ctx.potential_to_hamiltonian(phi2, h1b)
+    assert np.abs(h1b.cpu().numpy() - h1.cpu().numpy()).max() <= 1e-15 * scale
+
+
+def test_blas1(cuda):
+    import torch
+    from vivlasov1d_amd.core import Grid as PGrid
+    from vivlasov1d_amd.solvers import Context
+    ctx = Context(PGrid(64, 128, 4 * np.pi, 8.0))
+    rng = np.random.default_rng(5)
+    for n in (1, 7, 1000, 1 << 20):
+        a, b = rng.standard_normal(n), rng.standard_normal(n)
+        A, B = torch.from_numpy(a).to(cuda), torch.from_numpy(b).to(cuda)
+        assert abs(ctx.norm2(A) - np.linalg.norm(a)) <= 1e-13 * np.linalg.norm(a)
+        assert abs(ctx.dot(A, B) - a @ b) <= 1e-12 * np.linalg.norm(a) * np.linalg.norm(b)
+        ctx.axpy(-0.5, A, B)
+        assert np.abs(B.cpu().numpy() - (b - 0.5 * a)).max() <= 1e-15 * (np.abs(b).max() + np.abs(a).max())
+
+
+@pytest.mark.parametrize("nx,nv,kind,restart", [(16, 32, 'arakawaj4', 30), (64, 128, 'arakawaj1', 30),
+                                                (64, 128, 'arakawaj4', 3), (256, 512, 'arakawaj2', 30)])
+def test_gmres_matches_oracle(nx, nv, kind, restart, cuda):
+    print("case", nx, nv, kind, restart, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, kind, 11, cuda, restart=restart)
+    pc = O.Preconditioner(g, 'lapack')
+    f = rng.random((nx, nv))
+    f[:, :2] = 0
+    f[:, -2:] = 0
+    fh = f + 1e-3 * rng.standard_normal((nx, nv))
+    b = pc.apply(O.residual(kind, f, fh, H, g))
+    A = lambda v: pc.apply(O.jacobian(kind, v, H, g))
+    hist = []
+    x_ref, its_ref, reason_ref = O.gmres(A, b, rtol=1e-10, atol=1e-50, max_it=1000, restart=restart,
+                                         history=hist)
+    xd = ctx.new_field()
+    res = ctx.ksp_solve(T(_nat(b)), xd, 1e-10, 1e-50, 1e5, 1000)
+    assert res.reason == reason_ref, (res.reason, reason_ref)
+    assert abs(res.its - its_ref) <= 1, (res.its, its_ref)
+    x = xd.cpu().numpy().T
+    assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+    # the solution really solves the preconditioned system
+    r = b - A(x)
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(b)
+
+
+def test_gmres_max_it(cuda):
+    g, ctx, H, rng, T = _setup(64, 128, 'arakawaj4', 13, cuda, restart=5)
+    b = rng.standard_normal((64, 128))
+    xd = ctx.new_field()
+    res = ctx.ksp_solve(T(_nat(b)), xd, 1e-14, 1e-50, 1e5, 3)
+    assert res.reason == O.KSP_DIVERGED_ITS and res.its == 3
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_picard_steps_match_oracle(kind, cuda):
+    """Three implicit timesteps of the Landau problem on 32x64: the Picard
+    residual history and f agree with the oracle run to the solver
+    tolerance (ksp_rtol 1e-10, snes_atol 1e-10)."""
+    from vivlasov1d_amd.core import Config
+    from vivlasov1d_amd.runner import Runscript
+    nx, nv, L, vmax, ht = 32, 64, 4 * np.pi, 8.0, 0.05
+    prof = lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * np.exp(-0.5 * v ** 2) / math.sqrt(2 * np.pi)
+    g = O.Grid(nx, nv, L, vmax, ht=ht)
+    orun = O.OracleRun(g, O.initial_distribution(g, prof), bracket=kind, ksp_rtol=1e-10, snes_atol=1e-10)
+    cfg = Config({'grid': {'nx': nx, 'nv': nv, 'L': L, 'vmax': vmax, 'ht': ht, 'nt': 3},
+                  'solver': {'poisson_bracket': kind, 'laplace_operator': 'CFD4', 'preconditioner_type': 'tensor',
+                             'preconditioner_scheme': 'tensor_kinetic', 'petsc_ksp_rtol': 1e-10,
+                             'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
+                  'initial_data': {'charge': -1.0}})
+    from vivlasov1d_amd.core import initial_distribution
+    run = Runscript(cfg, profile=prof)
+    assert np.abs(run.fc.cpu().numpy().T - orun.fc).max() <= 1e-15
+    for s in range(3):
+        eo = orun.step()
+        eg = run.step()
+        print(s, eo['initial'], eg['initial'], len(eo['picard']), len(eg['picard']), flush=True)
+        assert abs(eg['initial'] - eo['initial']) <= 1e-9 * eo['initial']
+        assert len(eg['picard']) == len(eo['picard'])
+        for po, pg in zip(eo['picard'], eg['picard']):
+            assert abs(pg['gmres'] - po['gmres_its']) <= 1
+        f = run.fc.cpu().numpy().T
+        assert np.abs(f - orun.fc).max() <= 1e-9 * np.abs(orun.fc).max()
+    inv = run.invariants()
+    ref = O.invariants(orun.fc, orun.h0, orun.h1c, np.zeros(nx), orun.phi, g, -1.0)
+    for k in ('mass', 'l2', 'energy', 'ekin', 'epot'):
+        assert abs(inv[k] - ref[k]) <= 1e-9 * abs(ref[k]) + 1e-14, k

@@ -1,0 +1,205 @@
+"""Pins of the oracle's preconditioner, Poisson solve, moments, invariants,
+GMRES and Picard driver (CPU only)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import stencil_tables as ST
+from oracle import vlasov_oracle as O
+
+
+# ---------------------------------------------------------------- preconditioner
+def test_eigenvalues_are_centered_difference_symbol():
+    """ifftshift(exp(2 pi i k (n-1)/n) - exp(2 pi i k/n)) = 2 i sin(2 pi k/n)
+    for even n (the symbol of f_{i+1} - f_{i-1})."""
+    for n in (8, 64, 1024):
+        e = O.kinetic_eigenvalues(n)
+        k = np.arange(n)
+        np.testing.assert_allclose(e, 2j * np.sin(2 * np.pi * k / n), rtol=0, atol=1e-11)
+
+
+def test_lapack_and_scipy_statements_agree():
+    g = O.Grid(32, 64, 4 * np.pi, 8.0, ht=0.05)
+    a, b = O.Preconditioner(g, 'lapack'), O.Preconditioner(g, 'scipy')
+    x = np.random.default_rng(0).standard_normal((g.nx, g.nv))
+    ya, yb = a.apply(x), b.apply(x)
+    assert np.abs(ya - yb).max() <= 1e-13 * np.abs(ya).max()
+    assert a.pivots_are_identity()
+
+
+def test_pivots_identity_at_benchmark_sizes():
+    for (nx, nv, L, vmax, ht) in [(64, 128, 4 * np.pi, 10., 0.1), (256, 512, 4 * np.pi, 8., 0.05)]:
+        assert O.Preconditioner(O.Grid(nx, nv, L, vmax, ht=ht)).pivots_are_identity()
+
+
+def test_p_times_pinv_is_identity():
+    g = O.Grid(8, 16, 2 * np.pi, 4.0, ht=0.1)
+    pc = O.Preconditioner(g)
+    P = pc.dense_matrix()
+    x = np.random.default_rng(1).standard_normal((g.nx, g.nv))
+    y = pc.apply(x)
+    Py = (P @ y.T.reshape(-1)).reshape((g.nv, g.nx)).T
+    assert np.abs(Py - x).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_preconditioner_is_half_j1_of_h0_plus_identity_over_dt():
+    """On interior rows P u = u/dt + 1/2 B_J1(u, h0): the bands of
+    KineticFast.formBandedPreconditionerMatrix are the Fourier symbol of the
+    J1 bracket with the kinetic Hamiltonian (checked through the stencil
+    tables, not the oracle's bracket)."""
+    g = O.Grid(8, 16, 2 * np.pi, 4.0, ht=0.1)
+    pc = O.Preconditioner(g)
+    P = pc.dense_matrix()
+    H0 = np.broadcast_to(O.kinetic_hamiltonian(g)[None, :], (g.nx, g.nv)).copy()
+    M = ST.bracket_matrix('arakawaj1', H0, g, factor=0.5) + np.eye(g.nx * g.nv) / g.ht
+    interior = np.zeros((g.nv, g.nx), bool)
+    interior[2:-2, :] = True
+    rows = interior.reshape(-1)
+    assert np.abs(P[rows] - M[rows]).max() <= 1e-12 * np.abs(M).max()
+
+
+# ---------------------------------------------------------------- Poisson
+@pytest.mark.parametrize("laplace", ['cfd4', 'cfd2'])
+def test_poisson_solve_vs_dense_lstsq(laplace):
+    nx = 16
+    hx = 0.37
+    A = np.zeros((nx, nx))
+    for c in range(nx):
+        e = np.zeros(nx)
+        e[c] = 1
+        A[:, c] = O.poisson_matrix_mult(e, 1 / hx ** 2, laplace)
+    # stencil of PETScPoissonCFD4.formMat / CFD2.formMat
+    row = A[5]
+    if laplace == 'cfd4':
+        np.testing.assert_allclose(row[[3, 4, 5, 6, 7]], np.array([5, -32, 54, -32, 5]) / (12 * hx ** 2))
+    else:
+        np.testing.assert_allclose(row[[4, 5, 6]], np.array([-1, 2, -1]) / hx ** 2)
+    n = 1 + 0.3 * np.random.default_rng(2).standard_normal(nx)
+    b = O.poisson_rhs(n, -1.0, nx)
+    phi = O.poisson_solve(b, hx, laplace)
+    ref = np.linalg.lstsq(A, b, rcond=None)[0]
+    np.testing.assert_allclose(phi - phi.mean(), ref - ref.mean(), rtol=0, atol=1e-12 * np.abs(ref).max())
+    r = O.poisson_function_mult(phi, n, -1.0, 1 / hx ** 2, laplace)
+    assert np.linalg.norm(r) <= 1e-12 * np.linalg.norm(n)
+
+
+def test_poisson_eigenvalues():
+    nx, hx = 32, 0.2
+    lam = O.poisson_eigenvalues(nx, hx, 'cfd4')
+    for k in (1, 5, 16):
+        e = np.cos(2 * np.pi * k * np.arange(nx) / nx)
+        np.testing.assert_allclose(O.poisson_matrix_mult(e, 1 / hx ** 2, 'cfd4'), lam[k] * e, atol=1e-10)
+
+
+# ---------------------------------------------------------------- moments
+def test_density_and_normalisation_of_maxwellian():
+    g = O.Grid(16, 257, 4 * np.pi, 10.0)
+    f = O.initial_distribution(g, lambda x, v: O.maxwellian(v) * (1 + 0.1 * np.cos(0.5 * x)))
+    n = O.density(f, g)
+    np.testing.assert_allclose(n, 1 + 0.1 * np.cos(0.5 * g.x), rtol=1e-12)
+    f2 = O.normalise(f, g)
+    assert abs(f2.sum() * g.hv / g.nx - 1.0) <= 1e-14
+    assert np.all(f2[:, :2] == 0) and np.all(f2[:, -2:] == 0)
+
+
+def test_invariants_closed_form():
+    g = O.Grid(32, 401, 2 * np.pi, 10.0)
+    f = np.broadcast_to(O.maxwellian(g.v)[None, :], (g.nx, g.nv)).copy()
+    h0 = O.kinetic_hamiltonian(g)
+    z = np.zeros(g.nx)
+    inv = O.invariants(f, h0, z, z, z, g, -1.0)
+    L = g.L
+    assert abs(inv['mass'] - L) <= 1e-10 * L
+    assert abs(inv['l2'] - L / (2 * math.sqrt(np.pi))) <= 1e-10 * L
+    assert abs(inv['energy'] - 0.5 * L) <= 1e-10 * L
+    phi = np.sin(2 * np.pi * g.x / L)
+    inv2 = O.invariants(f, h0, z, z, phi, g, -1.0)
+    # 1/2 q sum (dphi)^2 / hx  ~  q/2 * int phi'^2 dx = q/2 * (2pi/L)^2 * L/2
+    exact = -0.5 * (2 * np.pi / L) ** 2 * L / 2
+    assert abs(inv2['efield'] - exact) <= 1e-2 * abs(exact)
+
+
+# ---------------------------------------------------------------- GMRES
+def _system(n, seed, cond=10.0):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = Q @ np.diag(np.linspace(1, cond, n)) @ Q.T + 0.3 * rng.standard_normal((n, n)) / math.sqrt(n)
+    return A, rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("restart", [50, 10, 3])
+def test_gmres_vs_dense_solve(restart):
+    A, b = _system(40, 3)
+    hist = []
+    x, its, reason = O.gmres(lambda v: A @ v, b, rtol=1e-12, atol=1e-50, max_it=5000, restart=restart,
+                             history=hist)
+    assert reason == O.KSP_CONVERGED_RTOL
+    ref = np.linalg.solve(A, b)
+    assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max()
+    assert np.linalg.norm(b - A @ x) <= 1e-11 * np.linalg.norm(b)
+    if restart >= 40:
+        assert its <= 41
+    # residual estimates never increase (GMRES minimises over a growing space;
+    # a restart recomputes b - Ax, equal up to rounding of size eps ||b||)
+    assert all(hist[i + 1] <= hist[i] * (1 + 1e-12) + 1e-13 * hist[0] for i in range(len(hist) - 1))
+
+
+def test_gmres_full_restart_equivalence():
+    A, b = _system(25, 4)
+    x1, i1, _ = O.gmres(lambda v: A @ v, b, rtol=1e-10, restart=30)
+    x2, i2, _ = O.gmres(lambda v: A @ v, b, rtol=1e-10, restart=100)
+    assert i1 == i2 and np.abs(x1 - x2).max() <= 1e-12 * np.abs(x1).max()
+
+
+def test_gmres_edge_cases():
+    A, b = _system(10, 5)
+    x, its, reason = O.gmres(lambda v: A @ v, np.zeros(10))
+    assert reason == O.KSP_CONVERGED_ATOL and its == 0 and not x.any()
+    x, its, reason = O.gmres(lambda v: A @ v, b, rtol=1e-14, max_it=3, restart=30)
+    assert reason == O.KSP_DIVERGED_ITS and its == 3
+    x, its, reason = O.gmres(lambda v: v, b, rtol=1e-10)     # identity: happy breakdown / one step
+    assert reason > 0 and its == 1 and np.abs(x - b).max() <= 1e-14
+
+
+# ---------------------------------------------------------------- driver
+def _landau_run(**kw):
+    g = O.Grid(16, 32, 4 * np.pi, 8.0, ht=0.05)
+    prof = lambda x, v: (1 + 0.3 * np.cos(0.5 * x)) * O.maxwellian(v)
+    return O.OracleRun(g, O.initial_distribution(g, prof), **kw)
+
+
+def test_driver_converges_and_conserves():
+    run = _landau_run()
+    inv0 = O.invariants(run.fc, run.h0, run.h1c, run.h2c, run.phi, run.grid, -1.0)
+    for _ in range(3):
+        e = run.step()
+        assert e['picard'][-1]['residual'] < 1e-10 or len(e['picard']) == 20
+    inv1 = O.invariants(run.fc, run.h0, run.h1c, run.h2c, run.phi, run.grid, -1.0)
+    # f vanishes near the v-boundaries, so the J4 bracket conserves mass and L2
+    assert abs(inv1['mass'] - inv0['mass']) <= 1e-10 * inv0['mass']
+    assert abs(inv1['l2'] - inv0['l2']) <= 1e-8 * inv0['l2']
+
+
+def test_driver_max_iter_branch():
+    run = _landau_run(snes_max_iter=1)
+    e = run.step()
+    assert len(e['picard']) == 1
+
+
+def test_driver_revert_branch():
+    run = _landau_run(snes_atol=0.0, snes_max_iter=50)
+    e = run.step()
+    # with snes_atol = 0 the loop only ends when the residual stops decreasing
+    # (rounding floor) or after max_iter; a rise reverts f to the last iterate
+    assert len(e['picard']) < 50
+    last = e['picard'][-1]['residual']
+    assert last >= e['picard'][-2]['residual']
+
+
+def test_driver_abort_branch(monkeypatch):
+    run = _landau_run()
+    monkeypatch.setattr(run, 'calculate_residual', lambda: 100.0)
+    with pytest.raises(RuntimeError):
+        run.step()

@@ -1,0 +1,227 @@
+// Arakawa J1 / J2 / J4 Poisson-bracket kernels: the implicit-midpoint
+// residual F(f), the Jacobian-vector product J u and the plain accumulation
+// y += factor {x, H}.
+//
+// This translation unit is compiled with -ffp-contract=off: every stencil
+// expression below keeps the reference's IEEE operation order
+// (PoissonBracket.pyx:100-113, 221-234, 360-389; C left-to-right), so the
+// result is bit-identical to the reference's unfused double arithmetic.
+//
+// Layout: natural order, element (i, j) at i + nx*j (x contiguous).
+// Tiling: one workgroup computes a 64 (x) x 32 (v) tile.  The tile of the
+// bracket's first argument and of H_ave, both with a 2-cell halo, is staged
+// in LDS; H_ave is composed on the fly from its 1D factors
+// ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i), exactly the
+// value the reference's 2D Have vector holds (PETScVlasovSolver.pyx:145-149),
+// so it costs no HBM traffic.  Each wavefront owns 8 rows of the tile; lane
+// l computes column i0 + l, so global loads and stores are 512-byte rows.
+
+#include "vv1d_internal.h"
+
+namespace vv {
+namespace {
+
+constexpr int TX = 64;
+constexpr int TY = 32;
+constexpr int HX = TX + 4;
+constexpr int HY = TY + 4;
+constexpr int NT = 256;
+constexpr int ROWS_PER_WAVE = TY / (NT / 64);
+
+enum { MODE_RESIDUAL = 0, MODE_JACOBIAN = 1, MODE_ACCUMULATE = 2 };
+
+struct BrArgs {
+    int nx, nv;
+    const double *f;       // residual: f; jacobian: u; accumulate: x
+    const double *fh;      // residual: f at the previous time level
+    const double *uscale;  // jacobian: optional device scale of u (unnormalised Krylov vector)
+    double *y;
+    const double *h0, *h1p, *h1h, *h2p, *h2h;
+    double ht_inv, af1, af2, factor;
+    const int32_t *done;   // optional early-exit flag (GMRES cycle finished)
+};
+
+template <int KIND, int MODE>
+__global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
+{
+    if (a.done != nullptr && *a.done) return;
+    __shared__ double sF[HY][HX];
+    __shared__ double sH[HY][HX];
+
+    const int nx = a.nx, nv = a.nv;
+    const int ntx = (nx + TX - 1) / TX;
+    const int nty = (nv + TY - 1) / TY;
+    const int t = xcd_tile(blockIdx.x, ntx * nty);
+    const int i0 = (t % ntx) * TX;
+    const int j0 = (t / ntx) * TY;
+    const double us = (MODE == MODE_JACOBIAN && a.uscale != nullptr) ? *a.uscale : 1.0;
+
+    for (int idx = threadIdx.x; idx < HY * HX; idx += NT) {
+        const int r = idx / HX;
+        const int c = idx - r * HX;
+        const int gj = j0 - 2 + r;
+        int gi = (i0 - 2 + c) % nx;
+        if (gi < 0) gi += nx;
+        double fv = 0.0, hv = 0.0;
+        if (gj >= 0 && gj < nv) {
+            const size_t o = (size_t)gj * nx + gi;
+            if (MODE == MODE_RESIDUAL) {
+                fv = 0.5 * a.fh[o] + 0.5 * a.f[o];           // Fave (PETScNLVlasovMP.pyx:44-46)
+            } else if (MODE == MODE_JACOBIAN) {
+                fv = a.f[o] * us;                            // == VecScale'd Krylov vector
+            } else {
+                fv = a.f[o];
+            }
+            hv = (((a.h0[gj] + 0.5 * a.h1p[gi]) + 0.5 * a.h1h[gi]) + 0.5 * a.h2p[gi]) + 0.5 * a.h2h[gi];
+        }
+        sF[r][c] = fv;
+        sH[r][c] = hv;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int i = i0 + lane;
+    if (i >= nx) return;
+    const int cx = lane + 2;
+
+#define F(di, dj) sF[cy + (dj)][cx + (di)]
+#define H(di, dj) sH[cy + (dj)][cx + (di)]
+    for (int q = 0; q < ROWS_PER_WAVE; ++q) {
+        const int rr = wave * ROWS_PER_WAVE + q;
+        const int j = j0 + rr;
+        if (j >= nv) break;
+        const int cy = rr + 2;
+        const size_t o = (size_t)j * nx + i;
+
+        double base;
+        if (MODE == MODE_RESIDUAL) {
+            const double fder = a.f[o] - a.fh[o];            // Fder = (0 + F) - Fh
+            base = 0.0 + a.ht_inv * fder;                    // Y = 0; Y += Fder/dt
+        } else if (MODE == MODE_JACOBIAN) {
+            base = 0.0 + a.ht_inv * F(0, 0);                 // Y = 0; Y += u/dt
+        } else {
+            base = a.y[o];
+        }
+        double out = base;
+        if (j >= 2 && j < nv - 2) {
+            if (KIND == VV_BRACKET_J1 || KIND == VV_BRACKET_J4) {
+                // PoissonBracket.pyx:100-111 (J1 / second order)
+                const double jpp = (F(1, 0) - F(-1, 0)) * (H(0, 1) - H(0, -1))
+                                 - (F(0, 1) - F(0, -1)) * (H(1, 0) - H(-1, 0));
+                const double jpc = F(1, 0) * (H(1, 1) - H(1, -1))
+                                 - F(-1, 0) * (H(-1, 1) - H(-1, -1))
+                                 - F(0, 1) * (H(1, 1) - H(-1, 1))
+                                 + F(0, -1) * (H(1, -1) - H(-1, -1));
+                const double jcp = F(1, 1) * (H(0, 1) - H(1, 0))
+                                 - F(-1, -1) * (H(-1, 0) - H(0, -1))
+                                 - F(-1, 1) * (H(0, 1) - H(-1, 0))
+                                 + F(1, -1) * (H(1, 0) - H(0, -1));
+                if (KIND == VV_BRACKET_J1) {
+                    out = base + a.factor * (jpp + jpc + jcp) * a.af1;
+                } else {
+                    // PoissonBracket.pyx:373-389 (J4 = 2 J1 - J2)
+                    const double jcc2 = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
+                                      - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
+                    const double jpc2 = F(2, 0) * (H(1, 1) - H(1, -1))
+                                      - F(-2, 0) * (H(-1, 1) - H(-1, -1))
+                                      - F(0, 2) * (H(1, 1) - H(-1, 1))
+                                      + F(0, -2) * (H(1, -1) - H(-1, -1));
+                    const double jcp2 = F(1, 1) * (H(0, 2) - H(2, 0))
+                                      - F(-1, -1) * (H(-2, 0) - H(0, -2))
+                                      - F(-1, 1) * (H(0, 2) - H(-2, 0))
+                                      + F(1, -1) * (H(2, 0) - H(0, -2));
+                    const double r1 = (jpp + jpc + jcp) * a.af1;
+                    const double r2 = (jcc2 + jpc2 + jcp2) * a.af2;
+                    out = base + a.factor * (2. * r1 - r2);
+                }
+            } else {
+                // PoissonBracket.pyx:221-234 (J2)
+                const double jcc = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
+                                 - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
+                const double jpc = F(2, 0) * (H(1, 1) - H(1, -1))
+                                 - F(-2, 0) * (H(-1, 1) - H(-1, -1))
+                                 - F(0, 2) * (H(1, 1) - H(-1, 1))
+                                 + F(0, -2) * (H(1, -1) - H(-1, -1));
+                const double jcp = F(1, 1) * (H(0, 2) - H(2, 0))
+                                 - F(-1, -1) * (H(-2, 0) - H(0, -2))
+                                 - F(-1, 1) * (H(0, 2) - H(-2, 0))
+                                 + F(1, -1) * (H(2, 0) - H(0, -2));
+                out = base + a.factor * (jcc + jpc + jcp) * a.af2;
+            }
+        }
+        a.y[o] = out;
+    }
+#undef F
+#undef H
+}
+
+template <int MODE>
+hipError_t launch_mode(vv_ctx *c, BrArgs a)
+{
+    const int ntx = (c->p.nx + TX - 1) / TX;
+    const int nty = (c->p.nv + TY - 1) / TY;
+    dim3 grid(ntx * nty), block(NT);
+    switch (c->p.bracket) {
+    case VV_BRACKET_J1:
+        hipLaunchKernelGGL((bracket_kernel<VV_BRACKET_J1, MODE>), grid, block, 0, c->stream, a);
+        break;
+    case VV_BRACKET_J2:
+        hipLaunchKernelGGL((bracket_kernel<VV_BRACKET_J2, MODE>), grid, block, 0, c->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL((bracket_kernel<VV_BRACKET_J4, MODE>), grid, block, 0, c->stream, a);
+        break;
+    }
+    return hipGetLastError();
+}
+
+BrArgs base_args(vv_ctx *c)
+{
+    BrArgs a{};
+    a.nx = c->p.nx;
+    a.nv = c->p.nv;
+    a.h0 = c->h0;
+    a.h1p = c->h1p;
+    a.h1h = c->h1h;
+    a.h2p = c->h2p;
+    a.h2h = c->h2h;
+    a.ht_inv = c->ht_inv;
+    a.af1 = c->af1;
+    a.af2 = c->af2;
+    return a;
+}
+
+}  // namespace
+
+hipError_t launch_residual(vv_ctx *c, const double *f, const double *fh, double *y)
+{
+    BrArgs a = base_args(c);
+    a.f = f;
+    a.fh = fh;
+    a.y = y;
+    a.factor = 1.0;                 // B(Fave, H, 1.0)  (PETScNLVlasovMP.pyx:55)
+    return launch_mode<MODE_RESIDUAL>(c, a);
+}
+
+hipError_t launch_jacobian(vv_ctx *c, const double *u, const double *uscale, double *y)
+{
+    BrArgs a = base_args(c);
+    a.f = u;
+    a.uscale = uscale;
+    a.y = y;
+    a.factor = 0.5;                 // B(u, H, 0.5)  (PETScNLVlasovMP.pyx:37)
+    a.done = uscale != nullptr ? &c->gs->st.done : nullptr;
+    return launch_mode<MODE_JACOBIAN>(c, a);
+}
+
+hipError_t launch_bracket_add(vv_ctx *c, const double *x, double *y, double factor)
+{
+    BrArgs a = base_args(c);
+    a.f = x;
+    a.y = y;
+    a.factor = factor;
+    return launch_mode<MODE_ACCUMULATE>(c, a);
+}
+
+}  // namespace vv

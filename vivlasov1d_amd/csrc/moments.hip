@@ -1,0 +1,276 @@
+// Moments, the periodic Poisson solve and the invariants.
+//
+//  density_kernel    column partial sums of f over chunks of v-rows
+//                    (n_i = hv * sum_j f_ij, Toolbox.pyx:79-98); coalesced
+//                    along x, one chunk of rows per block row.
+//  poisson_kernel    one workgroup: finishes n, forms b = -(n - mean n) q
+//                    (PETScPoissonSolver.pyx:58-74), solves A phi = b by the
+//                    mean-free circulant inverse (the periodic CFD4/CFD2
+//                    matrix is circulant; its pseudo-inverse's first row is
+//                    precomputed on the host), forms h1 = phi - mean phi
+//                    (Toolbox.pyx:38-74) and the Poisson residual norm
+//                    ||A phi + (n - mean n) q|| of run_base.py:631-637.
+//  invariants_kernel mass, L2, kinetic / potential energy over f, plus the
+//                    field energy over phi (distribution_function.pyx:130-199,
+//                    hamiltonian.pyx:67-100, potential.pyx:48-59).
+#include "vv1d_internal.h"
+
+namespace vv {
+namespace {
+
+constexpr int NT = 256;
+constexpr int PT = 1024;     // threads of the single-workgroup Poisson kernel
+constexpr int DCH = 64;      // row chunks of the density reduction
+
+__global__ __launch_bounds__(NT) void density_kernel(const double *__restrict__ f, int nx, int nv,
+                                                     double *__restrict__ npart)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= nx) return;
+    const int rows = (nv + DCH - 1) / DCH;
+    const int j0 = blockIdx.y * rows;
+    const int j1 = min(nv, j0 + rows);
+    double s = 0.0;
+    for (int j = j0; j < j1; ++j) s += f[(size_t)j * nx + i];
+    npart[(size_t)blockIdx.y * nx + i] = s;
+}
+
+__device__ double block_sum_big(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < PT / 64; ++w) s += red[w];
+        red[PT / 64] = s;
+    }
+    __syncthreads();
+    return red[PT / 64];    // valid in every thread
+}
+
+struct PoiArgs {
+    int nx, laplace, from_part, nch;
+    double hv, charge, hx2_inv;
+    const double *npart;   // density partials (from_part = 1)
+    const double *n_in;    // density (from_part = 0)
+    const double *pinv;
+    double *n_out, *phi, *h1, *pres;
+};
+
+// A x for the periodic CFD4 / CFD2 stencils, in the reference's order
+// (PETScPoissonCFD4.pyx:78, PETScPoissonCFD2.pyx:78).
+__device__ __forceinline__ double lap(const double *x, int i, int nx, int laplace, double hx2_inv)
+{
+    const int m = nx - 1;
+    const double xm2 = x[(i - 2) & m], xm1 = x[(i - 1) & m], x0 = x[i], xp1 = x[(i + 1) & m], xp2 = x[(i + 2) & m];
+    if (laplace == VV_LAPLACE_CFD4)
+        return (5. * xm2 - 32. * xm1 + 54. * x0 - 32. * xp1 + 5. * xp2) * hx2_inv / 12.;
+    return (2. * x0 - xm1 - xp1) * hx2_inv;
+}
+
+__global__ __launch_bounds__(PT) void poisson_kernel(PoiArgs a)
+{
+    extern __shared__ double sm[];
+    double *sn = sm, *sb = sm + a.nx, *sc = sm + 2 * a.nx, *sp = sm + 3 * a.nx;
+    __shared__ double red[PT / 64 + 1];
+    const int nx = a.nx;
+    double part = 0.0;
+    for (int i = threadIdx.x; i < nx; i += PT) {
+        double n;
+        if (a.from_part) {
+            double s = 0.0;
+            for (int ch = 0; ch < a.nch; ++ch) s += a.npart[(size_t)ch * nx + i];
+            n = s * a.hv;
+        } else {
+            n = a.n_in[i];
+        }
+        sn[i] = n;
+        sc[i] = a.pinv[i];
+        if (a.n_out) a.n_out[i] = n;
+        part += n;
+    }
+    const double nmean = block_sum_big(part, red) / nx;
+    for (int i = threadIdx.x; i < nx; i += PT) sb[i] = -(sn[i] - nmean) * a.charge;
+    __syncthreads();
+    // phi = C b, C the mean-free circulant inverse of A
+    part = 0.0;
+    for (int i = threadIdx.x; i < nx; i += PT) {
+        double s = 0.0;
+        for (int m = 0; m < nx; ++m) s += sc[m] * sb[(i - m) & (nx - 1)];
+        sp[i] = s;
+        part += s;
+    }
+    const double phiave = block_sum_big(part, red) / nx;
+    part = 0.0;
+    for (int i = threadIdx.x; i < nx; i += PT) {
+        if (a.phi) a.phi[i] = sp[i];
+        if (a.h1) a.h1[i] = sp[i] - phiave;
+        const double r = lap(sp, i, nx, a.laplace, a.hx2_inv) + (sn[i] - nmean) * a.charge;
+        part += r * r;
+    }
+    const double rr = block_sum_big(part, red);
+    if (threadIdx.x == 0 && a.pres) *a.pres = sqrt(rr);
+}
+
+// y = A x (+ (n - mean n) q when n != null)
+__global__ __launch_bounds__(PT) void poisson_mult_kernel(const double *x, const double *n, double *y, int nx,
+                                                          int laplace, double hx2_inv, double charge)
+{
+    extern __shared__ double sm[];
+    __shared__ double red[PT / 64 + 1];
+    double part = 0.0;
+    for (int i = threadIdx.x; i < nx; i += PT) {
+        sm[i] = x[i];
+        if (n) part += n[i];
+    }
+    const double nmean = block_sum_big(part, red) / nx;
+    for (int i = threadIdx.x; i < nx; i += PT) {
+        double v = lap(sm, i, nx, laplace, hx2_inv);
+        if (n) v = v + (n[i] - nmean) * charge;
+        y[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(PT) void pot_to_ham_kernel(const double *phi, double *h1, int nx)
+{
+    __shared__ double red[PT / 64 + 1];
+    double part = 0.0;
+    for (int i = threadIdx.x; i < nx; i += PT) part += phi[i];
+    const double ave = block_sum_big(part, red) / nx;
+    for (int i = threadIdx.x; i < nx; i += PT) h1[i] = phi[i] - ave;
+}
+
+__global__ __launch_bounds__(NT) void density_finish_kernel(const double *npart, int nch, int nx, double hv,
+                                                            double *n)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= nx) return;
+    double s = 0.0;
+    for (int ch = 0; ch < nch; ++ch) s += npart[(size_t)ch * nx + i];
+    n[i] = s * hv;
+}
+
+struct InvArgs {
+    const double *f, *h0, *h1, *h2, *phi;
+    int nx, nv;
+    double hx, hv, charge;
+    double *partial;      // [gridDim.x][5]
+    uint32_t *counter;
+    double *out;          // 6 doubles
+};
+
+__global__ __launch_bounds__(NT) void invariants_kernel(InvArgs a)
+{
+    __shared__ double red[5][NT / 64];
+    __shared__ int s_last;
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t N = (int64_t)a.nx * a.nv;
+    for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < N; e += (int64_t)gridDim.x * NT) {
+        const int j = (int)(e / a.nx);
+        const int i = (int)(e - (int64_t)j * a.nx);
+        const double fv = a.f[e];
+        if (j < a.nv - 1) {
+            const int ip = (i + 1 == a.nx) ? 0 : i + 1;
+            s[0] += (fv + a.f[(size_t)j * a.nx + ip] + a.f[(size_t)(j + 1) * a.nx + ip] + a.f[e + a.nx]);
+            s[1] += fv * fv;
+        }
+        s[2] += fv * a.h0[j];
+        s[3] += fv * a.h1[i];
+        s[4] += fv * a.h2[i];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < 5; ++q) {
+        double v = s[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (lane == 0) red[q][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double v = 0.0;
+        for (int w = 0; w < NT / 64; ++w) v += red[threadIdx.x][w];
+        a.partial[(size_t)blockIdx.x * 5 + threadIdx.x] = v;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(a.counter, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (threadIdx.x != 0) return;
+    double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < (int)gridDim.x; ++b)
+        for (int q = 0; q < 5; ++q) t[q] += a.partial[(size_t)b * 5 + q];
+    *a.counter = 0u;
+    const double hxhv = a.hx * a.hv;
+    const double mass = t[0] * 0.25 * a.hx * a.hv;
+    const double l2 = hxhv * t[1];
+    const double ekin = t[2] * a.hx * a.hv;
+    const double epot = (t[3] + t[4]) * a.hx * a.hv;
+    double ef = 0.0;
+    for (int i = 0; i < a.nx; ++i) {
+        const double d = a.phi[(i + 1) % a.nx] - a.phi[i];
+        ef += d * d;
+    }
+    a.out[0] = mass;
+    a.out[1] = l2;
+    a.out[2] = ekin + 0.5 * epot;
+    a.out[3] = ekin;
+    a.out[4] = epot;
+    a.out[5] = 0.5 * a.charge * ef / a.hx;
+}
+
+}  // namespace
+
+hipError_t launch_density_partial(vv_ctx *c, const double *f)
+{
+    dim3 grid((c->p.nx + NT - 1) / NT, DCH);
+    hipLaunchKernelGGL(density_kernel, grid, dim3(NT), 0, c->stream, f, c->p.nx, c->p.nv, c->npart);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(vv_ctx *c, const double *f, double *n)
+{
+    hipError_t e = launch_density_partial(c, f);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(density_finish_kernel, dim3((c->p.nx + NT - 1) / NT), dim3(NT), 0, c->stream, c->npart, DCH,
+                       c->p.nx, c->p.hv, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_poisson(vv_ctx *c, const double *n_in, double *n_out, double *phi, double *h1, double *pres,
+                          int from_part)
+{
+    PoiArgs a{c->p.nx, c->p.laplace, from_part, DCH, c->p.hv, c->p.charge, c->hx2_inv,
+              c->npart, n_in, c->pinv, n_out, phi, h1, pres};
+    const size_t lds = (size_t)4 * c->p.nx * sizeof(double);
+    hipLaunchKernelGGL(poisson_kernel, dim3(1), dim3(PT), lds, c->stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_poisson_mult(vv_ctx *c, const double *x, const double *n, double *y)
+{
+    const size_t lds = (size_t)c->p.nx * sizeof(double);
+    hipLaunchKernelGGL(poisson_mult_kernel, dim3(1), dim3(PT), lds, c->stream, x, n, y, c->p.nx, c->p.laplace,
+                       c->hx2_inv, c->p.charge);
+    return hipGetLastError();
+}
+
+hipError_t launch_pot_to_ham(vv_ctx *c, const double *phi, double *h1)
+{
+    hipLaunchKernelGGL(pot_to_ham_kernel, dim3(1), dim3(PT), 0, c->stream, phi, h1, c->p.nx);
+    return hipGetLastError();
+}
+
+hipError_t launch_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1, const double *h2,
+                             const double *phi, double *out_dev)
+{
+    InvArgs a{f, h0, h1, h2, phi, c->p.nx, c->p.nv, c->p.hx, c->p.hv, c->p.charge,
+              c->partial, &c->gs->counter[CNT_INV], out_dev};
+    hipLaunchKernelGGL(invariants_kernel, dim3(kReduceBlocks), dim3(NT), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace vv

@@ -1,0 +1,142 @@
+"""Time loop of the split-mode, matrix-free, nonlinear implicit-midpoint run.
+
+``Runscript`` mirrors ``viVlasov1Drunscript`` of
+vlasov/run/run_nonlinear_matrixfree_split_mp.py:15-174 together with the
+parts of ``viVlasov1Dbase`` (vlasov/run/run_base.py:35-807) the timestep
+uses: initial data and normalisation, moments + Poisson, residual, history,
+and the Picard loop with its stop / revert / abort rules.  HDF5 output and
+the other run modes are not part of this library.
+"""
+
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import Config, Grid, initial_distribution, kinetic_hamiltonian, normalise
+from .solvers import (Context, PETScPoissonSolver, PETScVlasovSolver, SNES, Toolbox)
+
+
+class ResidualTooLarge(RuntimeError):
+    """Raised where the reference exits: residual > 10 after a step (:165-168)."""
+
+
+class Runscript:
+    def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False):
+        if not isinstance(cfg, Config):
+            cfg = Config(cfg)
+        self.cfg = cfg
+        self.grid = grid = cfg.make_grid()
+        self.verbose = verbose
+        self.charge = cfg['initial_data']['charge']
+        self.mass = cfg['initial_data']['mass']
+        sol = cfg['solver']
+        self.ctx = Context(grid, bracket=cfg.get_poisson_bracket(), preconditioner=cfg.get_preconditioner(),
+                           laplace=cfg.get_laplace_operator(), charge=self.charge, restart=restart)
+        ctx = self.ctx
+        # vectors (run_base.py:192-233); x-vectors are 1D, the Hamiltonians are their factors
+        self.h0 = torch.from_numpy(kinetic_hamiltonian(grid, self.mass)).cuda()
+        self.h1c, self.h1h = ctx.new_x(), ctx.new_x()
+        self.h2c, self.h2h = ctx.new_x(), ctx.new_x()
+        self.fc, self.fh, self.fl, self.fb = ctx.new_field(), ctx.new_field(), ctx.new_field(), ctx.new_field()
+        self.N = ctx.new_x()
+        self.pc_int, self.ph_int = ctx.new_x(), ctx.new_x()
+        self.pc_ext = ctx.new_x()
+        self.toolbox = Toolbox(None, None, grid, ctx)
+        # initial data (run_base.py:408-506)
+        if f0 is None:
+            if profile is None:
+                raise ValueError("give an initial profile(x, v) or f0")
+            f0 = initial_distribution(grid, profile)
+        f0 = normalise(np.asarray(f0, dtype=np.float64).reshape(grid.nv, grid.nx), grid)
+        self.fc.copy_(torch.from_numpy(f0))
+        self.pnorm = 0.0
+        self.calculate_moments()
+        # solver objects (run_nonlinear_matrixfree_split_mp.py:26-80)
+        self.vlasov_solver = PETScVlasovSolver(cfg, None, None, grid, self.h0, self.h1c, self.h1h, self.h2c,
+                                               self.h2h, self.pc_int, self.charge, context=ctx)
+        self.poisson_solver = PETScPoissonSolver(None, grid.nx, grid.hx, self.charge,
+                                                 laplace=cfg.get_laplace_operator(), context=ctx)
+        self.snes = SNES()
+        self.snes.setType('ksponly')
+        self.snes.setFunction(self.vlasov_solver.function_snes_mult, self.fb)
+        self.snes.getKSP().setTolerances(rtol=sol['petsc_ksp_rtol'], atol=sol['petsc_ksp_atol'],
+                                         max_it=sol['petsc_ksp_max_iter'])
+        self.snes_atol = sol['petsc_snes_atol']
+        self.snes_max_iter = sol['petsc_snes_max_iter']
+        self.itime = 0
+        self.history = []
+        self.log = log
+
+    # run_base.py:631-637 --------------------------------------------------
+    def calculate_residual(self):
+        """||F(fc)||_2 + ||A phi + (n - mean n) q||_2.  The Poisson part is the
+        residual of the last moments/Poisson solve, which the fused moments
+        kernel returns (the reference recomputes it from the same phi, n)."""
+        fnorm = self.vlasov_solver.function_norm(self.fc, self.fb)
+        return fnorm + self.pnorm
+
+    # run_base.py:640-704, 741-742 -------------------------------------------
+    def calculate_moments(self):
+        self.pnorm = self.ctx.moments(self.fc, self.N, self.pc_int, self.h1c, want_norm=True)
+
+    # run_base.py:749-763 ----------------------------------------------------
+    def make_history(self):
+        self.fh.copy_(self.fc)
+        self.h1h.copy_(self.h1c)
+        self.h2h.copy_(self.h2c)
+        self.ph_int.copy_(self.pc_int)
+        self.vlasov_solver.update_history(self.fc)
+
+    def step(self):
+        """One timestep (run_nonlinear_matrixfree_split_mp.py:112-171)."""
+        self.itime += 1
+        self.make_history()
+        # calculate_external: no external potential -> h2c stays 0 (run_base.py:707-725)
+        self.vlasov_solver.update_previous(self.fc)
+        pred = self.calculate_residual()
+        rec = dict(step=self.itime, initial=pred, picard=[])
+        i = 0
+        while True:
+            i += 1
+            self.fl.copy_(self.fc)
+            self.snes.solve(None, self.fc)
+            self.calculate_moments()
+            self.vlasov_solver.update_previous(self.fc)
+            prev, pred = pred, self.calculate_residual()
+            ksp = self.snes.getKSP()
+            rec['picard'].append(dict(gmres=ksp.getIterationNumber(), reason=ksp.getConvergedReason(),
+                                      residual=pred))
+            if self.verbose:
+                print("  Nonlinear Solver: %5i GMRES  iterations, residual = %24.16E"
+                      % (ksp.getIterationNumber(), pred), flush=True)
+            if (pred > prev and i > 1) or pred < self.snes_atol or i >= self.snes_max_iter:
+                if pred > prev:
+                    self.fc.copy_(self.fl)
+                    self.calculate_moments()
+                    rec['reverted'] = True
+                break
+        rec['final'] = pred
+        self.history.append(rec)
+        if self.log is not None:
+            self.log.write(json.dumps(rec) + "\n")
+        if pred > 10.:
+            raise ResidualTooLarge("ERROR: Residual of nonlinear solver too large.")
+        return rec
+
+    def run(self, nt=None):
+        nt = self.grid.nt if nt is None else nt
+        for _ in range(nt):
+            self.step()
+
+    def invariants(self):
+        return self.ctx.invariants(self.fc, self.h0, self.h1c, self.h2c, self.pc_int)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.ctx.synchronize()

@@ -1,0 +1,522 @@
+"""Solver objects with the reference's class and callback interface, backed
+by libvv1d.so on the GPU.
+
+Reference counterparts (paths in the viVlasov1D tree):
+
+* ``PETScVlasovSolver``      vlasov/solvers/vlasov/PETScVlasovSolver.pyx:19-205
+                             + PETScNLVlasovMP.pyx:15-58
+* ``TensorProductPreconditioner``  vlasov/solvers/preconditioner/TensorProduct.pyx:66-86
+* ``PoissonBracket``, ``TimeDerivative``  vlasov/solvers/components/*.pyx factories
+* ``PETScPoissonSolver``     vlasov/solvers/poisson/PETScPoissonSolver.pyx,
+                             PETScPoissonCFD4.pyx, PETScPoissonCFD2.pyx
+* ``Toolbox``                vlasov/toolbox/Toolbox.pyx:38-224
+* ``KSP``, ``SNES``          the PETSc objects configured at
+                             run_nonlinear_matrixfree_split_mp.py:42-80
+
+Vectors are torch float64 tensors on the GPU.  A 2D field is C-contiguous
+with shape (nv, nx): element (i, j) at i + nx*j, the reference's PETSc
+natural order.  x-vectors have shape (nx,), v-vectors (nv,).  The
+Hamiltonian pieces are passed by their 1D factors: H0 as the (nv,) kinetic
+profile, H1/H2 as (nx,) potentials -- the only form the reference ever
+builds (Toolbox.pyx:59, :187).
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import Grid, kinetic_eigenvalues
+
+DOUBLE = torch.float64
+
+
+def _dev_ptr(t, numel=None, name="vector"):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor" % name)
+    if t.dtype != DOUBLE or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("%s must be a contiguous float64 CUDA tensor" % name)
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s has %d elements, expected %d" % (name, t.numel(), numel))
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One libvv1d context: grid, operator choice and device scratch."""
+
+    def __init__(self, grid, bracket='arakawaj4', preconditioner='tensor_kinetic', laplace='cfd4',
+                 charge=-1.0, restart=30, stream=None):
+        lib = _lib.load()
+        if bracket not in _lib.BRACKETS:
+            raise _lib.VVError("Poisson bracket %r is not provided (arakawaj1/2/4)" % (bracket,))
+        if preconditioner not in _lib.PRECONDITIONERS:
+            raise _lib.VVError("preconditioner %r is not provided (tensor_kinetic or none)" % (preconditioner,))
+        if laplace not in _lib.LAPLACE:
+            raise _lib.VVError("laplace operator %r is not provided (cfd2/cfd4)" % (laplace,))
+        self.grid = grid
+        self.bracket, self.preconditioner, self.laplace = bracket, preconditioner, laplace
+        self.charge = float(charge)
+        self._v = np.ascontiguousarray(grid.v, dtype=np.float64)
+        eig = kinetic_eigenvalues(grid.nx)
+        self._eig = np.ascontiguousarray(np.stack([eig.real, eig.imag], axis=1).reshape(-1))
+        p = _lib.VVParams()
+        p.nx, p.nv = grid.nx, grid.nv
+        p.hx, p.hv, p.ht = grid.hx, grid.hv, grid.ht
+        p.charge = self.charge
+        p.bracket = _lib.BRACKETS[bracket]
+        p.precond = _lib.PRECONDITIONERS[preconditioner]
+        p.laplace = _lib.LAPLACE[laplace]
+        p.restart = int(restart)
+        p.v = self._v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        p.eigen = self._eig.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._ptr = ctypes.c_void_p()
+        ret = lib.vv_ctx_create(ctypes.byref(p), ctypes.c_void_p(stream), ctypes.byref(self._ptr))
+        _lib.check(ret, None)
+        self.lib = lib
+        self.N = grid.nx * grid.nv
+
+    @property
+    def handle(self):
+        return self._ptr
+
+    def call(self, name, *args):
+        return _lib.check(getattr(self.lib, name)(self._ptr, *args), self._ptr)
+
+    def close(self):
+        if getattr(self, "_ptr", None) and self._ptr.value:
+            self.lib.vv_ctx_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # fields ----------------------------------------------------------------
+    def F(self, t, name="field"):
+        return _dev_ptr(t, self.N, name)
+
+    def X(self, t, name="x-vector"):
+        return _dev_ptr(t, self.grid.nx, name)
+
+    def V(self, t, name="v-vector"):
+        return _dev_ptr(t, self.grid.nv, name)
+
+    def new_field(self):
+        return torch.zeros((self.grid.nv, self.grid.nx), dtype=DOUBLE, device='cuda')
+
+    def new_x(self):
+        return torch.zeros(self.grid.nx, dtype=DOUBLE, device='cuda')
+
+    # operators ---------------------------------------------------------------
+    def set_hamiltonian(self, h0, h1p, h1h, h2p, h2h):
+        self.call("vv_set_hamiltonian", self.V(h0, "H0"), self.X(h1p, "H1p"), self.X(h1h, "H1h"),
+                  self.X(h2p, "H2p"), self.X(h2h, "H2h"))
+
+    def residual(self, f, fh, y):
+        self.call("vv_residual", self.F(f), self.F(fh), self.F(y))
+
+    def jacobian_apply(self, u, y):
+        self.call("vv_jacobian_apply", self.F(u), self.F(y))
+
+    def bracket_add(self, x, y, factor):
+        self.call("vv_bracket_add", self.F(x), self.F(y), ctypes.c_double(factor))
+
+    def precond(self, x, y):
+        self.call("vv_precond_apply", self.F(x), self.F(y))
+
+    def function(self, f, fh, y):
+        self.call("vv_function", self.F(f), self.F(fh), self.F(y))
+
+    def jacobian(self, u, y):
+        self.call("vv_jacobian", self.F(u), self.F(y))
+
+    def function_norm(self, f, fh, y):
+        out = ctypes.c_double()
+        self.call("vv_function_norm", self.F(f), self.F(fh), self.F(y), ctypes.byref(out))
+        return out.value
+
+    def density(self, f, n):
+        self.call("vv_density", self.F(f), self.X(n))
+
+    def poisson_solve(self, n, phi):
+        self.call("vv_poisson_solve", self.X(n), self.X(phi))
+
+    def poisson_matrix_mult(self, x, y):
+        self.call("vv_poisson_matrix_mult", self.X(x), self.X(y))
+
+    def poisson_function_mult(self, x, n, y):
+        self.call("vv_poisson_function_mult", self.X(x), self.X(n), self.X(y))
+
+    def potential_to_hamiltonian(self, phi, h1):
+        self.call("vv_potential_to_hamiltonian", self.X(phi), self.X(h1))
+
+    def moments(self, f, n, phi, h1, want_norm=True):
+        out = ctypes.c_double()
+        self.call("vv_moments", self.F(f), self.X(n), self.X(phi), self.X(h1),
+                  ctypes.byref(out) if want_norm else None)
+        return out.value if want_norm else None
+
+    def norm2(self, x):
+        out = ctypes.c_double()
+        self.call("vv_norm2", _dev_ptr(x), ctypes.c_int64(x.numel()), ctypes.byref(out))
+        return out.value
+
+    def dot(self, x, y):
+        out = ctypes.c_double()
+        if x.numel() != y.numel():
+            raise ValueError("dot: size mismatch")
+        self.call("vv_dot", _dev_ptr(x), _dev_ptr(y), ctypes.c_int64(x.numel()), ctypes.byref(out))
+        return out.value
+
+    def axpy(self, alpha, x, y):
+        if x.numel() != y.numel():
+            raise ValueError("axpy: size mismatch")
+        self.call("vv_axpy", ctypes.c_double(alpha), _dev_ptr(x), _dev_ptr(y), ctypes.c_int64(x.numel()))
+
+    def ksp_solve(self, b, x, rtol, atol, dtol, max_it):
+        kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
+        res = _lib.KspResult()
+        self.call("vv_ksp_solve", self.F(b), self.F(x), ctypes.byref(kp), ctypes.byref(res))
+        return res
+
+    def snes_ksponly(self, f, fh, rtol, atol, dtol, max_it):
+        kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
+        res = _lib.KspResult()
+        self.call("vv_snes_ksponly", self.F(f), self.F(fh), ctypes.byref(kp), ctypes.byref(res))
+        return res
+
+    def invariants(self, f, h0, h1, h2, phi):
+        out = (ctypes.c_double * 6)()
+        self.call("vv_invariants", self.F(f), self.V(h0), self.X(h1), self.X(h2), self.X(phi), out)
+        keys = ("mass", "l2", "energy", "ekin", "epot", "efield")
+        return dict(zip(keys, list(out)))
+
+    def profile(self, enable=True):
+        self.call("vv_profile_enable", ctypes.c_int(1 if enable else 0))
+
+    def profile_read(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self.call("vv_profile_read", kernel.encode(), ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    def synchronize(self):
+        self.call("vv_synchronize")
+
+
+# ---------------------------------------------------------------------------
+# component factories (names and keys of the reference)
+# ---------------------------------------------------------------------------
+class PoissonBracket:
+    """PoissonBracket.create (PoissonBracket.pyx:37-50): 'arakawaj1' | 'arakawaj2' | 'arakawaj4'."""
+
+    KINDS = ('arakawaj1', 'arakawaj2', 'arakawaj4')
+
+    def __init__(self, kind, context):
+        self.kind = kind
+        self.ctx = context
+
+    @staticmethod
+    def create(type, da1, grid, context=None):
+        if type not in PoissonBracket.KINDS:
+            if type == 'simpson':
+                raise _lib.VVError("the Simpson bracket is not provided by this library")
+            return None
+        ctx = context if context is not None else Context(grid, bracket=type, preconditioner='')
+        return PoissonBracket(type, ctx)
+
+    def function(self, F, H, Y, factor):
+        """Y += factor {F, H} (PoissonBracket.pyx:54-63).  H must be the
+        context's H_ave (set through update_previous / set_hamiltonian);
+        H is accepted for signature parity."""
+        self.ctx.bracket_add(F, Y, factor)
+
+
+class TimeDerivative:
+    """TimeDerivative.create (TimeDerivative.pyx:36-53): only 'point'."""
+
+    def __init__(self, grid):
+        self.grid = grid
+
+    @staticmethod
+    def create(type, da1, grid):
+        if type == 'point':
+            return TimeDerivative(grid)
+        if type in ('midpoint', 'simpson', 'arakawa_J1', 'arakawa_J2', 'arakawa_J4'):
+            raise _lib.VVError("averaged time derivative %r is not provided by this library" % type)
+        return None
+
+    def function(self, F, Y):
+        """Y += F / dt (TimeDerivative.pyx:56-57)."""
+        Y.add_(F, alpha=self.grid.ht_inv)
+
+
+class TensorProductPreconditioner:
+    """TensorProductPreconditioner.create (TensorProduct.pyx:66-86)."""
+
+    def __init__(self, context):
+        self.ctx = context
+
+    @staticmethod
+    def create(type, da1, daphi, grid, phi, context=None):
+        if type == 'tensor_kinetic':
+            ctx = context if context is not None else Context(grid)
+            return TensorProductPreconditioner(ctx)
+        if type in ('tensor_kinetic_scipy', 'tensor_potential', 'tensor_potential_scipy', 'tensor_diagonal'):
+            raise _lib.VVError("preconditioner %r is not provided by this library" % type)
+        return None
+
+    def tensorProduct(self, X, Y):
+        """Y = irfft_x(T_k^-1 rfft_x X) (TensorProductKinetic.pyx:146-160)."""
+        self.ctx.precond(X, Y)
+
+    def update_matrices(self):
+        pass       # the kinetic preconditioner depends on the grid only (KineticFast.pyx:244-245)
+
+
+class PETScVlasovSolver:
+    """PETScVlasovSolverBase + PETScNLVlasovMP on the GPU.
+
+    Constructor arguments follow PETScVlasovSolver.pyx:27-42.  ``config`` is
+    a ``Config`` (or anything with the same getters); ``da1``/``daph`` are
+    accepted for signature parity and unused; ``grid`` is a ``Grid``.
+    """
+
+    def __init__(self, config, da1, daph, grid, H0, H1p, H1h, H2p, H2h, phi, charge=-1., coll_freq=0.,
+                 coll_diff=1., coll_drag=1., regularisation=0., restart=30, context=None):
+        if config.get_averaging_operator() != 'point':
+            raise _lib.VVError("averaging operator %r is not provided" % config.get_averaging_operator())
+        if config.get_collision_operator() or config.get_double_bracket():
+            raise _lib.VVError("collision / double-bracket dissipation is not provided")
+        if regularisation != 0.:
+            raise _lib.VVError("regularisation is not provided")
+        self.grid = grid
+        self.charge = charge
+        self.H0, self.H1p, self.H1h, self.H2p, self.H2h = H0, H1p, H1h, H2p, H2h
+        self.phi = phi
+        pc = config.get_preconditioner()
+        self.ctx = context if context is not None else Context(
+            grid, bracket=config.get_poisson_bracket(), preconditioner=pc,
+            laplace=config.get_laplace_operator(), charge=charge, restart=restart)
+        self.poisson_bracket = PoissonBracket(config.get_poisson_bracket(), self.ctx)
+        self.time_derivative = TimeDerivative(grid)
+        self.preconditioner = TensorProductPreconditioner(self.ctx) if pc else None
+        self.Fp = self.ctx.new_field()
+        self.Fh = self.ctx.new_field()
+        self.Np = self.Up = self.Ep = self.Ap = None
+        self.Nh = self.Uh = self.Eh = self.Ah = None
+
+    def set_moments(self, Np, Up, Ep, Ap, Nh, Uh, Eh, Ah):
+        """PETScVlasovSolver.pyx:123-132 (used only by collision operators)."""
+        self.Np, self.Up, self.Ep, self.Ap = Np, Up, Ep, Ap
+        self.Nh, self.Uh, self.Eh, self.Ah = Nh, Uh, Eh, Ah
+
+    def update_history(self, F):
+        """Fh = F; the kinetic preconditioner needs no update (:135-139)."""
+        self.Fh.copy_(F)
+        if self.preconditioner is not None:
+            self.preconditioner.update_matrices()
+
+    def update_previous(self, F):
+        """Fp = F; H_ave from H0 + .5(H1p + H1h + H2p + H2h) (:142-157)."""
+        self.Fp.copy_(F)
+        self.ctx.set_hamiltonian(self.H0, self.H1p, self.H1h, self.H2p, self.H2h)
+
+    # PETScNLVlasovMP.pyx:33-58 ----------------------------------------------
+    def jacobian_solver(self, F, Y):
+        self.ctx.jacobian_apply(F, Y)
+
+    def function_solver(self, F, Y):
+        self.ctx.residual(F, self.Fh, Y)
+
+    # PETScVlasovSolver.pyx:160-198 ------------------------------------------
+    def jacobian(self, F, Y):
+        if self.preconditioner is None:
+            self.ctx.jacobian_apply(F, Y)
+        else:
+            self.ctx.jacobian(F, Y)
+
+    def function(self, F, Y):
+        if self.preconditioner is None:
+            self.ctx.residual(F, self.Fh, Y)
+        else:
+            self.ctx.function(F, self.Fh, Y)
+
+    def snes_mult(self, snes, F, Y):
+        self.jacobian(F, Y)
+
+    def mult(self, mat, F, Y):
+        self.jacobian(F, Y)
+
+    def jacobian_mult(self, F, Y):
+        self.jacobian(F, Y)
+
+    def function_snes_mult(self, snes, F, Y):
+        self.function(F, Y)
+
+    def function_mult(self, F, Y):
+        self.function(F, Y)
+
+    def function_norm(self, F, Y):
+        return self.ctx.function_norm(F, self.Fh, Y)
+
+    def formJacobian(self, J):
+        raise _lib.VVError("the assembled Jacobian (direct-solver path) is not provided")
+
+
+class PETScPoissonSolver:
+    """Periodic Poisson solver (PETScPoissonSolver.pyx:18-82 with the CFD4 /
+    CFD2 stencils of PETScPoissonCFD4.pyx / PETScPoissonCFD2.pyx).
+
+    ``solve(B, X)`` replaces the reference's CG + LU KSP on the singular
+    matrix: an exact mean-free solve of A phi = b."""
+
+    def __init__(self, dax, nx, hx, charge, laplace='cfd4', context=None, grid=None):
+        self.nx, self.hx, self.charge = int(nx), float(hx), float(charge)
+        self.hx2 = self.hx ** 2
+        self.hx2_inv = 1. / self.hx2
+        self.laplace = laplace
+        if context is None:
+            g = grid if grid is not None else Grid(nx, 8, nx * hx, 1.0)
+            context = Context(g, preconditioner='', laplace=laplace, charge=charge)
+        self.ctx = context
+
+    def mult(self, mat, X, Y):
+        self.matrix_mult(X, Y)
+
+    def matrix_mult(self, X, Y):
+        self.ctx.poisson_matrix_mult(X, Y)
+
+    def function_mult(self, X, N, Y):
+        self.ctx.poisson_function_mult(X, N, Y)
+
+    def formRHS(self, N, B):
+        """b = -(n - mean n) q  (PETScPoissonSolver.pyx:58-74)."""
+        nmean = N.sum() / self.nx
+        B.copy_(-(N - nmean) * self.charge)
+
+    def solve_density(self, N, X):
+        """KSPSolve of A x = formRHS(N): phi from the density."""
+        self.ctx.poisson_solve(N, X)
+
+    def formMat(self, A):
+        raise _lib.VVError("the assembled Poisson matrix is not used: the solve is exact")
+
+
+class Toolbox:
+    """Moments and initialisation (Toolbox.pyx:38-224) on the GPU."""
+
+    def __init__(self, da1, dax, grid, context):
+        self.grid = grid
+        self.ctx = context
+
+    def compute_density(self, F, N):
+        self.ctx.density(F, N)
+
+    def potential_to_hamiltonian(self, P, H):
+        self.ctx.potential_to_hamiltonian(P, H)
+
+    def initialise_kinetic_hamiltonian(self, H, mass):
+        H.copy_(torch.from_numpy(0.5 * self.grid.v ** 2 * mass))
+
+    def initialise_distribution_function(self, F, init_function):
+        from .core import initial_distribution
+        F.copy_(torch.from_numpy(initial_distribution(self.grid, init_function)))
+
+
+# ---------------------------------------------------------------------------
+# PETSc KSP / SNES stand-ins for the objects the run script configures
+# ---------------------------------------------------------------------------
+class KSP:
+    """GMRES on the MatPython context (the solver's ``mult``), PETSc defaults:
+    restart = the context's restart (30), CGS, x0 = 0, dtol 1e5."""
+
+    def __init__(self):
+        self.rtol, self.atol, self.dtol, self.max_it = 1e-5, 1e-50, 1e5, 10000
+        self.ctx = None
+        self.type = 'gmres'
+        self._res = None
+
+    def setType(self, t):
+        if t != 'gmres':
+            raise _lib.VVError("only KSP type gmres is provided")
+        self.type = t
+
+    def setTolerances(self, rtol=None, atol=None, divtol=None, max_it=None):
+        if rtol is not None:
+            self.rtol = rtol
+        if atol is not None:
+            self.atol = atol
+        if divtol is not None:
+            self.dtol = divtol
+        if max_it is not None:
+            self.max_it = max_it
+
+    def setOperators(self, A, P=None):
+        ctx = getattr(A, 'ctx', None)
+        if ctx is None:
+            raise _lib.VVError("KSP operator must be a PETScVlasovSolver (its mult runs on the GPU)")
+        self.ctx = ctx
+
+    def solve(self, b, x):
+        self._res = self.ctx.ksp_solve(b, x, self.rtol, self.atol, self.dtol, self.max_it)
+
+    def getIterationNumber(self):
+        return self._res.its if self._res else 0
+
+    def getConvergedReason(self):
+        return self._res.reason if self._res else 0
+
+    def getResidualNorm(self):
+        return self._res.rnorm if self._res else 0.0
+
+
+class SNES:
+    """SNES of type 'ksponly' with the solver's function and matrix-free
+    Jacobian (run_nonlinear_matrixfree_split_mp.py:47-53)."""
+
+    def __init__(self):
+        self.ksp = KSP()
+        self.solver = None
+        self.type = 'ksponly'
+        self.linear_its = 0
+        self.last = None
+
+    def setType(self, t):
+        if t != 'ksponly':
+            raise _lib.VVError("only SNES type ksponly is provided")
+
+    def setFunction(self, function, fb):
+        solver = getattr(function, '__self__', None)
+        if not isinstance(solver, PETScVlasovSolver):
+            raise _lib.VVError("the SNES function must be PETScVlasovSolver.function_snes_mult")
+        self.solver = solver
+        self.ksp.setOperators(solver)
+
+    def setJacobian(self, update, J, P=None):
+        pass
+
+    def setFromOptions(self):
+        pass
+
+    def getKSP(self):
+        return self.ksp
+
+    def solve(self, b, x):
+        """SNESSolve_KSPONLY: F = P^-1 F(x); KSPSolve; x -= delta unless the
+        linear solve diverged."""
+        s = self.solver
+        if s.preconditioner is None:
+            raise _lib.VVError("the native SNES path needs the tensor_kinetic preconditioner")
+        k = self.ksp
+        self.last = s.ctx.snes_ksponly(x, s.Fh, k.rtol, k.atol, k.dtol, k.max_it)
+        k._res = self.last
+        self.linear_its = self.last.its
+
+    def getLinearSolveIterations(self):
+        return self.linear_its
