@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Throughput of the implicit Vlasov-Poisson timestep on one MI355X.
+
+Workload (default): two-stream instability, nx x nv = 1024 x 2048,
+x in [0, 2 pi/0.2), v in [-10, 10], f0 = (1 + 0.001 cos 0.2x) 1/2 [M(v-2.4) +
+M(v+2.4)], Arakawa J4, dt = 0.05, FP64, kinetic tensor preconditioner,
+GMRES(30) rtol 1e-10, Picard atol 1e-10 (at most 20 iterations).  A "step"
+is one full implicit timestep: Picard loop of SNES(ksponly) + GMRES on the
+preconditioned Jacobian, moments and Poisson after every solve.
+
+Timing: W untimed warm-up steps, then K steps between two
+torch.cuda.synchronize() calls.  value = K / elapsed  (timesteps/s).
+roofline: the dominant kernel's algorithmic bytes / its HIP-event time on
+the library stream over the timed region.  cpu_baseline: the C (OpenMP)
+restatement of the same timestep from oracle/, on a bounded sample.
+
+Prints ONE JSON line.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm")
+
+
+def workloads():
+    import numpy as np
+    from vivlasov1d_amd.core import landau, twostream
+    return {
+        "twostream_1024x2048_J4": dict(nx=1024, nv=2048, L=2 * np.pi / 0.2, vmax=10.0, ht=0.05,
+                                       bracket="arakawaj4", profile=twostream(0.2, 0.001),
+                                       desc="two-stream k=0.2 amp=1e-3 vd=2.4"),
+        "landau_256x512_J4": dict(nx=256, nv=512, L=4 * np.pi, vmax=8.0, ht=0.05, bracket="arakawaj4",
+                                  profile=landau(0.5, 0.5), desc="nonlinear Landau k=0.5 amp=0.5"),
+        "landau_64x128_J4": dict(nx=64, nv=128, L=4 * np.pi, vmax=10.0, ht=0.1, bracket="arakawaj4",
+                                 profile=landau(0.5, 0.01), desc="linear Landau k=0.5 amp=0.01"),
+    }
+
+
+def make_cfg(w, nt):
+    from vivlasov1d_amd.core import Config
+    return Config({
+        "grid": {"nx": w["nx"], "nv": w["nv"], "L": w["L"], "vmax": w["vmax"], "ht": w["ht"], "nt": nt},
+        "solver": {"method": "matrixfree", "type": "nonlinear", "mode": "split", "timestepping": "mp",
+                   "poisson_bracket": w["bracket"], "laplace_operator": "CFD4",
+                   "preconditioner_type": "tensor", "preconditioner_scheme": "tensor_kinetic",
+                   "petsc_ksp_rtol": 1e-10, "petsc_ksp_atol": 1e-50, "petsc_ksp_max_iter": 1000,
+                   "petsc_snes_atol": 1e-10, "petsc_snes_max_iter": 20},
+        "initial_data": {"charge": -1.0, "mass": 1.0},
+    })
+
+
+def step_bytes(N, rec, restart=30):
+    """Algorithmic HBM bytes of one timestep from its Picard / GMRES counts."""
+    total = 24.0
+    for p in rec["picard"]:
+        m = p["gmres"]
+        s = 16.0
+        for k in range(1, m + 1):
+            kk = ((k - 1) % restart) + 1
+            s += 32.0 + 8.0 * (2 * kk + 3)
+        s += 8.0 * (m + 2) + 8.0 + 24.0
+        total += s
+    return N * total
+
+
+def cpu_baseline(w, sample_steps, threads):
+    """C restatement of the timestep (oracle/c), timed on the host cores."""
+    from oracle import cbaseline
+    return cbaseline.time_sample(w, sample_steps, threads)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="twostream_1024x2048_J4")
+    ap.add_argument("--kernel", default="mdot",
+                    help="kernel timed with HIP events for the roofline ('all', 'none' or one of %s)" % (KERNELS,))
+    ap.add_argument("--cpu-steps", type=int, default=1, help="timesteps of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the timestep is not sharded across GPUs in this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from vivlasov1d_amd.runner import Runscript
+
+    w = workloads()[args.workload]
+    nt = args.warmup + args.steps
+    run = Runscript(make_cfg(w, nt), profile=w["profile"])
+    ctx = run.ctx
+    N = w["nx"] * w["nv"]
+
+    for _ in range(args.warmup):
+        run.step()
+    torch.cuda.synchronize()
+    if args.kernel == "all":
+        ctx.profile(None)
+    elif args.kernel not in (None, "none"):
+        ctx.profile((args.kernel,))
+    nrec0 = len(run.history)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile(())
+    elapsed = t1 - t0
+    recs = run.history[nrec0:]
+
+    kstats = {}
+    for k in KERNELS:
+        ms, n, by = ctx.profile_read(k)
+        if n:
+            kstats[k] = dict(ms=ms, launches=n, bytes=by, avg_us=1e3 * ms / n, gbs=by / (ms * 1e-3) / 1e9)
+    dom = max(kstats, key=lambda k: kstats[k]["ms"]) if kstats else None
+    roof = None
+    if dom:
+        d = kstats[dom]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": d["bytes"] / d["launches"], "avg_launch_us": round(d["avg_us"], 3),
+                "launches": d["launches"]}
+    sb = sum(step_bytes(N, r) for r in recs)
+    picard = [len(r["picard"]) for r in recs]
+    gm = [p["gmres"] for r in recs for p in r["picard"]]
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(w, args.cpu_steps, args.cpu_threads)
+        except Exception as e:      # reported, never silently replaced
+            cpu = {"error": "%s: %s" % (type(e).__name__, e)}
+
+    inv = run.invariants()
+    value = args.steps / elapsed
+    line = {
+        "metric": "timesteps/s at 1024x2048 J4 (plain FP64)" if args.workload.startswith("twostream_1024")
+        else "timesteps/s %s" % args.workload,
+        "value": round(value, 3), "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic initial condition (%s), generated in-process" % w["desc"],
+        "config": {"workload": args.workload, "nx": w["nx"], "nv": w["nv"], "bracket": w["bracket"],
+                   "dt": w["ht"], "ksp_rtol": 1e-10, "snes_atol": 1e-10, "restart": 30,
+                   "preconditioner": "tensor_kinetic", "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_roofline": {"bytes_per_step": sb / max(1, len(recs)),
+                          "achieved": round(sb / elapsed / 1e9, 1), "unit": "GB/s",
+                          "frac": round(sb / elapsed / 1e9 / HBM_PEAK_GBS, 4)},
+        "iterations": {"picard_per_step": float(np.mean(picard)), "gmres_per_solve": float(np.mean(gm)),
+                       "gmres_per_step": float(sum(gm)) / max(1, len(recs))},
+        "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
+                        "ms": round(v["ms"], 3)} for k, v in kstats.items()},
+        "cpu_baseline": cpu,
+        "invariants": {k: inv[k] for k in ("mass", "l2", "energy")},
+    }
+    if cpu and "value" in cpu and cpu["value"]:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,7 +67,7 @@ typedef struct vv_ctx vv_ctx;
  * (vlasov/core/Grid.pyx:14-51) and the solver options of run_base.py:35-196. */
 typedef struct {
     int32_t nx;            /* points in x (periodic); power of two, 8..4096   */
-    int32_t nv;            /* points in v; 8..8192                            */
+    int32_t nv;            /* points in v; 8..4096                            */
     double  hx;            /* L / nx                                          */
     double  hv;            /* (vmax - vmin) / (nv - 1)                        */
     double  ht;            /* time step                                       */
@@ -205,10 +205,16 @@ int vv_invariants(vv_ctx *ctx, const double *f, const double *h0,
 
 /* Per-kernel timing with HIP events on the context stream (for the
  * roofline report).  kernel: "jacobian", "residual", "rfft", "tridiag",
- * "irfft_mdot", "maxpy_norm".  Enable, run, then read the summed time in ms
- * and the launch count. */
+ * "irfft", "mdot", "maxpy_norm".  Enable, run, then read the summed time in ms,
+ * the launch count and the summed algorithmic HBM bytes of those launches
+ * (Krylov launches that returned at once after convergence are not
+ * counted).  Reading clears that kernel's records.  enable is a bit mask
+ * over the kernels in the order listed (jacobian = 1, residual = 2,
+ * rfft = 4, tridiag = 8, irfft = 16, mdot = 32, maxpy_norm = 64; -1 all,
+ * 0 off). */
 int vv_profile_enable(vv_ctx *ctx, int enable);
-int vv_profile_read(vv_ctx *ctx, const char *kernel, double *ms, int64_t *launches);
+int vv_profile_read(vv_ctx *ctx, const char *kernel, double *ms, int64_t *launches,
+                    double *bytes);
 
 #ifdef __cplusplus
 }

@@ -17,11 +17,13 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum ProfKind { P_JAC = 0, P_RES, P_RFFT, P_TRI, P_IRFFT, P_MAXPY, P_NKINDS };
-const char *kProfNames[P_NKINDS] = {"jacobian", "residual", "rfft", "tridiag", "irfft_mdot", "maxpy_norm"};
+enum ProfKind { P_JAC = 0, P_RES, P_RFFT, P_TRI, P_IRFFT, P_MDOT, P_MAXPY, P_NKINDS };
+const char *kProfNames[P_NKINDS] = {"jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm"};
 
 struct ProfRec {
     int kind;
+    int tag;          // Arnoldi iteration within the cycle, -1 outside GMRES
+    double bytes;     // algorithmic HBM bytes of this launch
     hipEvent_t a, b;
 };
 
@@ -48,10 +50,12 @@ hipEvent_t prof_event(Prof *p)
 struct ProfScope {
     vv_ctx *c;
     int kind;
+    int tag;
+    double bytes;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(vv_ctx *cc, int k) : c(cc), kind(k)
+    ProfScope(vv_ctx *cc, int k, double by, int tg = -1) : c(cc), kind(k), tag(tg), bytes(by)
     {
-        if (c->prof) {
+        if ((c->prof >> kind) & 1) {
             Prof *p = prof_of(c);
             a = prof_event(p);
             b = prof_event(p);
@@ -60,9 +64,9 @@ struct ProfScope {
     }
     ~ProfScope()
     {
-        if (c->prof && a && b) {
+        if (a && b) {
             (void)hipEventRecord(b, c->stream);
-            prof_of(c)->recs.push_back({kind, a, b});
+            prof_of(c)->recs.push_back({kind, tag, bytes, a, b});
         }
     }
 };
@@ -85,9 +89,9 @@ int hipfail(vv_ctx *c, hipError_t e, const char *where)
         if (_e != hipSuccess) return hipfail(c, _e, where);    \
     } while (0)
 
-#define VV_PLAUNCH(kind, call, where)                          \
+#define VV_PLAUNCH(kind, bytes, tag, call, where)              \
     do {                                                       \
-        ProfScope _ps(c, kind);                                \
+        ProfScope _ps(c, kind, (double)(bytes), tag);          \
         hipError_t _e = (call);                                \
         if (_e != hipSuccess) return hipfail(c, _e, where);    \
     } while (0)
@@ -115,6 +119,29 @@ int check_ctx(vv_ctx *c)
     return VV_OK;
 }
 
+// algorithmic HBM bytes per launch (compulsory reads + writes of each kernel)
+double b_field(vv_ctx *c) { return 8.0 * (double)c->N; }
+double b_spec(vv_ctx *c) { return 16.0 * (double)c->nmodes * (double)c->p.nv; }
+
+// drop the records of Arnoldi iterations that returned at once because the
+// cycle had already finished (tag >= iterations actually done)
+void prof_drop_idle(vv_ctx *c, size_t from, int done_its)
+{
+    if (!c->prof) return;
+    Prof *p = prof_of(c);
+    std::vector<ProfRec> keep(p->recs.begin(), p->recs.begin() + from);
+    for (size_t q = from; q < p->recs.size(); ++q) {
+        const ProfRec &r = p->recs[q];
+        if (r.tag >= done_its) {
+            p->pool.push_back(r.a);
+            p->pool.push_back(r.b);
+        } else {
+            keep.push_back(r);
+        }
+    }
+    p->recs.swap(keep);
+}
+
 // y = P^-1 x (spectrum in c->spec); done: optional early-exit flag
 int precond(vv_ctx *c, const double *x, double *y, const int32_t *done)
 {
@@ -122,9 +149,9 @@ int precond(vv_ctx *c, const double *x, double *y, const int32_t *done)
         if (x != y) VV_CHECK(launch_copy(c, x, y, c->N), "copy");
         return VV_OK;
     }
-    VV_PLAUNCH(P_RFFT, launch_rfft(c, x, c->spec, done), "rfft");
-    VV_PLAUNCH(P_TRI, launch_tridiag_k(c, c->spec, done), "tridiag");
-    VV_PLAUNCH(P_IRFFT, launch_irfft_mdot(c, c->spec, y, 0, 0, done), "irfft");
+    VV_PLAUNCH(P_RFFT, b_field(c) + b_spec(c), -1, launch_rfft(c, x, c->spec, done), "rfft");
+    VV_PLAUNCH(P_TRI, 2 * b_spec(c), -1, launch_tridiag_k(c, c->spec, done), "tridiag");
+    VV_PLAUNCH(P_IRFFT, b_field(c) + b_spec(c), -1, launch_irfft(c, c->spec, y, done), "irfft");
     return VV_OK;
 }
 
@@ -134,11 +161,13 @@ int arnoldi_step(vv_ctx *c, int it)
     const int32_t *done = &c->gs->st.done;
     double *v = c->basis + (size_t)it * c->N;
     double *w = c->basis + (size_t)(it + 1) * c->N;
-    VV_PLAUNCH(P_JAC, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
-    VV_PLAUNCH(P_RFFT, launch_rfft(c, c->X, c->spec, done), "rfft");
-    VV_PLAUNCH(P_TRI, launch_tridiag_k(c, c->spec, done), "tridiag");
-    VV_PLAUNCH(P_IRFFT, launch_irfft_mdot(c, c->spec, w, it + 1, it, done), "irfft_mdot");
-    VV_PLAUNCH(P_MAXPY, launch_maxpy_norm(c, it), "maxpy_norm");
+    const double F = b_field(c), S = b_spec(c);
+    VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
+    VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, c->X, c->spec, done), "rfft");
+    VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
+    VV_PLAUNCH(P_IRFFT, S + F, it, launch_irfft(c, c->spec, w, done), "irfft");
+    VV_PLAUNCH(P_MDOT, F * (it + 2), it, launch_mdot(c, it), "mdot");
+    VV_PLAUNCH(P_MAXPY, F * (it + 3), it, launch_maxpy_norm(c, it), "maxpy_norm");
     return VV_OK;
 }
 
@@ -157,13 +186,14 @@ int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *k
     for (;;) {
         if (!first) {
             // KSPInitialResidual after a restart: r = b - (P^-1 J) x
-            VV_PLAUNCH(P_JAC, launch_jacobian(c, x, nullptr, c->X), "jacobian");
+            VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
             int r = precond(c, c->X, c->basis, nullptr);
             if (r) return r;
             VV_CHECK(launch_residual_restart(c, b), "restart residual");
         }
         first = false;
         VV_CHECK(launch_norm_start(c), "cycle start");
+        const size_t prof_from = c->prof ? prof_of(c)->recs.size() : 0;
         int it = 0, pend = -1, slot = 0;
         bool stop = false;
         while (it < max_k && !stop) {
@@ -187,6 +217,7 @@ int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *k
                                 c->stream), "status copy");
         VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
         st = c->host_status[0];
+        prof_drop_idle(c, prof_from, st.it);
         if (st.reason != 0) break;
         if (st.its >= kp->max_it) {
             st.reason = VV_KSP_DIVERGED_ITS;
@@ -266,8 +297,9 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
         if (_e != hipSuccess) return bail(_e, where); \
     } while (0)
 
-    const int nfb = fft_blocks(c);
-    const int nrows = (nfb > kReduceBlocks ? nfb : kReduceBlocks);
+    const int nfb = fft_blocks(c), ncb = cgs_blocks(c->N);
+    int nrows = (nfb > kReduceBlocks ? nfb : kReduceBlocks);
+    if (ncb > nrows) nrows = ncb;
     VV_CCHECK(dalloc(&c->h0, nv), "alloc h0");
     VV_CCHECK(dalloc(&c->h1p, nx), "alloc h1p");
     VV_CCHECK(dalloc(&c->h1h, nx), "alloc h1h");
@@ -292,6 +324,7 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming), "event");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming), "event");
     VV_CCHECK(hipMemset(c->gs, 0, sizeof(GmresState)), "zero gmres state");
+    VV_CCHECK(hipMemset(c->partial, 0, (size_t)nrows * (kMaxRestart + 2) * sizeof(double)), "zero partials");
 
     // host tables
     std::vector<double> hzero(nx > nv ? nx : nv, 0.0);
@@ -411,7 +444,7 @@ int vv_residual(vv_ctx *c, const double *f, const double *fh, double *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !fh || !y) return fail(c, VV_ERR_INVALID, "null field");
-    VV_PLAUNCH(P_RES, launch_residual(c, f, fh, y), "residual");
+    VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, y), "residual");
     return VV_OK;
 }
 
@@ -419,7 +452,7 @@ int vv_jacobian_apply(vv_ctx *c, const double *u, double *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
-    VV_PLAUNCH(P_JAC, launch_jacobian(c, u, nullptr, y), "jacobian");
+    VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, u, nullptr, y), "jacobian");
     return VV_OK;
 }
 
@@ -443,7 +476,7 @@ int vv_function(vv_ctx *c, const double *f, const double *fh, double *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !fh || !y) return fail(c, VV_ERR_INVALID, "null field");
-    VV_PLAUNCH(P_RES, launch_residual(c, f, fh, c->X), "residual");
+    VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, c->X), "residual");
     return precond(c, c->X, y, nullptr);
 }
 
@@ -451,7 +484,7 @@ int vv_jacobian(vv_ctx *c, const double *u, double *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
-    VV_PLAUNCH(P_JAC, launch_jacobian(c, u, nullptr, c->X), "jacobian");
+    VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, u, nullptr, c->X), "jacobian");
     return precond(c, c->X, y, nullptr);
 }
 
@@ -459,7 +492,7 @@ int vv_function_norm(vv_ctx *c, const double *f, const double *fh, double *y, do
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !fh || !y || !out) return fail(c, VV_ERR_INVALID, "null argument");
-    VV_PLAUNCH(P_RES, launch_residual(c, f, fh, y), "residual");
+    VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, y), "residual");
     VV_CHECK(launch_norm2(c, y, c->N, c->dscalar), "norm");
     return read_scalar(c, c->dscalar, out);
 }
@@ -572,21 +605,21 @@ int vv_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1
 int vv_profile_enable(vv_ctx *c, int enable)
 {
     if (int r = check_ctx(c)) return r;
-    c->prof = enable != 0;
+    c->prof = enable;
     return VV_OK;
 }
 
-int vv_profile_read(vv_ctx *c, const char *kernel, double *ms, int64_t *launches)
+int vv_profile_read(vv_ctx *c, const char *kernel, double *ms, int64_t *launches, double *bytes)
 {
     if (int r = check_ctx(c)) return r;
-    if (!kernel || !ms || !launches) return fail(c, VV_ERR_INVALID, "null argument");
+    if (!kernel || !ms || !launches || !bytes) return fail(c, VV_ERR_INVALID, "null argument");
     int kind = -1;
     for (int k = 0; k < P_NKINDS; ++k)
         if (std::strcmp(kernel, kProfNames[k]) == 0) kind = k;
     if (kind < 0) return fail(c, VV_ERR_INVALID, std::string("unknown kernel ") + kernel);
     VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
     Prof *p = prof_of(c);
-    double tot = 0.0;
+    double tot = 0.0, by = 0.0;
     int64_t cnt = 0;
     std::vector<ProfRec> keep;
     for (auto &r : p->recs) {
@@ -594,6 +627,7 @@ int vv_profile_read(vv_ctx *c, const char *kernel, double *ms, int64_t *launches
             float t = 0.f;
             if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
                 tot += t;
+                by += r.bytes;
                 ++cnt;
             }
             p->pool.push_back(r.a);
@@ -605,6 +639,7 @@ int vv_profile_read(vv_ctx *c, const char *kernel, double *ms, int64_t *launches
     p->recs.swap(keep);
     *ms = tot;
     *launches = cnt;
+    *bytes = by;
     return VV_OK;
 }
 

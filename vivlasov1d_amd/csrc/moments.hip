@@ -158,14 +158,12 @@ struct InvArgs {
     int nx, nv;
     double hx, hv, charge;
     double *partial;      // [gridDim.x][5]
-    uint32_t *counter;
     double *out;          // 6 doubles
 };
 
 __global__ __launch_bounds__(NT) void invariants_kernel(InvArgs a)
 {
     __shared__ double red[5][NT / 64];
-    __shared__ int s_last;
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int64_t N = (int64_t)a.nx * a.nv;
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < N; e += (int64_t)gridDim.x * NT) {
@@ -193,17 +191,21 @@ __global__ __launch_bounds__(NT) void invariants_kernel(InvArgs a)
         for (int w = 0; w < NT / 64; ++w) v += red[threadIdx.x][w];
         a.partial[(size_t)blockIdx.x * 5 + threadIdx.x] = v;
     }
-    __threadfence();
+}
+
+// one workgroup: sums the per-block partials (fixed order) and the field energy
+__global__ __launch_bounds__(NT) void invariants_finish_kernel(InvArgs a, int nb)
+{
+    __shared__ double red[5][NT];
+    double t5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += NT)
+        for (int q = 0; q < 5; ++q) t5[q] += a.partial[(size_t)b * 5 + q];
+    for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = t5[q];
     __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(a.counter, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
     if (threadIdx.x != 0) return;
     double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int b = 0; b < (int)gridDim.x; ++b)
-        for (int q = 0; q < 5; ++q) t[q] += a.partial[(size_t)b * 5 + q];
-    *a.counter = 0u;
+    for (int b = 0; b < NT; ++b)
+        for (int q = 0; q < 5; ++q) t[q] += red[q][b];
     const double hxhv = a.hx * a.hv;
     const double mass = t[0] * 0.25 * a.hx * a.hv;
     const double l2 = hxhv * t[1];
@@ -267,9 +269,9 @@ hipError_t launch_pot_to_ham(vv_ctx *c, const double *phi, double *h1)
 hipError_t launch_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1, const double *h2,
                              const double *phi, double *out_dev)
 {
-    InvArgs a{f, h0, h1, h2, phi, c->p.nx, c->p.nv, c->p.hx, c->p.hv, c->p.charge,
-              c->partial, &c->gs->counter[CNT_INV], out_dev};
+    InvArgs a{f, h0, h1, h2, phi, c->p.nx, c->p.nv, c->p.hx, c->p.hv, c->p.charge, c->partial, out_dev};
     hipLaunchKernelGGL(invariants_kernel, dim3(kReduceBlocks), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL(invariants_finish_kernel, dim3(1), dim3(NT), 0, c->stream, a, kReduceBlocks);
     return hipGetLastError();
 }
 

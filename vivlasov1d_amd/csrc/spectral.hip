@@ -15,8 +15,6 @@
 //  irfft_kernel   spectrum -> real rows, backward-normalised like
 //                 numpy.fft.irfft / pyFFTW (imaginary parts of the k = 0 and
 //                 k = nx/2 coefficients are ignored, as in a c2r transform).
-//                 Optionally fused with GMRES's classical Gram-Schmidt dot
-//                 products (VecMDot) against the Krylov basis.
 #include "vv1d_internal.h"
 
 namespace vv {
@@ -172,23 +170,11 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs a, const double *__res
     }
 }
 
-struct MdotArgs {
-    int nvec;               // dot against basis vectors 0..nvec-1 (0: none)
-    int col;                // Hessenberg column receiving the dots
-    const double *basis;
-    int64_t N;
-    double *partial;        // [gridDim.x][stride]
-    int stride;
-    GmresState *gs;
-};
-
-__global__ __launch_bounds__(NT) void irfft_kernel(FftArgs a, MdotArgs m, const double2 *__restrict__ spec,
+__global__ __launch_bounds__(NT) void irfft_kernel(FftArgs a, const double2 *__restrict__ spec,
                                                    double *__restrict__ y)
 {
     if (a.done != nullptr && *a.done) return;
     extern __shared__ double2 buf[];
-    __shared__ double red[NT / 64][kMaxRestart + 1];
-    __shared__ int s_last;
     const int nx = a.nx, R = a.rows, nh = nx >> 1;
     const int j0 = blockIdx.x * R;
     const int npair = R >> 1;
@@ -209,68 +195,15 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs a, MdotArgs m, const 
     }
     __syncthreads();
     lds_fft<true>(buf, nx, a.logn, npair, a.tw);
-
     const double inv_n = 1.0 / (double)nx;   // exact: nx is a power of two
-    constexpr int PER = MAXE * 2 / NT;        // real outputs per thread at most
-    double w[PER];
-    int cnt = 0;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int idx = threadIdx.x + q * NT;
-        w[q] = 0.0;
-        if (idx < R * nx) {
-            const int r = idx >> a.logn;
-            const int i = idx & (nx - 1);
-            if (j0 + r < a.nv) {
-                const double2 z = buf[(r >> 1) * nx + i];
-                w[q] = ((r & 1) ? z.y : z.x) * inv_n;
-                y[(size_t)(j0 + r) * nx + i] = w[q];
-                cnt = q + 1;
-            }
+    for (int idx = threadIdx.x; idx < R * nx; idx += NT) {
+        const int r = idx >> a.logn;
+        const int i = idx & (nx - 1);
+        if (j0 + r < a.nv) {
+            const double2 z = buf[(r >> 1) * nx + i];
+            y[(size_t)(j0 + r) * nx + i] = ((r & 1) ? z.y : z.x) * inv_n;
         }
     }
-    if (m.nvec == 0) return;
-
-    // fused VecMDot: partial sums of w . (scale_v V_v) over this tile
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int v = 0; v < m.nvec; ++v) {
-        const double *V = m.basis + (size_t)v * m.N;
-        const double sv = m.gs->scale[v];
-        double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            if (q < cnt) {
-                const int idx = threadIdx.x + q * NT;
-                const int r = idx >> a.logn;
-                const int i = idx & (nx - 1);
-                if (j0 + r < a.nv) acc += w[q] * (V[(size_t)(j0 + r) * nx + i] * sv);
-            }
-        }
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-        if (lane == 0) red[wave][v] = acc;
-    }
-    __syncthreads();
-    if (threadIdx.x < m.nvec) {
-        double s = 0.0;
-        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][threadIdx.x];
-        m.partial[(size_t)blockIdx.x * m.stride + threadIdx.x] = s;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int ticket = atomicAdd(&m.gs->counter[CNT_MDOT], 1u);
-        s_last = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (threadIdx.x < m.nvec) {
-        double s = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) s += m.partial[(size_t)b * m.stride + threadIdx.x];
-        m.gs->hh[m.col * (kMaxRestart + 1) + threadIdx.x] = s;   // HH(v, it) = <w, v_v>
-        m.gs->coef[threadIdx.x] = -s;                           // lhh of VecMAXPY
-    }
-    if (threadIdx.x == 0) m.gs->counter[CNT_MDOT] = 0u;
 }
 
 struct TriArgs {
@@ -442,14 +375,12 @@ hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_irfft_mdot(vv_ctx *c, const double2 *spec, double *y, int nvec, int col,
-                             const int32_t *done)
+hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done)
 {
     FftArgs a{c->p.nx, c->logn, c->p.nv, fft_rows(c->p.nx, c->p.nv), c->twiddle, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
-    MdotArgs m{nvec, col, c->basis, c->N, c->partial, kMaxRestart + 2, c->gs};
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(double2);
-    hipLaunchKernelGGL(irfft_kernel, dim3(nb), dim3(NT), lds, c->stream, a, m, spec, y);
+    hipLaunchKernelGGL(irfft_kernel, dim3(nb), dim3(NT), lds, c->stream, a, spec, y);
     return hipGetLastError();
 }
 

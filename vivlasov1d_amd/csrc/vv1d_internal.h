@@ -80,7 +80,7 @@ struct vv_ctx {
     vv::GmresStatus *host_status;   // pinned, 2 slots
     hipEvent_t ev[2];
     // profiling (per-kernel HIP event pairs)
-    bool prof;
+    int prof;           // bit k set: time kernel kind k
     void *prof_state;
     std::string err;
 };
@@ -96,11 +96,12 @@ hipError_t launch_bracket_add(vv_ctx *c, const double *x, double *y, double fact
 int fft_blocks(vv_ctx *c);
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done);
 hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done);
-// irfft with the GMRES mdot against basis vectors 0..nvec-1 fused (nvec = 0: none)
-hipError_t launch_irfft_mdot(vv_ctx *c, const double2 *spec, double *y, int nvec, int col, const int32_t *done);
+hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done);
 // krylov.hip
 hipError_t launch_gmres_init(vv_ctx *c, double rtol, double atol, double dtol, int max_it);
 hipError_t launch_norm_start(vv_ctx *c);                  // cycle start on basis[0]
+int cgs_blocks(int64_t N);
+hipError_t launch_mdot(vv_ctx *c, int col);               // CGS dots of basis[col+1] vs 0..col
 hipError_t launch_maxpy_norm(vv_ctx *c, int it);          // CGS update of basis[it+1] + Givens
 hipError_t launch_build_solution(vv_ctx *c, double *x);  // x += V y
 hipError_t launch_residual_restart(vv_ctx *c, const double *b);  // basis[0] = b - basis[0]

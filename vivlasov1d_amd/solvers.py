@@ -196,14 +196,25 @@ class Context:
         keys = ("mass", "l2", "energy", "ekin", "epot", "efield")
         return dict(zip(keys, list(out)))
 
-    def profile(self, enable=True):
-        self.call("vv_profile_enable", ctypes.c_int(1 if enable else 0))
+    PROFILE_KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm")
+
+    def profile(self, kernels=None):
+        """Time the named kernels (all when None; () switches timing off)."""
+        if kernels is None:
+            mask = -1
+        else:
+            mask = 0
+            for k in kernels:
+                mask |= 1 << self.PROFILE_KERNELS.index(k)
+        self.call("vv_profile_enable", ctypes.c_int(mask))
 
     def profile_read(self, kernel):
+        """(total ms, launches, algorithmic bytes) of one kernel since the last read."""
         ms = ctypes.c_double()
         n = ctypes.c_int64()
-        self.call("vv_profile_read", kernel.encode(), ctypes.byref(ms), ctypes.byref(n))
-        return ms.value, n.value
+        by = ctypes.c_double()
+        self.call("vv_profile_read", kernel.encode(), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by))
+        return ms.value, n.value, by.value
 
     def synchronize(self):
         self.call("vv_synchronize")
