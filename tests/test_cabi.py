@@ -1,0 +1,67 @@
+"""The C-ABI library loads on the CPU and exports every function include/vv1d.h
+declares; argument validation runs without a GPU (no compute calls here)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vv1d.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vv_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from vivlasov1d_amd import _lib
+    if not os.path.exists(_lib.LIBPATH):
+        from vivlasov1d_amd import build
+        build.build()
+    return _lib.load()
+
+
+def test_header_declares_the_api():
+    names = declared_functions()
+    for n in ("vv_ctx_create", "vv_residual", "vv_jacobian_apply", "vv_precond_apply", "vv_ksp_solve",
+              "vv_snes_ksponly", "vv_moments", "vv_poisson_solve", "vv_density"):
+        assert n in names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_the_header():
+    from vivlasov1d_amd import _lib
+    assert set(declared_functions()) == set(_lib.SIGNATURES)
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("nx", 100, "power of two"), ("nv", 4, "nv must be"), ("bracket", 3, "bracket"),
+    ("laplace", 6, "laplace"), ("restart", 0, "restart"), ("precond", 7, "preconditioner")])
+def test_create_rejects_bad_parameters(lib, field, value, msg):
+    from vivlasov1d_amd import _lib
+    import numpy as np
+    v = np.linspace(-1, 1, 64)
+    eig = np.zeros(2 * 64)
+    p = _lib.VVParams(64, 64, 0.1, 0.1, 0.05, -1.0, 4, 1, 4, 30,
+                      v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      eig.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    setattr(p, field, value)
+    out = ctypes.c_void_p()
+    ret = lib.vv_ctx_create(ctypes.byref(p), None, ctypes.byref(out))
+    assert ret == -1 or ret == -3
+    assert out.value is None
+    assert msg in lib.vv_last_error(None).decode()
+
+
+def test_null_context_is_an_error(lib):
+    assert lib.vv_residual(None, None, None, None) == -1
+    assert lib.vv_synchronize(None) == -1

@@ -1,0 +1,45 @@
+"""The C restatement (CPU baseline) against the NumPy oracle (CPU only)."""
+
+import numpy as np
+import pytest
+
+from oracle import vlasov_oracle as O
+
+
+@pytest.fixture(scope="module")
+def cb():
+    from oracle import build, cbaseline
+    import os
+    if not os.path.exists(cbaseline.LIB) or os.path.getmtime(cbaseline.LIB) < os.path.getmtime(build.SRC):
+        build.build()
+    return cbaseline
+
+
+@pytest.mark.parametrize("kind", ['arakawaj1', 'arakawaj2', 'arakawaj4'])
+def test_c_operators_match_oracle(cb, kind):
+    g = O.Grid(16, 40, 4 * np.pi, 8.0, ht=0.05)
+    rng = np.random.default_rng(3)
+    f0 = O.normalise(O.initial_distribution(g, lambda x, v: (1 + 0.2 * np.cos(0.5 * x)) * O.maxwellian(v)), g)
+    run = cb.CRun(g, f0.T, bracket=kind)
+    h1p, h1h, h2p, h2h = 0.1 * rng.standard_normal((4, g.nx))
+    run.set_h(h1p, h1h, h2p, h2h)
+    H = O.h_average(O.kinetic_hamiltonian(g), h1p, h1h, h2p, h2h)
+    f, fh, u = rng.random((3, g.nx, g.nv))
+    assert np.array_equal(run.residual(f.T, fh.T).T, O.residual(kind, f, fh, H, g))
+    assert np.array_equal(run.jacobian(u.T).T, O.jacobian(kind, u, H, g))
+    ref = O.Preconditioner(g).apply(u)
+    got = run.precond(u.T).T
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_c_timesteps_match_oracle(cb):
+    g = O.Grid(32, 64, 4 * np.pi, 8.0, ht=0.05)
+    prof = lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * O.maxwellian(v)
+    orun = O.OracleRun(g, O.initial_distribution(g, prof))
+    crun = cb.CRun(g, O.normalise(O.initial_distribution(g, prof), g).T)
+    for _ in range(3):
+        eo = orun.step()
+        ec = crun.step()
+        assert ec['picard'] == len(eo['picard'])
+        assert abs(ec['gmres'] - sum(p['gmres_its'] for p in eo['picard'])) <= 2
+        assert np.abs(crun.f().T - orun.fc).max() <= 1e-10 * np.abs(orun.fc).max()

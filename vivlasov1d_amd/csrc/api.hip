@@ -123,23 +123,14 @@ int check_ctx(vv_ctx *c)
 double b_field(vv_ctx *c) { return 8.0 * (double)c->N; }
 double b_spec(vv_ctx *c) { return 16.0 * (double)c->nmodes * (double)c->p.nv; }
 
-// drop the records of Arnoldi iterations that returned at once because the
-// cycle had already finished (tag >= iterations actually done)
-void prof_drop_idle(vv_ctx *c, size_t from, int done_its)
+// Arnoldi iterations enqueued past the end of the cycle return at once: their
+// launches keep their (short) time but move no algorithmic bytes.
+void prof_mark_idle(vv_ctx *c, size_t from, int done_its)
 {
     if (!c->prof) return;
     Prof *p = prof_of(c);
-    std::vector<ProfRec> keep(p->recs.begin(), p->recs.begin() + from);
-    for (size_t q = from; q < p->recs.size(); ++q) {
-        const ProfRec &r = p->recs[q];
-        if (r.tag >= done_its) {
-            p->pool.push_back(r.a);
-            p->pool.push_back(r.b);
-        } else {
-            keep.push_back(r);
-        }
-    }
-    p->recs.swap(keep);
+    for (size_t q = from; q < p->recs.size(); ++q)
+        if (p->recs[q].tag >= done_its) p->recs[q].bytes = 0.0;
 }
 
 // y = P^-1 x (spectrum in c->spec); done: optional early-exit flag
@@ -217,7 +208,7 @@ int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *k
                                 c->stream), "status copy");
         VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
         st = c->host_status[0];
-        prof_drop_idle(c, prof_from, st.it);
+        prof_mark_idle(c, prof_from, st.it);
         if (st.reason != 0) break;
         if (st.its >= kp->max_it) {
             st.reason = VV_KSP_DIVERGED_ITS;
@@ -312,6 +303,7 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
     VV_CCHECK(dalloc(&c->twiddle, nx), "alloc twiddle");
     VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * nv), "alloc spectrum");
     VV_CCHECK(dalloc(&c->pinv, nx), "alloc pinv");
+    VV_CCHECK(dalloc(&c->pwork, 3 * nx + 1), "alloc Poisson scratch");
     VV_CCHECK(dalloc(&c->X, c->N), "alloc X");
     VV_CCHECK(dalloc(&c->B, c->N), "alloc B");
     VV_CCHECK(dalloc(&c->D, c->N), "alloc D");
@@ -385,7 +377,7 @@ int vv_ctx_destroy(vv_ctx *c)
     if (!c) return VV_OK;
     (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->h0, c->h1p, c->h1h, c->h2p, c->h2h, c->eigen, c->gm, c->g0, c->gp, c->twiddle, c->spec,
-                    c->pinv, c->X, c->B, c->D, c->basis, c->partial, c->dscalar, c->npart, c->gs};
+                    c->pinv, c->pwork, c->X, c->B, c->D, c->basis, c->partial, c->dscalar, c->npart, c->gs};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);

@@ -8,12 +8,12 @@
 // result is bit-identical to the reference's unfused double arithmetic.
 //
 // Layout: natural order, element (i, j) at i + nx*j (x contiguous).
-// Tiling: one workgroup computes a 64 (x) x 32 (v) tile.  The tile of the
+// Tiling: one workgroup computes a 64 (x) x 28 (v) tile.  The tile of the
 // bracket's first argument and of H_ave, both with a 2-cell halo, is staged
 // in LDS; H_ave is composed on the fly from its 1D factors
 // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i), exactly the
 // value the reference's 2D Have vector holds (PETScVlasovSolver.pyx:145-149),
-// so it costs no HBM traffic.  Each wavefront owns 8 rows of the tile; lane
+// so it costs no HBM traffic.  Each wavefront owns 7 rows of the tile; lane
 // l computes column i0 + l, so global loads and stores are 512-byte rows.
 
 #include "vv1d_internal.h"
@@ -22,7 +22,7 @@ namespace vv {
 namespace {
 
 constexpr int TX = 64;
-constexpr int TY = 32;
+constexpr int TY = 28;      // 4 workgroups per CU fit in LDS
 constexpr int HX = TX + 4;
 constexpr int HY = TY + 4;
 constexpr int NT = 256;
@@ -47,6 +47,9 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
     if (a.done != nullptr && *a.done) return;
     __shared__ double sF[HY][HX];
     __shared__ double sH[HY][HX];
+    __shared__ double sX[4][HX];      // h1p, h1h, h2p, h2h of the tile's columns
+    __shared__ double sV[HY];         // h0 of the tile's rows
+    __shared__ int sI[HX];            // periodic column index of the tile's columns
 
     const int nx = a.nx, nv = a.nv;
     const int ntx = (nx + TX - 1) / TX;
@@ -56,26 +59,59 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
     const int j0 = (t / ntx) * TY;
     const double us = (MODE == MODE_JACOBIAN && a.uscale != nullptr) ? *a.uscale : 1.0;
 
-    for (int idx = threadIdx.x; idx < HY * HX; idx += NT) {
-        const int r = idx / HX;
-        const int c = idx - r * HX;
-        const int gj = j0 - 2 + r;
+    // stage the 1D Hamiltonian factors of the tile (H_ave never touches HBM as a 2D field)
+    if (threadIdx.x < HX) {
+        const int c = threadIdx.x;
         int gi = (i0 - 2 + c) % nx;
         if (gi < 0) gi += nx;
-        double fv = 0.0, hv = 0.0;
-        if (gj >= 0 && gj < nv) {
-            const size_t o = (size_t)gj * nx + gi;
-            if (MODE == MODE_RESIDUAL) {
-                fv = 0.5 * a.fh[o] + 0.5 * a.f[o];           // Fave (PETScNLVlasovMP.pyx:44-46)
-            } else if (MODE == MODE_JACOBIAN) {
-                fv = a.f[o] * us;                            // == VecScale'd Krylov vector
-            } else {
-                fv = a.f[o];
+        sI[c] = gi;
+        sX[0][c] = a.h1p[gi];
+        sX[1][c] = a.h1h[gi];
+        sX[2][c] = a.h2p[gi];
+        sX[3][c] = a.h2h[gi];
+    } else if (threadIdx.x < HX + HY) {
+        const int r = threadIdx.x - HX;
+        const int gj = j0 - 2 + r;
+        sV[r] = (gj >= 0 && gj < nv) ? a.h0[gj] : 0.0;
+    }
+    __syncthreads();
+
+    // all global loads of the tile are issued before any LDS store
+    constexpr int NQ = (HY * HX + NT - 1) / NT;
+    double fv[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        fv[q] = 0.0;
+        if (idx < HY * HX) {
+            const int r = idx / HX;
+            const int c = idx - r * HX;
+            const int gj = j0 - 2 + r;
+            if (gj >= 0 && gj < nv) {
+                const size_t o = (size_t)gj * nx + sI[c];
+                if (MODE == MODE_RESIDUAL) {
+                    fv[q] = 0.5 * a.fh[o] + 0.5 * a.f[o];        // Fave (PETScNLVlasovMP.pyx:44-46)
+                } else if (MODE == MODE_JACOBIAN) {
+                    fv[q] = a.f[o] * us;                         // == VecScale'd Krylov vector
+                } else {
+                    fv[q] = a.f[o];
+                }
             }
-            hv = (((a.h0[gj] + 0.5 * a.h1p[gi]) + 0.5 * a.h1h[gi]) + 0.5 * a.h2p[gi]) + 0.5 * a.h2h[gi];
         }
-        sF[r][c] = fv;
-        sH[r][c] = hv;
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < HY * HX) {
+            const int r = idx / HX;
+            const int c = idx - r * HX;
+            const int gj = j0 - 2 + r;
+            sF[r][c] = fv[q];
+            // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i)  (PETScVlasovSolver.pyx:145-149)
+            sH[r][c] = (gj >= 0 && gj < nv)
+                           ? (((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c]
+                           : 0.0;
+        }
     }
     __syncthreads();
 

@@ -3,11 +3,12 @@
 //  density_kernel    column partial sums of f over chunks of v-rows
 //                    (n_i = hv * sum_j f_ij, Toolbox.pyx:79-98); coalesced
 //                    along x, one chunk of rows per block row.
-//  poisson_kernel    one workgroup: finishes n, forms b = -(n - mean n) q
-//                    (PETScPoissonSolver.pyx:58-74), solves A phi = b by the
-//                    mean-free circulant inverse (the periodic CFD4/CFD2
-//                    matrix is circulant; its pseudo-inverse's first row is
-//                    precomputed on the host), forms h1 = phi - mean phi
+//  poisson_*_kernel  prep (one workgroup): finishes n, forms
+//                    b = -(n - mean n) q (PETScPoissonSolver.pyx:58-74);
+//                    conv: solves A phi = b by the mean-free circulant inverse
+//                    (the periodic CFD4/CFD2 matrix is circulant; its
+//                    pseudo-inverse's first row is precomputed on the host);
+//                    post (one workgroup): h1 = phi - mean phi
 //                    (Toolbox.pyx:38-74) and the Poisson residual norm
 //                    ||A phi + (n - mean n) q|| of run_base.py:631-637.
 //  invariants_kernel mass, L2, kinetic / potential energy over f, plus the
@@ -52,11 +53,12 @@ __device__ double block_sum_big(double v, double *red)
 }
 
 struct PoiArgs {
-    int nx, laplace, from_part, nch;
+    int nx, laplace, from_part;
     double hv, charge, hx2_inv;
     const double *npart;   // density partials (from_part = 1)
     const double *n_in;    // density (from_part = 0)
     const double *pinv;
+    double *work;          // 3*nx + 1 scratch: b | phi | n | mean n
     double *n_out, *phi, *h1, *pres;
 };
 
@@ -71,44 +73,72 @@ __device__ __forceinline__ double lap(const double *x, int i, int nx, int laplac
     return (2. * x0 - xm1 - xp1) * hx2_inv;
 }
 
-__global__ __launch_bounds__(PT) void poisson_kernel(PoiArgs a)
+// one workgroup: n (from the density partials), mean n, b = -(n - mean n) q
+__global__ __launch_bounds__(PT) void poisson_prep_kernel(PoiArgs a)
 {
-    extern __shared__ double sm[];
-    double *sn = sm, *sb = sm + a.nx, *sc = sm + 2 * a.nx, *sp = sm + 3 * a.nx;
     __shared__ double red[PT / 64 + 1];
     const int nx = a.nx;
+    double *b = a.work, *wn = a.work + 2 * nx;
     double part = 0.0;
     for (int i = threadIdx.x; i < nx; i += PT) {
         double n;
         if (a.from_part) {
+            double v[DCH];
+#pragma unroll
+            for (int ch = 0; ch < DCH; ++ch) v[ch] = a.npart[(size_t)ch * nx + i];
             double s = 0.0;
-            for (int ch = 0; ch < a.nch; ++ch) s += a.npart[(size_t)ch * nx + i];
+#pragma unroll
+            for (int ch = 0; ch < DCH; ++ch) s += v[ch];
             n = s * a.hv;
         } else {
             n = a.n_in[i];
         }
-        sn[i] = n;
-        sc[i] = a.pinv[i];
+        wn[i] = n;
         if (a.n_out) a.n_out[i] = n;
         part += n;
     }
     const double nmean = block_sum_big(part, red) / nx;
-    for (int i = threadIdx.x; i < nx; i += PT) sb[i] = -(sn[i] - nmean) * a.charge;
+    for (int i = threadIdx.x; i < nx; i += PT) b[i] = -(wn[i] - nmean) * a.charge;
+    if (threadIdx.x == 0) a.work[3 * nx] = nmean;
+}
+
+// phi_i = sum_m c_m b_{i-m}: C the mean-free circulant inverse of A
+__global__ __launch_bounds__(NT) void poisson_conv_kernel(PoiArgs a)
+{
+    extern __shared__ double sm[];
+    const int nx = a.nx;
+    double *sb = sm, *sc = sm + nx;
+    for (int i = threadIdx.x; i < nx; i += NT) {
+        sb[i] = a.work[i];
+        sc[i] = a.pinv[i];
+    }
     __syncthreads();
-    // phi = C b, C the mean-free circulant inverse of A
-    part = 0.0;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= nx) return;
+    double s = 0.0;
+    for (int m = 0; m < nx; ++m) s += sc[m] * sb[(i - m) & (nx - 1)];
+    a.work[nx + i] = s;
+}
+
+// one workgroup: h1 = phi - mean phi and ||A phi + (n - mean n) q||
+__global__ __launch_bounds__(PT) void poisson_post_kernel(PoiArgs a)
+{
+    extern __shared__ double sm[];
+    __shared__ double red[PT / 64 + 1];
+    const int nx = a.nx;
+    const double *wphi = a.work + nx, *wn = a.work + 2 * nx;
+    const double nmean = a.work[3 * nx];
+    double part = 0.0;
     for (int i = threadIdx.x; i < nx; i += PT) {
-        double s = 0.0;
-        for (int m = 0; m < nx; ++m) s += sc[m] * sb[(i - m) & (nx - 1)];
-        sp[i] = s;
-        part += s;
+        sm[i] = wphi[i];
+        part += wphi[i];
     }
     const double phiave = block_sum_big(part, red) / nx;
     part = 0.0;
     for (int i = threadIdx.x; i < nx; i += PT) {
-        if (a.phi) a.phi[i] = sp[i];
-        if (a.h1) a.h1[i] = sp[i] - phiave;
-        const double r = lap(sp, i, nx, a.laplace, a.hx2_inv) + (sn[i] - nmean) * a.charge;
+        if (a.phi) a.phi[i] = sm[i];
+        if (a.h1) a.h1[i] = sm[i] - phiave;
+        const double r = lap(sm, i, nx, a.laplace, a.hx2_inv) + (wn[i] - nmean) * a.charge;
         part += r * r;
     }
     const double rr = block_sum_big(part, red);
@@ -245,10 +275,13 @@ hipError_t launch_density(vv_ctx *c, const double *f, double *n)
 hipError_t launch_poisson(vv_ctx *c, const double *n_in, double *n_out, double *phi, double *h1, double *pres,
                           int from_part)
 {
-    PoiArgs a{c->p.nx, c->p.laplace, from_part, DCH, c->p.hv, c->p.charge, c->hx2_inv,
-              c->npart, n_in, c->pinv, n_out, phi, h1, pres};
-    const size_t lds = (size_t)4 * c->p.nx * sizeof(double);
-    hipLaunchKernelGGL(poisson_kernel, dim3(1), dim3(PT), lds, c->stream, a);
+    PoiArgs a{c->p.nx, c->p.laplace, from_part, c->p.hv, c->p.charge, c->hx2_inv,
+              c->npart, n_in, c->pinv, c->pwork, n_out, phi, h1, pres};
+    const int nx = c->p.nx;
+    hipLaunchKernelGGL(poisson_prep_kernel, dim3(1), dim3(PT), 0, c->stream, a);
+    hipLaunchKernelGGL(poisson_conv_kernel, dim3((nx + NT - 1) / NT), dim3(NT), 2 * nx * sizeof(double), c->stream,
+                       a);
+    hipLaunchKernelGGL(poisson_post_kernel, dim3(1), dim3(PT), nx * sizeof(double), c->stream, a);
     return hipGetLastError();
 }
 

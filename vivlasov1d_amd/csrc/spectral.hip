@@ -146,12 +146,25 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs a, const double *__res
     extern __shared__ double2 buf[];
     const int nx = a.nx, R = a.rows;
     const int j0 = blockIdx.x * R;
-    for (int idx = threadIdx.x; idx < R * nx; idx += NT) {
+    // every global load of the thread is issued before the LDS stores
+    constexpr int PER = 2 * MAXE / NT;   // R * nx <= 2 * MAXE real inputs
+    double val[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
         const int r = idx >> a.logn;
         const int i = idx & (nx - 1);
-        const double val = (j0 + r < a.nv) ? x[(size_t)(j0 + r) * nx + i] : 0.0;
-        double *slot = reinterpret_cast<double *>(&buf[(r >> 1) * nx + i]);
-        slot[r & 1] = val;
+        val[q] = (idx < R * nx && j0 + r < a.nv) ? x[(size_t)(j0 + r) * nx + i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < R * nx) {
+            const int r = idx >> a.logn;
+            const int i = idx & (nx - 1);
+            double *slot = reinterpret_cast<double *>(&buf[(r >> 1) * nx + i]);
+            slot[r & 1] = val[q];
+        }
     }
     __syncthreads();
     lds_fft<false>(buf, nx, a.logn, R >> 1, a.tw);
@@ -178,20 +191,34 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs a, const double2 *__r
     const int nx = a.nx, R = a.rows, nh = nx >> 1;
     const int j0 = blockIdx.x * R;
     const int npair = R >> 1;
-    // Z_k = A_k + i B_k (k <= nx/2) and its Hermitian extension for k > nx/2.
-    for (int idx = threadIdx.x; idx < (nh + 1) * npair; idx += NT) {
+    // Z_k = A_k + i B_k (k <= nx/2) and its Hermitian extension for k > nx/2;
+    // every global load of the thread is issued before the LDS stores.
+    const int total = (nh + 1) * npair;
+    constexpr int PER = (MAXE / 2 + 64 + NT - 1) / NT;   // (nx/2 + 1) R/2 <= MAXE/2 + 64
+    double2 va[PER], vb[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
         const int k = idx / npair;
-        const int p = idx - k * npair;
-        const int ja = j0 + 2 * p;
-        double2 A = (ja < a.nv) ? spec[(size_t)k * a.nv + ja] : make_double2(0.0, 0.0);
-        double2 B = (ja + 1 < a.nv) ? spec[(size_t)k * a.nv + ja + 1] : make_double2(0.0, 0.0);
-        if (k == 0 || k == nh) {
-            A.y = 0.0;
-            B.y = 0.0;
+        const int ja = j0 + 2 * (idx - k * npair);
+        va[q] = (idx < total && ja < a.nv) ? spec[(size_t)k * a.nv + ja] : make_double2(0.0, 0.0);
+        vb[q] = (idx < total && ja + 1 < a.nv) ? spec[(size_t)k * a.nv + ja + 1] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < total) {
+            const int k = idx / npair;
+            const int p = idx - k * npair;
+            double2 A = va[q], B = vb[q];
+            if (k == 0 || k == nh) {
+                A.y = 0.0;
+                B.y = 0.0;
+            }
+            double2 *z = buf + p * nx;
+            z[k] = make_double2(A.x - B.y, A.y + B.x);
+            if (k != 0 && k != nh) z[nx - k] = make_double2(A.x + B.y, B.x - A.y);
         }
-        double2 *z = buf + p * nx;
-        z[k] = make_double2(A.x - B.y, A.y + B.x);
-        if (k != 0 && k != nh) z[nx - k] = make_double2(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
     lds_fft<true>(buf, nx, a.logn, npair, a.tw);
@@ -239,26 +266,39 @@ template <int M>
 __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restrict__ spec)
 {
     if (a.done != nullptr && *a.done) return;
-    extern __shared__ double2 sd[];          // P*M right-hand sides / solution
+    extern __shared__ double2 sd[];          // P*(M+1): right-hand sides / solution, chunk c at c*(M+1)
     __shared__ double2 sa[NT], sb[NT], sc[NT], se[NT];
     const int k = blockIdx.x;
     const double2 e = a.eigen[k];
     const int P = a.P, n = P * M;
     double2 *S = spec + (size_t)k * a.nv;
-    for (int j = threadIdx.x; j < n; j += NT) sd[j] = (j < a.nv) ? S[j] : make_double2(0.0, 0.0);
+    {
+        // n = P*M <= NT*M: every load of the thread in flight before the LDS stores
+        double2 v[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int j = threadIdx.x + q * NT;
+            v[q] = (j < a.nv) ? S[j] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int j = threadIdx.x + q * NT;
+            if (j < n) sd[j + j / M] = v[q];      // one pad slot per chunk: no bank conflicts
+        }
+    }
     __syncthreads();
 
     const int c = threadIdx.x;
     double2 Am[M], Cm[M], Dm[M];
     double2 ae, be, ce;          // coefficients of the chunk's last row
     if (c < P) {
-        const int s = c * M;
+        const int s = c * M, ls = c * (M + 1);
         // forward sweep: y_j + c'_j y_{j+1} = d'_j - a'_j y_{s-1}
         double2 aj, bj, cj;
         tri_row(a, e, s, aj, bj, cj);
         double2 r = crecip(bj);
         Cm[0] = cmul(cj, r);
-        Dm[0] = cmul(sd[s], r);
+        Dm[0] = cmul(sd[ls], r);
         Am[0] = cmul(aj, r);
 #pragma unroll
         for (int q = 1; q < M; ++q) {
@@ -271,7 +311,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restr
             if (q < M - 1) {
                 r = crecip(csub(bj, cmul(aj, Cm[q - 1])));
                 Cm[q] = cmul(cj, r);
-                Dm[q] = cmul(csub(sd[s + q], cmul(aj, Dm[q - 1])), r);
+                Dm[q] = cmul(csub(sd[ls + q], cmul(aj, Dm[q - 1])), r);
                 Am[q] = cneg(cmul(cmul(aj, Am[q - 1]), r));
             }
         }
@@ -293,7 +333,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restr
     double2 al = make_double2(0.0, 0.0), bt = make_double2(1.0, 0.0), gm = make_double2(0.0, 0.0),
             dl = make_double2(0.0, 0.0);
     if (c < P) {
-        const double2 dE = sd[c * M + M - 1];
+        const double2 dE = sd[c * (M + 1) + M - 1];
         al = cneg(cmul(ae, Am[M - 2]));
         bt = csub(be, cmul(ae, Cm[M - 2]));
         dl = csub(dE, cmul(ae, Dm[M - 2]));
@@ -338,16 +378,16 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restr
     if (c < P) se[c] = cdiv(dl, bt);     // E_c = y at the chunk's last row
     __syncthreads();
     if (c < P) {
-        const int s = c * M;
+        const int ls = c * (M + 1);
         const double2 Eprev = (c > 0) ? se[c - 1] : make_double2(0.0, 0.0);
         const double2 Eown = se[c];
 #pragma unroll
         for (int q = 0; q < M - 1; ++q)
-            sd[s + q] = csub(csub(Dm[q], cmul(Am[q], Eprev)), cmul(Cm[q], Eown));
-        sd[s + M - 1] = Eown;
+            sd[ls + q] = csub(csub(Dm[q], cmul(Am[q], Eprev)), cmul(Cm[q], Eown));
+        sd[ls + M - 1] = Eown;
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < a.nv; j += NT) S[j] = sd[j];
+    for (int j = threadIdx.x; j < a.nv; j += NT) S[j] = sd[j + j / M];
 }
 
 int fft_rows(int nx, int nv)
@@ -403,7 +443,7 @@ hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done)
     int P, M;
     tridiag_shape(c->p.nv, &P, &M);
     TriArgs a{c->p.nv, P, c->eigen, c->gm, c->g0, c->gp, c->afpc, c->ht_inv, done};
-    const size_t lds = (size_t)P * M * sizeof(double2);
+    const size_t lds = (size_t)P * (M + 1) * sizeof(double2);
     dim3 grid(c->nmodes), block(NT);
     switch (M) {
     case 2: hipLaunchKernelGGL(tridiag_kernel<2>, grid, block, lds, c->stream, a, spec); break;

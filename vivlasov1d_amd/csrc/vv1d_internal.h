@@ -67,6 +67,7 @@ struct vv_ctx {
     double2 *spec;      // nmodes * nv complex spectrum, mode-major
     // Poisson
     double *pinv;       // first row of the mean-free circulant inverse (nx)
+    double *pwork;      // Poisson scratch (3*nx + 1)
     // work vectors (N)
     double *X;          // bracket / residual output before P^-1
     double *B;          // SNES right-hand side P^-1 F
