@@ -9,7 +9,11 @@ is one full implicit timestep: Picard loop of SNES(ksponly) + GMRES on the
 preconditioned Jacobian, moments and Poisson after every solve.
 
 Timing: W untimed warm-up steps, then K steps between two
-torch.cuda.synchronize() calls.  value = K / elapsed  (timesteps/s).
+torch.cuda.synchronize() calls.  value = K / elapsed  (timesteps/s) of the
+plain FP64 form.  The mixed-precision form (FP32 GMRES cycles + FP32
+preconditioner inside FP64 iterative refinement, FP64 residual / update /
+Poisson) is then timed the same way from the same initial state and reported
+under "mixed" with its L-infinity distance to the FP64 run after W + K steps.
 roofline: the dominant kernel's algorithmic bytes / its HIP-event time on
 the library stream over the timed region.  cpu_baseline: the C (OpenMP)
 restatement of the same timestep from oracle/, on a bounded sample.
@@ -28,7 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm")
+KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
+           "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32")
 
 
 def workloads():
@@ -78,6 +83,35 @@ def cpu_baseline(w, sample_steps, threads):
     return cbaseline.time_sample(w, sample_steps, threads)
 
 
+def timed_run(w, args, precision, kernel):
+    """W warm-up steps, then K timed steps; returns the run and its figures."""
+    import torch
+    from vivlasov1d_amd.runner import Runscript
+    nt = args.warmup + args.steps
+    run = Runscript(make_cfg(w, nt), profile=w["profile"], precision=precision, inner_rtol=args.inner_rtol)
+    ctx = run.ctx
+    for _ in range(args.warmup):
+        run.step()
+    torch.cuda.synchronize()
+    if kernel == "all":
+        ctx.profile(None)
+    elif kernel not in (None, "none"):
+        ctx.profile((kernel,))
+    nrec0 = len(run.history)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile(())
+    kstats = {}
+    for k in KERNELS:
+        ms, n, by = ctx.profile_read(k)
+        if n:
+            kstats[k] = dict(ms=ms, launches=n, bytes=by, avg_us=1e3 * ms / n, gbs=by / (ms * 1e-3) / 1e9)
+    return run, t1 - t0, run.history[nrec0:], kstats
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +123,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="timesteps of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
+    ap.add_argument("--mixed-kernel", default="none", help="kernel timed in the mixed run (e.g. mdot32)")
+    ap.add_argument("--inner-rtol", type=float, default=1e-5, help="FP32 cycle reduction of the mixed form")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -98,36 +135,10 @@ def main():
 
     import numpy as np
     import torch
-    from vivlasov1d_amd.runner import Runscript
 
     w = workloads()[args.workload]
-    nt = args.warmup + args.steps
-    run = Runscript(make_cfg(w, nt), profile=w["profile"])
-    ctx = run.ctx
     N = w["nx"] * w["nv"]
-
-    for _ in range(args.warmup):
-        run.step()
-    torch.cuda.synchronize()
-    if args.kernel == "all":
-        ctx.profile(None)
-    elif args.kernel not in (None, "none"):
-        ctx.profile((args.kernel,))
-    nrec0 = len(run.history)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run.step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ctx.profile(())
-    elapsed = t1 - t0
-    recs = run.history[nrec0:]
-
-    kstats = {}
-    for k in KERNELS:
-        ms, n, by = ctx.profile_read(k)
-        if n:
-            kstats[k] = dict(ms=ms, launches=n, bytes=by, avg_us=1e3 * ms / n, gbs=by / (ms * 1e-3) / 1e9)
+    run, elapsed, recs, kstats = timed_run(w, args, "double", args.kernel)
     dom = max(kstats, key=lambda k: kstats[k]["ms"]) if kstats else None
     roof = None
     if dom:
@@ -149,6 +160,33 @@ def main():
 
     inv = run.invariants()
     value = args.steps / elapsed
+
+    mixed = None
+    if not args.no_mixed:
+        f64 = run.fc.clone()
+        phi64 = run.pc_int.clone()
+        del run
+        torch.cuda.empty_cache()
+        mrun, melapsed, mrecs, mk = timed_run(w, args, "mixed", args.mixed_kernel)
+        minv = mrun.invariants()
+        phim = mrun.pc_int - mrun.pc_int.mean()
+        phid = phi64 - phi64.mean()
+        mgm = [p["gmres"] for r in mrecs for p in r["picard"]]
+        mcy = [p["cycles"] for r in mrecs for p in r["picard"]]
+        mval = args.steps / melapsed
+        mixed = {
+            "value": round(mval, 3), "unit": "timesteps/s", "ms_per_step": round(1e3 * melapsed / args.steps, 3),
+            "speedup_vs_fp64": round(mval / value, 3), "dtype": "f32 Krylov/P^-1/J.u in f64 refinement",
+            "inner_rtol": args.inner_rtol,
+            "linf_f_vs_fp64": float((mrun.fc - f64).abs().max() / f64.abs().max()),
+            "linf_phi_vs_fp64": float((phim - phid).abs().max() / max(phid.abs().max().item(), 1e-300)),
+            "steps_compared": args.warmup + args.steps,
+            "invariants": {k: minv[k] for k in ("mass", "l2", "energy")},
+            "iterations": {"picard_per_step": float(np.mean([len(r["picard"]) for r in mrecs])),
+                           "fp32_its_per_solve": float(np.mean(mgm)), "cycles_per_solve": float(np.mean(mcy))},
+            "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
+                            "GB/s": round(v["gbs"], 1), "ms": round(v["ms"], 3)} for k, v in mk.items()},
+        }
     line = {
         "metric": "timesteps/s at 1024x2048 J4 (plain FP64)" if args.workload.startswith("twostream_1024")
         else "timesteps/s %s" % args.workload,
@@ -169,6 +207,7 @@ def main():
                         "ms": round(v["ms"], 3)} for k, v in kstats.items()},
         "cpu_baseline": cpu,
         "invariants": {k: inv[k] for k in ("mass", "l2", "energy")},
+        "mixed": mixed,
     }
     if cpu and "value" in cpu and cpu["value"]:
         line["gpu_over_cpu"] = round(value / cpu["value"], 2)

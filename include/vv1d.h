@@ -195,6 +195,35 @@ int vv_ksp_solve(vv_ctx *ctx, const double *b, double *x, const vv_ksp_params *k
 int vv_snes_ksponly(vv_ctx *ctx, double *f, const double *fh,
                     const vv_ksp_params *kp, vv_ksp_result *res);
 
+/* Mixed precision (no counterpart in the reference, whose solve is FP64
+ * throughout; it replaces the same KSPSolve / SNESSolve_KSPONLY call sites):
+ * iterative refinement on (P^-1 J) x = b.  Each correction equation is solved
+ * by one FP32 GMRES cycle (FP32 Jacobian-apply, FP32 preconditioner, FP32
+ * Krylov basis; dots, norms, Hessenberg and Givens in FP64) to a relative
+ * reduction of inner_rtol (less when less meets the target); the residual
+ * r = b - P^-1 J x, the solution x and the stopping test
+ * ||r|| <= max(rtol ||b||, atol) stay FP64, on the true preconditioned
+ * residual.  res->its counts FP32 iterations, res->rnorm is the true final
+ * residual norm; *cycles (may be NULL) the FP32 cycles run. */
+typedef struct {
+    double  inner_rtol;    /* relative reduction per FP32 cycle, e.g. 1e-5  */
+    int32_t max_cycles;    /* cap on FP32 cycles per solve                  */
+} vv_mixed_params;
+
+int vv_ksp_solve_mixed(vv_ctx *ctx, const double *b, double *x, const vv_ksp_params *kp,
+                       const vv_mixed_params *mp, vv_ksp_result *res, int32_t *cycles);
+
+/* vv_snes_ksponly with the linear solve done by vv_ksp_solve_mixed;
+ * b = P^-1 F(f) and the update f -= delta are FP64. */
+int vv_snes_ksponly_mixed(vv_ctx *ctx, double *f, const double *fh, const vv_ksp_params *kp,
+                          const vv_mixed_params *mp, vv_ksp_result *res, int32_t *cycles);
+
+/* The FP32 operators of the mixed solver on FP32 device fields (same layout):
+ * y = u/dt + .5 B(u, H_ave) and y = P^-1 x.  x and y may alias in the
+ * preconditioner. */
+int vv_jacobian_apply32(vv_ctx *ctx, const float *u, float *y);
+int vv_precond_apply32(vv_ctx *ctx, const float *x, float *y);
+
 /* Invariants of core/distribution_function.pyx:130-199,
  * core/hamiltonian.pyx:67-100 and core/potential.pyx:48-59:
  * out[0] mass, out[1] L2, out[2] total energy, out[3] kinetic energy,
@@ -203,15 +232,16 @@ int vv_invariants(vv_ctx *ctx, const double *f, const double *h0,
                   const double *h1, const double *h2, const double *phi,
                   double *out);
 
-/* Per-kernel timing with HIP events on the context stream (for the
+/* Per-kernel timing with HIP events attached to each timed kernel's own dispatch (for the
  * roofline report).  kernel: "jacobian", "residual", "rfft", "tridiag",
- * "irfft", "mdot", "maxpy_norm".  Enable, run, then read the summed time in ms,
- * the launch count and the summed algorithmic HBM bytes of those launches
- * (Krylov launches that returned at once after convergence are not
- * counted).  Reading clears that kernel's records.  enable is a bit mask
- * over the kernels in the order listed (jacobian = 1, residual = 2,
- * rfft = 4, tridiag = 8, irfft = 16, mdot = 32, maxpy_norm = 64; -1 all,
- * 0 off). */
+ * "irfft", "mdot", "maxpy_norm" and the FP32 kernels of the mixed solver
+ * "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32".
+ * Enable, run, then read the summed time in ms, the launch count and the
+ * summed algorithmic HBM bytes of those launches (Krylov launches that
+ * returned at once after convergence are not counted).  Reading clears that
+ * kernel's records.  enable is a bit mask over the kernels in the order
+ * listed (jacobian = 1, residual = 2, rfft = 4, ..., maxpy_norm = 64,
+ * jacobian32 = 128, ..., maxpy_norm32 = 4096; -1 all, 0 off). */
 int vv_profile_enable(vv_ctx *ctx, int enable);
 int vv_profile_read(vv_ctx *ctx, const char *kernel, double *ms, int64_t *launches,
                     double *bytes);

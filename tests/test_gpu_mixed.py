@@ -1,0 +1,166 @@
+"""Mixed-precision solver on the GPU against the FP64 oracle.
+
+The mixed form runs the GMRES cycles in FP32 (FP32 Jacobian-apply, FP32
+P^-1, FP32 Krylov basis with FP64 dots/norms/Hessenberg) inside FP64
+iterative refinement; residual, update, moments and Poisson stay FP64.
+Bars (each written in the test):
+  * FP32 operators: max |gpu - oracle| <= 2e-6 * max |oracle| (a few float
+    ulps of the largest term; the oracle is FP64).
+  * Mixed KSP: converged reason, true FP64 preconditioned residual
+    <= rtol ||b||, and the solution within 1e-8 (relative, max norm) of the
+    oracle's FP64 GMRES solution.
+  * Mixed Picard steps: f within 1e-9 relative of the FP64 oracle run and
+    the invariants within 1e-9 relative, at ksp_rtol = snes_atol = 1e-10.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import vlasov_oracle as O
+
+from test_gpu_parity import KINDS, _nat, _setup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nx,nv", [(8, 16), (32, 50), (64, 128), (256, 512), (1024, 2048)])
+@pytest.mark.parametrize("kind", KINDS)
+def test_fp32_jacobian(nx, nv, kind, cuda):
+    import torch
+    print("case", nx, nv, kind, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, kind, 21 + nx, cuda)
+    u = rng.standard_normal((nx, nv)).astype(np.float32).astype(np.float64)
+    ref = O.jacobian(kind, u, H, g)
+    ud = T(_nat(u)).float()
+    y = torch.empty_like(ud)
+    ctx.jacobian_apply32(ud, y)
+    got = y.double().cpu().numpy().T
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= 2e-6, err
+
+
+@pytest.mark.parametrize("nx,nv", [(8, 16), (32, 50), (64, 128), (256, 512), (1024, 2048)])
+def test_fp32_preconditioner(nx, nv, cuda):
+    import torch
+    print("case", nx, nv, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, 'arakawaj4', 31 + nv, cuda)
+    pc = O.Preconditioner(g, 'lapack') if nx * nv <= 256 * 512 else O.Preconditioner(g, 'scipy')
+    x = rng.standard_normal((nx, nv)).astype(np.float32).astype(np.float64)
+    ref = pc.apply(x)
+    xd = T(_nat(x)).float()
+    y = torch.empty_like(xd)
+    ctx.precond32(xd, y)
+    got = y.double().cpu().numpy().T
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= 2e-6, err
+    ctx.precond32(xd, xd)      # in place
+    assert torch.equal(xd, y)
+
+
+@pytest.mark.parametrize("nx,nv,kind,restart", [(16, 32, 'arakawaj4', 30), (64, 128, 'arakawaj1', 30),
+                                                (64, 128, 'arakawaj4', 3), (256, 512, 'arakawaj2', 30),
+                                                (256, 512, 'arakawaj4', 30)])
+def test_mixed_gmres_matches_oracle(nx, nv, kind, restart, cuda):
+    print("case", nx, nv, kind, restart, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, kind, 11, cuda, restart=restart)
+    pc = O.Preconditioner(g, 'lapack')
+    f = rng.random((nx, nv))
+    f[:, :2] = 0
+    f[:, -2:] = 0
+    fh = f + 1e-3 * rng.standard_normal((nx, nv))
+    b = pc.apply(O.residual(kind, f, fh, H, g))
+    A = lambda v: pc.apply(O.jacobian(kind, v, H, g))
+    x_ref, its_ref, reason_ref = O.gmres(A, b, rtol=1e-10, atol=1e-50, max_it=1000, restart=restart)
+    assert reason_ref > 0
+    xd = ctx.new_field()
+    res, cycles = ctx.ksp_solve_mixed(T(_nat(b)), xd, 1e-10, 1e-50, 1e5, 1000, inner_rtol=1e-5)
+    print("its", res.its, "cycles", cycles, "fp64 its", its_ref, flush=True)
+    assert res.reason in (O.KSP_CONVERGED_RTOL, O.KSP_CONVERGED_ATOL), res.reason
+    assert cycles >= 1
+    x = xd.cpu().numpy().T
+    # the true FP64 residual of the returned solution meets the FP64 test
+    r = b - A(x)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b) * (1 + 1e-6)
+    assert abs(res.rnorm - np.linalg.norm(r)) <= 1e-3 * np.linalg.norm(r) + 1e-14 * np.linalg.norm(b)
+    assert abs(res.rnorm0 - np.linalg.norm(b)) <= 1e-13 * np.linalg.norm(b)
+    assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+
+
+def test_mixed_gmres_limits(cuda):
+    """max_it and max_cycles stop the refinement with DIVERGED_ITS; a zero
+    right-hand side converges at once with a zero solution."""
+    import torch
+    g, ctx, H, rng, T = _setup(64, 128, 'arakawaj4', 13, cuda, restart=5)
+    b = rng.standard_normal((64, 128))
+    xd = ctx.new_field()
+    res, cycles = ctx.ksp_solve_mixed(T(_nat(b)), xd, 1e-14, 1e-50, 1e5, 1000, inner_rtol=0.5, max_cycles=2)
+    assert res.reason == O.KSP_DIVERGED_ITS and cycles == 2
+    res, cycles = ctx.ksp_solve_mixed(T(_nat(b)), xd, 1e-14, 1e-50, 1e5, 3, inner_rtol=1e-5)
+    assert res.reason == O.KSP_DIVERGED_ITS and res.its >= 3
+    xd.fill_(1.0)
+    res, cycles = ctx.ksp_solve_mixed(ctx.new_field(), xd, 1e-10, 1e-50, 1e5, 100)
+    assert res.reason == O.KSP_CONVERGED_ATOL and cycles == 0
+    assert torch.count_nonzero(xd).item() == 0
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_mixed_picard_steps_match_oracle(kind, cuda):
+    """Three implicit timesteps of the Landau problem on 32x64 with the mixed
+    solver against the FP64 oracle run."""
+    from vivlasov1d_amd.core import Config
+    from vivlasov1d_amd.runner import Runscript
+    nx, nv, L, vmax, ht = 32, 64, 4 * np.pi, 8.0, 0.05
+    prof = lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * np.exp(-0.5 * v ** 2) / math.sqrt(2 * np.pi)
+    g = O.Grid(nx, nv, L, vmax, ht=ht)
+    orun = O.OracleRun(g, O.initial_distribution(g, prof), bracket=kind, ksp_rtol=1e-10, snes_atol=1e-10)
+    cfg = Config({'grid': {'nx': nx, 'nv': nv, 'L': L, 'vmax': vmax, 'ht': ht, 'nt': 3},
+                  'solver': {'poisson_bracket': kind, 'laplace_operator': 'CFD4', 'preconditioner_type': 'tensor',
+                             'preconditioner_scheme': 'tensor_kinetic', 'petsc_ksp_rtol': 1e-10,
+                             'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
+                  'initial_data': {'charge': -1.0}})
+    run = Runscript(cfg, profile=prof, precision='mixed')
+    for s in range(3):
+        eo = orun.step()
+        eg = run.step()
+        print(s, eo['initial'], eg['initial'], len(eo['picard']), len(eg['picard']),
+              [p['cycles'] for p in eg['picard']], flush=True)
+        assert abs(eg['initial'] - eo['initial']) <= 1e-8 * eo['initial']
+        for pg in eg['picard']:
+            assert pg['reason'] > 0 and pg['cycles'] >= 1
+        f = run.fc.cpu().numpy().T
+        assert np.abs(f - orun.fc).max() <= 1e-9 * np.abs(orun.fc).max()
+    inv = run.invariants()
+    ref = O.invariants(orun.fc, orun.h0, orun.h1c, np.zeros(nx), orun.phi, g, -1.0)
+    for k in ('mass', 'l2', 'energy', 'ekin', 'epot'):
+        assert abs(inv[k] - ref[k]) <= 1e-9 * abs(ref[k]) + 1e-14, k
+
+
+def test_mixed_twostream_step_matches_fp64(cuda):
+    """One timestep of the two-stream workload at 256x512 J4: mixed against
+    the FP64 GPU path (itself pinned to the oracle above)."""
+    from vivlasov1d_amd.core import Config
+    from vivlasov1d_amd.runner import Runscript
+    nx, nv, L, vmax, ht = 256, 512, 2 * np.pi / 0.2, 10.0, 0.05
+
+    def prof(x, v):
+        m = lambda w: np.exp(-0.5 * w ** 2) / math.sqrt(2 * np.pi)
+        return (1 + 0.001 * np.cos(0.2 * x)) * 0.5 * (m(v - 2.4) + m(v + 2.4))
+
+    cfg = lambda: Config({'grid': {'nx': nx, 'nv': nv, 'L': L, 'vmax': vmax, 'ht': ht, 'nt': 1},
+                          'solver': {'poisson_bracket': 'arakawaj4', 'laplace_operator': 'CFD4',
+                                     'preconditioner_type': 'tensor', 'preconditioner_scheme': 'tensor_kinetic',
+                                     'petsc_ksp_rtol': 1e-10, 'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
+                          'initial_data': {'charge': -1.0}})
+    r64 = Runscript(cfg(), profile=prof)
+    rmx = Runscript(cfg(), profile=prof, precision='mixed')
+    for s in range(2):
+        e64 = r64.step()
+        emx = rmx.step()
+        its64 = sum(p['gmres'] for p in e64['picard'])
+        itsmx = sum(p['gmres'] for p in emx['picard'])
+        print(s, len(e64['picard']), len(emx['picard']), its64, itsmx, flush=True)
+        f64 = r64.fc.cpu().numpy()
+        fmx = rmx.fc.cpu().numpy()
+        assert np.abs(fmx - f64).max() <= 1e-9 * np.abs(f64).max()

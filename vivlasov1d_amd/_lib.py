@@ -41,6 +41,10 @@ class KspResult(ctypes.Structure):
                 ("rnorm0", ctypes.c_double)]
 
 
+class MixedParams(ctypes.Structure):
+    _fields_ = [("inner_rtol", ctypes.c_double), ("max_cycles", ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _I64 = ctypes.c_int64
@@ -73,6 +77,12 @@ SIGNATURES = {
     "vv_axpy": (ctypes.c_int, [_P, _D, _P, _P, _I64]),
     "vv_ksp_solve": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(KspResult)]),
     "vv_snes_ksponly": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(KspResult)]),
+    "vv_ksp_solve_mixed": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(MixedParams),
+                                          ctypes.POINTER(KspResult), ctypes.POINTER(ctypes.c_int32)]),
+    "vv_snes_ksponly_mixed": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(MixedParams),
+                                             ctypes.POINTER(KspResult), ctypes.POINTER(ctypes.c_int32)]),
+    "vv_jacobian_apply32": (ctypes.c_int, [_P, _P, _P]),
+    "vv_precond_apply32": (ctypes.c_int, [_P, _P, _P]),
     "vv_invariants": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _PD]),
     "vv_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "vv_profile_read": (ctypes.c_int, [_P, ctypes.c_char_p, _PD, ctypes.POINTER(ctypes.c_int64), _PD]),

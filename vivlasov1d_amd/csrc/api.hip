@@ -17,8 +17,13 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum ProfKind { P_JAC = 0, P_RES, P_RFFT, P_TRI, P_IRFFT, P_MDOT, P_MAXPY, P_NKINDS };
-const char *kProfNames[P_NKINDS] = {"jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm"};
+enum ProfKind {
+    P_JAC = 0, P_RES, P_RFFT, P_TRI, P_IRFFT, P_MDOT, P_MAXPY,
+    P_JAC32, P_RFFT32, P_TRI32, P_IRFFT32, P_MDOT32, P_MAXPY32, P_NKINDS
+};
+const char *kProfNames[P_NKINDS] = {"jacobian",   "residual", "rfft",      "tridiag", "irfft",
+                                    "mdot",       "maxpy_norm", "jacobian32", "rfft32",  "tridiag32",
+                                    "irfft32",    "mdot32",   "maxpy_norm32"};
 
 struct ProfRec {
     int kind;
@@ -53,21 +58,24 @@ struct ProfScope {
     int tag;
     double bytes;
     hipEvent_t a = nullptr, b = nullptr;
+    // The events are attached to the main kernel's own dispatch
+    // (hipExtLaunchKernelGGL in launch_timed), so they time the kernel itself.
     ProfScope(vv_ctx *cc, int k, double by, int tg = -1) : c(cc), kind(k), tag(tg), bytes(by)
     {
         if ((c->prof >> kind) & 1) {
             Prof *p = prof_of(c);
             a = prof_event(p);
             b = prof_event(p);
-            if (a) (void)hipEventRecord(a, c->stream);
+            if (a && b) {
+                c->prof_a = a;
+                c->prof_b = b;
+            }
         }
     }
     ~ProfScope()
     {
-        if (a && b) {
-            (void)hipEventRecord(b, c->stream);
-            prof_of(c)->recs.push_back({kind, tag, bytes, a, b});
-        }
+        c->prof_a = c->prof_b = nullptr;
+        if (a && b) prof_of(c)->recs.push_back({kind, tag, bytes, a, b});
     }
 };
 
@@ -162,6 +170,71 @@ int arnoldi_step(vv_ctx *c, int it)
     return VV_OK;
 }
 
+// y = P^-1 x in FP32 (complex64 spectrum in c->spec)
+int precond32(vv_ctx *c, const float *x, float *y, const int32_t *done)
+{
+    float2 *spec = reinterpret_cast<float2 *>(c->spec);
+    const double F = 4.0 * (double)c->N, S = 0.5 * b_spec(c);
+    VV_PLAUNCH(P_RFFT32, F + S, -1, launch_rfft32(c, x, spec, done), "rfft32");
+    VV_PLAUNCH(P_TRI32, 2 * S, -1, launch_tridiag32(c, spec, done), "tridiag32");
+    VV_PLAUNCH(P_IRFFT32, S + F, -1, launch_irfft32(c, spec, y, done), "irfft32");
+    return VV_OK;
+}
+
+// one Arnoldi step of the FP32 cycle: V_{it+1} = P32^-1 J32 v_it, CGS in FP64
+// arithmetic on the FP32 basis
+int arnoldi_step32(vv_ctx *c, int it)
+{
+    const int32_t *done = &c->gs->st.done;
+    float *v = c->basis32 + (size_t)it * c->N;
+    float *w = c->basis32 + (size_t)(it + 1) * c->N;
+    float *X = reinterpret_cast<float *>(c->X);
+    float2 *spec = reinterpret_cast<float2 *>(c->spec);
+    const double F = 4.0 * (double)c->N, S = 0.5 * b_spec(c);
+    VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], X), "jacobian32");
+    VV_PLAUNCH(P_RFFT32, F + S, it, launch_rfft32(c, X, spec, done), "rfft32");
+    VV_PLAUNCH(P_TRI32, 2 * S, it, launch_tridiag32(c, spec, done), "tridiag32");
+    VV_PLAUNCH(P_IRFFT32, S + F, it, launch_irfft32(c, spec, w, done), "irfft32");
+    VV_PLAUNCH(P_MDOT32, F * (it + 2), it, launch_mdot32(c, it), "mdot32");
+    VV_PLAUNCH(P_MAXPY32, F * (it + 3), it, launch_maxpy_norm32(c, it), "maxpy_norm32");
+    return VV_OK;
+}
+
+// Arnoldi steps of one GMRES cycle, polling the device status every two
+// steps (the status copy of step k is read while steps k+1, k+2 run); returns
+// once the cycle has finished on the device, with its status in
+// host_status[0].  Steps enqueued past the end of the cycle return at once on
+// the device.
+int run_cycle(vv_ctx *c, bool fp32)
+{
+    const int max_k = c->p.restart;
+    const int chunk = 2;
+    const size_t prof_from = c->prof ? prof_of(c)->recs.size() : 0;
+    int it = 0, pend = -1, slot = 0;
+    bool stop = false;
+    while (it < max_k && !stop) {
+        const int end = it + chunk < max_k ? it + chunk : max_k;
+        for (; it < end; ++it) {
+            int r = fp32 ? arnoldi_step32(c, it) : arnoldi_step(c, it);
+            if (r) return r;
+        }
+        VV_CHECK(hipMemcpyAsync(&c->host_status[slot], &c->gs->st, sizeof(GmresStatus), hipMemcpyDeviceToHost,
+                                c->stream), "status copy");
+        VV_CHECK(hipEventRecord(c->ev[slot], c->stream), "event");
+        if (pend >= 0) {
+            VV_CHECK(hipEventSynchronize(c->ev[pend]), "event sync");
+            if (c->host_status[pend].done) stop = true;
+        }
+        pend = slot;
+        slot ^= 1;
+    }
+    VV_CHECK(hipMemcpyAsync(&c->host_status[0], &c->gs->st, sizeof(GmresStatus), hipMemcpyDeviceToHost, c->stream),
+             "status copy");
+    VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+    prof_mark_idle(c, prof_from, c->host_status[0].it);
+    return VV_OK;
+}
+
 int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, vv_ksp_result *res)
 {
     if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
@@ -170,8 +243,6 @@ int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *k
     VV_CHECK(hipMemsetAsync(x, 0, N * sizeof(double), c->stream), "zero x");
     VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
     VV_CHECK(launch_copy(c, b, c->basis, N), "copy b");
-    const int max_k = c->p.restart;
-    const int chunk = 2;
     bool first = true;
     GmresStatus st{};
     for (;;) {
@@ -180,35 +251,16 @@ int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *k
             VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
             int r = precond(c, c->X, c->basis, nullptr);
             if (r) return r;
-            VV_CHECK(launch_residual_restart(c, b), "restart residual");
+            VV_CHECK(launch_residual_restart(c, b, c->basis), "restart residual");
         }
         first = false;
         VV_CHECK(launch_norm_start(c), "cycle start");
-        const size_t prof_from = c->prof ? prof_of(c)->recs.size() : 0;
-        int it = 0, pend = -1, slot = 0;
-        bool stop = false;
-        while (it < max_k && !stop) {
-            const int end = it + chunk < max_k ? it + chunk : max_k;
-            for (; it < end; ++it) {
-                int r = arnoldi_step(c, it);
-                if (r) return r;
-            }
-            VV_CHECK(hipMemcpyAsync(&c->host_status[slot], &c->gs->st, sizeof(GmresStatus), hipMemcpyDeviceToHost,
-                                    c->stream), "status copy");
-            VV_CHECK(hipEventRecord(c->ev[slot], c->stream), "event");
-            if (pend >= 0) {
-                VV_CHECK(hipEventSynchronize(c->ev[pend]), "event sync");
-                if (c->host_status[pend].done) stop = true;
-            }
-            pend = slot;
-            slot ^= 1;
-        }
+        if (int r = run_cycle(c, false)) return r;
         VV_CHECK(launch_build_solution(c, x), "build solution");
         VV_CHECK(hipMemcpyAsync(&c->host_status[0], &c->gs->st, sizeof(GmresStatus), hipMemcpyDeviceToHost,
                                 c->stream), "status copy");
         VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
         st = c->host_status[0];
-        prof_mark_idle(c, prof_from, st.it);
         if (st.reason != 0) break;
         if (st.its >= kp->max_it) {
             st.reason = VV_KSP_DIVERGED_ITS;
@@ -221,6 +273,71 @@ int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *k
         res->rnorm = st.res;
         res->rnorm0 = st.rnorm0;
     }
+    return VV_OK;
+}
+
+// Mixed-precision solve of (P^-1 J) x = b: iterative refinement whose
+// correction equations are solved by FP32 GMRES cycles (FP32 J u, FP32 P^-1,
+// FP32 basis; Hessenberg, dots and norms in FP64) and whose residual
+// r = b - P^-1 J x, solution x and stopping test are FP64.  The stopping test
+// is the reference's (||r|| <= max(rtol ||b||, atol)) applied to the true
+// FP64 preconditioned residual; every cycle restarts from that residual.
+int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, const vv_mixed_params *mp,
+                         vv_ksp_result *res, int32_t *cycles)
+{
+    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
+        return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve_mixed needs the tensor_kinetic preconditioner");
+    const int64_t N = c->N;
+    VV_CHECK(hipMemsetAsync(x, 0, N * sizeof(double), c->stream), "zero x");
+    VV_CHECK(launch_norm2(c, b, N, c->dscalar), "norm");
+    double rnorm0 = 0.0;
+    if (int r = read_scalar(c, c->dscalar, &rnorm0)) return r;
+    const double ttol = std::fmax(kp->rtol * rnorm0, kp->atol);
+    double rnorm = rnorm0;
+    const double *rsrc = b;
+    int its = 0, rounds = 0, reason = 0;
+    for (;;) {
+        if (!std::isfinite(rnorm)) {
+            reason = VV_KSP_DIVERGED_NANORINF;
+            break;
+        }
+        if (rnorm <= ttol) {
+            reason = rnorm < kp->atol ? VV_KSP_CONVERGED_ATOL : VV_KSP_CONVERGED_RTOL;
+            break;
+        }
+        if (rnorm >= kp->dtol * rnorm0) {
+            reason = VV_KSP_DIVERGED_DTOL;
+            break;
+        }
+        if (its >= kp->max_it || rounds >= mp->max_cycles) {
+            reason = VV_KSP_DIVERGED_ITS;
+            break;
+        }
+        // FP32 cycle on the correction equation (P^-1 J) e = r, to a relative
+        // reduction of inner_rtol, or less when less already meets the target
+        double rt = 0.5 * ttol / rnorm;
+        if (rt < mp->inner_rtol) rt = mp->inner_rtol;
+        VV_CHECK(launch_gmres_init(c, rt, 0.0, 1.0e300, kp->max_it - its), "gmres init");
+        VV_CHECK(launch_start32(c, rsrc), "cycle start");
+        if (int r = run_cycle(c, true)) return r;
+        its += c->host_status[0].it;
+        ++rounds;
+        VV_CHECK(launch_build_solution32(c, x), "build solution");
+        // true FP64 residual r = b - P^-1 J x
+        VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
+        if (int r = precond(c, c->X, c->R, nullptr)) return r;
+        VV_CHECK(launch_residual_restart(c, b, c->R), "residual");
+        VV_CHECK(launch_norm2(c, c->R, N, c->dscalar), "norm");
+        if (int r = read_scalar(c, c->dscalar, &rnorm)) return r;
+        rsrc = c->R;
+    }
+    if (res) {
+        res->its = its;
+        res->reason = reason;
+        res->rnorm = rnorm;
+        res->rnorm0 = rnorm0;
+    }
+    if (cycles) *cycles = rounds;
     return VV_OK;
 }
 
@@ -301,6 +418,7 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
     VV_CCHECK(dalloc(&c->g0, nv), "alloc g0");
     VV_CCHECK(dalloc(&c->gp, nv), "alloc gp");
     VV_CCHECK(dalloc(&c->twiddle, nx), "alloc twiddle");
+    VV_CCHECK(dalloc(&c->twiddle32, nx), "alloc twiddle32");
     VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * nv), "alloc spectrum");
     VV_CCHECK(dalloc(&c->pinv, nx), "alloc pinv");
     VV_CCHECK(dalloc(&c->pwork, 3 * nx + 1), "alloc Poisson scratch");
@@ -308,6 +426,10 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
     VV_CCHECK(dalloc(&c->B, c->N), "alloc B");
     VV_CCHECK(dalloc(&c->D, c->N), "alloc D");
     VV_CCHECK(dalloc(&c->basis, (size_t)(p->restart + 1) * c->N), "alloc Krylov basis");
+    // the mixed solver's FP32 basis and FP64 residual share this allocation:
+    // (restart+1) N floats, then N doubles (fits for restart >= 1)
+    c->basis32 = reinterpret_cast<float *>(c->basis);
+    c->R = c->basis + (size_t)(p->restart + 1) * (size_t)(c->N / 2);
     VV_CCHECK(dalloc(&c->partial, (size_t)nrows * (kMaxRestart + 2)), "alloc partial");
     VV_CCHECK(dalloc(&c->dscalar, 16), "alloc scalars");
     VV_CCHECK(dalloc(&c->npart, (size_t)64 * nx), "alloc density partials");
@@ -333,10 +455,13 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
         eig[k].y = p->eigen ? p->eigen[2 * k + 1] : 0.0;
     }
     std::vector<double2> tw(nx);
+    std::vector<float2> tw32(nx);
     for (int m = 0; m < nx; ++m) {
         const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)nx;
         tw[m].x = (double)cosl(ang);
         tw[m].y = (double)sinl(ang);
+        tw32[m].x = (float)cosl(ang);
+        tw32[m].y = (float)sinl(ang);
     }
     // mean-free pseudo-inverse of the periodic Laplacian: c_m = 1/nx sum_{k>0} cos(2 pi k m/nx) / lambda_k
     std::vector<long double> ct(nx), lam(nx);
@@ -366,6 +491,7 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
     VV_CCHECK(hipMemcpy(c->gp, gp.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->eigen, eig.data(), c->nmodes * sizeof(double2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->twiddle, tw.data(), nx * sizeof(double2), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->twiddle32, tw32.data(), nx * sizeof(float2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->pinv, pinv.data(), nx * sizeof(double), hipMemcpyHostToDevice), "upload");
 #undef VV_CCHECK
     *out = c;
@@ -376,7 +502,8 @@ int vv_ctx_destroy(vv_ctx *c)
 {
     if (!c) return VV_OK;
     (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->h0, c->h1p, c->h1h, c->h2p, c->h2h, c->eigen, c->gm, c->g0, c->gp, c->twiddle, c->spec,
+    void *bufs[] = {c->h0, c->h1p, c->h1h, c->h2p, c->h2h, c->eigen, c->gm, c->g0, c->gp, c->twiddle, c->twiddle32,
+                    c->spec,
                     c->pinv, c->pwork, c->X, c->B, c->D, c->basis, c->partial, c->dscalar, c->npart, c->gs};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -415,7 +542,8 @@ int64_t vv_scratch_bytes(const vv_ctx *c)
     if (!c) return 0;
     const int64_t nx = c->p.nx, nv = c->p.nv;
     return (int64_t)sizeof(double) * ((c->p.restart + 4) * c->N + 64 * nx + 4 * nx + 4 * nv) +
-           (int64_t)sizeof(double2) * ((int64_t)c->nmodes * nv + nx + c->nmodes) + (int64_t)sizeof(GmresState);
+           (int64_t)sizeof(double2) * ((int64_t)c->nmodes * nv + nx + c->nmodes) + (int64_t)sizeof(float2) * nx +
+           (int64_t)sizeof(GmresState);
 }
 
 int vv_set_hamiltonian(vv_ctx *c, const double *h0, const double *h1p, const double *h1h, const double *h2p,
@@ -581,6 +709,52 @@ int vv_snes_ksponly(vv_ctx *c, double *f, const double *fh, const vv_ksp_params 
     return VV_OK;
 }
 
+int vv_ksp_solve_mixed(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, const vv_mixed_params *mp,
+                       vv_ksp_result *res, int32_t *cycles)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!b || !x || !kp || !mp) return fail(c, VV_ERR_INVALID, "null argument");
+    if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
+    if (!(mp->inner_rtol > 0.0 && mp->inner_rtol < 1.0) || mp->max_cycles < 1)
+        return fail(c, VV_ERR_INVALID, "inner_rtol must be in (0, 1) and max_cycles >= 1");
+    return ksp_solve_mixed_impl(c, b, x, kp, mp, res, cycles);
+}
+
+int vv_snes_ksponly_mixed(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, const vv_mixed_params *mp,
+                          vv_ksp_result *res, int32_t *cycles)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!f || !fh || !kp || !mp) return fail(c, VV_ERR_INVALID, "null argument");
+    if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
+    if (!(mp->inner_rtol > 0.0 && mp->inner_rtol < 1.0) || mp->max_cycles < 1)
+        return fail(c, VV_ERR_INVALID, "inner_rtol must be in (0, 1) and max_cycles >= 1");
+    if (int r = vv_function(c, f, fh, c->B)) return r;
+    vv_ksp_result kr{};
+    if (int r = ksp_solve_mixed_impl(c, c->B, c->D, kp, mp, &kr, cycles)) return r;
+    // SNESSolve_KSPONLY takes the step unless the linear solve diverged
+    if (kr.reason > 0) VV_CHECK(launch_axpy(c, -1.0, c->D, f, c->N), "update");
+    if (res) *res = kr;
+    return VV_OK;
+}
+
+int vv_jacobian_apply32(vv_ctx *c, const float *u, float *y)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (u == y) return fail(c, VV_ERR_INVALID, "jacobian_apply32: u and y must not alias");
+    VV_PLAUNCH(P_JAC32, 8.0 * (double)c->N, -1, launch_jacobian32(c, u, nullptr, y), "jacobian32");
+    return VV_OK;
+}
+
+int vv_precond_apply32(vv_ctx *c, const float *x, float *y)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!x || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
+        return fail(c, VV_ERR_UNSUPPORTED, "precond_apply32 needs the tensor_kinetic preconditioner");
+    return precond32(c, x, y, nullptr);
+}
+
 int vv_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1, const double *h2,
                   const double *phi, double *out)
 {
@@ -617,7 +791,8 @@ int vv_profile_read(vv_ctx *c, const char *kernel, double *ms, int64_t *launches
     for (auto &r : p->recs) {
         if (r.kind == kind) {
             float t = 0.f;
-            if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            // launches that returned at once (cycle already finished) are not counted
+            if (r.bytes > 0.0 && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
                 tot += t;
                 by += r.bytes;
                 ++cnt;

@@ -30,23 +30,27 @@ constexpr int ROWS_PER_WAVE = TY / (NT / 64);
 
 enum { MODE_RESIDUAL = 0, MODE_JACOBIAN = 1, MODE_ACCUMULATE = 2 };
 
+// T is the field type: double (bit-exact FP64 path) or float (Jacobian-apply
+// of the FP32 GMRES cycle of the mixed-precision solver; H_ave is composed in
+// FP64 and rounded once, the stencil runs in float).
+template <typename T>
 struct BrArgs {
     int nx, nv;
-    const double *f;       // residual: f; jacobian: u; accumulate: x
-    const double *fh;      // residual: f at the previous time level
+    const T *f;            // residual: f; jacobian: u; accumulate: x
+    const T *fh;           // residual: f at the previous time level
     const double *uscale;  // jacobian: optional device scale of u (unnormalised Krylov vector)
-    double *y;
+    T *y;
     const double *h0, *h1p, *h1h, *h2p, *h2h;
     double ht_inv, af1, af2, factor;
     const int32_t *done;   // optional early-exit flag (GMRES cycle finished)
 };
 
-template <int KIND, int MODE>
-__global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
+template <int KIND, int MODE, typename T>
+__global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 {
     if (a.done != nullptr && *a.done) return;
-    __shared__ double sF[HY][HX];
-    __shared__ double sH[HY][HX];
+    __shared__ T sF[HY][HX];
+    __shared__ T sH[HY][HX];
     __shared__ double sX[4][HX];      // h1p, h1h, h2p, h2h of the tile's columns
     __shared__ double sV[HY];         // h0 of the tile's rows
     __shared__ int sI[HX];            // periodic column index of the tile's columns
@@ -57,7 +61,8 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
     const int t = xcd_tile(blockIdx.x, ntx * nty);
     const int i0 = (t % ntx) * TX;
     const int j0 = (t / ntx) * TY;
-    const double us = (MODE == MODE_JACOBIAN && a.uscale != nullptr) ? *a.uscale : 1.0;
+    const T us = (MODE == MODE_JACOBIAN && a.uscale != nullptr) ? (T)*a.uscale : T(1);
+    const T half = T(0.5);
 
     // stage the 1D Hamiltonian factors of the tile (H_ave never touches HBM as a 2D field)
     if (threadIdx.x < HX) {
@@ -76,13 +81,24 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
     }
     __syncthreads();
 
+    // FP32: H_ave enters every Arakawa term only through differences, so the
+    // float tile holds H_ave - H_ave(tile centre), formed in FP64 and rounded
+    // once; the rounding error is then relative to H's variation over the
+    // tile, not to |H| (which would cost ~|H|/(hv dH/dv) ulps on fine v grids)
+    double href = 0.0;
+    if (sizeof(T) == 4) {
+        const int rc = HY / 2, cc = HX / 2;
+        const int gj = min(max(j0 - 2 + rc, 0), nv - 1);
+        href = (((a.h0[gj] + 0.5 * sX[0][cc]) + 0.5 * sX[1][cc]) + 0.5 * sX[2][cc]) + 0.5 * sX[3][cc];
+    }
+
     // all global loads of the tile are issued before any LDS store
     constexpr int NQ = (HY * HX + NT - 1) / NT;
-    double fv[NQ];
+    T fv[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int idx = threadIdx.x + q * NT;
-        fv[q] = 0.0;
+        fv[q] = T(0);
         if (idx < HY * HX) {
             const int r = idx / HX;
             const int c = idx - r * HX;
@@ -90,7 +106,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
             if (gj >= 0 && gj < nv) {
                 const size_t o = (size_t)gj * nx + sI[c];
                 if (MODE == MODE_RESIDUAL) {
-                    fv[q] = 0.5 * a.fh[o] + 0.5 * a.f[o];        // Fave (PETScNLVlasovMP.pyx:44-46)
+                    fv[q] = half * a.fh[o] + half * a.f[o];      // Fave (PETScNLVlasovMP.pyx:44-46)
                 } else if (MODE == MODE_JACOBIAN) {
                     fv[q] = a.f[o] * us;                         // == VecScale'd Krylov vector
                 } else {
@@ -108,9 +124,18 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
             const int gj = j0 - 2 + r;
             sF[r][c] = fv[q];
             // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i)  (PETScVlasovSolver.pyx:145-149)
-            sH[r][c] = (gj >= 0 && gj < nv)
-                           ? (((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c]
-                           : 0.0;
+            if (sizeof(T) == 8) {
+                sH[r][c] = (gj >= 0 && gj < nv)
+                               ? (T)((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
+                               : T(0);
+            } else {
+                // outside the grid H is 0 in the reference; those cells only meet
+                // rows whose result is not formed (j < 2 or j >= nv - 2)
+                sH[r][c] = (gj >= 0 && gj < nv)
+                               ? (T)(((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
+                                     - href)
+                               : T(0);
+            }
         }
     }
     __syncthreads();
@@ -120,6 +145,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
     const int i = i0 + lane;
     if (i >= nx) return;
     const int cx = lane + 2;
+    const T ht_inv = (T)a.ht_inv, af1 = (T)a.af1, af2 = (T)a.af2, factor = (T)a.factor;
 
 #define F(di, dj) sF[cy + (dj)][cx + (di)]
 #define H(di, dj) sH[cy + (dj)][cx + (di)]
@@ -130,60 +156,60 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
         const int cy = rr + 2;
         const size_t o = (size_t)j * nx + i;
 
-        double base;
+        T base;
         if (MODE == MODE_RESIDUAL) {
-            const double fder = a.f[o] - a.fh[o];            // Fder = (0 + F) - Fh
-            base = 0.0 + a.ht_inv * fder;                    // Y = 0; Y += Fder/dt
+            const T fder = a.f[o] - a.fh[o];                 // Fder = (0 + F) - Fh
+            base = T(0) + ht_inv * fder;                     // Y = 0; Y += Fder/dt
         } else if (MODE == MODE_JACOBIAN) {
-            base = 0.0 + a.ht_inv * F(0, 0);                 // Y = 0; Y += u/dt
+            base = T(0) + ht_inv * F(0, 0);                  // Y = 0; Y += u/dt
         } else {
             base = a.y[o];
         }
-        double out = base;
+        T out = base;
         if (j >= 2 && j < nv - 2) {
             if (KIND == VV_BRACKET_J1 || KIND == VV_BRACKET_J4) {
                 // PoissonBracket.pyx:100-111 (J1 / second order)
-                const double jpp = (F(1, 0) - F(-1, 0)) * (H(0, 1) - H(0, -1))
-                                 - (F(0, 1) - F(0, -1)) * (H(1, 0) - H(-1, 0));
-                const double jpc = F(1, 0) * (H(1, 1) - H(1, -1))
-                                 - F(-1, 0) * (H(-1, 1) - H(-1, -1))
-                                 - F(0, 1) * (H(1, 1) - H(-1, 1))
-                                 + F(0, -1) * (H(1, -1) - H(-1, -1));
-                const double jcp = F(1, 1) * (H(0, 1) - H(1, 0))
-                                 - F(-1, -1) * (H(-1, 0) - H(0, -1))
-                                 - F(-1, 1) * (H(0, 1) - H(-1, 0))
-                                 + F(1, -1) * (H(1, 0) - H(0, -1));
+                const T jpp = (F(1, 0) - F(-1, 0)) * (H(0, 1) - H(0, -1))
+                            - (F(0, 1) - F(0, -1)) * (H(1, 0) - H(-1, 0));
+                const T jpc = F(1, 0) * (H(1, 1) - H(1, -1))
+                            - F(-1, 0) * (H(-1, 1) - H(-1, -1))
+                            - F(0, 1) * (H(1, 1) - H(-1, 1))
+                            + F(0, -1) * (H(1, -1) - H(-1, -1));
+                const T jcp = F(1, 1) * (H(0, 1) - H(1, 0))
+                            - F(-1, -1) * (H(-1, 0) - H(0, -1))
+                            - F(-1, 1) * (H(0, 1) - H(-1, 0))
+                            + F(1, -1) * (H(1, 0) - H(0, -1));
                 if (KIND == VV_BRACKET_J1) {
-                    out = base + a.factor * (jpp + jpc + jcp) * a.af1;
+                    out = base + factor * (jpp + jpc + jcp) * af1;
                 } else {
                     // PoissonBracket.pyx:373-389 (J4 = 2 J1 - J2)
-                    const double jcc2 = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
-                                      - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
-                    const double jpc2 = F(2, 0) * (H(1, 1) - H(1, -1))
-                                      - F(-2, 0) * (H(-1, 1) - H(-1, -1))
-                                      - F(0, 2) * (H(1, 1) - H(-1, 1))
-                                      + F(0, -2) * (H(1, -1) - H(-1, -1));
-                    const double jcp2 = F(1, 1) * (H(0, 2) - H(2, 0))
-                                      - F(-1, -1) * (H(-2, 0) - H(0, -2))
-                                      - F(-1, 1) * (H(0, 2) - H(-2, 0))
-                                      + F(1, -1) * (H(2, 0) - H(0, -2));
-                    const double r1 = (jpp + jpc + jcp) * a.af1;
-                    const double r2 = (jcc2 + jpc2 + jcp2) * a.af2;
-                    out = base + a.factor * (2. * r1 - r2);
-                }
-            } else {
-                // PoissonBracket.pyx:221-234 (J2)
-                const double jcc = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
+                    const T jcc2 = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
                                  - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
-                const double jpc = F(2, 0) * (H(1, 1) - H(1, -1))
+                    const T jpc2 = F(2, 0) * (H(1, 1) - H(1, -1))
                                  - F(-2, 0) * (H(-1, 1) - H(-1, -1))
                                  - F(0, 2) * (H(1, 1) - H(-1, 1))
                                  + F(0, -2) * (H(1, -1) - H(-1, -1));
-                const double jcp = F(1, 1) * (H(0, 2) - H(2, 0))
+                    const T jcp2 = F(1, 1) * (H(0, 2) - H(2, 0))
                                  - F(-1, -1) * (H(-2, 0) - H(0, -2))
                                  - F(-1, 1) * (H(0, 2) - H(-2, 0))
                                  + F(1, -1) * (H(2, 0) - H(0, -2));
-                out = base + a.factor * (jcc + jpc + jcp) * a.af2;
+                    const T r1 = (jpp + jpc + jcp) * af1;
+                    const T r2 = (jcc2 + jpc2 + jcp2) * af2;
+                    out = base + factor * (T(2) * r1 - r2);
+                }
+            } else {
+                // PoissonBracket.pyx:221-234 (J2)
+                const T jcc = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
+                            - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
+                const T jpc = F(2, 0) * (H(1, 1) - H(1, -1))
+                            - F(-2, 0) * (H(-1, 1) - H(-1, -1))
+                            - F(0, 2) * (H(1, 1) - H(-1, 1))
+                            + F(0, -2) * (H(1, -1) - H(-1, -1));
+                const T jcp = F(1, 1) * (H(0, 2) - H(2, 0))
+                            - F(-1, -1) * (H(-2, 0) - H(0, -2))
+                            - F(-1, 1) * (H(0, 2) - H(-2, 0))
+                            + F(1, -1) * (H(2, 0) - H(0, -2));
+                out = base + factor * (jcc + jpc + jcp) * af2;
             }
         }
         a.y[o] = out;
@@ -192,29 +218,30 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs a)
 #undef H
 }
 
-template <int MODE>
-hipError_t launch_mode(vv_ctx *c, BrArgs a)
+template <int MODE, typename T>
+hipError_t launch_mode(vv_ctx *c, BrArgs<T> a)
 {
     const int ntx = (c->p.nx + TX - 1) / TX;
     const int nty = (c->p.nv + TY - 1) / TY;
     dim3 grid(ntx * nty), block(NT);
     switch (c->p.bracket) {
     case VV_BRACKET_J1:
-        hipLaunchKernelGGL((bracket_kernel<VV_BRACKET_J1, MODE>), grid, block, 0, c->stream, a);
+        launch_timed(c, bracket_kernel<VV_BRACKET_J1, MODE, T>, grid, block, 0, a);
         break;
     case VV_BRACKET_J2:
-        hipLaunchKernelGGL((bracket_kernel<VV_BRACKET_J2, MODE>), grid, block, 0, c->stream, a);
+        launch_timed(c, bracket_kernel<VV_BRACKET_J2, MODE, T>, grid, block, 0, a);
         break;
     default:
-        hipLaunchKernelGGL((bracket_kernel<VV_BRACKET_J4, MODE>), grid, block, 0, c->stream, a);
+        launch_timed(c, bracket_kernel<VV_BRACKET_J4, MODE, T>, grid, block, 0, a);
         break;
     }
     return hipGetLastError();
 }
 
-BrArgs base_args(vv_ctx *c)
+template <typename T>
+BrArgs<T> base_args(vv_ctx *c)
 {
-    BrArgs a{};
+    BrArgs<T> a{};
     a.nx = c->p.nx;
     a.nv = c->p.nv;
     a.h0 = c->h0;
@@ -232,7 +259,7 @@ BrArgs base_args(vv_ctx *c)
 
 hipError_t launch_residual(vv_ctx *c, const double *f, const double *fh, double *y)
 {
-    BrArgs a = base_args(c);
+    BrArgs<double> a = base_args<double>(c);
     a.f = f;
     a.fh = fh;
     a.y = y;
@@ -242,7 +269,7 @@ hipError_t launch_residual(vv_ctx *c, const double *f, const double *fh, double 
 
 hipError_t launch_jacobian(vv_ctx *c, const double *u, const double *uscale, double *y)
 {
-    BrArgs a = base_args(c);
+    BrArgs<double> a = base_args<double>(c);
     a.f = u;
     a.uscale = uscale;
     a.y = y;
@@ -253,11 +280,22 @@ hipError_t launch_jacobian(vv_ctx *c, const double *u, const double *uscale, dou
 
 hipError_t launch_bracket_add(vv_ctx *c, const double *x, double *y, double factor)
 {
-    BrArgs a = base_args(c);
+    BrArgs<double> a = base_args<double>(c);
     a.f = x;
     a.y = y;
     a.factor = factor;
     return launch_mode<MODE_ACCUMULATE>(c, a);
+}
+
+hipError_t launch_jacobian32(vv_ctx *c, const float *u, const double *uscale, float *y)
+{
+    BrArgs<float> a = base_args<float>(c);
+    a.f = u;
+    a.uscale = uscale;
+    a.y = y;
+    a.factor = 0.5;
+    a.done = uscale != nullptr ? &c->gs->st.done : nullptr;
+    return launch_mode<MODE_JACOBIAN>(c, a);
 }
 
 }  // namespace vv

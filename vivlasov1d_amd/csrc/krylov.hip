@@ -114,7 +114,6 @@ __global__ void gmres_init_kernel(GmresState *g, double rtol, double atol, doubl
     g->max_it = max_it;
     g->max_k = max_k;
     g->first = 1;
-    for (int i = 0; i < 8; ++i) g->counter[i] = 0u;
 }
 
 // Per-block partial sums of x*y over len elements (x == y: sum of squares).
@@ -171,33 +170,67 @@ __global__ __launch_bounds__(NT) void reduce_finish_kernel(const double *partial
     }
 }
 
+// 16-byte vector of the basis storage type: double2 (FP64 cycle) or float4
+// (FP32 cycle of the mixed-precision solver).  Arithmetic is FP64 either way.
+template <typename S> struct V16;
+template <> struct V16<double> {
+    using type = double2;
+    static constexpr int L = 2;
+};
+template <> struct V16<float> {
+    using type = float4;
+    static constexpr int L = 4;
+};
+__device__ __forceinline__ double vget(const double2 &v, int l) { return l == 0 ? v.x : v.y; }
+__device__ __forceinline__ double vget(const float4 &v, int l)
+{
+    return (double)(l == 0 ? v.x : (l == 1 ? v.y : (l == 2 ? v.z : v.w)));
+}
+__device__ __forceinline__ void vset(double2 &v, int l, double x)
+{
+    if (l == 0) v.x = x;
+    else v.y = x;
+}
+__device__ __forceinline__ void vset(float4 &v, int l, double x)
+{
+    const float f = (float)x;
+    if (l == 0) v.x = f;
+    else if (l == 1) v.y = f;
+    else if (l == 2) v.z = f;
+    else v.w = f;
+}
+
+template <typename S>
 struct CgsArgs {
-    const double *basis;
+    const S *basis;
     int64_t N;
     double *part;
     GmresState *g;
 };
 
-template <int EPT, int G>
-__device__ __forceinline__ void load_group(const double *basis, int64_t N, int v0, int64_t e0, int64_t n2,
-                                           double2 (&x)[G][EPT])
+template <int EPT, int G, typename S>
+__device__ __forceinline__ void load_group(const S *basis, int64_t N, int v0, int64_t e0, int64_t nvv,
+                                           typename V16<S>::type (&x)[G][EPT])
 {
+    using V = typename V16<S>::type;
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
-        const double2 *V = reinterpret_cast<const double2 *>(basis + (size_t)(v0 + gg) * N);
+        const V *Vp = reinterpret_cast<const V *>(basis + (size_t)(v0 + gg) * N);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int64_t e = e0 + (int64_t)q * NT;
-            x[gg][q] = (e < n2) ? V[e] : make_double2(0.0, 0.0);
+            x[gg][q] = (e < nvv) ? Vp[e] : V{};
         }
     }
 }
 
 // VecMDot of w = V_{col+1} against v_0..v_col, written as HH(:, col) and the
 // VecMAXPY coefficients -HH(:, col).
-template <int EPT, int PV>
-__global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs a, int col)
+template <int EPT, int PV, typename S>
+__global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 {
+    using V = typename V16<S>::type;
+    constexpr int L = V16<S>::L;
     GmresState *g = a.g;
     if (g->st.done) return;
     __shared__ double red[NW][PV];
@@ -205,42 +238,42 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs a, int col)
     const int nvec = col + 1;
     if (threadIdx.x < nvec) s_scale[threadIdx.x] = g->scale[threadIdx.x];
     __syncthreads();
-    const int64_t n2 = a.N >> 1;
+    const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
-    const double2 *W = reinterpret_cast<const double2 *>(a.basis + (size_t)nvec * a.N);
-    double2 w[EPT];
+    const V *W = reinterpret_cast<const V *>(a.basis + (size_t)nvec * a.N);
+    V w[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int64_t e = e0 + (int64_t)q * NT;
-        w[q] = (e < n2) ? W[e] : make_double2(0.0, 0.0);
+        w[q] = (e < nvv) ? W[e] : V{};
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int v = 0;
     for (; v + 2 <= nvec; v += 2) {
-        double2 x[2][EPT];
-        load_group<EPT, 2>(a.basis, a.N, v, e0, n2, x);
+        V x[2][EPT];
+        load_group<EPT, 2, S>(a.basis, a.N, v, e0, nvv, x);
 #pragma unroll
         for (int gg = 0; gg < 2; ++gg) {
             const double sv = s_scale[v + gg];
             double acc = 0.0;
 #pragma unroll
             for (int q = 0; q < EPT; ++q) {
-                acc += w[q].x * (x[gg][q].x * sv);
-                acc += w[q].y * (x[gg][q].y * sv);
+#pragma unroll
+                for (int l = 0; l < L; ++l) acc += vget(w[q], l) * (vget(x[gg][q], l) * sv);
             }
             acc = wave_sum(acc);
             if (lane == 0) red[wave][v + gg] = acc;
         }
     }
     if (v < nvec) {
-        double2 x[1][EPT];
-        load_group<EPT, 1>(a.basis, a.N, v, e0, n2, x);
+        V x[1][EPT];
+        load_group<EPT, 1, S>(a.basis, a.N, v, e0, nvv, x);
         const double sv = s_scale[v];
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
-            acc += w[q].x * (x[0][q].x * sv);
-            acc += w[q].y * (x[0][q].y * sv);
+#pragma unroll
+            for (int l = 0; l < L; ++l) acc += vget(w[q], l) * (vget(x[0][q], l) * sv);
         }
         acc = wave_sum(acc);
         if (lane == 0) red[wave][v] = acc;
@@ -271,10 +304,12 @@ __global__ __launch_bounds__(NT) void mdot_finish_kernel(const double *part, int
 }
 
 // CGS update  V_{it+1} <- w - sum_j h_j v_j  (VecMAXPY with lhh = -h) and
-// per-block partial sums of its squared norm.
-template <int EPT>
-__global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs a, int it)
+// per-block partial sums of its squared norm (of the stored values).
+template <int EPT, typename S>
+__global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 {
+    using V = typename V16<S>::type;
+    constexpr int L = V16<S>::L;
     GmresState *g = a.g;
     if (g->st.done) return;
     __shared__ double red[NW + 1];
@@ -285,47 +320,55 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs a, int it)
         s_scale[threadIdx.x] = g->scale[threadIdx.x];
     }
     __syncthreads();
-    const int64_t n2 = a.N >> 1;
+    const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
-    double2 *W = reinterpret_cast<double2 *>(const_cast<double *>(a.basis) + (size_t)nvec * a.N);
-    double2 w[EPT];
+    V *W = reinterpret_cast<V *>(const_cast<S *>(a.basis) + (size_t)nvec * a.N);
+    double w[EPT][L];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int64_t e = e0 + (int64_t)q * NT;
-        w[q] = (e < n2) ? W[e] : make_double2(0.0, 0.0);
+        const V t = (e < nvv) ? W[e] : V{};
+#pragma unroll
+        for (int l = 0; l < L; ++l) w[q][l] = vget(t, l);
     }
     int j = 0;
     for (; j + 2 <= nvec; j += 2) {
-        double2 x[2][EPT];
-        load_group<EPT, 2>(a.basis, a.N, j, e0, n2, x);
+        V x[2][EPT];
+        load_group<EPT, 2, S>(a.basis, a.N, j, e0, nvv, x);
 #pragma unroll
         for (int gg = 0; gg < 2; ++gg) {
             const double sj = s_scale[j + gg], cj = s_coef[j + gg];
 #pragma unroll
             for (int q = 0; q < EPT; ++q) {
-                w[q].x = w[q].x + cj * (x[gg][q].x * sj);
-                w[q].y = w[q].y + cj * (x[gg][q].y * sj);
+#pragma unroll
+                for (int l = 0; l < L; ++l) w[q][l] = w[q][l] + cj * (vget(x[gg][q], l) * sj);
             }
         }
     }
     if (j < nvec) {
-        double2 x[1][EPT];
-        load_group<EPT, 1>(a.basis, a.N, j, e0, n2, x);
+        V x[1][EPT];
+        load_group<EPT, 1, S>(a.basis, a.N, j, e0, nvv, x);
         const double sj = s_scale[j], cj = s_coef[j];
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
-            w[q].x = w[q].x + cj * (x[0][q].x * sj);
-            w[q].y = w[q].y + cj * (x[0][q].y * sj);
+#pragma unroll
+            for (int l = 0; l < L; ++l) w[q][l] = w[q][l] + cj * (vget(x[0][q], l) * sj);
         }
     }
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int64_t e = e0 + (int64_t)q * NT;
-        if (e < n2) {
-            W[e] = w[q];
-            acc += w[q].x * w[q].x;
-            acc += w[q].y * w[q].y;
+        if (e < nvv) {
+            V o;
+#pragma unroll
+            for (int l = 0; l < L; ++l) vset(o, l, w[q][l]);
+            W[e] = o;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                const double r = vget(o, l);
+                acc += r * r;
+            }
         }
     }
     const double s = block_sum_all(acc, red);
@@ -419,10 +462,13 @@ __global__ __launch_bounds__(NT) void backsolve_kernel(GmresState *g)
     for (int j = 0; j <= k; ++j) g->coef[j] = y[j];
 }
 
-// x += sum_j coef_j v_j  (VEC_TEMP = 0 + sum_j nrs_j v_j; x = x + VEC_TEMP)
-template <int EPT>
-__global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsArgs a)
+// x += sum_j coef_j v_j  (VEC_TEMP = 0 + sum_j nrs_j v_j; x = x + VEC_TEMP);
+// x is FP64 for either basis type.
+template <int EPT, typename S>
+__global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsArgs<S> a)
 {
+    using V = typename V16<S>::type;
+    constexpr int L = V16<S>::L;
     const GmresState *g = a.g;
     const int n = g->st.it;
     if (n <= 0) return;
@@ -432,46 +478,67 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
         s_scale[threadIdx.x] = g->scale[threadIdx.x];
     }
     __syncthreads();
-    const int64_t n2 = a.N >> 1;
+    const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
-    double2 t[EPT];
+    double t[EPT][L];
 #pragma unroll
-    for (int q = 0; q < EPT; ++q) t[q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < EPT; ++q)
+#pragma unroll
+        for (int l = 0; l < L; ++l) t[q][l] = 0.0;
     int j = 0;
     for (; j + 2 <= n; j += 2) {
-        double2 v[2][EPT];
-        load_group<EPT, 2>(a.basis, a.N, j, e0, n2, v);
+        V v[2][EPT];
+        load_group<EPT, 2, S>(a.basis, a.N, j, e0, nvv, v);
 #pragma unroll
         for (int gg = 0; gg < 2; ++gg) {
             const double sj = s_scale[j + gg], cj = s_coef[j + gg];
 #pragma unroll
             for (int q = 0; q < EPT; ++q) {
-                t[q].x = t[q].x + cj * (v[gg][q].x * sj);
-                t[q].y = t[q].y + cj * (v[gg][q].y * sj);
+#pragma unroll
+                for (int l = 0; l < L; ++l) t[q][l] = t[q][l] + cj * (vget(v[gg][q], l) * sj);
             }
         }
     }
     if (j < n) {
-        double2 v[1][EPT];
-        load_group<EPT, 1>(a.basis, a.N, j, e0, n2, v);
+        V v[1][EPT];
+        load_group<EPT, 1, S>(a.basis, a.N, j, e0, nvv, v);
         const double sj = s_scale[j], cj = s_coef[j];
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
-            t[q].x = t[q].x + cj * (v[0][q].x * sj);
-            t[q].y = t[q].y + cj * (v[0][q].y * sj);
+#pragma unroll
+            for (int l = 0; l < L; ++l) t[q][l] = t[q][l] + cj * (vget(v[0][q], l) * sj);
         }
     }
     double2 *X = reinterpret_cast<double2 *>(x);
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int64_t e = e0 + (int64_t)q * NT;
-        if (e < n2) {
-            double2 xv = X[e];
-            xv.x = xv.x + t[q].x;
-            xv.y = xv.y + t[q].y;
-            X[e] = xv;
+        if (e < nvv) {
+#pragma unroll
+            for (int h = 0; h < L / 2; ++h) {
+                double2 xv = X[e * (L / 2) + h];
+                xv.x = xv.x + t[q][2 * h];
+                xv.y = xv.y + t[q][2 * h + 1];
+                X[e * (L / 2) + h] = xv;
+            }
         }
     }
+}
+
+// Cycle start of the FP32 GMRES cycle: V_0 = (float) r and per-block partial
+// sums of the squares of the rounded values.
+__global__ __launch_bounds__(NT) void cvt_partial_kernel(const double *__restrict__ r, float *__restrict__ v0,
+                                                         int64_t len, double *partial)
+{
+    __shared__ double red[NW + 1];
+    double acc = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT) {
+        const float f = (float)r[e];
+        v0[e] = f;
+        acc += (double)f * (double)f;
+    }
+    const double s = block_sum_all(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // y = y + alpha*x (VecAXPY).  mode 1: only when the GMRES reason is positive
@@ -502,23 +569,24 @@ int grid_for(int64_t len)
     return (int)(want < kReduceBlocks ? (want > 0 ? want : 1) : kReduceBlocks);
 }
 
-// elements per thread of the CGS kernels: the largest of 8,4,2,1 that still
-// gives >= 256 workgroups
-int cgs_ept(int64_t N)
+// 16-byte vectors per thread of the CGS kernels: the largest of 8,4,2,1 that
+// still gives >= 256 workgroups (nvv: 16-byte vectors per basis vector)
+int cgs_ept_v(int64_t nvv)
 {
-    const int64_t n2 = N >> 1;
     for (int ept = 8; ept > 1; ept >>= 1)
-        if ((n2 + (int64_t)NT * ept - 1) / ((int64_t)NT * ept) >= 256) return ept;
+        if ((nvv + (int64_t)NT * ept - 1) / ((int64_t)NT * ept) >= 256) return ept;
     return 1;
+}
+
+int cgs_blocks_v(int64_t nvv)
+{
+    const int ept = cgs_ept_v(nvv);
+    return (int)((nvv + (int64_t)NT * ept - 1) / ((int64_t)NT * ept));
 }
 
 }  // namespace
 
-int cgs_blocks(int64_t N)
-{
-    const int ept = cgs_ept(N);
-    return (int)(((N >> 1) + (int64_t)NT * ept - 1) / ((int64_t)NT * ept));
-}
+int cgs_blocks(int64_t N) { return cgs_blocks_v(N >> 1); }
 
 hipError_t launch_gmres_init(vv_ctx *c, double rtol, double atol, double dtol, int max_it)
 {
@@ -538,39 +606,39 @@ hipError_t launch_norm_start(vv_ctx *c)
 
 namespace {
 
-template <int E>
-void cgs_launch(vv_ctx *c, int which, int arg, double *x)
+template <int E, typename S>
+void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 {
-    CgsArgs a{c->basis, c->N, c->partial, c->gs};
-    const dim3 grid(cgs_blocks(c->N)), block(NT);
+    CgsArgs<S> a{basis, c->N, c->partial, c->gs};
+    const dim3 grid(cgs_blocks_v(c->N / V16<S>::L)), block(NT);
     if (which == 0) {
-        if (c->p.restart < 32) hipLaunchKernelGGL((mdot_kernel<E, 32>), grid, block, 0, c->stream, a, arg);
-        else hipLaunchKernelGGL((mdot_kernel<E, 64>), grid, block, 0, c->stream, a, arg);
+        if (c->p.restart < 32) launch_timed(c, mdot_kernel<E, 32, S>, grid, block, 0, a, arg);
+        else launch_timed(c, mdot_kernel<E, 64, S>, grid, block, 0, a, arg);
     } else if (which == 1) {
-        hipLaunchKernelGGL((maxpy_norm_kernel<E>), grid, block, 0, c->stream, a, arg);
+        launch_timed(c, maxpy_norm_kernel<E, S>, grid, block, 0, a, arg);
     } else {
-        hipLaunchKernelGGL((update_kernel<E>), grid, block, 0, c->stream, x, a);
+        launch_timed(c, update_kernel<E, S>, grid, block, 0, x, a);
     }
 }
 
-hipError_t cgs_dispatch(vv_ctx *c, int which, int arg, double *x)
+template <typename S>
+hipError_t cgs_dispatch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 {
-    switch (cgs_ept(c->N)) {
-    case 8: cgs_launch<8>(c, which, arg, x); break;
-    case 4: cgs_launch<4>(c, which, arg, x); break;
-    case 2: cgs_launch<2>(c, which, arg, x); break;
-    default: cgs_launch<1>(c, which, arg, x); break;
+    switch (cgs_ept_v(c->N / V16<S>::L)) {
+    case 8: cgs_launch<8>(c, basis, which, arg, x); break;
+    case 4: cgs_launch<4>(c, basis, which, arg, x); break;
+    case 2: cgs_launch<2>(c, basis, which, arg, x); break;
+    default: cgs_launch<1>(c, basis, which, arg, x); break;
     }
     return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_mdot(vv_ctx *c, int col)
+template <typename S>
+hipError_t mdot_t(vv_ctx *c, const S *basis, int col)
 {
-    hipError_t e = cgs_dispatch(c, 0, col, nullptr);
+    hipError_t e = cgs_dispatch(c, basis, 0, col, nullptr);
     if (e != hipSuccess) return e;
-    const int nb = cgs_blocks(c->N);
+    const int nb = cgs_blocks_v(c->N / V16<S>::L);
     if (c->p.restart < 32)
         hipLaunchKernelGGL((mdot_finish_kernel<32>), dim3(1), dim3(NT), 0, c->stream, c->partial, nb, col, c->gs);
     else
@@ -578,23 +646,44 @@ hipError_t launch_mdot(vv_ctx *c, int col)
     return hipGetLastError();
 }
 
-hipError_t launch_maxpy_norm(vv_ctx *c, int it)
+template <typename S>
+hipError_t maxpy_norm_t(vv_ctx *c, const S *basis, int it)
 {
-    hipError_t e = cgs_dispatch(c, 1, it, nullptr);
+    hipError_t e = cgs_dispatch(c, basis, 1, it, nullptr);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(gmres_update_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, cgs_blocks(c->N), it, c->gs);
+    hipLaunchKernelGGL(gmres_update_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial,
+                       cgs_blocks_v(c->N / V16<S>::L), it, c->gs);
     return hipGetLastError();
 }
 
-hipError_t launch_build_solution(vv_ctx *c, double *x)
+template <typename S>
+hipError_t build_solution_t(vv_ctx *c, const S *basis, double *x)
 {
     hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(NT), 0, c->stream, c->gs);
-    return cgs_dispatch(c, 2, 0, x);
+    return cgs_dispatch(c, basis, 2, 0, x);
 }
 
-hipError_t launch_residual_restart(vv_ctx *c, const double *b)
+}  // namespace
+
+hipError_t launch_mdot(vv_ctx *c, int col) { return mdot_t<double>(c, c->basis, col); }
+hipError_t launch_maxpy_norm(vv_ctx *c, int it) { return maxpy_norm_t<double>(c, c->basis, it); }
+hipError_t launch_build_solution(vv_ctx *c, double *x) { return build_solution_t<double>(c, c->basis, x); }
+hipError_t launch_mdot32(vv_ctx *c, int col) { return mdot_t<float>(c, c->basis32, col); }
+hipError_t launch_maxpy_norm32(vv_ctx *c, int it) { return maxpy_norm_t<float>(c, c->basis32, it); }
+hipError_t launch_build_solution32(vv_ctx *c, double *x) { return build_solution_t<float>(c, c->basis32, x); }
+
+hipError_t launch_start32(vv_ctx *c, const double *r)
 {
-    hipLaunchKernelGGL(rsub_kernel, dim3(grid_for(c->N)), dim3(NT), 0, c->stream, b, c->basis, c->N);
+    const int nb = grid_for(c->N);
+    hipLaunchKernelGGL(cvt_partial_kernel, dim3(nb), dim3(NT), 0, c->stream, r, c->basis32, c->N, c->partial);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, nb, c->gs,
+                       (double *)nullptr, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_restart(vv_ctx *c, const double *b, double *y)
+{
+    hipLaunchKernelGGL(rsub_kernel, dim3(grid_for(c->N)), dim3(NT), 0, c->stream, b, y, c->N);
     return hipGetLastError();
 }
 

@@ -15,6 +15,11 @@
 //  irfft_kernel   spectrum -> real rows, backward-normalised like
 //                 numpy.fft.irfft / pyFFTW (imaginary parts of the k = 0 and
 //                 k = nx/2 coefficients are ignored, as in a c2r transform).
+//
+// Every kernel is a template on the real type: double for the FP64 path,
+// float for the FP32 preconditioner inside the mixed-precision GMRES cycle
+// (complex64 spectrum, float twiddles; T_k's coefficients are formed in FP64
+// and rounded).
 #include "vv1d_internal.h"
 
 namespace vv {
@@ -23,43 +28,60 @@ namespace {
 constexpr int NT = 256;
 constexpr int MAXE = 4096;   // complex elements of one FFT workgroup (64 KiB of LDS)
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+// complex arithmetic on double2 / float2
+template <typename C> __device__ __forceinline__ C cmk(typename C::value_type re, typename C::value_type im)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    C r;
+    r.x = re;
+    r.y = im;
+    return r;
 }
-__device__ __forceinline__ double2 cneg(double2 a) { return make_double2(-a.x, -a.y); }
-__device__ __forceinline__ double2 cdiv(double2 z, double2 w)
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
-    const double inv = 1.0 / (w.x * w.x + w.y * w.y);
-    return make_double2((z.x * w.x + z.y * w.y) * inv, (z.y * w.x - z.x * w.y) * inv);
+    return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ double2 crecip(double2 w)
+template <typename C> __device__ __forceinline__ C cneg(C a) { return cmk<C>(-a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C cdiv(C z, C w)
 {
-    const double inv = 1.0 / (w.x * w.x + w.y * w.y);
-    return make_double2(w.x * inv, -w.y * inv);
+    using R = typename C::value_type;
+    const R inv = R(1) / (w.x * w.x + w.y * w.y);
+    return cmk<C>((z.x * w.x + z.y * w.y) * inv, (z.y * w.x - z.x * w.y) * inv);
+}
+template <typename C> __device__ __forceinline__ C crecip(C w)
+{
+    using R = typename C::value_type;
+    const R inv = R(1) / (w.x * w.x + w.y * w.y);
+    return cmk<C>(w.x * inv, -w.y * inv);
 }
 
-template <bool INV>
-__device__ __forceinline__ double2 twiddle(const double2 *__restrict__ tw, int m)
+// complex type of a real type
+template <typename T> struct CxOf;
+template <> struct CxOf<double> { using type = double2; };
+template <> struct CxOf<float> { using type = float2; };
+template <typename T> using cx_t = typename CxOf<T>::type;
+
+template <bool INV, typename C>
+__device__ __forceinline__ C twiddle(const C *__restrict__ tw, int m)
 {
-    const double2 t = tw[m];
-    return INV ? make_double2(t.x, -t.y) : t;
+    const C t = tw[m];
+    return INV ? cmk<C>(t.x, -t.y) : t;
 }
 
 // Batched in-LDS Stockham FFT of `nb` transforms of length n = 2^logn stored
 // back to back in buf.  Forward uses exp(-2 pi i m/n), inverse the conjugate
 // (unnormalised).  Each stage loads every butterfly of the thread into
-// registers, synchronises, then stores, so one buffer suffices.
-template <bool INV>
-__device__ void lds_fft(double2 *buf, int n, int logn, int nb, const double2 *__restrict__ tw)
+// registers, synchronises, then stores, so one buffer suffices.  C is
+// double2 (FP64 path) or float2 (FP32 preconditioner of the mixed solver).
+template <bool INV, typename C>
+__device__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw)
 {
     int Ns = 1;
     const int nq = n >> 2;
     for (int s = 0; s < (logn >> 1); ++s) {
         const int total = nb * nq;
-        double2 v[MAXE / 4 / NT][4];
+        C v[MAXE / 4 / NT][4];
         int dst[MAXE / 4 / NT];
 #pragma unroll
         for (int q = 0; q < MAXE / 4 / NT; ++q) {
@@ -69,19 +91,19 @@ __device__ void lds_fft(double2 *buf, int n, int logn, int nb, const double2 *__
                 const int b = t / nq;
                 const int j = t - b * nq;
                 const int k = j & (Ns - 1);
-                const double2 *src = buf + b * n;
+                const C *src = buf + b * n;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) v[q][r] = src[j + r * nq];
                 const int st = (n / (Ns * 4)) * k;
                 v[q][1] = cmul(v[q][1], twiddle<INV>(tw, st));
                 v[q][2] = cmul(v[q][2], twiddle<INV>(tw, 2 * st));
                 v[q][3] = cmul(v[q][3], twiddle<INV>(tw, 3 * st));
-                const double2 a0 = cadd(v[q][0], v[q][2]);
-                const double2 a1 = csub(v[q][0], v[q][2]);
-                const double2 a2 = cadd(v[q][1], v[q][3]);
-                const double2 d = csub(v[q][1], v[q][3]);
+                const C a0 = cadd(v[q][0], v[q][2]);
+                const C a1 = csub(v[q][0], v[q][2]);
+                const C a2 = cadd(v[q][1], v[q][3]);
+                const C d = csub(v[q][1], v[q][3]);
                 // forward: (v1 - v3) * (-i); inverse: (v1 - v3) * (+i)
-                const double2 a3 = INV ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);
+                const C a3 = INV ? cmk<C>(-d.y, d.x) : cmk<C>(d.y, -d.x);
                 v[q][0] = cadd(a0, a2);
                 v[q][1] = cadd(a1, a3);
                 v[q][2] = csub(a0, a2);
@@ -103,7 +125,7 @@ __device__ void lds_fft(double2 *buf, int n, int logn, int nb, const double2 *__
     if (logn & 1) {
         const int nh = n >> 1;
         const int total = nb * nh;
-        double2 v[MAXE / 2 / NT][2];
+        C v[MAXE / 2 / NT][2];
         int dst[MAXE / 2 / NT];
 #pragma unroll
         for (int q = 0; q < MAXE / 2 / NT; ++q) {
@@ -113,9 +135,9 @@ __device__ void lds_fft(double2 *buf, int n, int logn, int nb, const double2 *__
                 const int b = t / nh;
                 const int j = t - b * nh;
                 const int k = j & (Ns - 1);
-                const double2 *src = buf + b * n;
-                const double2 x0 = src[j];
-                const double2 x1 = cmul(src[j + nh], twiddle<INV>(tw, (n / (Ns * 2)) * k));
+                const C *src = buf + b * n;
+                const C x0 = src[j];
+                const C x1 = cmul(src[j + nh], twiddle<INV>(tw, (n / (Ns * 2)) * k));
                 v[q][0] = cadd(x0, x1);
                 v[q][1] = csub(x0, x1);
                 dst[q] = b * n + (j - k) * 2 + k;
@@ -133,28 +155,31 @@ __device__ void lds_fft(double2 *buf, int n, int logn, int nb, const double2 *__
     }
 }
 
+template <typename T>
 struct FftArgs {
     int nx, logn, nv, rows;     // rows: v-rows per workgroup (even)
-    const double2 *tw;
+    const cx_t<T> *tw;
     const int32_t *done;
 };
 
-__global__ __launch_bounds__(NT) void rfft_kernel(FftArgs a, const double *__restrict__ x,
-                                                  double2 *__restrict__ spec)
+template <typename T>
+__global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
 {
+    using C = cx_t<T>;
     if (a.done != nullptr && *a.done) return;
-    extern __shared__ double2 buf[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows;
     const int j0 = blockIdx.x * R;
     // every global load of the thread is issued before the LDS stores
     constexpr int PER = 2 * MAXE / NT;   // R * nx <= 2 * MAXE real inputs
-    double val[PER];
+    T val[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
         const int r = idx >> a.logn;
         const int i = idx & (nx - 1);
-        val[q] = (idx < R * nx && j0 + r < a.nv) ? x[(size_t)(j0 + r) * nx + i] : 0.0;
+        val[q] = (idx < R * nx && j0 + r < a.nv) ? x[(size_t)(j0 + r) * nx + i] : T(0);
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -162,32 +187,35 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs a, const double *__res
         if (idx < R * nx) {
             const int r = idx >> a.logn;
             const int i = idx & (nx - 1);
-            double *slot = reinterpret_cast<double *>(&buf[(r >> 1) * nx + i]);
+            T *slot = reinterpret_cast<T *>(&buf[(r >> 1) * nx + i]);
             slot[r & 1] = val[q];
         }
     }
     __syncthreads();
     lds_fft<false>(buf, nx, a.logn, R >> 1, a.tw);
     const int nk = (nx >> 1) + 1;
+    const T half = T(0.5);
     for (int idx = threadIdx.x; idx < nk * R; idx += NT) {
         const int k = idx / R;
         const int r = idx - k * R;
         if (j0 + r >= a.nv) continue;
-        const double2 *z = buf + (r >> 1) * nx;
-        const double2 zk = z[k];
-        const double2 zm = z[(nx - k) & (nx - 1)];
-        double2 out;
-        if ((r & 1) == 0) out = make_double2((zk.x + zm.x) * 0.5, (zk.y - zm.y) * 0.5);
-        else out = make_double2((zk.y + zm.y) * 0.5, (zm.x - zk.x) * 0.5);
+        const C *z = buf + (r >> 1) * nx;
+        const C zk = z[k];
+        const C zm = z[(nx - k) & (nx - 1)];
+        C out;
+        if ((r & 1) == 0) out = cmk<C>((zk.x + zm.x) * half, (zk.y - zm.y) * half);
+        else out = cmk<C>((zk.y + zm.y) * half, (zm.x - zk.x) * half);
         spec[(size_t)k * a.nv + j0 + r] = out;
     }
 }
 
-__global__ __launch_bounds__(NT) void irfft_kernel(FftArgs a, const double2 *__restrict__ spec,
-                                                   double *__restrict__ y)
+template <typename T>
+__global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
 {
+    using C = cx_t<T>;
     if (a.done != nullptr && *a.done) return;
-    extern __shared__ double2 buf[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows, nh = nx >> 1;
     const int j0 = blockIdx.x * R;
     const int npair = R >> 1;
@@ -195,14 +223,15 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs a, const double2 *__r
     // every global load of the thread is issued before the LDS stores.
     const int total = (nh + 1) * npair;
     constexpr int PER = (MAXE / 2 + 64 + NT - 1) / NT;   // (nx/2 + 1) R/2 <= MAXE/2 + 64
-    double2 va[PER], vb[PER];
+    C va[PER], vb[PER];
+    const C zero = cmk<C>(T(0), T(0));
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
         const int k = idx / npair;
         const int ja = j0 + 2 * (idx - k * npair);
-        va[q] = (idx < total && ja < a.nv) ? spec[(size_t)k * a.nv + ja] : make_double2(0.0, 0.0);
-        vb[q] = (idx < total && ja + 1 < a.nv) ? spec[(size_t)k * a.nv + ja + 1] : make_double2(0.0, 0.0);
+        va[q] = (idx < total && ja < a.nv) ? spec[(size_t)k * a.nv + ja] : zero;
+        vb[q] = (idx < total && ja + 1 < a.nv) ? spec[(size_t)k * a.nv + ja + 1] : zero;
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -210,24 +239,24 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs a, const double2 *__r
         if (idx < total) {
             const int k = idx / npair;
             const int p = idx - k * npair;
-            double2 A = va[q], B = vb[q];
+            C A = va[q], B = vb[q];
             if (k == 0 || k == nh) {
-                A.y = 0.0;
-                B.y = 0.0;
+                A.y = T(0);
+                B.y = T(0);
             }
-            double2 *z = buf + p * nx;
-            z[k] = make_double2(A.x - B.y, A.y + B.x);
-            if (k != 0 && k != nh) z[nx - k] = make_double2(A.x + B.y, B.x - A.y);
+            C *z = buf + p * nx;
+            z[k] = cmk<C>(A.x - B.y, A.y + B.x);
+            if (k != 0 && k != nh) z[nx - k] = cmk<C>(A.x + B.y, B.x - A.y);
         }
     }
     __syncthreads();
     lds_fft<true>(buf, nx, a.logn, npair, a.tw);
-    const double inv_n = 1.0 / (double)nx;   // exact: nx is a power of two
+    const T inv_n = T(1) / (T)nx;   // exact: nx is a power of two
     for (int idx = threadIdx.x; idx < R * nx; idx += NT) {
         const int r = idx >> a.logn;
         const int i = idx & (nx - 1);
         if (j0 + r < a.nv) {
-            const double2 z = buf[(r >> 1) * nx + i];
+            const C z = buf[(r >> 1) * nx + i];
             y[(size_t)(j0 + r) * nx + i] = ((r & 1) ? z.y : z.x) * inv_n;
         }
     }
@@ -244,41 +273,47 @@ struct TriArgs {
 
 // Row j of T_k (KineticFast.pyx:248-272): sub a_j = diagm[j] (j >= 1),
 // diagonal b_j = diag[j], super c_j = diagp[j] (j <= nv-2); rows past nv are
-// identity rows that pad the system to P*M unknowns.
-__device__ __forceinline__ void tri_row(const TriArgs &a, double2 e, int j, double2 &aj, double2 &bj, double2 &cj)
+// identity rows that pad the system to P*M unknowns.  The coefficients are
+// formed in FP64 (as the reference forms its band matrix) and rounded to C.
+template <typename C>
+__device__ __forceinline__ void tri_row(const TriArgs &a, double2 e, int j, C &aj, C &bj, C &cj)
 {
+    using R = typename C::value_type;
     if (j < a.nv) {
         const double g = a.gm[j];
-        aj = (j > 0) ? make_double2(((e.x * 0.5) * g) * a.af, ((e.y * 0.5) * g) * a.af) : make_double2(0.0, 0.0);
+        aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * g) * a.af), (R)(((e.y * 0.5) * g) * a.af)) : cmk<C>(R(0), R(0));
         const double g0 = a.g0[j];
-        bj = make_double2(((e.x * 1.0) * g0) * a.af + a.ht_inv, ((e.y * 1.0) * g0) * a.af);
+        bj = cmk<C>((R)(((e.x * 1.0) * g0) * a.af + a.ht_inv), (R)(((e.y * 1.0) * g0) * a.af));
         const double gp = a.gp[j];
-        cj = (j < a.nv - 1) ? make_double2(((e.x * 0.5) * gp) * a.af, ((e.y * 0.5) * gp) * a.af)
-                            : make_double2(0.0, 0.0);
+        cj = (j < a.nv - 1) ? cmk<C>((R)(((e.x * 0.5) * gp) * a.af), (R)(((e.y * 0.5) * gp) * a.af))
+                            : cmk<C>(R(0), R(0));
     } else {
-        aj = make_double2(0.0, 0.0);
-        bj = make_double2(1.0, 0.0);
-        cj = make_double2(0.0, 0.0);
+        aj = cmk<C>(R(0), R(0));
+        bj = cmk<C>(R(1), R(0));
+        cj = cmk<C>(R(0), R(0));
     }
 }
 
-template <int M>
-__global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restrict__ spec)
+template <int M, typename T>
+__global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restrict__ spec)
 {
+    using C = cx_t<T>;
     if (a.done != nullptr && *a.done) return;
-    extern __shared__ double2 sd[];          // P*(M+1): right-hand sides / solution, chunk c at c*(M+1)
-    __shared__ double2 sa[NT], sb[NT], sc[NT], se[NT];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *sd = reinterpret_cast<C *>(lds_raw);   // P*(M+1): right-hand sides / solution, chunk c at c*(M+1)
+    __shared__ C sa[NT], sb[NT], sc[NT], se[NT];
     const int k = blockIdx.x;
     const double2 e = a.eigen[k];
     const int P = a.P, n = P * M;
-    double2 *S = spec + (size_t)k * a.nv;
+    const C zero = cmk<C>(T(0), T(0));
+    C *S = spec + (size_t)k * a.nv;
     {
         // n = P*M <= NT*M: every load of the thread in flight before the LDS stores
-        double2 v[M];
+        C v[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) {
             const int j = threadIdx.x + q * NT;
-            v[q] = (j < a.nv) ? S[j] : make_double2(0.0, 0.0);
+            v[q] = (j < a.nv) ? S[j] : zero;
         }
 #pragma unroll
         for (int q = 0; q < M; ++q) {
@@ -289,14 +324,14 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restr
     __syncthreads();
 
     const int c = threadIdx.x;
-    double2 Am[M], Cm[M], Dm[M];
-    double2 ae, be, ce;          // coefficients of the chunk's last row
+    C Am[M], Cm[M], Dm[M];
+    C ae, be, ce;          // coefficients of the chunk's last row
     if (c < P) {
         const int s = c * M, ls = c * (M + 1);
         // forward sweep: y_j + c'_j y_{j+1} = d'_j - a'_j y_{s-1}
-        double2 aj, bj, cj;
+        C aj, bj, cj;
         tri_row(a, e, s, aj, bj, cj);
-        double2 r = crecip(bj);
+        C r = crecip(bj);
         Cm[0] = cmul(cj, r);
         Dm[0] = cmul(sd[ls], r);
         Am[0] = cmul(aj, r);
@@ -330,10 +365,9 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restr
     __syncthreads();
     // reduced equation of chunk c in the chunk-last unknowns E:
     //   alpha E_{c-1} + beta E_c + gamma E_{c+1} = delta
-    double2 al = make_double2(0.0, 0.0), bt = make_double2(1.0, 0.0), gm = make_double2(0.0, 0.0),
-            dl = make_double2(0.0, 0.0);
+    C al = zero, bt = cmk<C>(T(1), T(0)), gm = zero, dl = zero;
     if (c < P) {
-        const double2 dE = sd[c * (M + 1) + M - 1];
+        const C dE = sd[c * (M + 1) + M - 1];
         al = cneg(cmul(ae, Am[M - 2]));
         bt = csub(be, cmul(ae, Cm[M - 2]));
         dl = csub(dE, cmul(ae, Dm[M - 2]));
@@ -354,16 +388,16 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restr
         }
         __syncthreads();
         if (c < P) {
-            double2 nal = make_double2(0.0, 0.0), ngm = make_double2(0.0, 0.0);
-            double2 nbt = bt, ndl = dl;
+            C nal = zero, ngm = zero;
+            C nbt = bt, ndl = dl;
             if (c - st >= 0) {
-                const double2 k1 = cdiv(al, sb[c - st]);
+                const C k1 = cdiv(al, sb[c - st]);
                 nal = cneg(cmul(sa[c - st], k1));
                 nbt = csub(nbt, cmul(sc[c - st], k1));
                 ndl = csub(ndl, cmul(se[c - st], k1));
             }
             if (c + st < P) {
-                const double2 k2 = cdiv(gm, sb[c + st]);
+                const C k2 = cdiv(gm, sb[c + st]);
                 ngm = cneg(cmul(sc[c + st], k2));
                 nbt = csub(nbt, cmul(sa[c + st], k2));
                 ndl = csub(ndl, cmul(se[c + st], k2));
@@ -379,8 +413,8 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, double2 *__restr
     __syncthreads();
     if (c < P) {
         const int ls = c * (M + 1);
-        const double2 Eprev = (c > 0) ? se[c - 1] : make_double2(0.0, 0.0);
-        const double2 Eown = se[c];
+        const C Eprev = (c > 0) ? se[c - 1] : zero;
+        const C Eown = se[c];
 #pragma unroll
         for (int q = 0; q < M - 1; ++q)
             sd[ls + q] = csub(csub(Dm[q], cmul(Am[q], Eprev)), cmul(Cm[q], Eown));
@@ -398,29 +432,23 @@ int fft_rows(int nx, int nv)
     return R;
 }
 
-}  // namespace
-
-int fft_blocks(vv_ctx *c)
+template <typename T>
+hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const int32_t *done)
 {
-    const int R = fft_rows(c->p.nx, c->p.nv);
-    return (c->p.nv + R - 1) / R;
-}
-
-hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done)
-{
-    FftArgs a{c->p.nx, c->logn, c->p.nv, fft_rows(c->p.nx, c->p.nv), c->twiddle, done};
+    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c->p.nx, c->p.nv), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
-    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(double2);
-    hipLaunchKernelGGL(rfft_kernel, dim3(nb), dim3(NT), lds, c->stream, a, x, spec);
+    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
+    launch_timed(c, rfft_kernel<T>, dim3(nb), dim3(NT), (uint32_t)lds, a, x, spec);
     return hipGetLastError();
 }
 
-hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done)
+template <typename T>
+hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, const int32_t *done)
 {
-    FftArgs a{c->p.nx, c->logn, c->p.nv, fft_rows(c->p.nx, c->p.nv), c->twiddle, done};
+    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c->p.nx, c->p.nv), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
-    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(double2);
-    hipLaunchKernelGGL(irfft_kernel, dim3(nb), dim3(NT), lds, c->stream, a, spec, y);
+    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
+    launch_timed(c, irfft_kernel<T>, dim3(nb), dim3(NT), (uint32_t)lds, a, spec, y);
     return hipGetLastError();
 }
 
@@ -438,20 +466,59 @@ void tridiag_shape(int nv, int *P, int *M)
     }
 }
 
-hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done)
+template <typename T>
+hipError_t tridiag_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
 {
     int P, M;
     tridiag_shape(c->p.nv, &P, &M);
     TriArgs a{c->p.nv, P, c->eigen, c->gm, c->g0, c->gp, c->afpc, c->ht_inv, done};
-    const size_t lds = (size_t)P * (M + 1) * sizeof(double2);
+    const size_t lds = (size_t)P * (M + 1) * sizeof(cx_t<T>);
     dim3 grid(c->nmodes), block(NT);
     switch (M) {
-    case 2: hipLaunchKernelGGL(tridiag_kernel<2>, grid, block, lds, c->stream, a, spec); break;
-    case 4: hipLaunchKernelGGL(tridiag_kernel<4>, grid, block, lds, c->stream, a, spec); break;
-    case 8: hipLaunchKernelGGL(tridiag_kernel<8>, grid, block, lds, c->stream, a, spec); break;
-    default: hipLaunchKernelGGL(tridiag_kernel<16>, grid, block, lds, c->stream, a, spec); break;
+    case 2: launch_timed(c, tridiag_kernel<2, T>, grid, block, (uint32_t)lds, a, spec); break;
+    case 4: launch_timed(c, tridiag_kernel<4, T>, grid, block, (uint32_t)lds, a, spec); break;
+    case 8: launch_timed(c, tridiag_kernel<8, T>, grid, block, (uint32_t)lds, a, spec); break;
+    default: launch_timed(c, tridiag_kernel<16, T>, grid, block, (uint32_t)lds, a, spec); break;
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+int fft_blocks(vv_ctx *c)
+{
+    const int R = fft_rows(c->p.nx, c->p.nv);
+    return (c->p.nv + R - 1) / R;
+}
+
+hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done)
+{
+    return rfft_t<double>(c, c->twiddle, x, spec, done);
+}
+
+hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done)
+{
+    return irfft_t<double>(c, c->twiddle, spec, y, done);
+}
+
+hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done)
+{
+    return tridiag_t<double>(c, spec, done);
+}
+
+hipError_t launch_rfft32(vv_ctx *c, const float *x, float2 *spec, const int32_t *done)
+{
+    return rfft_t<float>(c, c->twiddle32, x, spec, done);
+}
+
+hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t *done)
+{
+    return irfft_t<float>(c, c->twiddle32, spec, y, done);
+}
+
+hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done)
+{
+    return tridiag_t<float>(c, spec, done);
 }
 
 }  // namespace vv

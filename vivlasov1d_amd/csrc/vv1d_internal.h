@@ -1,6 +1,7 @@
 // Internal declarations shared by the HIP translation units of libvv1d.so.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -26,9 +27,9 @@ struct GmresStatus {
     double rnorm0;     // norm of the right-hand side
 };
 
-// Device-resident GMRES state.  Written only by one thread of the "last
-// block" of a reduction or by one-thread control kernels; read by every
-// Krylov kernel.
+// Device-resident GMRES state.  Written only by the one-workgroup control
+// kernels (cycle start, mdot finish, Hessenberg update, back substitution);
+// read by every Krylov kernel.
 struct GmresState {
     GmresStatus st;
     double hh[(kMaxRestart + 1) * kMaxRestart];  // rotated Hessenberg, column it at hh[it*(kMaxRestart+1)]
@@ -42,11 +43,7 @@ struct GmresState {
     int32_t max_k;
     int32_t first;                   // first cycle: sets rnorm0 and ttol
     int32_t pad;
-    uint32_t counter[8];             // last-block tickets, one per kernel kind
 };
-
-// counters of GmresState::counter
-enum { CNT_MDOT = 0, CNT_MAXPY = 1, CNT_NORM = 2, CNT_DOT = 3, CNT_INV = 4, CNT_START = 5 };
 
 }  // namespace vv
 
@@ -64,6 +61,7 @@ struct vv_ctx {
     double2 *eigen;     // nmodes used (k = 0..nx/2)
     double *gm, *g0, *gp;   // per-row real factors of the bands
     double2 *twiddle;   // exp(-2 pi i m / nx), m = 0..nx-1
+    float2 *twiddle32;  // the same, rounded to float (FP32 preconditioner)
     double2 *spec;      // nmodes * nv complex spectrum, mode-major
     // Poisson
     double *pinv;       // first row of the mean-free circulant inverse (nx)
@@ -73,6 +71,9 @@ struct vv_ctx {
     double *B;          // SNES right-hand side P^-1 F
     double *D;          // Newton update
     double *basis;      // (restart+1) * N
+    // mixed precision: the FP32 cycle reuses the FP64 basis allocation
+    float *basis32;     // (restart+1) * N floats at the start of basis
+    double *R;          // FP64 true residual of the refinement loop, after basis32
     // reductions
     double *partial;    // kReduceBlocks * (kMaxRestart+2)
     double *dscalar;    // device scalars
@@ -83,6 +84,7 @@ struct vv_ctx {
     // profiling (per-kernel HIP event pairs)
     int prof;           // bit k set: time kernel kind k
     void *prof_state;
+    hipEvent_t prof_a, prof_b;   // events of the next main-kernel launch (null: none)
     std::string err;
 };
 
@@ -93,11 +95,15 @@ namespace vv {
 hipError_t launch_residual(vv_ctx *c, const double *f, const double *fh, double *y);
 hipError_t launch_jacobian(vv_ctx *c, const double *u, const double *uscale, double *y);
 hipError_t launch_bracket_add(vv_ctx *c, const double *x, double *y, double factor);
+hipError_t launch_jacobian32(vv_ctx *c, const float *u, const double *uscale, float *y);
 // spectral.hip
 int fft_blocks(vv_ctx *c);
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done);
 hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done);
 hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done);
+hipError_t launch_rfft32(vv_ctx *c, const float *x, float2 *spec, const int32_t *done);
+hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done);
+hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t *done);
 // krylov.hip
 hipError_t launch_gmres_init(vv_ctx *c, double rtol, double atol, double dtol, int max_it);
 hipError_t launch_norm_start(vv_ctx *c);                  // cycle start on basis[0]
@@ -105,7 +111,11 @@ int cgs_blocks(int64_t N);
 hipError_t launch_mdot(vv_ctx *c, int col);               // CGS dots of basis[col+1] vs 0..col
 hipError_t launch_maxpy_norm(vv_ctx *c, int it);          // CGS update of basis[it+1] + Givens
 hipError_t launch_build_solution(vv_ctx *c, double *x);  // x += V y
-hipError_t launch_residual_restart(vv_ctx *c, const double *b);  // basis[0] = b - basis[0]
+hipError_t launch_start32(vv_ctx *c, const double *r);    // FP32 cycle start: V_0 = (float) r
+hipError_t launch_mdot32(vv_ctx *c, int col);
+hipError_t launch_maxpy_norm32(vv_ctx *c, int it);
+hipError_t launch_build_solution32(vv_ctx *c, double *x);  // x (FP64) += V y (FP32 basis)
+hipError_t launch_residual_restart(vv_ctx *c, const double *b, double *y);  // y = b - y
 hipError_t launch_norm2(vv_ctx *c, const double *x, int64_t n, double *out_dev);
 hipError_t launch_dot(vv_ctx *c, const double *x, const double *y, int64_t n, double *out_dev);
 hipError_t launch_axpy(vv_ctx *c, double alpha, const double *x, double *y, int64_t n);
@@ -120,6 +130,20 @@ hipError_t launch_poisson_mult(vv_ctx *c, const double *x, const double *n, doub
 hipError_t launch_pot_to_ham(vv_ctx *c, const double *phi, double *h1);
 hipError_t launch_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1,
                              const double *h2, const double *phi, double *out_dev);
+
+// Launch a timed kernel: when the profiler armed c->prof_a / c->prof_b, the
+// events are attached to this kernel's own dispatch packet (start / end
+// timestamps of the kernel, as a kernel trace reports them).
+template <typename K, typename... Args>
+inline void launch_timed(vv_ctx *c, K kernel, dim3 grid, dim3 block, uint32_t shmem, Args... args)
+{
+    if (c->prof_a != nullptr) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, c->stream, c->prof_a, c->prof_b, 0u, args...);
+        c->prof_a = c->prof_b = nullptr;
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, args...);
+    }
+}
 
 // XCD-aware block order: spread consecutive tiles so that each of the 8 XCDs
 // works on one contiguous range of tiles (halo lines stay in that XCD's L2).

@@ -25,7 +25,8 @@ class ResidualTooLarge(RuntimeError):
 
 
 class Runscript:
-    def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False):
+    def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False, precision='double',
+                 inner_rtol=1e-5):
         if not isinstance(cfg, Config):
             cfg = Config(cfg)
         self.cfg = cfg
@@ -65,6 +66,10 @@ class Runscript:
         self.snes.setFunction(self.vlasov_solver.function_snes_mult, self.fb)
         self.snes.getKSP().setTolerances(rtol=sol['petsc_ksp_rtol'], atol=sol['petsc_ksp_atol'],
                                          max_it=sol['petsc_ksp_max_iter'])
+        # 'mixed': FP32 GMRES cycles inside FP64 iterative refinement; the
+        # residual, update, moments and Poisson solve stay FP64
+        self.snes.getKSP().setPrecision(precision, inner_rtol=inner_rtol)
+        self.precision = precision
         self.snes_atol = sol['petsc_snes_atol']
         self.snes_max_iter = sol['petsc_snes_max_iter']
         self.itime = 0
@@ -108,8 +113,10 @@ class Runscript:
             self.vlasov_solver.update_previous(self.fc)
             prev, pred = pred, self.calculate_residual()
             ksp = self.snes.getKSP()
-            rec['picard'].append(dict(gmres=ksp.getIterationNumber(), reason=ksp.getConvergedReason(),
-                                      residual=pred))
+            it_rec = dict(gmres=ksp.getIterationNumber(), reason=ksp.getConvergedReason(), residual=pred)
+            if ksp.precision == 'mixed':
+                it_rec['cycles'] = ksp.cycles
+            rec['picard'].append(it_rec)
             if self.verbose:
                 print("  Nonlinear Solver: %5i GMRES  iterations, residual = %24.16E"
                       % (ksp.getIterationNumber(), pred), flush=True)

@@ -32,11 +32,11 @@ from .core import Grid, kinetic_eigenvalues
 DOUBLE = torch.float64
 
 
-def _dev_ptr(t, numel=None, name="vector"):
+def _dev_ptr(t, numel=None, name="vector", dtype=DOUBLE):
     if not isinstance(t, torch.Tensor):
         raise TypeError("%s must be a torch tensor" % name)
-    if t.dtype != DOUBLE or not t.is_cuda or not t.is_contiguous():
-        raise ValueError("%s must be a contiguous float64 CUDA tensor" % name)
+    if t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("%s must be a contiguous %s CUDA tensor" % (name, str(dtype).replace("torch.", "")))
     if numel is not None and t.numel() != numel:
         raise ValueError("%s has %d elements, expected %d" % (name, t.numel(), numel))
     return ctypes.c_void_p(t.data_ptr())
@@ -190,13 +190,43 @@ class Context:
         self.call("vv_snes_ksponly", self.F(f), self.F(fh), ctypes.byref(kp), ctypes.byref(res))
         return res
 
+    # mixed precision ----------------------------------------------------------
+    def F32(self, t, name="field"):
+        return _dev_ptr(t, self.N, name, torch.float32)
+
+    def jacobian_apply32(self, u, y):
+        self.call("vv_jacobian_apply32", self.F32(u), self.F32(y))
+
+    def precond32(self, x, y):
+        self.call("vv_precond_apply32", self.F32(x), self.F32(y))
+
+    def ksp_solve_mixed(self, b, x, rtol, atol, dtol, max_it, inner_rtol=1e-5, max_cycles=50):
+        """Iterative refinement with FP32 GMRES cycles; returns (KspResult, cycles)."""
+        kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
+        mp = _lib.MixedParams(inner_rtol, int(max_cycles))
+        res = _lib.KspResult()
+        cyc = ctypes.c_int32()
+        self.call("vv_ksp_solve_mixed", self.F(b), self.F(x), ctypes.byref(kp), ctypes.byref(mp),
+                  ctypes.byref(res), ctypes.byref(cyc))
+        return res, cyc.value
+
+    def snes_ksponly_mixed(self, f, fh, rtol, atol, dtol, max_it, inner_rtol=1e-5, max_cycles=50):
+        kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
+        mp = _lib.MixedParams(inner_rtol, int(max_cycles))
+        res = _lib.KspResult()
+        cyc = ctypes.c_int32()
+        self.call("vv_snes_ksponly_mixed", self.F(f), self.F(fh), ctypes.byref(kp), ctypes.byref(mp),
+                  ctypes.byref(res), ctypes.byref(cyc))
+        return res, cyc.value
+
     def invariants(self, f, h0, h1, h2, phi):
         out = (ctypes.c_double * 6)()
         self.call("vv_invariants", self.F(f), self.V(h0), self.X(h1), self.X(h2), self.X(phi), out)
         keys = ("mass", "l2", "energy", "ekin", "epot", "efield")
         return dict(zip(keys, list(out)))
 
-    PROFILE_KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm")
+    PROFILE_KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
+                       "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32")
 
     def profile(self, kernels=None):
         """Time the named kernels (all when None; () switches timing off)."""
@@ -452,6 +482,21 @@ class KSP:
         self.ctx = None
         self.type = 'gmres'
         self._res = None
+        # 'double': the reference's FP64 GMRES; 'mixed': FP64 iterative
+        # refinement around FP32 GMRES cycles (same stopping test)
+        self.precision = 'double'
+        self.inner_rtol = 1e-5
+        self.max_cycles = 50
+        self.cycles = 0
+
+    def setPrecision(self, precision, inner_rtol=None, max_cycles=None):
+        if precision not in ('double', 'mixed'):
+            raise _lib.VVError("precision must be 'double' or 'mixed'")
+        self.precision = precision
+        if inner_rtol is not None:
+            self.inner_rtol = float(inner_rtol)
+        if max_cycles is not None:
+            self.max_cycles = int(max_cycles)
 
     def setType(self, t):
         if t != 'gmres':
@@ -475,7 +520,11 @@ class KSP:
         self.ctx = ctx
 
     def solve(self, b, x):
-        self._res = self.ctx.ksp_solve(b, x, self.rtol, self.atol, self.dtol, self.max_it)
+        if self.precision == 'mixed':
+            self._res, self.cycles = self.ctx.ksp_solve_mixed(b, x, self.rtol, self.atol, self.dtol, self.max_it,
+                                                              self.inner_rtol, self.max_cycles)
+        else:
+            self._res = self.ctx.ksp_solve(b, x, self.rtol, self.atol, self.dtol, self.max_it)
 
     def getIterationNumber(self):
         return self._res.its if self._res else 0
@@ -525,7 +574,11 @@ class SNES:
         if s.preconditioner is None:
             raise _lib.VVError("the native SNES path needs the tensor_kinetic preconditioner")
         k = self.ksp
-        self.last = s.ctx.snes_ksponly(x, s.Fh, k.rtol, k.atol, k.dtol, k.max_it)
+        if k.precision == 'mixed':
+            self.last, k.cycles = s.ctx.snes_ksponly_mixed(x, s.Fh, k.rtol, k.atol, k.dtol, k.max_it,
+                                                           k.inner_rtol, k.max_cycles)
+        else:
+            self.last = s.ctx.snes_ksponly(x, s.Fh, k.rtol, k.atol, k.dtol, k.max_it)
         k._res = self.last
         self.linear_its = self.last.its
 
