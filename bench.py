@@ -14,6 +14,11 @@ plain FP64 form.  The mixed-precision form (FP32 GMRES cycles + FP32
 preconditioner inside FP64 iterative refinement, FP64 residual / update /
 Poisson) is then timed the same way from the same initial state and reported
 under "mixed" with its L-infinity distance to the FP64 run after W + K steps.
+The batched form (256 independent instances with uneven grids, bucketed by
+shape into batched contexts, one captured graph replay per Newton step,
+autotuned launch shapes) is reported under "batched": aggregate
+timesteps/s over all instances next to the same instances run one after
+another through the single-instance path.
 roofline: the dominant kernel's algorithmic bytes / its HIP-event time on
 the library stream over the timed region.  cpu_baseline: the C (OpenMP)
 restatement of the same timestep from oracle/, on a bounded sample.
@@ -112,6 +117,54 @@ def timed_run(w, args, precision, kernel):
     return run, t1 - t0, run.history[nrec0:], kstats
 
 
+def batched_figures(args):
+    """Aggregate timesteps/s of the uneven-grid ensemble, batched and sequential."""
+    import numpy as np
+    import torch
+    from vivlasov1d_amd.batch import BatchRun, make_instances, run_sequential
+    specs = make_instances(args.batched_instances, seed=0)
+    t_setup = time.perf_counter()
+    br = BatchRun(specs, graph=True, autotune=True)
+    for _ in range(args.batched_warmup):
+        br.step()
+    br.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    t0 = time.perf_counter()
+    for _ in range(args.batched_steps):
+        br.step()
+    br.synchronize()
+    el = time.perf_counter() - t0
+    n = len(specs)
+    recs = [r for b in br.buckets for r in b.history[args.batched_warmup:]]
+    picard = np.concatenate([r["picard"] for r in recs])
+    gm = np.concatenate([r["gmres"] for r in recs])
+    out = {
+        "instances": n, "buckets": len(br.buckets), "steps": args.batched_steps, "warmup": args.batched_warmup,
+        "value": round(n * args.batched_steps / el, 2), "unit": "timesteps/s (all instances)",
+        "ms_per_step": round(1e3 * el / args.batched_steps, 3), "setup_s": round(t_setup, 2),
+        "points": int(sum(s["nx"] * s["nv"] for s in specs)),
+        "picard_per_step": float(picard.mean()), "gmres_per_step": float(gm.mean()),
+        "tuning": {"%dx%d" % (b.grid.nx, b.grid.nv): list(b.tuning) for b in br.buckets},
+        "graph": True,
+    }
+    if not args.no_sequential:
+        total = args.batched_warmup + args.batched_steps
+        el_seq, states = run_sequential(specs, total)
+        err = 0.0
+        for s_ in specs:
+            f, _ = br.state(s_["index"])
+            fs, _ = states[s_["index"]]
+            err = max(err, float(np.abs(f - fs).max() / np.abs(fs).max()))
+        seq_rate = n * total / el_seq
+        out["sequential"] = {"value": round(seq_rate, 2), "unit": "timesteps/s (all instances)",
+                             "steps": total, "seconds": round(el_seq, 2)}
+        out["speedup_vs_sequential"] = round(out["value"] / seq_rate, 2)
+        out["linf_f_vs_sequential"] = err
+    del br
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +179,11 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
     ap.add_argument("--mixed-kernel", default="none", help="kernel timed in the mixed run (e.g. mdot32)")
     ap.add_argument("--inner-rtol", type=float, default=1e-5, help="FP32 cycle reduction of the mixed form")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble timing")
+    ap.add_argument("--no-sequential", action="store_true", help="skip the sequential run of the ensemble")
+    ap.add_argument("--batched-instances", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--batched-warmup", type=int, default=1)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -187,6 +245,10 @@ def main():
             "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
                             "GB/s": round(v["gbs"], 1), "ms": round(v["ms"], 3)} for k, v in mk.items()},
         }
+    batched = None
+    if not args.no_batched:
+        batched = batched_figures(args)
+
     line = {
         "metric": "timesteps/s at 1024x2048 J4 (plain FP64)" if args.workload.startswith("twostream_1024")
         else "timesteps/s %s" % args.workload,
@@ -208,6 +270,7 @@ def main():
         "cpu_baseline": cpu,
         "invariants": {k: inv[k] for k in ("mass", "l2", "energy")},
         "mixed": mixed,
+        "batched": batched,
     }
     if cpu and "value" in cpu and cpu["value"]:
         line["gpu_over_cpu"] = round(value / cpu["value"], 2)

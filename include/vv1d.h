@@ -89,6 +89,47 @@ typedef struct {
  * Poisson solver (PETScPoissonSolver.pyx:23-45).  stream: a hipStream_t, or
  * NULL for the null stream. */
 int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out);
+
+/* Batched context (no counterpart in the reference, which runs one instance
+ * per process): `batch` independent instances that share nx, nv, the v-grid
+ * (hv, v), dt, charge and the operator choice, but each with its own x-period
+ * (hx[b], host array of `batch`; p->hx is ignored).  Every kernel runs all
+ * instances in one launch (instance = second grid dimension).  Layout: a
+ * field argument holds the instances back to back (instance b at b*nx*nv, a
+ * C (batch, nv, nx) array); x-arrays likewise (b*nx); h0 and the v-arrays are
+ * shared.  Functions that return host scalars return one per instance, in
+ * instance order (vv_function_norm, vv_moments' pnorm: batch values;
+ * vv_invariants: 6*batch; vv_ksp_solve / vv_snes_ksponly: batch results).
+ * vv_norm2 / vv_dot / vv_axpy act on plain vectors of len.  The
+ * mixed-precision solve takes one instance. */
+int vv_ctx_create_batched(const vv_params *p, int32_t batch, const double *hx,
+                          void *stream, vv_ctx **out);
+int32_t vv_batch_size(const vv_ctx *ctx);
+
+/* Instances that take part in the next linear solves (host array of `batch`
+ * flags; NULL: all).  A masked instance's solve ends at once with reason 0 and
+ * vv_snes_ksponly leaves its f unchanged (a finished Picard loop). */
+int vv_set_active(vv_ctx *ctx, const int32_t *active);
+
+/* Graph replay of vv_snes_ksponly: the residual, P^-1, GMRES set-up and the
+ * first `restart` Arnoldi steps are captured once into a HIP graph (per f/fh
+ * pair and tolerances) and replayed as one launch per Newton step; Arnoldi
+ * steps past an instance's convergence return at once.  Needs a non-null
+ * stream.  Restart cycles, if any instance needs one, run as ordinary
+ * launches. */
+int vv_set_graph(vv_ctx *ctx, int enable);
+
+/* Launch-shape tuning (no counterpart in the reference): fft_rows = v-rows
+ * per FFT workgroup (even, fft_rows * nx <= 8192), cgs_ept = 16-byte vectors
+ * per thread of the Gram-Schmidt kernels (1, 2, 4, 8); 0 restores the
+ * built-in choice.  vv_autotune times the candidates on the context's own
+ * buffers (the FP64 preconditioner for fft_rows, mdot + maxpy at a
+ * mid-cycle basis for cgs_ept), keeps the fastest and returns them; it
+ * overwrites the context's work buffers and GMRES state. */
+int vv_set_tuning(vv_ctx *ctx, int32_t fft_rows, int32_t cgs_ept);
+int vv_get_tuning(vv_ctx *ctx, int32_t *fft_rows, int32_t *cgs_ept);
+int vv_autotune(vv_ctx *ctx, int32_t *fft_rows, int32_t *cgs_ept);
+
 int vv_ctx_destroy(vv_ctx *ctx);
 const char *vv_last_error(const vv_ctx *ctx);
 int vv_set_stream(vv_ctx *ctx, void *stream);
@@ -223,6 +264,34 @@ int vv_snes_ksponly_mixed(vv_ctx *ctx, double *f, const double *fh, const vv_ksp
  * preconditioner. */
 int vv_jacobian_apply32(vv_ctx *ctx, const float *u, float *y);
 int vv_precond_apply32(vv_ctx *ctx, const float *x, float *y);
+
+/* Linearisation and adjoint of the implicit timestep (no counterpart in the
+ * reference, which has no derivatives).  With
+ *   G(f1, f0) = (f1 - f0)/dt + R {(f0 + f1)/2, H},  H = h0 + h1[f1]/2 + h1[f0]/2,
+ * R = keep rows 2..nv-3, h1[f] the h1 of vv_moments (linear in f) and
+ * H_ave of vv_set_hamiltonian standing for H, about a converged step
+ * (f0, f1) with fbar = (f0 + f1)/2:
+ *   A = dG/df1 = I/dt + 1/2 R {., H} + 1/2 R {fbar, h1[.]},
+ *   A0 = dG/df0 = -I/dt + (the same two terms).
+ * The transposes use the skew-symmetry of the Arakawa stencils:
+ *   A^T u = u/dt - 1/2 {R^T u, H} + 1/2 h1[{R^T u, fbar}] on all rows.
+ * fbar = NULL drops the field-coupling term (A is then the Picard
+ * Jacobian J).  vv_linear_apply: y = A u, A0 u, A^T u or A0^T u.
+ * vv_precond_apply_t: y = P^-T x (mode blocks T_k^H).
+ * vv_adjoint_solve: GMRES on P^-T A^T mu = P^-T lam (same KSP options).
+ * vv_adjoint_step: mu from vv_adjoint_solve, then lam0 = -A0^T mu, the
+ * sensitivity with respect to f0 of a function whose sensitivity with
+ * respect to f1 is lam1. */
+#define VV_LIN_A   0
+#define VV_LIN_A0  1
+#define VV_LIN_AT  2
+#define VV_LIN_A0T 3
+int vv_linear_apply(vv_ctx *ctx, const double *u, const double *fbar, double *y, int32_t which);
+int vv_precond_apply_t(vv_ctx *ctx, const double *x, double *y);
+int vv_adjoint_solve(vv_ctx *ctx, const double *lam, const double *fbar, double *mu,
+                     const vv_ksp_params *kp, vv_ksp_result *res);
+int vv_adjoint_step(vv_ctx *ctx, const double *lam1, const double *fbar, double *lam0,
+                    double *mu, const vv_ksp_params *kp, vv_ksp_result *res);
 
 /* Invariants of core/distribution_function.pyx:130-199,
  * core/hamiltonian.pyx:67-100 and core/potential.pyx:48-59:

@@ -1,0 +1,205 @@
+"""Batched contexts on the GPU.
+
+A batched context runs the same kernels as a single-instance one, with the
+instance as the second grid dimension and per-instance partial sums, so every
+result must be bit-identical to the single-instance context of that
+instance's grid (which the parity tests pin to the oracle).  Bars:
+  * operators, moments, invariants, GMRES and SNES steps: bit-identical per
+    instance, including restarts (restart = 3) and masked instances;
+  * graph replay of the SNES step: bit-identical to the eager path on every
+    replay, with eager work and synchronisations between the replays;
+  * the bucketed multi-instance driver: per-instance states after two
+    timesteps bit-identical to running each instance alone;
+  * autotuned launch shapes: a valid choice, results within 1e-12 (the
+    Gram-Schmidt summation tree changes with the shape).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NX, NV = 32, 64
+LS = [4 * math.pi, 2 * math.pi / 0.3, 10 * math.pi]
+
+
+def _ctx(cuda, L, restart=30, hx=None):
+    from vivlasov1d_amd.core import Grid
+    from vivlasov1d_amd.solvers import Context
+    import torch
+    g = Grid(NX, NV, L, 8.0, ht=0.05)
+    s = torch.cuda.current_stream().cuda_stream
+    return g, Context(g, bracket='arakawaj4', charge=-1.0, restart=restart, hx=hx, stream=s)
+
+
+def _fields(rng, B):
+    f = rng.random((B, NV, NX))
+    f[:, :2] = 0
+    f[:, -2:] = 0
+    fh = f + 1e-3 * rng.standard_normal((B, NV, NX))
+    h1 = [0.05 * rng.standard_normal((B, NX)) for _ in range(4)]
+    return f, fh, h1
+
+
+def _setup(cuda, restart=30):
+    import torch
+    from vivlasov1d_amd.core import kinetic_hamiltonian
+    rng = np.random.default_rng(3)
+    B = len(LS)
+    gb, cb = _ctx(cuda, LS[0], restart, hx=[L / NX for L in LS])
+    singles = [_ctx(cuda, L, restart) for L in LS]
+    f, fh, h1 = _fields(rng, B)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    h0 = T(kinetic_hamiltonian(gb))
+    cb.set_hamiltonian(h0, *(T(h) for h in h1))
+    for b, (_, cs) in enumerate(singles):
+        cs.set_hamiltonian(h0, *(T(h[b]) for h in h1))
+    return cb, singles, f, fh, h1, T, h0, rng
+
+
+def test_batched_operators_bitwise(cuda):
+    import torch
+    cb, singles, f, fh, h1, T, h0, rng = _setup(cuda)
+    B = len(LS)
+    u = rng.standard_normal((B, NV, NX))
+    yb = cb.new_field()
+    ys = torch.zeros((NV, NX), dtype=torch.float64, device=cuda)
+    for name in ("residual", "jacobian", "precond", "function", "jacobian_pc"):
+        if name == "residual":
+            cb.residual(T(f), T(fh), yb)
+        elif name == "jacobian":
+            cb.jacobian_apply(T(u), yb)
+        elif name == "precond":
+            cb.precond(T(u), yb)
+        elif name == "function":
+            cb.function(T(f), T(fh), yb)
+        else:
+            cb.jacobian(T(u), yb)
+        for b, (_, cs) in enumerate(singles):
+            if name == "residual":
+                cs.residual(T(f[b]), T(fh[b]), ys)
+            elif name == "jacobian":
+                cs.jacobian_apply(T(u[b]), ys)
+            elif name == "precond":
+                cs.precond(T(u[b]), ys)
+            elif name == "function":
+                cs.function(T(f[b]), T(fh[b]), ys)
+            else:
+                cs.jacobian(T(u[b]), ys)
+            assert torch.equal(yb[b], ys), (name, b)
+    nb = cb.function_norm(T(f), T(fh), yb)
+    for b, (_, cs) in enumerate(singles):
+        assert nb[b] == cs.function_norm(T(f[b]), T(fh[b]), ys)
+    # moments / Poisson / invariants
+    n, phi, h1o = cb.new_x(), cb.new_x(), cb.new_x()
+    pb = cb.moments(T(f), n, phi, h1o)
+    invb = cb.invariants(T(f), h0, h1o, T(h1[2]), phi)
+    for b, (_, cs) in enumerate(singles):
+        ns, ps, hs = cs.new_x(), cs.new_x(), cs.new_x()
+        p = cs.moments(T(f[b]), ns, ps, hs)
+        assert p == pb[b]
+        assert torch.equal(ns, n[b]) and torch.equal(ps, phi[b]) and torch.equal(hs, h1o[b])
+        inv = cs.invariants(T(f[b]), h0, hs, T(h1[2][b]), ps)
+        assert inv == invb[b]
+
+
+@pytest.mark.parametrize("restart", [30, 3])
+def test_batched_gmres_and_snes_bitwise(restart, cuda):
+    import torch
+    print("restart", restart, flush=True)
+    cb, singles, f, fh, h1, T, h0, rng = _setup(cuda, restart)
+    B = len(LS)
+    b_ = cb.new_field()
+    cb.function(T(f), T(fh), b_)
+    xb = cb.new_field()
+    rb = cb.ksp_solve(b_, xb, 1e-10, 1e-50, 1e5, 1000)
+    xs = torch.zeros((NV, NX), dtype=torch.float64, device=cuda)
+    for b, (_, cs) in enumerate(singles):
+        r = cs.ksp_solve(b_[b].contiguous(), xs, 1e-10, 1e-50, 1e5, 1000)
+        print(b, r.its, rb[b].its, flush=True)
+        assert (r.its, r.reason, r.rnorm, r.rnorm0) == (rb[b].its, rb[b].reason, rb[b].rnorm, rb[b].rnorm0)
+        assert torch.equal(xs, xb[b])
+    # SNES step with instance 1 masked out
+    fb = T(f)
+    cb.set_active([True, False, True])
+    rs = cb.snes_ksponly(fb, T(fh), 1e-10, 1e-50, 1e5, 1000)
+    cb.set_active(None)
+    assert torch.equal(fb[1], T(f[1])) and rs[1].its == 0 and rs[1].reason == 0
+    for b in (0, 2):
+        fs = T(f[b])
+        r = singles[b][1].snes_ksponly(fs, T(fh[b]), 1e-10, 1e-50, 1e5, 1000)
+        assert r.its == rs[b].its and r.reason == rs[b].reason
+        assert torch.equal(fs, fb[b])
+
+
+def test_graph_replay_matches_eager(cuda):
+    """Two batched contexts on one stream, one eager and one replaying the
+    captured SNES step; three steps, with eager work on the graph context and
+    host synchronisation between the replays."""
+    import torch
+    from vivlasov1d_amd.core import Grid, kinetic_hamiltonian
+    from vivlasov1d_amd.solvers import Context
+    rng = np.random.default_rng(5)
+    f, fh, h1 = _fields(rng, len(LS))
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        g = Grid(NX, NV, LS[0], 8.0, ht=0.05)
+        hx = [L / NX for L in LS]
+        ce = Context(g, charge=-1.0, hx=hx, stream=stream.cuda_stream)
+        cg = Context(g, charge=-1.0, hx=hx, stream=stream.cuda_stream)
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+        h0 = T(kinetic_hamiltonian(g))
+        for c in (ce, cg):
+            c.set_hamiltonian(h0, *(T(h) for h in h1))
+        cg.set_graph(True)
+        fe, fg, fhd = T(f), T(f), T(fh)
+        work = cg.new_field()
+        for rep in range(3):
+            re = ce.snes_ksponly(fe, fhd, 1e-10, 1e-50, 1e5, 1000)
+            rg = cg.snes_ksponly(fg, fhd, 1e-10, 1e-50, 1e5, 1000)   # captured at rep 0, replayed after
+            stream.synchronize()
+            print(rep, [r.its for r in re], [r.its for r in rg], flush=True)
+            assert [(r.its, r.reason, r.rnorm) for r in re] == [(r.its, r.reason, r.rnorm) for r in rg]
+            assert torch.equal(fe, fg)
+            # eager work on the graph context between the replays
+            nrm = cg.function_norm(fg, fhd, work)
+            assert np.array_equal(nrm, ce.function_norm(fe, fhd, ce.new_field()))
+
+
+def test_bucketed_driver_matches_single_runs(cuda):
+    from vivlasov1d_amd.batch import BatchRun, make_instances, run_sequential
+    specs = make_instances(64, seed=0)
+    # a few instances of two shapes, kept small
+    pick = [s for s in specs if (s['nx'], s['nv']) in ((64, 128), (128, 256))][:5]
+    assert len({(s['nx'], s['nv']) for s in pick}) == 2
+    br = BatchRun(pick, graph=True, autotune=False)
+    for _ in range(2):
+        br.step()
+    br.synchronize()
+    _, states = run_sequential(pick, 2)
+    for s in pick:
+        f, phi = br.state(s['index'])
+        fs, phis = states[s['index']]
+        print(s['index'], s['nx'], s['nv'], np.abs(f - fs).max(), flush=True)
+        assert np.array_equal(f, fs) and np.array_equal(phi, phis)
+
+
+def test_autotune(cuda):
+    import torch
+    cb, singles, f, fh, h1, T, h0, rng = _setup(cuda)
+    b_ = cb.new_field()
+    cb.function(T(f), T(fh), b_)
+    x0 = cb.new_field()
+    cb.ksp_solve(b_, x0, 1e-10, 1e-50, 1e5, 1000)
+    rows, ept = cb.autotune()
+    assert rows >= 2 and rows % 2 == 0 and rows * NX <= 8192 and ept in (1, 2, 4, 8)
+    assert cb.get_tuning() == (rows, ept)
+    cb.function(T(f), T(fh), b_)          # autotune overwrote the work buffers
+    x1 = cb.new_field()
+    cb.ksp_solve(b_, x1, 1e-10, 1e-50, 1e5, 1000)
+    assert (x1 - x0).abs().max().item() <= 1e-12 * x0.abs().max().item()
+    cb.set_tuning(0, 0)
+    assert cb.get_tuning() == (0, 0)

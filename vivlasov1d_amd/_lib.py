@@ -17,6 +17,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "unsupported opti
 BRACKETS = {"arakawaj1": 1, "arakawaj2": 2, "arakawaj4": 4}
 PRECONDITIONERS = {"": 0, None: 0, "none": 0, "tensor_kinetic": 1}
 LAPLACE = {"cfd2": 2, "cfd4": 4}
+LINEAR = {"A": 0, "A0": 1, "AT": 2, "A0T": 3}
 
 KSP_REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", -3: "DIVERGED_ITS", -4: "DIVERGED_DTOL",
                -5: "DIVERGED_BREAKDOWN", -9: "DIVERGED_NANORINF", 0: "ITERATING"}
@@ -53,6 +54,14 @@ _PD = ctypes.POINTER(ctypes.c_double)
 # name -> (restype, argtypes)
 SIGNATURES = {
     "vv_ctx_create": (ctypes.c_int, [ctypes.POINTER(VVParams), _P, ctypes.POINTER(_P)]),
+    "vv_ctx_create_batched": (ctypes.c_int, [ctypes.POINTER(VVParams), ctypes.c_int32, _PD, _P,
+                                             ctypes.POINTER(_P)]),
+    "vv_batch_size": (ctypes.c_int32, [_P]),
+    "vv_set_active": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "vv_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
+    "vv_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "vv_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "vv_autotune": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "vv_ctx_destroy": (ctypes.c_int, [_P]),
     "vv_last_error": (ctypes.c_char_p, [_P]),
     "vv_set_stream": (ctypes.c_int, [_P, _P]),
@@ -83,6 +92,10 @@ SIGNATURES = {
                                              ctypes.POINTER(KspResult), ctypes.POINTER(ctypes.c_int32)]),
     "vv_jacobian_apply32": (ctypes.c_int, [_P, _P, _P]),
     "vv_precond_apply32": (ctypes.c_int, [_P, _P, _P]),
+    "vv_linear_apply": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int32]),
+    "vv_precond_apply_t": (ctypes.c_int, [_P, _P, _P]),
+    "vv_adjoint_solve": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(KspResult)]),
+    "vv_adjoint_step": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(KspResult)]),
     "vv_invariants": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _PD]),
     "vv_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "vv_profile_read": (ctypes.c_int, [_P, ctypes.c_char_p, _PD, ctypes.POINTER(ctypes.c_int64), _PD]),

@@ -1,5 +1,6 @@
 // C ABI of libvv1d.so: context set-up, the operator entry points and the
-// native GMRES / SNES(ksponly) drivers.  See include/vv1d.h.
+// native GMRES / SNES(ksponly) drivers, for one instance or a batch of
+// instances that share nx, nv, the v-grid and dt.  See include/vv1d.h.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -24,6 +25,13 @@ enum ProfKind {
 const char *kProfNames[P_NKINDS] = {"jacobian",   "residual", "rfft",      "tridiag", "irfft",
                                     "mdot",       "maxpy_norm", "jacobian32", "rfft32",  "tridiag32",
                                     "irfft32",    "mdot32",   "maxpy_norm32"};
+
+// device scalar regions (doubles): per-instance norms | Poisson residuals
+// (16 apart) | invariants (6 per instance)
+inline double *ds_norms(vv_ctx *c) { return c->dscalar; }
+inline double *ds_pres(vv_ctx *c) { return c->dscalar + c->B; }
+inline double *ds_inv(vv_ctx *c) { return c->dscalar + 17 * (size_t)c->B; }
+inline size_t ds_count(int B) { return 24 * (size_t)B + 16; }
 
 struct ProfRec {
     int kind;
@@ -60,9 +68,10 @@ struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     // The events are attached to the main kernel's own dispatch
     // (hipExtLaunchKernelGGL in launch_timed), so they time the kernel itself.
+    // Nothing is timed while a graph is being captured.
     ProfScope(vv_ctx *cc, int k, double by, int tg = -1) : c(cc), kind(k), tag(tg), bytes(by)
     {
-        if ((c->prof >> kind) & 1) {
+        if (((c->prof >> kind) & 1) && !c->capturing) {
             Prof *p = prof_of(c);
             a = prof_event(p);
             b = prof_event(p);
@@ -112,12 +121,13 @@ hipError_t dalloc(T **p, size_t count)
     return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T) > 0 ? count * sizeof(T) : 16);
 }
 
-int read_scalar(vv_ctx *c, const double *dev, double *out)
+// count doubles from device to host (through the pinned staging buffer)
+int read_scalars(vv_ctx *c, const double *dev, size_t count, double *out)
 {
-    VV_CHECK(hipMemcpyAsync(&c->host_status[1].res, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream),
-             "copy scalar");
+    VV_CHECK(hipMemcpyAsync(c->host_scalars, dev, count * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+             "copy scalars");
     VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
-    *out = c->host_status[1].res;
+    for (size_t q = 0; q < count; ++q) out[q] = c->host_scalars[q];
     return VV_OK;
 }
 
@@ -127,9 +137,10 @@ int check_ctx(vv_ctx *c)
     return VV_OK;
 }
 
-// algorithmic HBM bytes per launch (compulsory reads + writes of each kernel)
-double b_field(vv_ctx *c) { return 8.0 * (double)c->N; }
-double b_spec(vv_ctx *c) { return 16.0 * (double)c->nmodes * (double)c->p.nv; }
+// algorithmic HBM bytes per launch (compulsory reads + writes of each kernel,
+// all instances of the context)
+double b_field(vv_ctx *c) { return 8.0 * (double)c->N * c->B; }
+double b_spec(vv_ctx *c) { return 16.0 * (double)c->nmodes * (double)c->p.nv * c->B; }
 
 // Arnoldi iterations enqueued past the end of the cycle return at once: their
 // launches keep their (short) time but move no algorithmic bytes.
@@ -145,7 +156,7 @@ void prof_mark_idle(vv_ctx *c, size_t from, int done_its)
 int precond(vv_ctx *c, const double *x, double *y, const int32_t *done)
 {
     if (c->p.precond == VV_PRECOND_NONE) {
-        if (x != y) VV_CHECK(launch_copy(c, x, y, c->N), "copy");
+        if (x != y) VV_CHECK(launch_copy(c, x, y, c->N * c->B), "copy");
         return VV_OK;
     }
     VV_PLAUNCH(P_RFFT, b_field(c) + b_spec(c), -1, launch_rfft(c, x, c->spec, done), "rfft");
@@ -154,17 +165,101 @@ int precond(vv_ctx *c, const double *x, double *y, const int32_t *done)
     return VV_OK;
 }
 
-// one Arnoldi step of GMRES on P^-1 J: V_{it+1} from V_it
+// y = P^-T x: the transposed preconditioner (mode blocks T_k^H)
+int precond_t(vv_ctx *c, const double *x, double *y, const int32_t *done)
+{
+    VV_CHECK(launch_rfft(c, x, c->spec, done), "rfft");
+    VV_CHECK(launch_tridiag_t(c, c->spec, done), "tridiag (transposed)");
+    VV_CHECK(launch_irfft(c, c->spec, y, done), "irfft");
+    return VV_OK;
+}
+
+int ensure_linear(vv_ctx *c)
+{
+    if (c->T2) return VV_OK;
+    VV_CHECK(dalloc(&c->T2, (size_t)c->N * c->B), "alloc linearisation scratch");
+    VV_CHECK(dalloc(&c->xs, (size_t)4 * c->p.nx * c->B), "alloc x scratch");
+    return VV_OK;
+}
+
+// x scratch regions of B nx each: n | phi | h1
+inline double *xs_h1(vv_ctx *c) { return c->xs + 2 * (size_t)c->p.nx * c->B; }
+
+// h1[g] of a field g: the h1 output of the moments pipeline,
+// (I - mean) C (-q) (I - mean) hv sum_j g (C: the symmetric mean-free inverse
+// Laplacian), into xs_h1.  It is the derivative of h1 with respect to f, and,
+// composed with the broadcast over v-rows, its own transpose.
+int h1_of(vv_ctx *c, const double *g)
+{
+    const size_t bx = (size_t)c->p.nx * c->B;
+    VV_CHECK(launch_density_partial(c, g), "density");
+    VV_CHECK(launch_poisson(c, nullptr, c->xs, c->xs + bx, c->xs + 2 * bx, nullptr, 1), "poisson");
+    return VV_OK;
+}
+
+// The linearisation of the implicit-midpoint residual
+//   G(f1, f0) = (f1 - f0)/dt + R {(f0 + f1)/2, H(f0, f1)},
+//   H = h0 + h1[f1]/2 + h1[f0]/2 (+ h2 terms, constant),
+// about (f0, f1) with f_bar = (f0 + f1)/2:
+//   A  = dG/df1 =  I/dt + 1/2 R {., H} + 1/2 R {f_bar, h1[.]}
+//   A0 = dG/df0 = -I/dt + 1/2 R {., H} + 1/2 R {f_bar, h1[.]}
+// (R keeps rows 2..nv-3; the coupling term is dropped when fbar is null).
+// Transposes, from the skew-symmetry of the stencils in their first
+// argument and their antisymmetry in the two arguments:
+//   A^T u  =  u/dt - 1/2 {u~, H} + 1/2 h1[{u~, f_bar}]   (all rows; u~ = R^T u)
+//   A0^T u = -u/dt - 1/2 {u~, H} + 1/2 h1[{u~, f_bar}]
+// with h1[g] (x-only) broadcast over the rows.  linear_apply forms
+//   forward:    y = alpha u + beta R {u, H} + gamma R {f_bar, h1[u]}
+//   transposed: y = alpha u + beta {u~, H} + gamma h1[{u~, f_bar}]
+int linear_apply(vv_ctx *c, const double *u, const double *uscale, const double *fbar, double *y, double alpha,
+                 double beta, double gamma, bool transposed)
+{
+    if (int r = ensure_linear(c)) return r;
+    const double *h1 = xs_h1(c);
+    if (transposed) {
+        VV_CHECK(launch_bracket_t(c, u, uscale, nullptr, nullptr, y, alpha, beta), "bracket (transposed)");
+        if (fbar && gamma != 0.0) {
+            VV_CHECK(launch_bracket_t(c, u, uscale, fbar, nullptr, c->T2, 0.0, 1.0), "coupling bracket");
+            if (int r = h1_of(c, c->T2)) return r;
+            VV_CHECK(launch_xadd(c, y, h1, gamma), "coupling");
+        }
+    } else {
+        if (uscale) return fail(c, VV_ERR_INVALID, "internal: scaled input of the forward linearisation");
+        VV_CHECK(launch_zero(c, y, c->N * c->B), "zero");
+        VV_CHECK(launch_axpy(c, alpha, u, y, c->N * c->B), "axpy");
+        VV_CHECK(launch_bracket_add(c, u, y, beta), "bracket");
+        if (fbar && gamma != 0.0) {
+            if (int r = h1_of(c, u)) return r;
+            VV_CHECK(launch_bracket_gen(c, fbar, nullptr, h1, y, gamma), "coupling bracket");
+        }
+    }
+    return VV_OK;
+}
+
+// the preconditioner of the GMRES cycle: P^-1, or P^-T for the adjoint
+int apply_pc(vv_ctx *c, const double *x, double *y, const int32_t *done)
+{
+    if (c->op_mode == 1) return precond_t(c, x, y, done);
+    return precond(c, x, y, done);
+}
+
+// one Arnoldi step of GMRES on P^-1 J: V_{it+1} from V_it (every instance)
 int arnoldi_step(vv_ctx *c, int it)
 {
     const int32_t *done = &c->gs->st.done;
-    double *v = c->basis + (size_t)it * c->N;
-    double *w = c->basis + (size_t)(it + 1) * c->N;
+    const size_t vs = (size_t)c->N * c->B;
+    double *v = c->basis + (size_t)it * vs;
+    double *w = c->basis + (size_t)(it + 1) * vs;
     const double F = b_field(c), S = b_spec(c);
-    VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
-    VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, c->X, c->spec, done), "rfft");
-    VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
-    VV_PLAUNCH(P_IRFFT, S + F, it, launch_irfft(c, c->spec, w, done), "irfft");
+    if (c->op_mode != 0) {
+        if (int r = linear_apply(c, v, &c->gs->scale[it], c->op_fbar, c->X, c->ht_inv, -0.5, 0.5, true)) return r;
+        if (int r = apply_pc(c, c->X, w, done)) return r;
+    } else {
+        VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
+        VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, c->X, c->spec, done), "rfft");
+        VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
+        VV_PLAUNCH(P_IRFFT, S + F, it, launch_irfft(c, c->spec, w, done), "irfft");
+    }
     VV_PLAUNCH(P_MDOT, F * (it + 2), it, launch_mdot(c, it), "mdot");
     VV_PLAUNCH(P_MAXPY, F * (it + 3), it, launch_maxpy_norm(c, it), "maxpy_norm");
     return VV_OK;
@@ -174,7 +269,7 @@ int arnoldi_step(vv_ctx *c, int it)
 int precond32(vv_ctx *c, const float *x, float *y, const int32_t *done)
 {
     float2 *spec = reinterpret_cast<float2 *>(c->spec);
-    const double F = 4.0 * (double)c->N, S = 0.5 * b_spec(c);
+    const double F = 4.0 * (double)c->N * c->B, S = 0.5 * b_spec(c);
     VV_PLAUNCH(P_RFFT32, F + S, -1, launch_rfft32(c, x, spec, done), "rfft32");
     VV_PLAUNCH(P_TRI32, 2 * S, -1, launch_tridiag32(c, spec, done), "tridiag32");
     VV_PLAUNCH(P_IRFFT32, S + F, -1, launch_irfft32(c, spec, y, done), "irfft32");
@@ -186,11 +281,12 @@ int precond32(vv_ctx *c, const float *x, float *y, const int32_t *done)
 int arnoldi_step32(vv_ctx *c, int it)
 {
     const int32_t *done = &c->gs->st.done;
-    float *v = c->basis32 + (size_t)it * c->N;
-    float *w = c->basis32 + (size_t)(it + 1) * c->N;
+    const size_t vs = (size_t)c->N * c->B;
+    float *v = c->basis32 + (size_t)it * vs;
+    float *w = c->basis32 + (size_t)(it + 1) * vs;
     float *X = reinterpret_cast<float *>(c->X);
     float2 *spec = reinterpret_cast<float2 *>(c->spec);
-    const double F = 4.0 * (double)c->N, S = 0.5 * b_spec(c);
+    const double F = 4.0 * (double)c->N * c->B, S = 0.5 * b_spec(c);
     VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], X), "jacobian32");
     VV_PLAUNCH(P_RFFT32, F + S, it, launch_rfft32(c, X, spec, done), "rfft32");
     VV_PLAUNCH(P_TRI32, 2 * S, it, launch_tridiag32(c, spec, done), "tridiag32");
@@ -200,11 +296,35 @@ int arnoldi_step32(vv_ctx *c, int it)
     return VV_OK;
 }
 
+GmresStatus *status_slot(vv_ctx *c, int slot) { return c->host_status + (size_t)slot * c->B; }
+
+bool all_done(const GmresStatus *s, int B)
+{
+    for (int b = 0; b < B; ++b)
+        if (!s[b].done) return false;
+    return true;
+}
+
+int max_it_of(const GmresStatus *s, int B)
+{
+    int m = 0;
+    for (int b = 0; b < B; ++b) m = s[b].it > m ? s[b].it : m;
+    return m;
+}
+
+int copy_status(vv_ctx *c, int slot)
+{
+    VV_CHECK(hipMemcpy2DAsync(status_slot(c, slot), sizeof(GmresStatus), &c->gs->st, sizeof(GmresState),
+                              sizeof(GmresStatus), c->B, hipMemcpyDeviceToHost, c->stream),
+             "status copy");
+    return VV_OK;
+}
+
 // Arnoldi steps of one GMRES cycle, polling the device status every two
 // steps (the status copy of step k is read while steps k+1, k+2 run); returns
-// once the cycle has finished on the device, with its status in
-// host_status[0].  Steps enqueued past the end of the cycle return at once on
-// the device.
+// once the cycle has finished on the device for every instance, with the
+// statuses in host slot 0.  Steps enqueued past the end of an instance's
+// cycle return at once on the device.
 int run_cycle(vv_ctx *c, bool fp32)
 {
     const int max_k = c->p.restart;
@@ -218,61 +338,170 @@ int run_cycle(vv_ctx *c, bool fp32)
             int r = fp32 ? arnoldi_step32(c, it) : arnoldi_step(c, it);
             if (r) return r;
         }
-        VV_CHECK(hipMemcpyAsync(&c->host_status[slot], &c->gs->st, sizeof(GmresStatus), hipMemcpyDeviceToHost,
-                                c->stream), "status copy");
+        if (int r = copy_status(c, slot)) return r;
         VV_CHECK(hipEventRecord(c->ev[slot], c->stream), "event");
         if (pend >= 0) {
             VV_CHECK(hipEventSynchronize(c->ev[pend]), "event sync");
-            if (c->host_status[pend].done) stop = true;
+            if (all_done(status_slot(c, pend), c->B)) stop = true;
         }
         pend = slot;
         slot ^= 1;
     }
-    VV_CHECK(hipMemcpyAsync(&c->host_status[0], &c->gs->st, sizeof(GmresStatus), hipMemcpyDeviceToHost, c->stream),
-             "status copy");
+    if (int r = copy_status(c, 0)) return r;
     VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
-    prof_mark_idle(c, prof_from, c->host_status[0].it);
+    prof_mark_idle(c, prof_from, max_it_of(status_slot(c, 0), c->B));
     return VV_OK;
+}
+
+// V_0 = b (per instance) and the cycle start
+int cycle_start_from(vv_ctx *c, const double *b)
+{
+    VV_CHECK(launch_copy(c, b, c->basis, c->N * c->B), "copy b");
+    VV_CHECK(launch_norm_start(c), "cycle start");
+    return VV_OK;
+}
+
+void fill_results(vv_ctx *c, vv_ksp_result *res)
+{
+    if (!res) return;
+    const GmresStatus *st = status_slot(c, 0);
+    for (int b = 0; b < c->B; ++b) {
+        res[b].its = st[b].its;
+        res[b].reason = st[b].reason;
+        res[b].rnorm = st[b].res;
+        res[b].rnorm0 = st[b].rnorm0;
+    }
+}
+
+// Cycles after the first: while some instance still iterates, restart from
+// r = b - (P^-1 J) x (KSPInitialResidual) and run another cycle.  Instances
+// that finished stay done (their cycle start keeps done set).
+int restart_cycles(vv_ctx *c, const double *b, double *x)
+{
+    for (;;) {
+        const GmresStatus *st = status_slot(c, 0);
+        bool more = false;
+        for (int q = 0; q < c->B; ++q)
+            if (st[q].active && st[q].reason == 0) more = true;
+        if (!more) return VV_OK;
+        if (c->op_mode != 0) {
+            if (int r = linear_apply(c, x, nullptr, c->op_fbar, c->X, c->ht_inv, -0.5, 0.5, true)) return r;
+        } else {
+            VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
+        }
+        if (int r = apply_pc(c, c->X, c->basis, nullptr)) return r;
+        VV_CHECK(launch_residual_restart(c, b, c->basis), "restart residual");
+        VV_CHECK(launch_norm_start(c), "cycle start");
+        if (int r = run_cycle(c, false)) return r;
+        VV_CHECK(launch_build_solution(c, x), "build solution");
+        if (int r = copy_status(c, 0)) return r;
+        VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+    }
 }
 
 int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, vv_ksp_result *res)
 {
     if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
         return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve needs the tensor_kinetic preconditioner");
-    const int64_t N = c->N;
-    VV_CHECK(hipMemsetAsync(x, 0, N * sizeof(double), c->stream), "zero x");
+    VV_CHECK(launch_zero(c, x, c->N * c->B), "zero x");
     VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
-    VV_CHECK(launch_copy(c, b, c->basis, N), "copy b");
-    bool first = true;
-    GmresStatus st{};
-    for (;;) {
-        if (!first) {
-            // KSPInitialResidual after a restart: r = b - (P^-1 J) x
-            VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
-            int r = precond(c, c->X, c->basis, nullptr);
-            if (r) return r;
-            VV_CHECK(launch_residual_restart(c, b, c->basis), "restart residual");
+    if (int r = cycle_start_from(c, b)) return r;
+    if (int r = run_cycle(c, false)) return r;
+    VV_CHECK(launch_build_solution(c, x), "build solution");
+    if (int r = copy_status(c, 0)) return r;
+    VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+    if (int r = restart_cycles(c, b, x)) return r;
+    fill_results(c, res);
+    return VV_OK;
+}
+
+// The first GMRES cycle of SNESSolve_KSPONLY as one captured graph:
+// b = P^-1 F(f); x = 0; GMRES init; V_0 = b; cycle start; `restart` Arnoldi
+// steps (steps past an instance's convergence return at once); x += V y.
+// The graph is kept per (f, fh) pair and replayed while they stay the same;
+// the KSP tolerances are kernel arguments, so a change re-captures too.
+int snes_graph(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp)
+{
+    const bool same = c->graph_exec != nullptr && c->graph_f == f && c->graph_fh == fh &&
+                      c->graph_kp[0] == kp->rtol && c->graph_kp[1] == kp->atol && c->graph_kp[2] == kp->dtol &&
+                      c->graph_kp[3] == (double)kp->max_it;
+    if (!same) {
+        if (c->graph_exec) {
+            (void)hipGraphExecDestroy(c->graph_exec);
+            c->graph_exec = nullptr;
         }
-        first = false;
-        VV_CHECK(launch_norm_start(c), "cycle start");
+        if (c->stream == nullptr) return fail(c, VV_ERR_INVALID, "graph capture needs a non-null stream");
+        VV_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        c->capturing = true;
+        int r = VV_OK;
+        hipError_t le = hipSuccess;
+        do {
+            if ((le = launch_residual(c, f, fh, c->X)) != hipSuccess) break;
+            if ((r = precond(c, c->X, c->rhs, nullptr))) break;
+            if ((le = launch_zero(c, c->D, c->N * c->B)) != hipSuccess) break;
+            if ((le = launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it)) != hipSuccess) break;
+            if ((r = cycle_start_from(c, c->rhs))) break;
+            for (int it = 0; it < c->p.restart && r == VV_OK; ++it) r = arnoldi_step(c, it);
+            if (r) break;
+            le = launch_build_solution(c, c->D);
+        } while (0);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->capturing = false;
+        if (r != VV_OK || le != hipSuccess || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (r != VV_OK) return r;
+            return hipfail(c, le != hipSuccess ? le : e, "graph capture");
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->graph_exec = nullptr;
+            return hipfail(c, ei, "graph instantiate");
+        }
+        c->graph_f = f;
+        c->graph_fh = fh;
+        c->graph_kp[0] = kp->rtol;
+        c->graph_kp[1] = kp->atol;
+        c->graph_kp[2] = kp->dtol;
+        c->graph_kp[3] = (double)kp->max_it;
+    }
+    VV_CHECK(hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
+    if (int r = copy_status(c, 0)) return r;
+    VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+    return VV_OK;
+}
+
+int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, vv_ksp_result *res)
+{
+    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
+        return fail(c, VV_ERR_UNSUPPORTED, "vv_snes_ksponly needs the tensor_kinetic preconditioner");
+    if (c->use_graph) {
+        if (int r = snes_graph(c, f, fh, kp)) return r;
+    } else {
+        VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, c->X), "residual");
+        if (int r = precond(c, c->X, c->rhs, nullptr)) return r;
+        VV_CHECK(launch_zero(c, c->D, c->N * c->B), "zero x");
+        VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
+        if (int r = cycle_start_from(c, c->rhs)) return r;
         if (int r = run_cycle(c, false)) return r;
-        VV_CHECK(launch_build_solution(c, x), "build solution");
-        VV_CHECK(hipMemcpyAsync(&c->host_status[0], &c->gs->st, sizeof(GmresStatus), hipMemcpyDeviceToHost,
-                                c->stream), "status copy");
+        VV_CHECK(launch_build_solution(c, c->D), "build solution");
+        if (int r = copy_status(c, 0)) return r;
         VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
-        st = c->host_status[0];
-        if (st.reason != 0) break;
-        if (st.its >= kp->max_it) {
-            st.reason = VV_KSP_DIVERGED_ITS;
-            break;
-        }
     }
-    if (res) {
-        res->its = st.its;
-        res->reason = st.reason;
-        res->rnorm = st.res;
-        res->rnorm0 = st.rnorm0;
-    }
+    if (int r = restart_cycles(c, c->rhs, c->D)) return r;
+    // SNESSolve_KSPONLY takes the step unless the linear solve diverged
+    VV_CHECK(launch_axpy_cond(c, -1.0, c->D, f), "update");
+    fill_results(c, res);
+    return VV_OK;
+}
+
+// lazily allocated buffers of the mixed-precision solver (FP32 basis, FP64 residual)
+int ensure_mixed(vv_ctx *c)
+{
+    if (c->basis32) return VV_OK;
+    VV_CHECK(dalloc(&c->basis32, (size_t)(c->p.restart + 1) * c->N * c->B), "alloc FP32 basis");
+    VV_CHECK(dalloc(&c->R, (size_t)c->N * c->B), "alloc residual");
     return VV_OK;
 }
 
@@ -287,11 +516,13 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
 {
     if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
         return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve_mixed needs the tensor_kinetic preconditioner");
+    if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the mixed-precision solve takes one instance");
+    if (int r = ensure_mixed(c)) return r;
     const int64_t N = c->N;
-    VV_CHECK(hipMemsetAsync(x, 0, N * sizeof(double), c->stream), "zero x");
-    VV_CHECK(launch_norm2(c, b, N, c->dscalar), "norm");
+    VV_CHECK(launch_zero(c, x, N), "zero x");
+    VV_CHECK(launch_norm2(c, b, N, ds_norms(c)), "norm");
     double rnorm0 = 0.0;
-    if (int r = read_scalar(c, c->dscalar, &rnorm0)) return r;
+    if (int r = read_scalars(c, ds_norms(c), 1, &rnorm0)) return r;
     const double ttol = std::fmax(kp->rtol * rnorm0, kp->atol);
     double rnorm = rnorm0;
     const double *rsrc = b;
@@ -320,15 +551,15 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
         VV_CHECK(launch_gmres_init(c, rt, 0.0, 1.0e300, kp->max_it - its), "gmres init");
         VV_CHECK(launch_start32(c, rsrc), "cycle start");
         if (int r = run_cycle(c, true)) return r;
-        its += c->host_status[0].it;
+        its += status_slot(c, 0)[0].it;
         ++rounds;
         VV_CHECK(launch_build_solution32(c, x), "build solution");
         // true FP64 residual r = b - P^-1 J x
         VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
         if (int r = precond(c, c->X, c->R, nullptr)) return r;
         VV_CHECK(launch_residual_restart(c, b, c->R), "residual");
-        VV_CHECK(launch_norm2(c, c->R, N, c->dscalar), "norm");
-        if (int r = read_scalar(c, c->dscalar, &rnorm)) return r;
+        VV_CHECK(launch_norm2(c, c->R, N, ds_norms(c)), "norm");
+        if (int r = read_scalars(c, ds_norms(c), 1, &rnorm)) return r;
         rsrc = c->R;
     }
     if (res) {
@@ -341,21 +572,8 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
     return VV_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char *vv_last_error(const vv_ctx *ctx)
+int check_params(const vv_params *p)
 {
-    if (!ctx) return g_create_error.c_str();
-    return ctx->err.c_str();
-}
-
-int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
-{
-    vv_ctx *c = nullptr;
-    if (!p || !out) return fail(nullptr, VV_ERR_INVALID, "null argument");
-    *out = nullptr;
     if (!is_pow2(p->nx) || p->nx < 8 || p->nx > kMaxNx)
         return fail(nullptr, VV_ERR_INVALID, "nx must be a power of two in [8, 4096]");
     if (p->nv < 8 || p->nv > 4096) return fail(nullptr, VV_ERR_INVALID, "nv must be in [8, 4096]");
@@ -366,16 +584,30 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
     if (p->laplace != VV_LAPLACE_CFD2 && p->laplace != VV_LAPLACE_CFD4)
         return fail(nullptr, VV_ERR_UNSUPPORTED, "laplace operator must be cfd2 or cfd4");
     if (p->restart < 1 || p->restart > kMaxRestart) return fail(nullptr, VV_ERR_INVALID, "restart must be in [1, 64]");
-    if (!(p->hx > 0) || !(p->hv > 0) || !(p->ht > 0)) return fail(nullptr, VV_ERR_INVALID, "hx, hv, ht must be > 0");
+    if (!(p->hv > 0) || !(p->ht > 0)) return fail(nullptr, VV_ERR_INVALID, "hx, hv, ht must be > 0");
     if (!p->v) return fail(nullptr, VV_ERR_INVALID, "v grid missing");
     if (p->precond == VV_PRECOND_TENSOR_KINETIC && !p->eigen)
         return fail(nullptr, VV_ERR_INVALID, "kinetic eigenvalues missing");
+    return VV_OK;
+}
+
+int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_ctx **out)
+{
+    vv_ctx *c = nullptr;
+    if (!p || !out || !hxs) return fail(nullptr, VV_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (int r = check_params(p)) return r;
+    if (B < 1 || B > 65535) return fail(nullptr, VV_ERR_INVALID, "batch must be in [1, 65535]");
+    for (int b = 0; b < B; ++b)
+        if (!(hxs[b] > 0)) return fail(nullptr, VV_ERR_INVALID, "hx, hv, ht must be > 0");
 
     c = new (std::nothrow) vv_ctx();
     if (!c) return fail(nullptr, VV_ERR_INVALID, "out of host memory");
     c->p = *p;
     c->p.v = nullptr;
     c->p.eigen = nullptr;
+    c->p.hx = hxs[0];
+    c->B = B;
     c->stream = reinterpret_cast<hipStream_t>(stream);
     const int nx = p->nx, nv = p->nv;
     c->N = (int64_t)nx * nv;
@@ -383,15 +615,21 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
     c->logn = 0;
     while ((1 << c->logn) < nx) ++c->logn;
     // Grid.create (Grid.pyx:40-48) and the bracket / band factors
-    const double hx = p->hx, hv = p->hv;
+    const double hv = p->hv;
     c->ht_inv = 1. / p->ht;
-    c->hx_inv = 1. / hx;
     c->hv_inv = 1. / hv;
-    const double hx2 = hx * hx, hv2 = hv * hv;
-    c->hx2_inv = 1. / hx2;
-    c->af1 = c->hx_inv * c->hv_inv / 12.;          // PoissonBracket.pyx:88
-    c->af2 = c->hx_inv * c->hv_inv / 24.;          // PoissonBracket.pyx:209
-    c->afpc = 0.5 * c->hx_inv * c->hv_inv / 12.;   // KineticFast.pyx:251
+    const double hv2 = hv * hv;
+    c->inst_host.resize(B);
+    for (int b = 0; b < B; ++b) {
+        InstConst &ic = c->inst_host[b];
+        const double hx = hxs[b];
+        ic.hx = hx;
+        ic.hx_inv = 1. / hx;
+        ic.hx2_inv = 1. / (hx * hx);
+        ic.af1 = ic.hx_inv * c->hv_inv / 12.;          // PoissonBracket.pyx:88
+        ic.af2 = ic.hx_inv * c->hv_inv / 24.;          // PoissonBracket.pyx:209
+        ic.afpc = 0.5 * ic.hx_inv * c->hv_inv / 12.;   // KineticFast.pyx:251
+    }
     c->prof_state = new Prof();
 
     auto bail = [&](hipError_t e, const char *where) {
@@ -405,43 +643,50 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
         if (_e != hipSuccess) return bail(_e, where); \
     } while (0)
 
-    const int nfb = fft_blocks(c), ncb = cgs_blocks(c->N);
+    const int nfb = fft_blocks(c), ncb = cgs_blocks_max(c->N);
     int nrows = (nfb > kReduceBlocks ? nfb : kReduceBlocks);
     if (ncb > nrows) nrows = ncb;
+    c->part_stride = (int64_t)nrows * (kMaxRestart + 2);
+    const size_t Bz = (size_t)B;
+    VV_CCHECK(dalloc(&c->inst, Bz), "alloc instance constants");
+    VV_CCHECK(dalloc(&c->active, Bz), "alloc active flags");
     VV_CCHECK(dalloc(&c->h0, nv), "alloc h0");
-    VV_CCHECK(dalloc(&c->h1p, nx), "alloc h1p");
-    VV_CCHECK(dalloc(&c->h1h, nx), "alloc h1h");
-    VV_CCHECK(dalloc(&c->h2p, nx), "alloc h2p");
-    VV_CCHECK(dalloc(&c->h2h, nx), "alloc h2h");
+    VV_CCHECK(dalloc(&c->h1p, nx * Bz), "alloc h1p");
+    VV_CCHECK(dalloc(&c->h1h, nx * Bz), "alloc h1h");
+    VV_CCHECK(dalloc(&c->h2p, nx * Bz), "alloc h2p");
+    VV_CCHECK(dalloc(&c->h2h, nx * Bz), "alloc h2h");
     VV_CCHECK(dalloc(&c->eigen, c->nmodes), "alloc eigen");
     VV_CCHECK(dalloc(&c->gm, nv), "alloc gm");
     VV_CCHECK(dalloc(&c->g0, nv), "alloc g0");
     VV_CCHECK(dalloc(&c->gp, nv), "alloc gp");
     VV_CCHECK(dalloc(&c->twiddle, nx), "alloc twiddle");
     VV_CCHECK(dalloc(&c->twiddle32, nx), "alloc twiddle32");
-    VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * nv), "alloc spectrum");
-    VV_CCHECK(dalloc(&c->pinv, nx), "alloc pinv");
-    VV_CCHECK(dalloc(&c->pwork, 3 * nx + 1), "alloc Poisson scratch");
-    VV_CCHECK(dalloc(&c->X, c->N), "alloc X");
-    VV_CCHECK(dalloc(&c->B, c->N), "alloc B");
-    VV_CCHECK(dalloc(&c->D, c->N), "alloc D");
-    VV_CCHECK(dalloc(&c->basis, (size_t)(p->restart + 1) * c->N), "alloc Krylov basis");
-    // the mixed solver's FP32 basis and FP64 residual share this allocation:
-    // (restart+1) N floats, then N doubles (fits for restart >= 1)
-    c->basis32 = reinterpret_cast<float *>(c->basis);
-    c->R = c->basis + (size_t)(p->restart + 1) * (size_t)(c->N / 2);
-    VV_CCHECK(dalloc(&c->partial, (size_t)nrows * (kMaxRestart + 2)), "alloc partial");
-    VV_CCHECK(dalloc(&c->dscalar, 16), "alloc scalars");
-    VV_CCHECK(dalloc(&c->npart, (size_t)64 * nx), "alloc density partials");
-    VV_CCHECK(dalloc(&c->gs, 1), "alloc gmres state");
-    VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_status), 2 * sizeof(GmresStatus)), "pinned status");
+    VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * nv * Bz), "alloc spectrum");
+    VV_CCHECK(dalloc(&c->pinv, nx * Bz), "alloc pinv");
+    VV_CCHECK(dalloc(&c->pwork, (3 * (size_t)nx + 1) * Bz), "alloc Poisson scratch");
+    VV_CCHECK(dalloc(&c->X, c->N * Bz), "alloc X");
+    VV_CCHECK(dalloc(&c->rhs, c->N * Bz), "alloc rhs");
+    VV_CCHECK(dalloc(&c->D, c->N * Bz), "alloc D");
+    VV_CCHECK(dalloc(&c->basis, (size_t)(p->restart + 1) * c->N * Bz), "alloc Krylov basis");
+    VV_CCHECK(dalloc(&c->partial, (size_t)c->part_stride * Bz), "alloc partial");
+    VV_CCHECK(dalloc(&c->dscalar, ds_count(B)), "alloc scalars");
+    VV_CCHECK(dalloc(&c->npart, (size_t)64 * nx * Bz), "alloc density partials");
+    VV_CCHECK(dalloc(&c->gs, Bz), "alloc gmres state");
+    VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_status), 2 * Bz * sizeof(GmresStatus)),
+              "pinned status");
+    VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_scalars), ds_count(B) * sizeof(double)),
+              "pinned scalars");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming), "event");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming), "event");
-    VV_CCHECK(hipMemset(c->gs, 0, sizeof(GmresState)), "zero gmres state");
-    VV_CCHECK(hipMemset(c->partial, 0, (size_t)nrows * (kMaxRestart + 2) * sizeof(double)), "zero partials");
+    VV_CCHECK(hipMemset(c->gs, 0, Bz * sizeof(GmresState)), "zero gmres state");
+    VV_CCHECK(hipMemset(c->partial, 0, (size_t)c->part_stride * Bz * sizeof(double)), "zero partials");
+    VV_CCHECK(hipMemset(c->h1p, 0, nx * Bz * sizeof(double)), "zero h1p");
+    VV_CCHECK(hipMemset(c->h1h, 0, nx * Bz * sizeof(double)), "zero h1h");
+    VV_CCHECK(hipMemset(c->h2p, 0, nx * Bz * sizeof(double)), "zero h2p");
+    VV_CCHECK(hipMemset(c->h2h, 0, nx * Bz * sizeof(double)), "zero h2h");
+    VV_CCHECK(hipMemset(c->h0, 0, nv * sizeof(double)), "zero h0");
 
     // host tables
-    std::vector<double> hzero(nx > nv ? nx : nv, 0.0);
     std::vector<double> gm(nv), g0(nv), gp(nv);
     for (int j = 0; j < nv; ++j) {
         const double v = p->v[j];
@@ -463,51 +708,79 @@ int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
         tw32[m].x = (float)cosl(ang);
         tw32[m].y = (float)sinl(ang);
     }
-    // mean-free pseudo-inverse of the periodic Laplacian: c_m = 1/nx sum_{k>0} cos(2 pi k m/nx) / lambda_k
+    // mean-free pseudo-inverse of the periodic Laplacian, per instance:
+    // c_m = 1/nx sum_{k>0} cos(2 pi k m/nx) / lambda_k
     std::vector<long double> ct(nx), lam(nx);
     for (int q = 0; q < nx; ++q)
         ct[q] = cosl(2.0L * 3.141592653589793238462643383279502884L * (long double)q / (long double)nx);
-    for (int k = 1; k < nx; ++k) {
-        const long double th2 = 3.141592653589793238462643383279502884L * (long double)k / (long double)nx;
-        const long double s1 = sinl(th2), s2 = sinl(2.0L * th2);
-        const long double h2 = (long double)hx * (long double)hx;
-        // CFD4 symbol (54 - 64 cos t + 10 cos 2t)/12 = (128 sin^2(t/2) - 20 sin^2 t)/12; CFD2: 4 sin^2(t/2)
-        lam[k] = (p->laplace == VV_LAPLACE_CFD4) ? (128.0L * s1 * s1 - 20.0L * s2 * s2) / (12.0L * h2)
-                                                 : (4.0L * s1 * s1) / h2;
+    std::vector<double> pinv(nx * Bz);
+    for (int b = 0; b < B; ++b) {
+        const double hx = hxs[b];
+        for (int k = 1; k < nx; ++k) {
+            const long double th2 = 3.141592653589793238462643383279502884L * (long double)k / (long double)nx;
+            const long double s1 = sinl(th2), s2 = sinl(2.0L * th2);
+            const long double h2 = (long double)hx * (long double)hx;
+            // CFD4 symbol (54 - 64 cos t + 10 cos 2t)/12 = (128 sin^2(t/2) - 20 sin^2 t)/12; CFD2: 4 sin^2(t/2)
+            lam[k] = (p->laplace == VV_LAPLACE_CFD4) ? (128.0L * s1 * s1 - 20.0L * s2 * s2) / (12.0L * h2)
+                                                     : (4.0L * s1 * s1) / h2;
+        }
+        for (int m = 0; m < nx; ++m) {
+            long double s = 0.0L;
+            for (int k = 1; k < nx; ++k) s += ct[(int)(((long long)k * m) % nx)] / lam[k];
+            pinv[(size_t)b * nx + m] = (double)(s / (long double)nx);
+        }
     }
-    std::vector<double> pinv(nx);
-    for (int m = 0; m < nx; ++m) {
-        long double s = 0.0L;
-        for (int k = 1; k < nx; ++k) s += ct[(int)(((long long)k * m) % nx)] / lam[k];
-        pinv[m] = (double)(s / (long double)nx);
-    }
-    VV_CCHECK(hipMemcpy(c->h0, hzero.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->h1p, hzero.data(), nx * sizeof(double), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->h1h, hzero.data(), nx * sizeof(double), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->h2p, hzero.data(), nx * sizeof(double), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->h2h, hzero.data(), nx * sizeof(double), hipMemcpyHostToDevice), "upload");
+    std::vector<int32_t> act(Bz, 1);
+    VV_CCHECK(hipMemcpy(c->inst, c->inst_host.data(), Bz * sizeof(InstConst), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->active, act.data(), Bz * sizeof(int32_t), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->gm, gm.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->g0, g0.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->gp, gp.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->eigen, eig.data(), c->nmodes * sizeof(double2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->twiddle, tw.data(), nx * sizeof(double2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->twiddle32, tw32.data(), nx * sizeof(float2), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->pinv, pinv.data(), nx * sizeof(double), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->pinv, pinv.data(), nx * Bz * sizeof(double), hipMemcpyHostToDevice), "upload");
 #undef VV_CCHECK
     *out = c;
     return VV_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+const char *vv_last_error(const vv_ctx *ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+int vv_ctx_create(const vv_params *p, void *stream, vv_ctx **out)
+{
+    if (!p) return fail(nullptr, VV_ERR_INVALID, "null argument");
+    return create_impl(p, 1, &p->hx, stream, out);
+}
+
+int vv_ctx_create_batched(const vv_params *p, int32_t batch, const double *hx, void *stream, vv_ctx **out)
+{
+    return create_impl(p, batch, hx, stream, out);
+}
+
+int32_t vv_batch_size(const vv_ctx *c) { return c ? c->B : 0; }
+
 int vv_ctx_destroy(vv_ctx *c)
 {
     if (!c) return VV_OK;
     (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->h0, c->h1p, c->h1h, c->h2p, c->h2h, c->eigen, c->gm, c->g0, c->gp, c->twiddle, c->twiddle32,
-                    c->spec,
-                    c->pinv, c->pwork, c->X, c->B, c->D, c->basis, c->partial, c->dscalar, c->npart, c->gs};
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    void *bufs[] = {c->inst,   c->active, c->h0,      c->h1p,     c->h1h,  c->h2p,   c->h2h,    c->eigen,
+                    c->gm,     c->g0,     c->gp,      c->twiddle, c->twiddle32, c->spec, c->pinv, c->pwork,
+                    c->X,      c->rhs,    c->D,       c->basis,   c->basis32, c->R,   c->partial, c->dscalar,
+                    c->npart,  c->gs,     c->T2,      c->xs};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);
+    if (c->host_scalars) (void)hipHostFree(c->host_scalars);
     if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
     if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
     Prof *pr = prof_of(c);
@@ -526,6 +799,10 @@ int vv_ctx_destroy(vv_ctx *c)
 int vv_set_stream(vv_ctx *c, void *stream)
 {
     if (int r = check_ctx(c)) return r;
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
     c->stream = reinterpret_cast<hipStream_t>(stream);
     return VV_OK;
 }
@@ -540,10 +817,140 @@ int vv_synchronize(vv_ctx *c)
 int64_t vv_scratch_bytes(const vv_ctx *c)
 {
     if (!c) return 0;
-    const int64_t nx = c->p.nx, nv = c->p.nv;
-    return (int64_t)sizeof(double) * ((c->p.restart + 4) * c->N + 64 * nx + 4 * nx + 4 * nv) +
-           (int64_t)sizeof(double2) * ((int64_t)c->nmodes * nv + nx + c->nmodes) + (int64_t)sizeof(float2) * nx +
-           (int64_t)sizeof(GmresState);
+    const int64_t nx = c->p.nx, nv = c->p.nv, B = c->B;
+    int64_t bytes = (int64_t)sizeof(double) * B * ((c->p.restart + 4) * c->N + 64 * nx + 8 * nx + 1 + c->part_stride) +
+                    (int64_t)sizeof(double) * (4 * nv + (int64_t)ds_count(c->B)) +
+                    (int64_t)sizeof(double2) * (B * (int64_t)c->nmodes * nv + nx + c->nmodes) +
+                    (int64_t)sizeof(float2) * nx + (int64_t)(sizeof(GmresState) + sizeof(InstConst) + 4) * B;
+    if (c->basis32) bytes += (int64_t)sizeof(float) * B * (c->p.restart + 1) * c->N + 8 * B * c->N;
+    return bytes;
+}
+
+int vv_set_active(vv_ctx *c, const int32_t *active)
+{
+    if (int r = check_ctx(c)) return r;
+    std::vector<int32_t> a((size_t)c->B, 1);
+    if (active)
+        for (int b = 0; b < c->B; ++b) a[b] = active[b] ? 1 : 0;
+    VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+    VV_CHECK(hipMemcpy(c->active, a.data(), (size_t)c->B * sizeof(int32_t), hipMemcpyHostToDevice), "upload mask");
+    return VV_OK;
+}
+
+int vv_set_graph(vv_ctx *c, int enable)
+{
+    if (int r = check_ctx(c)) return r;
+    if (enable && c->stream == nullptr) return fail(c, VV_ERR_INVALID, "graph replay needs a non-null stream");
+    if (!enable && c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->use_graph = enable != 0;
+    return VV_OK;
+}
+
+int vv_set_tuning(vv_ctx *c, int32_t fft_rows, int32_t cgs_ept)
+{
+    if (int r = check_ctx(c)) return r;
+    if (fft_rows != 0 && (fft_rows < 2 || (fft_rows & 1) || fft_rows * c->p.nx > 8192))
+        return fail(c, VV_ERR_INVALID, "fft_rows must be 0 or even, >= 2, with fft_rows * nx <= 8192");
+    if (cgs_ept != 0 && cgs_ept != 1 && cgs_ept != 2 && cgs_ept != 4 && cgs_ept != 8)
+        return fail(c, VV_ERR_INVALID, "cgs_ept must be 0, 1, 2, 4 or 8");
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->tune_fft_rows = fft_rows;
+    c->tune_cgs_ept = cgs_ept;
+    return VV_OK;
+}
+
+int vv_get_tuning(vv_ctx *c, int32_t *fft_rows, int32_t *cgs_ept)
+{
+    if (int r = check_ctx(c)) return r;
+    if (fft_rows) *fft_rows = c->tune_fft_rows;
+    if (cgs_ept) *cgs_ept = c->tune_cgs_ept;
+    return VV_OK;
+}
+
+// Times the candidate launch shapes on this context's own buffers and keeps
+// the fastest: v-rows per FFT workgroup (over the FP64 preconditioner) and
+// vectors per thread of the CGS kernels (over mdot + maxpy at a mid-cycle
+// basis size).  Overwrites the work buffers and the GMRES state.
+int vv_autotune(vv_ctx *c, int32_t *fft_rows, int32_t *cgs_ept)
+{
+    if (int r = check_ctx(c)) return r;
+    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
+        return fail(c, VV_ERR_UNSUPPORTED, "autotune needs the tensor_kinetic preconditioner");
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    const int prof = c->prof;
+    c->prof = 0;
+    hipEvent_t e0, e1;
+    VV_CHECK(hipEventCreate(&e0), "event");
+    VV_CHECK(hipEventCreate(&e1), "event");
+    auto timed = [&](auto body, int reps, float *ms) -> int {
+        for (int q = 0; q < 2; ++q)
+            if (int r = body()) return r;
+        VV_CHECK(hipEventRecord(e0, c->stream), "event");
+        for (int q = 0; q < reps; ++q)
+            if (int r = body()) return r;
+        VV_CHECK(hipEventRecord(e1, c->stream), "event");
+        VV_CHECK(hipEventSynchronize(e1), "event sync");
+        VV_CHECK(hipEventElapsedTime(ms, e0, e1), "elapsed");
+        return VV_OK;
+    };
+    int rc = VV_OK;
+    do {
+        // FFT rows: even R with R nx <= 8192
+        if ((rc = launch_zero(c, c->X, c->N * c->B) == hipSuccess ? VV_OK : VV_ERR_HIP)) break;
+        int bestR = 0;
+        float bestT = 0.f;
+        for (int R = 2; R * c->p.nx <= 8192; R <<= 1) {
+            c->tune_fft_rows = R;
+            float ms = 0.f;
+            if ((rc = timed([&]() { return precond(c, c->X, c->D, nullptr); }, 5, &ms))) break;
+            if (bestR == 0 || ms < bestT) {
+                bestR = R;
+                bestT = ms;
+            }
+        }
+        if (rc) break;
+        c->tune_fft_rows = bestR;
+        // CGS vectors per thread, at basis size min(restart, 10)
+        const int col = (c->p.restart < 10 ? c->p.restart : 10) - 1;
+        if ((rc = launch_zero(c, c->basis, (size_t)(col + 2) * c->N * c->B) == hipSuccess ? VV_OK : VV_ERR_HIP))
+            break;
+        int bestE = 0;
+        bestT = 0.f;
+        for (int E = 1; E <= 8; E <<= 1) {
+            c->tune_cgs_ept = E;
+            float ms = 0.f;
+            auto body = [&]() -> int {
+                VV_CHECK(launch_gmres_init(c, 1e-300, 0.0, 1e300, 1 << 30), "gmres init");
+                VV_CHECK(launch_mdot(c, col), "mdot");
+                VV_CHECK(launch_maxpy_norm(c, col), "maxpy");
+                return VV_OK;
+            };
+            if ((rc = timed(body, 5, &ms))) break;
+            if (bestE == 0 || ms < bestT) {
+                bestE = E;
+                bestT = ms;
+            }
+        }
+        if (rc) break;
+        c->tune_cgs_ept = bestE;
+    } while (0);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->prof = prof;
+    if (rc == VV_ERR_HIP && c->err.empty()) c->err = "autotune: kernel launch failed";
+    if (rc) return rc;
+    if (fft_rows) *fft_rows = c->tune_fft_rows;
+    if (cgs_ept) *cgs_ept = c->tune_cgs_ept;
+    return VV_OK;
 }
 
 int vv_set_hamiltonian(vv_ctx *c, const double *h0, const double *h1p, const double *h1h, const double *h2p,
@@ -551,7 +958,7 @@ int vv_set_hamiltonian(vv_ctx *c, const double *h0, const double *h1p, const dou
 {
     if (int r = check_ctx(c)) return r;
     if (!h0 || !h1p || !h1h || !h2p || !h2h) return fail(c, VV_ERR_INVALID, "null Hamiltonian factor");
-    const size_t bx = c->p.nx * sizeof(double), bv = c->p.nv * sizeof(double);
+    const size_t bx = (size_t)c->p.nx * c->B * sizeof(double), bv = c->p.nv * sizeof(double);
     VV_CHECK(hipMemcpyAsync(c->h0, h0, bv, hipMemcpyDeviceToDevice, c->stream), "copy h0");
     VV_CHECK(hipMemcpyAsync(c->h1p, h1p, bx, hipMemcpyDeviceToDevice, c->stream), "copy h1p");
     VV_CHECK(hipMemcpyAsync(c->h1h, h1h, bx, hipMemcpyDeviceToDevice, c->stream), "copy h1h");
@@ -564,6 +971,7 @@ int vv_residual(vv_ctx *c, const double *f, const double *fh, double *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !fh || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (y == f || y == fh) return fail(c, VV_ERR_INVALID, "residual: y must not alias f or fh");
     VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, y), "residual");
     return VV_OK;
 }
@@ -572,6 +980,7 @@ int vv_jacobian_apply(vv_ctx *c, const double *u, double *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (u == y) return fail(c, VV_ERR_INVALID, "jacobian_apply: u and y must not alias");
     VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, u, nullptr, y), "jacobian");
     return VV_OK;
 }
@@ -612,9 +1021,10 @@ int vv_function_norm(vv_ctx *c, const double *f, const double *fh, double *y, do
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !fh || !y || !out) return fail(c, VV_ERR_INVALID, "null argument");
+    if (y == f || y == fh) return fail(c, VV_ERR_INVALID, "function_norm: y must not alias f or fh");
     VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, y), "residual");
-    VV_CHECK(launch_norm2(c, y, c->N, c->dscalar), "norm");
-    return read_scalar(c, c->dscalar, out);
+    VV_CHECK(launch_norms(c, y, ds_norms(c)), "norm");
+    return read_scalars(c, ds_norms(c), (size_t)c->B, out);
 }
 
 int vv_density(vv_ctx *c, const double *f, double *n)
@@ -662,8 +1072,11 @@ int vv_moments(vv_ctx *c, const double *f, double *n, double *phi, double *h1, d
     if (int r = check_ctx(c)) return r;
     if (!f || !n || !phi || !h1) return fail(c, VV_ERR_INVALID, "null argument");
     VV_CHECK(launch_density_partial(c, f), "density");
-    VV_CHECK(launch_poisson(c, nullptr, n, phi, h1, c->dscalar + 1, 1), "poisson");
-    if (pnorm) return read_scalar(c, c->dscalar + 1, pnorm);
+    VV_CHECK(launch_poisson(c, nullptr, n, phi, h1, ds_pres(c), 1), "poisson");
+    if (!pnorm) return VV_OK;
+    std::vector<double> tmp(16 * (size_t)c->B);
+    if (int r = read_scalars(c, ds_pres(c), 16 * (size_t)(c->B - 1) + 1, tmp.data())) return r;
+    for (int b = 0; b < c->B; ++b) pnorm[b] = tmp[16 * (size_t)b];
     return VV_OK;
 }
 
@@ -671,16 +1084,16 @@ int vv_norm2(vv_ctx *c, const double *x, int64_t len, double *out)
 {
     if (int r = check_ctx(c)) return r;
     if (!x || !out || len < 0) return fail(c, VV_ERR_INVALID, "bad argument");
-    VV_CHECK(launch_norm2(c, x, len, c->dscalar), "norm");
-    return read_scalar(c, c->dscalar, out);
+    VV_CHECK(launch_norm2(c, x, len, ds_norms(c)), "norm");
+    return read_scalars(c, ds_norms(c), 1, out);
 }
 
 int vv_dot(vv_ctx *c, const double *x, const double *y, int64_t len, double *out)
 {
     if (int r = check_ctx(c)) return r;
     if (!x || !y || !out || len < 0) return fail(c, VV_ERR_INVALID, "bad argument");
-    VV_CHECK(launch_dot(c, x, y, len, c->dscalar), "dot");
-    return read_scalar(c, c->dscalar, out);
+    VV_CHECK(launch_dot(c, x, y, len, ds_norms(c)), "dot");
+    return read_scalars(c, ds_norms(c), 1, out);
 }
 
 int vv_axpy(vv_ctx *c, double alpha, const double *x, double *y, int64_t len)
@@ -703,10 +1116,8 @@ int vv_snes_ksponly(vv_ctx *c, double *f, const double *fh, const vv_ksp_params 
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !fh || !kp) return fail(c, VV_ERR_INVALID, "null argument");
-    if (int r = vv_function(c, f, fh, c->B)) return r;
-    if (int r = ksp_solve_impl(c, c->B, c->D, kp, res)) return r;
-    VV_CHECK(launch_axpy_cond(c, -1.0, c->D, f, c->N), "update");
-    return VV_OK;
+    if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
+    return snes_impl(c, f, fh, kp, res);
 }
 
 int vv_ksp_solve_mixed(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, const vv_mixed_params *mp,
@@ -728,9 +1139,10 @@ int vv_snes_ksponly_mixed(vv_ctx *c, double *f, const double *fh, const vv_ksp_p
     if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
     if (!(mp->inner_rtol > 0.0 && mp->inner_rtol < 1.0) || mp->max_cycles < 1)
         return fail(c, VV_ERR_INVALID, "inner_rtol must be in (0, 1) and max_cycles >= 1");
-    if (int r = vv_function(c, f, fh, c->B)) return r;
+    if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the mixed-precision solve takes one instance");
+    if (int r = vv_function(c, f, fh, c->rhs)) return r;
     vv_ksp_result kr{};
-    if (int r = ksp_solve_mixed_impl(c, c->B, c->D, kp, mp, &kr, cycles)) return r;
+    if (int r = ksp_solve_mixed_impl(c, c->rhs, c->D, kp, mp, &kr, cycles)) return r;
     // SNESSolve_KSPONLY takes the step unless the linear solve diverged
     if (kr.reason > 0) VV_CHECK(launch_axpy(c, -1.0, c->D, f, c->N), "update");
     if (res) *res = kr;
@@ -742,7 +1154,7 @@ int vv_jacobian_apply32(vv_ctx *c, const float *u, float *y)
     if (int r = check_ctx(c)) return r;
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
     if (u == y) return fail(c, VV_ERR_INVALID, "jacobian_apply32: u and y must not alias");
-    VV_PLAUNCH(P_JAC32, 8.0 * (double)c->N, -1, launch_jacobian32(c, u, nullptr, y), "jacobian32");
+    VV_PLAUNCH(P_JAC32, 8.0 * (double)c->N * c->B, -1, launch_jacobian32(c, u, nullptr, y), "jacobian32");
     return VV_OK;
 }
 
@@ -755,17 +1167,68 @@ int vv_precond_apply32(vv_ctx *c, const float *x, float *y)
     return precond32(c, x, y, nullptr);
 }
 
+int vv_precond_apply_t(vv_ctx *c, const double *x, double *y)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!x || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
+        return fail(c, VV_ERR_UNSUPPORTED, "the transposed preconditioner needs tensor_kinetic");
+    return precond_t(c, x, y, nullptr);
+}
+
+int vv_linear_apply(vv_ctx *c, const double *u, const double *fbar, double *y, int32_t which)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (u == y || (fbar && (fbar == y))) return fail(c, VV_ERR_INVALID, "linear_apply: y must not alias u or fbar");
+    const double it = c->ht_inv;
+    switch (which) {
+    case VV_LIN_A: return linear_apply(c, u, nullptr, fbar, y, it, 0.5, 0.5, false);
+    case VV_LIN_A0: return linear_apply(c, u, nullptr, fbar, y, -it, 0.5, 0.5, false);
+    case VV_LIN_AT: return linear_apply(c, u, nullptr, fbar, y, it, -0.5, 0.5, true);
+    case VV_LIN_A0T: return linear_apply(c, u, nullptr, fbar, y, -it, -0.5, 0.5, true);
+    default: return fail(c, VV_ERR_INVALID, "which must be VV_LIN_A, _A0, _AT or _A0T");
+    }
+}
+
+int vv_adjoint_solve(vv_ctx *c, const double *lam, const double *fbar, double *mu, const vv_ksp_params *kp,
+                     vv_ksp_result *res)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!lam || !mu || !kp) return fail(c, VV_ERR_INVALID, "null argument");
+    if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
+    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
+        return fail(c, VV_ERR_UNSUPPORTED, "the adjoint solve needs the tensor_kinetic preconditioner");
+    if (c->use_graph) return fail(c, VV_ERR_UNSUPPORTED, "switch graph replay off for the adjoint solve");
+    if (int r = ensure_linear(c)) return r;
+    if (int r = precond_t(c, lam, c->rhs, nullptr)) return r;
+    c->op_mode = 1;
+    c->op_fbar = fbar;
+    const int r = ksp_solve_impl(c, c->rhs, mu, kp, res);
+    c->op_mode = 0;
+    c->op_fbar = nullptr;
+    return r;
+}
+
+int vv_adjoint_step(vv_ctx *c, const double *lam1, const double *fbar, double *lam0, double *mu,
+                    const vv_ksp_params *kp, vv_ksp_result *res)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!lam1 || !lam0 || !mu || !kp) return fail(c, VV_ERR_INVALID, "null argument");
+    if (lam0 == lam1 || lam0 == mu || mu == lam1 || (fbar && (fbar == lam0 || fbar == mu)))
+        return fail(c, VV_ERR_INVALID, "adjoint_step: lam1, lam0, mu and fbar must be distinct");
+    if (int r = vv_adjoint_solve(c, lam1, fbar, mu, kp, res)) return r;
+    // lam0 = -A0^T mu = mu/dt + 1/2 {mu~, H} - 1/2 h1[{mu~, f_bar}]
+    return linear_apply(c, mu, nullptr, fbar, lam0, c->ht_inv, 0.5, -0.5, true);
+}
+
 int vv_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1, const double *h2,
                   const double *phi, double *out)
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !h0 || !h1 || !h2 || !phi || !out) return fail(c, VV_ERR_INVALID, "null argument");
-    VV_CHECK(launch_invariants(c, f, h0, h1, h2, phi, c->dscalar + 2), "invariants");
-    VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
-    double tmp[6];
-    VV_CHECK(hipMemcpy(tmp, c->dscalar + 2, 6 * sizeof(double), hipMemcpyDeviceToHost), "copy");
-    for (int q = 0; q < 6; ++q) out[q] = tmp[q];
-    return VV_OK;
+    VV_CHECK(launch_invariants(c, f, h0, h1, h2, phi, ds_inv(c)), "invariants");
+    return read_scalars(c, ds_inv(c), 6 * (size_t)c->B, out);
 }
 
 int vv_profile_enable(vv_ctx *c, int enable)

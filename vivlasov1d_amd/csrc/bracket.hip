@@ -28,27 +28,43 @@ constexpr int HY = TY + 4;
 constexpr int NT = 256;
 constexpr int ROWS_PER_WAVE = TY / (NT / 64);
 
-enum { MODE_RESIDUAL = 0, MODE_JACOBIAN = 1, MODE_ACCUMULATE = 2 };
+// MODE_TRANSPOSED: y = alpha u + factor {u~, H} on every row, u~ = u with the
+// rows outside 2..nv-3 zeroed.  The Arakawa stencils are skew-symmetric in
+// their first argument ({f, H} summed against g equals -{g, H} summed
+// against f for fields that vanish outside the grid), so for the masked
+// operator M u = R {u, H} (R: keep rows 2..nv-3) the transpose is
+// M^T g = -{R g, H} evaluated on all rows: this mode with factor = -1.
+enum { MODE_RESIDUAL = 0, MODE_JACOBIAN = 1, MODE_ACCUMULATE = 2, MODE_TRANSPOSED = 3 };
 
 // T is the field type: double (bit-exact FP64 path) or float (Jacobian-apply
 // of the FP32 GMRES cycle of the mixed-precision solver; H_ave is composed in
 // FP64 and rounded once, the stencil runs in float).
+// Instance b of a batched context is blockIdx.y: its fields start at b * N,
+// its x-factors of H at b * nx, its GMRES state b * gstride bytes after
+// instance 0's (uscale, done), its hx-dependent factors in inst[b].
 template <typename T>
 struct BrArgs {
     int nx, nv;
+    int64_t N;
     const T *f;            // residual: f; jacobian: u; accumulate: x
     const T *fh;           // residual: f at the previous time level
     const double *uscale;  // jacobian: optional device scale of u (unnormalised Krylov vector)
     T *y;
     const double *h0, *h1p, *h1h, *h2p, *h2h;
-    double ht_inv, af1, af2, factor;
+    const double *h2d;     // optional second argument as a 2D field (instead of H_ave)
+    const double *hxv;     // optional second argument that depends on x only
+    const InstConst *inst;
+    double ht_inv, factor, alpha;
     const int32_t *done;   // optional early-exit flag (GMRES cycle finished)
+    int64_t gstride;
 };
 
 template <int KIND, int MODE, typename T>
 __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 {
-    if (a.done != nullptr && *a.done) return;
+    const int b = blockIdx.y;
+    if (a.done != nullptr && *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(a.done) + b * a.gstride))
+        return;
     __shared__ T sF[HY][HX];
     __shared__ T sH[HY][HX];
     __shared__ double sX[4][HX];      // h1p, h1h, h2p, h2h of the tile's columns
@@ -61,7 +77,13 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const int t = xcd_tile(blockIdx.x, ntx * nty);
     const int i0 = (t % ntx) * TX;
     const int j0 = (t / ntx) * TY;
-    const T us = (MODE == MODE_JACOBIAN && a.uscale != nullptr) ? (T)*a.uscale : T(1);
+    const size_t fo = (size_t)b * a.N, xo = (size_t)b * nx;
+    const T *__restrict__ fin = a.f + fo;
+    const T *__restrict__ fhin = (MODE == MODE_RESIDUAL) ? a.fh + fo : nullptr;
+    T *__restrict__ yout = a.y + fo;
+    const T us = ((MODE == MODE_JACOBIAN || MODE == MODE_TRANSPOSED) && a.uscale != nullptr)
+                     ? (T)*reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.uscale) + b * a.gstride)
+                     : T(1);
     const T half = T(0.5);
 
     // stage the 1D Hamiltonian factors of the tile (H_ave never touches HBM as a 2D field)
@@ -70,10 +92,10 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
         int gi = (i0 - 2 + c) % nx;
         if (gi < 0) gi += nx;
         sI[c] = gi;
-        sX[0][c] = a.h1p[gi];
-        sX[1][c] = a.h1h[gi];
-        sX[2][c] = a.h2p[gi];
-        sX[3][c] = a.h2h[gi];
+        sX[0][c] = a.h1p[xo + gi];
+        sX[1][c] = a.h1h[xo + gi];
+        sX[2][c] = a.h2p[xo + gi];
+        sX[3][c] = a.h2h[xo + gi];
     } else if (threadIdx.x < HX + HY) {
         const int r = threadIdx.x - HX;
         const int gj = j0 - 2 + r;
@@ -106,11 +128,13 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             if (gj >= 0 && gj < nv) {
                 const size_t o = (size_t)gj * nx + sI[c];
                 if (MODE == MODE_RESIDUAL) {
-                    fv[q] = half * a.fh[o] + half * a.f[o];      // Fave (PETScNLVlasovMP.pyx:44-46)
+                    fv[q] = half * fhin[o] + half * fin[o];      // Fave (PETScNLVlasovMP.pyx:44-46)
+                } else if (MODE == MODE_TRANSPOSED) {
+                    fv[q] = (gj >= 2 && gj < nv - 2) ? fin[o] * us : T(0);
                 } else if (MODE == MODE_JACOBIAN) {
-                    fv[q] = a.f[o] * us;                         // == VecScale'd Krylov vector
+                    fv[q] = fin[o] * us;                         // == VecScale'd Krylov vector
                 } else {
-                    fv[q] = a.f[o];
+                    fv[q] = fin[o];
                 }
             }
         }
@@ -124,7 +148,11 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             const int gj = j0 - 2 + r;
             sF[r][c] = fv[q];
             // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i)  (PETScVlasovSolver.pyx:145-149)
-            if (sizeof(T) == 8) {
+            if (a.h2d != nullptr) {
+                sH[r][c] = (gj >= 0 && gj < nv) ? (T)a.h2d[fo + (size_t)gj * nx + sI[c]] : T(0);
+            } else if (a.hxv != nullptr) {
+                sH[r][c] = (gj >= 0 && gj < nv) ? (T)a.hxv[xo + sI[c]] : T(0);
+            } else if (sizeof(T) == 8) {
                 sH[r][c] = (gj >= 0 && gj < nv)
                                ? (T)((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
                                : T(0);
@@ -145,7 +173,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const int i = i0 + lane;
     if (i >= nx) return;
     const int cx = lane + 2;
-    const T ht_inv = (T)a.ht_inv, af1 = (T)a.af1, af2 = (T)a.af2, factor = (T)a.factor;
+    const T ht_inv = (T)a.ht_inv, af1 = (T)a.inst[b].af1, af2 = (T)a.inst[b].af2, factor = (T)a.factor;
 
 #define F(di, dj) sF[cy + (dj)][cx + (di)]
 #define H(di, dj) sH[cy + (dj)][cx + (di)]
@@ -158,15 +186,17 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 
         T base;
         if (MODE == MODE_RESIDUAL) {
-            const T fder = a.f[o] - a.fh[o];                 // Fder = (0 + F) - Fh
+            const T fder = fin[o] - fhin[o];                 // Fder = (0 + F) - Fh
             base = T(0) + ht_inv * fder;                     // Y = 0; Y += Fder/dt
         } else if (MODE == MODE_JACOBIAN) {
             base = T(0) + ht_inv * F(0, 0);                  // Y = 0; Y += u/dt
+        } else if (MODE == MODE_TRANSPOSED) {
+            base = (T)a.alpha * (fin[o] * us);
         } else {
-            base = a.y[o];
+            base = yout[o];
         }
         T out = base;
-        if (j >= 2 && j < nv - 2) {
+        if (MODE == MODE_TRANSPOSED || (j >= 2 && j < nv - 2)) {
             if (KIND == VV_BRACKET_J1 || KIND == VV_BRACKET_J4) {
                 // PoissonBracket.pyx:100-111 (J1 / second order)
                 const T jpp = (F(1, 0) - F(-1, 0)) * (H(0, 1) - H(0, -1))
@@ -212,7 +242,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
                 out = base + factor * (jcc + jpc + jcp) * af2;
             }
         }
-        a.y[o] = out;
+        yout[o] = out;
     }
 #undef F
 #undef H
@@ -223,7 +253,7 @@ hipError_t launch_mode(vv_ctx *c, BrArgs<T> a)
 {
     const int ntx = (c->p.nx + TX - 1) / TX;
     const int nty = (c->p.nv + TY - 1) / TY;
-    dim3 grid(ntx * nty), block(NT);
+    dim3 grid(ntx * nty, c->B), block(NT);
     switch (c->p.bracket) {
     case VV_BRACKET_J1:
         launch_timed(c, bracket_kernel<VV_BRACKET_J1, MODE, T>, grid, block, 0, a);
@@ -244,14 +274,15 @@ BrArgs<T> base_args(vv_ctx *c)
     BrArgs<T> a{};
     a.nx = c->p.nx;
     a.nv = c->p.nv;
+    a.N = c->N;
+    a.inst = c->inst;
+    a.gstride = (int64_t)sizeof(GmresState);
     a.h0 = c->h0;
     a.h1p = c->h1p;
     a.h1h = c->h1h;
     a.h2p = c->h2p;
     a.h2h = c->h2h;
     a.ht_inv = c->ht_inv;
-    a.af1 = c->af1;
-    a.af2 = c->af2;
     return a;
 }
 
@@ -282,6 +313,33 @@ hipError_t launch_bracket_add(vv_ctx *c, const double *x, double *y, double fact
 {
     BrArgs<double> a = base_args<double>(c);
     a.f = x;
+    a.y = y;
+    a.factor = factor;
+    return launch_mode<MODE_ACCUMULATE>(c, a);
+}
+
+hipError_t launch_bracket_t(vv_ctx *c, const double *u, const double *uscale, const double *h2d, const double *hxv,
+                            double *y, double alpha, double factor)
+{
+    BrArgs<double> a = base_args<double>(c);
+    a.f = u;
+    a.uscale = uscale;
+    a.h2d = h2d;
+    a.hxv = hxv;
+    a.y = y;
+    a.alpha = alpha;
+    a.factor = factor;
+    a.done = uscale != nullptr ? &c->gs->st.done : nullptr;
+    return launch_mode<MODE_TRANSPOSED>(c, a);
+}
+
+hipError_t launch_bracket_gen(vv_ctx *c, const double *x, const double *h2d, const double *hxv, double *y,
+                              double factor)
+{
+    BrArgs<double> a = base_args<double>(c);
+    a.f = x;
+    a.h2d = h2d;
+    a.hxv = hxv;
     a.y = y;
     a.factor = factor;
     return launch_mode<MODE_ACCUMULATE>(c, a);

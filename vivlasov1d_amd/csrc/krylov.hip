@@ -98,13 +98,18 @@ __device__ __forceinline__ int ksp_test(const GmresState *g, double res)
     return 0;
 }
 
-__global__ void gmres_init_kernel(GmresState *g, double rtol, double atol, double dtol, int max_it, int max_k)
+// Instance b of a batched context is blockIdx.y in every kernel below.
+__global__ void gmres_init_kernel(GmresState *g, const int32_t *active, double rtol, double atol, double dtol,
+                                  int max_it, int max_k)
 {
     if (threadIdx.x != 0) return;
+    g += blockIdx.y;
+    const int32_t act = active != nullptr ? active[blockIdx.y] : 1;
     g->st.it = 0;
     g->st.its = 0;
     g->st.reason = 0;
-    g->st.done = 0;
+    g->st.done = act ? 0 : 1;
+    g->st.active = act;
     g->st.res = 0.0;
     g->st.rnorm0 = 0.0;
     g->rtol = rtol;
@@ -120,9 +125,13 @@ __global__ void gmres_init_kernel(GmresState *g, double rtol, double atol, doubl
 // Cross-block sums are finished by a one-block kernel: the kernel boundary
 // makes the partials visible without device-scope fences.
 __global__ __launch_bounds__(NT) void dot_partial_kernel(const double *__restrict__ x, const double *__restrict__ y,
-                                                         int64_t len, double *partial)
+                                                         int64_t len, int64_t xstride, double *partial,
+                                                         int64_t pstride)
 {
     __shared__ double red[NW + 1];
+    x += blockIdx.y * xstride;
+    y += blockIdx.y * xstride;
+    partial += blockIdx.y * pstride;
     double acc = 0.0;
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT)
         acc += x[e] * y[e];
@@ -132,10 +141,13 @@ __global__ __launch_bounds__(NT) void dot_partial_kernel(const double *__restric
 
 // mode 0: *out = sqrt(sum) (VecNorm); mode 2: *out = sum (VecDot);
 // mode 1: GMRES cycle start on V_0 with sum = ||V_0||^2.
-__global__ __launch_bounds__(NT) void reduce_finish_kernel(const double *partial, int nb, GmresState *g, double *out,
-                                                           int mode)
+__global__ __launch_bounds__(NT) void reduce_finish_kernel(const double *partial, int nb, int64_t pstride,
+                                                           GmresState *g, double *out, int mode)
 {
     __shared__ double red[NW + 1];
+    partial += blockIdx.y * pstride;
+    g += blockIdx.y;
+    if (out != nullptr) out += blockIdx.y;
     const double tot = final_sum_1(partial, nb, red);
     if (threadIdx.x != 0) return;
     if (mode == 2) {
@@ -145,6 +157,12 @@ __global__ __launch_bounds__(NT) void reduce_finish_kernel(const double *partial
     const double nrm = sqrt(tot);
     if (mode == 0) {
         *out = nrm;
+        return;
+    }
+    // an instance that is not solving, or whose solve has ended, stays done
+    if (!g->st.active || g->st.reason != 0) {
+        g->st.it = 0;
+        g->st.done = 1;
         return;
     }
     // KSPGMRESCycle entry: VecNormalize(VV(0)), zero-residual and convergence tests
@@ -200,22 +218,27 @@ __device__ __forceinline__ void vset(float4 &v, int l, double x)
     else v.w = f;
 }
 
+// The basis is vector-major across instances: V_j of instance b at
+// basis + j * vstride + b * N (vstride = B N), so V_j of all instances is one
+// contiguous field and the operator kernels see instance stride N.
 template <typename S>
 struct CgsArgs {
     const S *basis;
     int64_t N;
+    int64_t vstride;
     double *part;
+    int64_t pstride;
     GmresState *g;
 };
 
 template <int EPT, int G, typename S>
-__device__ __forceinline__ void load_group(const S *basis, int64_t N, int v0, int64_t e0, int64_t nvv,
+__device__ __forceinline__ void load_group(const S *basis, int64_t vstride, int v0, int64_t e0, int64_t nvv,
                                            typename V16<S>::type (&x)[G][EPT])
 {
     using V = typename V16<S>::type;
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
-        const V *Vp = reinterpret_cast<const V *>(basis + (size_t)(v0 + gg) * N);
+        const V *Vp = reinterpret_cast<const V *>(basis + (size_t)(v0 + gg) * vstride);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int64_t e = e0 + (int64_t)q * NT;
@@ -231,7 +254,9 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 {
     using V = typename V16<S>::type;
     constexpr int L = V16<S>::L;
-    GmresState *g = a.g;
+    GmresState *g = a.g + blockIdx.y;
+    a.basis += blockIdx.y * a.N;
+    a.part += blockIdx.y * a.pstride;
     if (g->st.done) return;
     __shared__ double red[NW][PV];
     __shared__ double s_scale[PV];
@@ -240,7 +265,7 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
     __syncthreads();
     const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
-    const V *W = reinterpret_cast<const V *>(a.basis + (size_t)nvec * a.N);
+    const V *W = reinterpret_cast<const V *>(a.basis + (size_t)nvec * a.vstride);
     V w[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
@@ -251,7 +276,7 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
     int v = 0;
     for (; v + 2 <= nvec; v += 2) {
         V x[2][EPT];
-        load_group<EPT, 2, S>(a.basis, a.N, v, e0, nvv, x);
+        load_group<EPT, 2, S>(a.basis, a.vstride, v, e0, nvv, x);
 #pragma unroll
         for (int gg = 0; gg < 2; ++gg) {
             const double sv = s_scale[v + gg];
@@ -267,7 +292,7 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
     }
     if (v < nvec) {
         V x[1][EPT];
-        load_group<EPT, 1, S>(a.basis, a.N, v, e0, nvv, x);
+        load_group<EPT, 1, S>(a.basis, a.vstride, v, e0, nvv, x);
         const double sv = s_scale[v];
         double acc = 0.0;
 #pragma unroll
@@ -290,8 +315,11 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 // Column sums of the mdot partials: HH(:, col) = <w, v_v> and the VecMAXPY
 // coefficients lhh = -HH(:, col).
 template <int PV>
-__global__ __launch_bounds__(NT) void mdot_finish_kernel(const double *part, int nb, int col, GmresState *g)
+__global__ __launch_bounds__(NT) void mdot_finish_kernel(const double *part, int nb, int64_t pstride, int col,
+                                                         GmresState *g)
 {
+    g += blockIdx.y;
+    part += blockIdx.y * pstride;
     if (g->st.done) return;
     __shared__ double s_red[NT];
     __shared__ double s_h[PV];
@@ -310,7 +338,9 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 {
     using V = typename V16<S>::type;
     constexpr int L = V16<S>::L;
-    GmresState *g = a.g;
+    GmresState *g = a.g + blockIdx.y;
+    a.basis += blockIdx.y * a.N;
+    a.part += blockIdx.y * a.pstride;
     if (g->st.done) return;
     __shared__ double red[NW + 1];
     __shared__ double s_coef[kMaxRestart + 1], s_scale[kMaxRestart + 1];
@@ -322,7 +352,7 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
     __syncthreads();
     const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
-    V *W = reinterpret_cast<V *>(const_cast<S *>(a.basis) + (size_t)nvec * a.N);
+    V *W = reinterpret_cast<V *>(const_cast<S *>(a.basis) + (size_t)nvec * a.vstride);
     double w[EPT][L];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
@@ -334,7 +364,7 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
     int j = 0;
     for (; j + 2 <= nvec; j += 2) {
         V x[2][EPT];
-        load_group<EPT, 2, S>(a.basis, a.N, j, e0, nvv, x);
+        load_group<EPT, 2, S>(a.basis, a.vstride, j, e0, nvv, x);
 #pragma unroll
         for (int gg = 0; gg < 2; ++gg) {
             const double sj = s_scale[j + gg], cj = s_coef[j + gg];
@@ -347,7 +377,7 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
     }
     if (j < nvec) {
         V x[1][EPT];
-        load_group<EPT, 1, S>(a.basis, a.N, j, e0, nvv, x);
+        load_group<EPT, 1, S>(a.basis, a.vstride, j, e0, nvv, x);
         const double sj = s_scale[j], cj = s_coef[j];
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
@@ -378,8 +408,11 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 // VecNormalize of V_{it+1} from the maxpy partials, then
 // KSPGMRESUpdateHessenberg (rotations, happy breakdown, residual estimate)
 // and the convergence test; sets done when the cycle ends.
-__global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, int nb, int it, GmresState *g)
+__global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, int nb, int64_t pstride, int it,
+                                                          GmresState *g)
 {
+    g += blockIdx.y;
+    part += blockIdx.y * pstride;
     if (g->st.done) return;
     __shared__ double red[NW + 1];
     const double tot = final_sum_1(part, nb, red);
@@ -421,6 +454,7 @@ __global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, in
     g->st.res = res;
     if (reason == 0) reason = ksp_test(g, res);
     if (hapend && reason == 0) reason = VV_KSP_DIVERGED_BREAKDOWN;
+    if (reason == 0 && g->st.its >= g->max_it) reason = VV_KSP_DIVERGED_ITS;
     g->st.reason = reason;
     if (reason != 0 || g->st.it >= g->max_k || g->st.its >= g->max_it) g->st.done = 1;
 }
@@ -429,6 +463,7 @@ __global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, in
 // the coefficients go to coef[0..it).  The triangle is staged in LDS first.
 __global__ __launch_bounds__(NT) void backsolve_kernel(GmresState *g)
 {
+    g += blockIdx.y;
     __shared__ double sh[kMaxRestart][kMaxRestart];
     __shared__ double sg[kMaxRestart];
     const int n = g->st.it;
@@ -469,7 +504,9 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
 {
     using V = typename V16<S>::type;
     constexpr int L = V16<S>::L;
-    const GmresState *g = a.g;
+    const GmresState *g = a.g + blockIdx.y;
+    a.basis += blockIdx.y * a.N;
+    x += blockIdx.y * a.N;
     const int n = g->st.it;
     if (n <= 0) return;
     __shared__ double s_coef[kMaxRestart + 1], s_scale[kMaxRestart + 1];
@@ -488,7 +525,7 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
     int j = 0;
     for (; j + 2 <= n; j += 2) {
         V v[2][EPT];
-        load_group<EPT, 2, S>(a.basis, a.N, j, e0, nvv, v);
+        load_group<EPT, 2, S>(a.basis, a.vstride, j, e0, nvv, v);
 #pragma unroll
         for (int gg = 0; gg < 2; ++gg) {
             const double sj = s_scale[j + gg], cj = s_coef[j + gg];
@@ -501,7 +538,7 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
     }
     if (j < n) {
         V v[1][EPT];
-        load_group<EPT, 1, S>(a.basis, a.N, j, e0, nvv, v);
+        load_group<EPT, 1, S>(a.basis, a.vstride, j, e0, nvv, v);
         const double sj = s_scale[j], cj = s_coef[j];
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
@@ -528,9 +565,13 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
 // Cycle start of the FP32 GMRES cycle: V_0 = (float) r and per-block partial
 // sums of the squares of the rounded values.
 __global__ __launch_bounds__(NT) void cvt_partial_kernel(const double *__restrict__ r, float *__restrict__ v0,
-                                                         int64_t len, double *partial)
+                                                         int64_t len, int64_t bstride, double *partial,
+                                                         int64_t pstride)
 {
     __shared__ double red[NW + 1];
+    r += blockIdx.y * len;
+    v0 += blockIdx.y * bstride;
+    partial += blockIdx.y * pstride;
     double acc = 0.0;
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT) {
         const float f = (float)r[e];
@@ -546,14 +587,20 @@ __global__ __launch_bounds__(NT) void cvt_partial_kernel(const double *__restric
 __global__ __launch_bounds__(NT) void axpy_kernel(double alpha, const double *__restrict__ x, double *y, int64_t len,
                                                   const GmresState *g, int mode)
 {
-    if (mode == 1 && g->st.reason <= 0) return;
+    x += blockIdx.y * len;
+    y += blockIdx.y * len;
+    if (mode == 1 && g[blockIdx.y].st.reason <= 0) return;
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT)
         y[e] = y[e] + alpha * x[e];
 }
 
-// y = b - y   (KSPInitialResidual after a restart: r = b - A x)
-__global__ __launch_bounds__(NT) void rsub_kernel(const double *__restrict__ b, double *y, int64_t len)
+// y = b - y   (KSPInitialResidual after a restart: r = b - A x); instance
+// blockIdx.y at b + blockIdx.y len, y + blockIdx.y ystride
+__global__ __launch_bounds__(NT) void rsub_kernel(const double *__restrict__ b, double *y, int64_t len,
+                                                  int64_t ystride)
 {
+    b += blockIdx.y * len;
+    y += blockIdx.y * ystride;
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT)
         y[e] = b[e] + (-1.0) * y[e];
 }
@@ -563,44 +610,64 @@ __global__ __launch_bounds__(NT) void copy_kernel(const double *__restrict__ x, 
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT) y[e] = x[e];
 }
 
+// y[b][j][i] += alpha w[b][i] (an x-only field added to every v-row)
+__global__ __launch_bounds__(NT) void xadd_kernel(double *__restrict__ y, const double *__restrict__ w, int nx,
+                                                  int64_t N, double alpha)
+{
+    y += blockIdx.y * N;
+    w += (size_t)blockIdx.y * nx;
+    for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < N; e += (int64_t)gridDim.x * NT)
+        y[e] = y[e] + alpha * w[e & (nx - 1)];
+}
+
+// y = 0 (a kernel rather than a memset node, so captured graphs replay it)
+__global__ __launch_bounds__(NT) void zero_kernel(double *__restrict__ y, int64_t len)
+{
+    for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT) y[e] = 0.0;
+}
+
 int grid_for(int64_t len)
 {
     const int64_t want = (len + NT - 1) / NT;
     return (int)(want < kReduceBlocks ? (want > 0 ? want : 1) : kReduceBlocks);
 }
 
-// 16-byte vectors per thread of the CGS kernels: the largest of 8,4,2,1 that
-// still gives >= 256 workgroups (nvv: 16-byte vectors per basis vector)
-int cgs_ept_v(int64_t nvv)
+// 16-byte vectors per thread of the CGS kernels: the context's tuned value,
+// else the largest of 8,4,2,1 that still gives >= 256 workgroups (nvv:
+// 16-byte vectors per basis vector)
+int cgs_ept_v(const vv_ctx *c, int64_t nvv)
 {
+    if (c != nullptr && c->tune_cgs_ept > 0) return c->tune_cgs_ept;
     for (int ept = 8; ept > 1; ept >>= 1)
         if ((nvv + (int64_t)NT * ept - 1) / ((int64_t)NT * ept) >= 256) return ept;
     return 1;
 }
 
-int cgs_blocks_v(int64_t nvv)
+int cgs_blocks_v(const vv_ctx *c, int64_t nvv)
 {
-    const int ept = cgs_ept_v(nvv);
+    const int ept = cgs_ept_v(c, nvv);
     return (int)((nvv + (int64_t)NT * ept - 1) / ((int64_t)NT * ept));
 }
 
 }  // namespace
 
-int cgs_blocks(int64_t N) { return cgs_blocks_v(N >> 1); }
+// partial-sum rows the CGS kernels can need (one vector per thread)
+int cgs_blocks_max(int64_t N) { return (int)(((N >> 1) + NT - 1) / NT); }
 
 hipError_t launch_gmres_init(vv_ctx *c, double rtol, double atol, double dtol, int max_it)
 {
-    hipLaunchKernelGGL(gmres_init_kernel, dim3(1), dim3(64), 0, c->stream, c->gs, rtol, atol, dtol, max_it,
-                       c->p.restart);
+    hipLaunchKernelGGL(gmres_init_kernel, dim3(1, c->B), dim3(64), 0, c->stream, c->gs, c->active, rtol, atol, dtol,
+                       max_it, c->p.restart);
     return hipGetLastError();
 }
 
 hipError_t launch_norm_start(vv_ctx *c)
 {
     const int nb = grid_for(c->N);
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(NT), 0, c->stream, c->basis, c->basis, c->N, c->partial);
-    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, nb, c->gs,
-                       (double *)nullptr, 1);
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb, c->B), dim3(NT), 0, c->stream, c->basis, c->basis, c->N, c->N,
+                       c->partial, c->part_stride);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial, nb, c->part_stride,
+                       c->gs, (double *)nullptr, 1);
     return hipGetLastError();
 }
 
@@ -609,8 +676,8 @@ namespace {
 template <int E, typename S>
 void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 {
-    CgsArgs<S> a{basis, c->N, c->partial, c->gs};
-    const dim3 grid(cgs_blocks_v(c->N / V16<S>::L)), block(NT);
+    CgsArgs<S> a{basis, c->N, c->N * c->B, c->partial, c->part_stride, c->gs};
+    const dim3 grid(cgs_blocks_v(c, c->N / V16<S>::L), c->B), block(NT);
     if (which == 0) {
         if (c->p.restart < 32) launch_timed(c, mdot_kernel<E, 32, S>, grid, block, 0, a, arg);
         else launch_timed(c, mdot_kernel<E, 64, S>, grid, block, 0, a, arg);
@@ -624,7 +691,7 @@ void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 template <typename S>
 hipError_t cgs_dispatch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 {
-    switch (cgs_ept_v(c->N / V16<S>::L)) {
+    switch (cgs_ept_v(c, c->N / V16<S>::L)) {
     case 8: cgs_launch<8>(c, basis, which, arg, x); break;
     case 4: cgs_launch<4>(c, basis, which, arg, x); break;
     case 2: cgs_launch<2>(c, basis, which, arg, x); break;
@@ -638,11 +705,13 @@ hipError_t mdot_t(vv_ctx *c, const S *basis, int col)
 {
     hipError_t e = cgs_dispatch(c, basis, 0, col, nullptr);
     if (e != hipSuccess) return e;
-    const int nb = cgs_blocks_v(c->N / V16<S>::L);
+    const int nb = cgs_blocks_v(c, c->N / V16<S>::L);
     if (c->p.restart < 32)
-        hipLaunchKernelGGL((mdot_finish_kernel<32>), dim3(1), dim3(NT), 0, c->stream, c->partial, nb, col, c->gs);
+        hipLaunchKernelGGL((mdot_finish_kernel<32>), dim3(1, c->B), dim3(NT), 0, c->stream, c->partial, nb,
+                           c->part_stride, col, c->gs);
     else
-        hipLaunchKernelGGL((mdot_finish_kernel<64>), dim3(1), dim3(NT), 0, c->stream, c->partial, nb, col, c->gs);
+        hipLaunchKernelGGL((mdot_finish_kernel<64>), dim3(1, c->B), dim3(NT), 0, c->stream, c->partial, nb,
+                           c->part_stride, col, c->gs);
     return hipGetLastError();
 }
 
@@ -651,15 +720,15 @@ hipError_t maxpy_norm_t(vv_ctx *c, const S *basis, int it)
 {
     hipError_t e = cgs_dispatch(c, basis, 1, it, nullptr);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(gmres_update_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial,
-                       cgs_blocks_v(c->N / V16<S>::L), it, c->gs);
+    hipLaunchKernelGGL(gmres_update_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial,
+                       cgs_blocks_v(c, c->N / V16<S>::L), c->part_stride, it, c->gs);
     return hipGetLastError();
 }
 
 template <typename S>
 hipError_t build_solution_t(vv_ctx *c, const S *basis, double *x)
 {
-    hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(NT), 0, c->stream, c->gs);
+    hipLaunchKernelGGL(backsolve_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->gs);
     return cgs_dispatch(c, basis, 2, 0, x);
 }
 
@@ -675,31 +744,46 @@ hipError_t launch_build_solution32(vv_ctx *c, double *x) { return build_solution
 hipError_t launch_start32(vv_ctx *c, const double *r)
 {
     const int nb = grid_for(c->N);
-    hipLaunchKernelGGL(cvt_partial_kernel, dim3(nb), dim3(NT), 0, c->stream, r, c->basis32, c->N, c->partial);
-    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, nb, c->gs,
-                       (double *)nullptr, 1);
+    hipLaunchKernelGGL(cvt_partial_kernel, dim3(nb, c->B), dim3(NT), 0, c->stream, r, c->basis32, c->N, c->N,
+                       c->partial, c->part_stride);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial, nb, c->part_stride,
+                       c->gs, (double *)nullptr, 1);
     return hipGetLastError();
 }
 
 hipError_t launch_residual_restart(vv_ctx *c, const double *b, double *y)
 {
-    hipLaunchKernelGGL(rsub_kernel, dim3(grid_for(c->N)), dim3(NT), 0, c->stream, b, y, c->N);
+    hipLaunchKernelGGL(rsub_kernel, dim3(grid_for(c->N), c->B), dim3(NT), 0, c->stream, b, y, c->N, c->N);
+    return hipGetLastError();
+}
+
+hipError_t launch_norms(vv_ctx *c, const double *x, double *out_dev)
+{
+    const int nb = grid_for(c->N);
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb, c->B), dim3(NT), 0, c->stream, x, x, c->N, c->N, c->partial,
+                       c->part_stride);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial, nb, c->part_stride,
+                       c->gs, out_dev, 0);
     return hipGetLastError();
 }
 
 hipError_t launch_norm2(vv_ctx *c, const double *x, int64_t n, double *out_dev)
 {
     const int nb = grid_for(n);
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(NT), 0, c->stream, x, x, n, c->partial);
-    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, nb, c->gs, out_dev, 0);
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(NT), 0, c->stream, x, x, n, (int64_t)0, c->partial,
+                       c->part_stride);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, nb, c->part_stride, c->gs,
+                       out_dev, 0);
     return hipGetLastError();
 }
 
 hipError_t launch_dot(vv_ctx *c, const double *x, const double *y, int64_t n, double *out_dev)
 {
     const int nb = grid_for(n);
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(NT), 0, c->stream, x, y, n, c->partial);
-    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, nb, c->gs, out_dev, 2);
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(NT), 0, c->stream, x, y, n, (int64_t)0, c->partial,
+                       c->part_stride);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, c->stream, c->partial, nb, c->part_stride, c->gs,
+                       out_dev, 2);
     return hipGetLastError();
 }
 
@@ -709,15 +793,27 @@ hipError_t launch_axpy(vv_ctx *c, double alpha, const double *x, double *y, int6
     return hipGetLastError();
 }
 
-hipError_t launch_axpy_cond(vv_ctx *c, double alpha, const double *x, double *y, int64_t n)
+hipError_t launch_axpy_cond(vv_ctx *c, double alpha, const double *x, double *y)
 {
-    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n)), dim3(NT), 0, c->stream, alpha, x, y, n, c->gs, 1);
+    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(c->N), c->B), dim3(NT), 0, c->stream, alpha, x, y, c->N, c->gs, 1);
     return hipGetLastError();
 }
 
 hipError_t launch_copy(vv_ctx *c, const double *x, double *y, int64_t n)
 {
     hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(NT), 0, c->stream, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_xadd(vv_ctx *c, double *y, const double *w, double alpha)
+{
+    hipLaunchKernelGGL(xadd_kernel, dim3(grid_for(c->N), c->B), dim3(NT), 0, c->stream, y, w, c->p.nx, c->N, alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(vv_ctx *c, double *y, int64_t n)
+{
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n)), dim3(NT), 0, c->stream, y, n);
     return hipGetLastError();
 }
 

@@ -14,6 +14,9 @@
 //  invariants_kernel mass, L2, kinetic / potential energy over f, plus the
 //                    field energy over phi (distribution_function.pyx:130-199,
 //                    hamiltonian.pyx:67-100, potential.pyx:48-59).
+//
+// In a batched context instance b is blockIdx.y (blockIdx.z for the density
+// partials); its fields start at b * N, its x-vectors at b * nx.
 #include "vv1d_internal.h"
 
 namespace vv {
@@ -26,6 +29,8 @@ constexpr int DCH = 64;      // row chunks of the density reduction
 __global__ __launch_bounds__(NT) void density_kernel(const double *__restrict__ f, int nx, int nv,
                                                      double *__restrict__ npart)
 {
+    f += (size_t)blockIdx.z * nx * nv;
+    npart += (size_t)blockIdx.z * DCH * nx;
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= nx) return;
     const int rows = (nv + DCH - 1) / DCH;
@@ -54,7 +59,8 @@ __device__ double block_sum_big(double v, double *red)
 
 struct PoiArgs {
     int nx, laplace, from_part;
-    double hv, charge, hx2_inv;
+    double hv, charge;
+    const InstConst *inst;
     const double *npart;   // density partials (from_part = 1)
     const double *n_in;    // density (from_part = 0)
     const double *pinv;
@@ -73,10 +79,25 @@ __device__ __forceinline__ double lap(const double *x, int i, int nx, int laplac
     return (2. * x0 - xm1 - xp1) * hx2_inv;
 }
 
+// per-instance views of the Poisson buffers (pres: 16 doubles per instance)
+__device__ __forceinline__ void poi_instance(PoiArgs &a)
+{
+    const size_t b = blockIdx.y, nx = a.nx;
+    if (a.npart) a.npart += b * DCH * nx;
+    if (a.n_in) a.n_in += b * nx;
+    a.pinv += b * nx;
+    a.work += b * (3 * nx + 1);
+    if (a.n_out) a.n_out += b * nx;
+    if (a.phi) a.phi += b * nx;
+    if (a.h1) a.h1 += b * nx;
+    if (a.pres) a.pres += b * 16;
+}
+
 // one workgroup: n (from the density partials), mean n, b = -(n - mean n) q
 __global__ __launch_bounds__(PT) void poisson_prep_kernel(PoiArgs a)
 {
     __shared__ double red[PT / 64 + 1];
+    poi_instance(a);
     const int nx = a.nx;
     double *b = a.work, *wn = a.work + 2 * nx;
     double part = 0.0;
@@ -106,6 +127,7 @@ __global__ __launch_bounds__(PT) void poisson_prep_kernel(PoiArgs a)
 __global__ __launch_bounds__(NT) void poisson_conv_kernel(PoiArgs a)
 {
     extern __shared__ double sm[];
+    poi_instance(a);
     const int nx = a.nx;
     double *sb = sm, *sc = sm + nx;
     for (int i = threadIdx.x; i < nx; i += NT) {
@@ -125,6 +147,8 @@ __global__ __launch_bounds__(PT) void poisson_post_kernel(PoiArgs a)
 {
     extern __shared__ double sm[];
     __shared__ double red[PT / 64 + 1];
+    poi_instance(a);
+    const double hx2_inv = a.inst[blockIdx.y].hx2_inv;
     const int nx = a.nx;
     const double *wphi = a.work + nx, *wn = a.work + 2 * nx;
     const double nmean = a.work[3 * nx];
@@ -138,7 +162,7 @@ __global__ __launch_bounds__(PT) void poisson_post_kernel(PoiArgs a)
     for (int i = threadIdx.x; i < nx; i += PT) {
         if (a.phi) a.phi[i] = sm[i];
         if (a.h1) a.h1[i] = sm[i] - phiave;
-        const double r = lap(sm, i, nx, a.laplace, a.hx2_inv) + (wn[i] - nmean) * a.charge;
+        const double r = lap(sm, i, nx, a.laplace, hx2_inv) + (wn[i] - nmean) * a.charge;
         part += r * r;
     }
     const double rr = block_sum_big(part, red);
@@ -147,10 +171,14 @@ __global__ __launch_bounds__(PT) void poisson_post_kernel(PoiArgs a)
 
 // y = A x (+ (n - mean n) q when n != null)
 __global__ __launch_bounds__(PT) void poisson_mult_kernel(const double *x, const double *n, double *y, int nx,
-                                                          int laplace, double hx2_inv, double charge)
+                                                          int laplace, const InstConst *inst, double charge)
 {
     extern __shared__ double sm[];
     __shared__ double red[PT / 64 + 1];
+    const double hx2_inv = inst[blockIdx.y].hx2_inv;
+    x += (size_t)blockIdx.y * nx;
+    y += (size_t)blockIdx.y * nx;
+    if (n) n += (size_t)blockIdx.y * nx;
     double part = 0.0;
     for (int i = threadIdx.x; i < nx; i += PT) {
         sm[i] = x[i];
@@ -167,6 +195,8 @@ __global__ __launch_bounds__(PT) void poisson_mult_kernel(const double *x, const
 __global__ __launch_bounds__(PT) void pot_to_ham_kernel(const double *phi, double *h1, int nx)
 {
     __shared__ double red[PT / 64 + 1];
+    phi += (size_t)blockIdx.y * nx;
+    h1 += (size_t)blockIdx.y * nx;
     double part = 0.0;
     for (int i = threadIdx.x; i < nx; i += PT) part += phi[i];
     const double ave = block_sum_big(part, red) / nx;
@@ -176,6 +206,8 @@ __global__ __launch_bounds__(PT) void pot_to_ham_kernel(const double *phi, doubl
 __global__ __launch_bounds__(NT) void density_finish_kernel(const double *npart, int nch, int nx, double hv,
                                                             double *n)
 {
+    npart += (size_t)blockIdx.y * DCH * nx;
+    n += (size_t)blockIdx.y * nx;
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= nx) return;
     double s = 0.0;
@@ -186,13 +218,27 @@ __global__ __launch_bounds__(NT) void density_finish_kernel(const double *npart,
 struct InvArgs {
     const double *f, *h0, *h1, *h2, *phi;
     int nx, nv;
-    double hx, hv, charge;
-    double *partial;      // [gridDim.x][5]
-    double *out;          // 6 doubles
+    double hv, charge;
+    const InstConst *inst;
+    double *partial;      // [gridDim.x][5] per instance, pstride apart
+    int64_t pstride;
+    double *out;          // 6 doubles per instance
 };
+
+__device__ __forceinline__ void inv_instance(InvArgs &a)
+{
+    const size_t b = blockIdx.y;
+    a.f += b * a.nx * a.nv;
+    a.h1 += b * a.nx;
+    a.h2 += b * a.nx;
+    a.phi += b * a.nx;
+    a.partial += b * a.pstride;
+    a.out += b * 6;
+}
 
 __global__ __launch_bounds__(NT) void invariants_kernel(InvArgs a)
 {
+    inv_instance(a);
     __shared__ double red[5][NT / 64];
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int64_t N = (int64_t)a.nx * a.nv;
@@ -227,6 +273,8 @@ __global__ __launch_bounds__(NT) void invariants_kernel(InvArgs a)
 __global__ __launch_bounds__(NT) void invariants_finish_kernel(InvArgs a, int nb)
 {
     __shared__ double red[5][NT];
+    inv_instance(a);
+    const double hx = a.inst[blockIdx.y].hx;
     double t5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nb; b += NT)
         for (int q = 0; q < 5; ++q) t5[q] += a.partial[(size_t)b * 5 + q];
@@ -236,11 +284,11 @@ __global__ __launch_bounds__(NT) void invariants_finish_kernel(InvArgs a, int nb
     double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int b = 0; b < NT; ++b)
         for (int q = 0; q < 5; ++q) t[q] += red[q][b];
-    const double hxhv = a.hx * a.hv;
-    const double mass = t[0] * 0.25 * a.hx * a.hv;
+    const double hxhv = hx * a.hv;
+    const double mass = t[0] * 0.25 * hx * a.hv;
     const double l2 = hxhv * t[1];
-    const double ekin = t[2] * a.hx * a.hv;
-    const double epot = (t[3] + t[4]) * a.hx * a.hv;
+    const double ekin = t[2] * hx * a.hv;
+    const double epot = (t[3] + t[4]) * hx * a.hv;
     double ef = 0.0;
     for (int i = 0; i < a.nx; ++i) {
         const double d = a.phi[(i + 1) % a.nx] - a.phi[i];
@@ -251,14 +299,14 @@ __global__ __launch_bounds__(NT) void invariants_finish_kernel(InvArgs a, int nb
     a.out[2] = ekin + 0.5 * epot;
     a.out[3] = ekin;
     a.out[4] = epot;
-    a.out[5] = 0.5 * a.charge * ef / a.hx;
+    a.out[5] = 0.5 * a.charge * ef / hx;
 }
 
 }  // namespace
 
 hipError_t launch_density_partial(vv_ctx *c, const double *f)
 {
-    dim3 grid((c->p.nx + NT - 1) / NT, DCH);
+    dim3 grid((c->p.nx + NT - 1) / NT, DCH, c->B);
     hipLaunchKernelGGL(density_kernel, grid, dim3(NT), 0, c->stream, f, c->p.nx, c->p.nv, c->npart);
     return hipGetLastError();
 }
@@ -267,7 +315,8 @@ hipError_t launch_density(vv_ctx *c, const double *f, double *n)
 {
     hipError_t e = launch_density_partial(c, f);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(density_finish_kernel, dim3((c->p.nx + NT - 1) / NT), dim3(NT), 0, c->stream, c->npart, DCH,
+    hipLaunchKernelGGL(density_finish_kernel, dim3((c->p.nx + NT - 1) / NT, c->B), dim3(NT), 0, c->stream, c->npart,
+                       DCH,
                        c->p.nx, c->p.hv, n);
     return hipGetLastError();
 }
@@ -275,36 +324,37 @@ hipError_t launch_density(vv_ctx *c, const double *f, double *n)
 hipError_t launch_poisson(vv_ctx *c, const double *n_in, double *n_out, double *phi, double *h1, double *pres,
                           int from_part)
 {
-    PoiArgs a{c->p.nx, c->p.laplace, from_part, c->p.hv, c->p.charge, c->hx2_inv,
-              c->npart, n_in, c->pinv, c->pwork, n_out, phi, h1, pres};
+    PoiArgs a{c->p.nx, c->p.laplace, from_part, c->p.hv, c->p.charge, c->inst,
+              from_part ? c->npart : nullptr, n_in, c->pinv, c->pwork, n_out, phi, h1, pres};
     const int nx = c->p.nx;
-    hipLaunchKernelGGL(poisson_prep_kernel, dim3(1), dim3(PT), 0, c->stream, a);
-    hipLaunchKernelGGL(poisson_conv_kernel, dim3((nx + NT - 1) / NT), dim3(NT), 2 * nx * sizeof(double), c->stream,
-                       a);
-    hipLaunchKernelGGL(poisson_post_kernel, dim3(1), dim3(PT), nx * sizeof(double), c->stream, a);
+    hipLaunchKernelGGL(poisson_prep_kernel, dim3(1, c->B), dim3(PT), 0, c->stream, a);
+    hipLaunchKernelGGL(poisson_conv_kernel, dim3((nx + NT - 1) / NT, c->B), dim3(NT), 2 * nx * sizeof(double),
+                       c->stream, a);
+    hipLaunchKernelGGL(poisson_post_kernel, dim3(1, c->B), dim3(PT), nx * sizeof(double), c->stream, a);
     return hipGetLastError();
 }
 
 hipError_t launch_poisson_mult(vv_ctx *c, const double *x, const double *n, double *y)
 {
     const size_t lds = (size_t)c->p.nx * sizeof(double);
-    hipLaunchKernelGGL(poisson_mult_kernel, dim3(1), dim3(PT), lds, c->stream, x, n, y, c->p.nx, c->p.laplace,
-                       c->hx2_inv, c->p.charge);
+    hipLaunchKernelGGL(poisson_mult_kernel, dim3(1, c->B), dim3(PT), lds, c->stream, x, n, y, c->p.nx, c->p.laplace,
+                       c->inst, c->p.charge);
     return hipGetLastError();
 }
 
 hipError_t launch_pot_to_ham(vv_ctx *c, const double *phi, double *h1)
 {
-    hipLaunchKernelGGL(pot_to_ham_kernel, dim3(1), dim3(PT), 0, c->stream, phi, h1, c->p.nx);
+    hipLaunchKernelGGL(pot_to_ham_kernel, dim3(1, c->B), dim3(PT), 0, c->stream, phi, h1, c->p.nx);
     return hipGetLastError();
 }
 
 hipError_t launch_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1, const double *h2,
                              const double *phi, double *out_dev)
 {
-    InvArgs a{f, h0, h1, h2, phi, c->p.nx, c->p.nv, c->p.hx, c->p.hv, c->p.charge, c->partial, out_dev};
-    hipLaunchKernelGGL(invariants_kernel, dim3(kReduceBlocks), dim3(NT), 0, c->stream, a);
-    hipLaunchKernelGGL(invariants_finish_kernel, dim3(1), dim3(NT), 0, c->stream, a, kReduceBlocks);
+    InvArgs a{f, h0, h1, h2, phi, c->p.nx, c->p.nv, c->p.hv, c->p.charge, c->inst, c->partial, c->part_stride,
+              out_dev};
+    hipLaunchKernelGGL(invariants_kernel, dim3(kReduceBlocks, c->B), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL(invariants_finish_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, a, kReduceBlocks);
     return hipGetLastError();
 }
 

@@ -155,18 +155,29 @@ __device__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ t
     }
 }
 
+// Instance b of a batched context is blockIdx.y: real fields at b * N,
+// spectra at b * S, GMRES state b * gstride bytes after instance 0's.
 template <typename T>
 struct FftArgs {
     int nx, logn, nv, rows;     // rows: v-rows per workgroup (even)
+    int64_t N, S, gstride;
     const cx_t<T> *tw;
     const int32_t *done;
 };
+
+__device__ __forceinline__ bool cycle_done(const int32_t *done, int64_t gstride)
+{
+    return done != nullptr &&
+           *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(done) + blockIdx.y * gstride) != 0;
+}
 
 template <typename T>
 __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
-    if (a.done != nullptr && *a.done) return;
+    if (cycle_done(a.done, a.gstride)) return;
+    x += blockIdx.y * a.N;
+    spec += blockIdx.y * a.S;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows;
@@ -213,7 +224,9 @@ template <typename T>
 __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
 {
     using C = cx_t<T>;
-    if (a.done != nullptr && *a.done) return;
+    if (cycle_done(a.done, a.gstride)) return;
+    spec += blockIdx.y * a.S;
+    y += blockIdx.y * a.N;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows, nh = nx >> 1;
@@ -265,9 +278,11 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
 struct TriArgs {
     int nv;           // system size
     int P;            // chunks (threads), power of two
+    int64_t S, gstride;
     const double2 *eigen;
     const double *gm, *g0, *gp;
-    double af, ht_inv;
+    const InstConst *inst;   // band factor afpc of instance blockIdx.y
+    double ht_inv;
     const int32_t *done;
 };
 
@@ -275,18 +290,32 @@ struct TriArgs {
 // diagonal b_j = diag[j], super c_j = diagp[j] (j <= nv-2); rows past nv are
 // identity rows that pad the system to P*M unknowns.  The coefficients are
 // formed in FP64 (as the reference forms its band matrix) and rounded to C.
-template <typename C>
-__device__ __forceinline__ void tri_row(const TriArgs &a, double2 e, int j, C &aj, C &bj, C &cj)
+// TR: row j of the conjugate transpose T_k^H (sub conj(c_{j-1}), diagonal
+// conj(b_j), super conj(a_{j+1})), the mode-k block of the transposed real
+// operator P^T (P^-T x = irfft(T_k^-H rfft(x)): transposing a real
+// x-circulant operator maps its symbol at k to the conjugate at -k).
+template <bool TR, typename C>
+__device__ __forceinline__ void tri_row(const TriArgs &a, double af, double2 e, int j, C &aj, C &bj, C &cj)
 {
     using R = typename C::value_type;
     if (j < a.nv) {
-        const double g = a.gm[j];
-        aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * g) * a.af), (R)(((e.y * 0.5) * g) * a.af)) : cmk<C>(R(0), R(0));
-        const double g0 = a.g0[j];
-        bj = cmk<C>((R)(((e.x * 1.0) * g0) * a.af + a.ht_inv), (R)(((e.y * 1.0) * g0) * a.af));
-        const double gp = a.gp[j];
-        cj = (j < a.nv - 1) ? cmk<C>((R)(((e.x * 0.5) * gp) * a.af), (R)(((e.y * 0.5) * gp) * a.af))
-                            : cmk<C>(R(0), R(0));
+        if (!TR) {
+            const double g = a.gm[j];
+            aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * g) * af), (R)(((e.y * 0.5) * g) * af)) : cmk<C>(R(0), R(0));
+            const double g0 = a.g0[j];
+            bj = cmk<C>((R)(((e.x * 1.0) * g0) * af + a.ht_inv), (R)(((e.y * 1.0) * g0) * af));
+            const double gp = a.gp[j];
+            cj = (j < a.nv - 1) ? cmk<C>((R)(((e.x * 0.5) * gp) * af), (R)(((e.y * 0.5) * gp) * af))
+                                : cmk<C>(R(0), R(0));
+        } else {
+            aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * a.gp[j - 1]) * af), (R)(-(((e.y * 0.5) * a.gp[j - 1]) * af)))
+                         : cmk<C>(R(0), R(0));
+            const double g0 = a.g0[j];
+            bj = cmk<C>((R)(((e.x * 1.0) * g0) * af + a.ht_inv), (R)(-(((e.y * 1.0) * g0) * af)));
+            cj = (j < a.nv - 1)
+                     ? cmk<C>((R)(((e.x * 0.5) * a.gm[j + 1]) * af), (R)(-(((e.y * 0.5) * a.gm[j + 1]) * af)))
+                     : cmk<C>(R(0), R(0));
+        }
     } else {
         aj = cmk<C>(R(0), R(0));
         bj = cmk<C>(R(1), R(0));
@@ -294,11 +323,13 @@ __device__ __forceinline__ void tri_row(const TriArgs &a, double2 e, int j, C &a
     }
 }
 
-template <int M, typename T>
+template <int M, typename T, bool TR>
 __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
-    if (a.done != nullptr && *a.done) return;
+    if (cycle_done(a.done, a.gstride)) return;
+    spec += blockIdx.y * a.S;
+    const double af = a.inst[blockIdx.y].afpc;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *sd = reinterpret_cast<C *>(lds_raw);   // P*(M+1): right-hand sides / solution, chunk c at c*(M+1)
     __shared__ C sa[NT], sb[NT], sc[NT], se[NT];
@@ -330,14 +361,14 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
         const int s = c * M, ls = c * (M + 1);
         // forward sweep: y_j + c'_j y_{j+1} = d'_j - a'_j y_{s-1}
         C aj, bj, cj;
-        tri_row(a, e, s, aj, bj, cj);
+        tri_row<TR>(a, af, e, s, aj, bj, cj);
         C r = crecip(bj);
         Cm[0] = cmul(cj, r);
         Dm[0] = cmul(sd[ls], r);
         Am[0] = cmul(aj, r);
 #pragma unroll
         for (int q = 1; q < M; ++q) {
-            tri_row(a, e, s + q, aj, bj, cj);
+            tri_row<TR>(a, af, e, s + q, aj, bj, cj);
             if (q == M - 1) {
                 ae = aj;
                 be = bj;
@@ -424,8 +455,12 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     for (int j = threadIdx.x; j < a.nv; j += NT) S[j] = sd[j + j / M];
 }
 
-int fft_rows(int nx, int nv)
+// v-rows per FFT workgroup: the context's tuned value, else the largest that
+// fits LDS while the launch keeps >= 512 workgroups
+int fft_rows(const vv_ctx *c)
 {
+    if (c->tune_fft_rows > 0) return c->tune_fft_rows;
+    const int nx = c->p.nx, nv = c->p.nv;
     int R = 8192 / nx;                 // LDS: R/2 complex rows of nx = R*nx*8 bytes <= 64 KiB
     if (R < 2) R = 2;
     while (R > 2 && (nv + R - 1) / R < 512) R >>= 1;
@@ -435,20 +470,22 @@ int fft_rows(int nx, int nv)
 template <typename T>
 hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const int32_t *done)
 {
-    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c->p.nx, c->p.nv), tw, done};
+    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
+                 (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    launch_timed(c, rfft_kernel<T>, dim3(nb), dim3(NT), (uint32_t)lds, a, x, spec);
+    launch_timed(c, rfft_kernel<T>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, const int32_t *done)
 {
-    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c->p.nx, c->p.nv), tw, done};
+    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
+                 (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    launch_timed(c, irfft_kernel<T>, dim3(nb), dim3(NT), (uint32_t)lds, a, spec, y);
+    launch_timed(c, irfft_kernel<T>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
     return hipGetLastError();
 }
 
@@ -466,19 +503,20 @@ void tridiag_shape(int nv, int *P, int *M)
     }
 }
 
-template <typename T>
+template <typename T, bool TR = false>
 hipError_t tridiag_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
 {
     int P, M;
     tridiag_shape(c->p.nv, &P, &M);
-    TriArgs a{c->p.nv, P, c->eigen, c->gm, c->g0, c->gp, c->afpc, c->ht_inv, done};
+    TriArgs a{c->p.nv, P, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp, c->inst,
+              c->ht_inv, done};
     const size_t lds = (size_t)P * (M + 1) * sizeof(cx_t<T>);
-    dim3 grid(c->nmodes), block(NT);
+    dim3 grid(c->nmodes, c->B), block(NT);
     switch (M) {
-    case 2: launch_timed(c, tridiag_kernel<2, T>, grid, block, (uint32_t)lds, a, spec); break;
-    case 4: launch_timed(c, tridiag_kernel<4, T>, grid, block, (uint32_t)lds, a, spec); break;
-    case 8: launch_timed(c, tridiag_kernel<8, T>, grid, block, (uint32_t)lds, a, spec); break;
-    default: launch_timed(c, tridiag_kernel<16, T>, grid, block, (uint32_t)lds, a, spec); break;
+    case 2: launch_timed(c, tridiag_kernel<2, T, TR>, grid, block, (uint32_t)lds, a, spec); break;
+    case 4: launch_timed(c, tridiag_kernel<4, T, TR>, grid, block, (uint32_t)lds, a, spec); break;
+    case 8: launch_timed(c, tridiag_kernel<8, T, TR>, grid, block, (uint32_t)lds, a, spec); break;
+    default: launch_timed(c, tridiag_kernel<16, T, TR>, grid, block, (uint32_t)lds, a, spec); break;
     }
     return hipGetLastError();
 }
@@ -487,7 +525,7 @@ hipError_t tridiag_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
 
 int fft_blocks(vv_ctx *c)
 {
-    const int R = fft_rows(c->p.nx, c->p.nv);
+    const int R = fft_rows(c);
     return (c->p.nv + R - 1) / R;
 }
 
@@ -504,6 +542,11 @@ hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t
 hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done)
 {
     return tridiag_t<double>(c, spec, done);
+}
+
+hipError_t launch_tridiag_t(vv_ctx *c, double2 *spec, const int32_t *done)
+{
+    return tridiag_t<double, true>(c, spec, done);
 }
 
 hipError_t launch_rfft32(vv_ctx *c, const float *x, float2 *spec, const int32_t *done)

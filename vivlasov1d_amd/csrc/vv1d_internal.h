@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/vv1d.h"
 
@@ -23,8 +24,18 @@ struct GmresStatus {
     int32_t its;       // total iterations
     int32_t reason;    // KSP converged reason (0 = iterating)
     int32_t done;      // current cycle finished: every Krylov kernel returns at once
+    int32_t active;    // instance takes part in this solve (batched contexts)
+    int32_t pad;
     double res;        // current residual estimate
     double rnorm0;     // norm of the right-hand side
+};
+
+// Per-instance grid constants (a batched context holds instances that share
+// nx, nv, v-grid and dt but not the x-period, so hx differs).
+struct InstConst {
+    double hx, hx_inv, hx2_inv;
+    double af1, af2;   // Arakawa factors hx^-1 hv^-1 / 12, / 24
+    double afpc;       // preconditioner band factor 0.5 hx^-1 hv^-1 / 12
 };
 
 // Device-resident GMRES state.  Written only by the one-workgroup control
@@ -50,12 +61,17 @@ struct GmresState {
 struct vv_ctx {
     vv_params p;
     hipStream_t stream;
-    int64_t N;          // nx * nv
+    int B;              // instances (1 unless created batched); instance b of every
+                        // per-instance buffer lives at base + b * stride
+    vv::InstConst *inst;        // device, B entries
+    std::vector<vv::InstConst> inst_host;
+    int32_t *active;            // device, B flags: instance takes part in the next solve
+    int64_t N;          // nx * nv (per instance)
     int nmodes;         // nx/2 + 1
     int logn;           // log2(nx)
-    // grid-derived constants
-    double ht_inv, hx_inv, hv_inv, hx2_inv, af1, af2, afpc;
-    // Hamiltonian factors (device, ctx-owned copies)
+    // grid-derived constants (x-dependent ones per instance in inst)
+    double ht_inv, hv_inv;
+    // Hamiltonian factors (device, ctx-owned copies): h0 shared, h1*/h2* per instance
     double *h0, *h1p, *h1h, *h2p, *h2h;
     // preconditioner data (device)
     double2 *eigen;     // nmodes used (k = 0..nx/2)
@@ -64,27 +80,45 @@ struct vv_ctx {
     float2 *twiddle32;  // the same, rounded to float (FP32 preconditioner)
     double2 *spec;      // nmodes * nv complex spectrum, mode-major
     // Poisson
-    double *pinv;       // first row of the mean-free circulant inverse (nx)
-    double *pwork;      // Poisson scratch (3*nx + 1)
+    double *pinv;       // first row of the mean-free circulant inverse (nx per instance)
+    double *pwork;      // Poisson scratch (3*nx + 1 per instance)
     // work vectors (N)
     double *X;          // bracket / residual output before P^-1
-    double *B;          // SNES right-hand side P^-1 F
+    double *rhs;        // SNES right-hand side P^-1 F
     double *D;          // Newton update
-    double *basis;      // (restart+1) * N
-    // mixed precision: the FP32 cycle reuses the FP64 basis allocation
-    float *basis32;     // (restart+1) * N floats at the start of basis
-    double *R;          // FP64 true residual of the refinement loop, after basis32
+    double *basis;      // (restart+1) vectors of B*N: V_j of instance b at basis + (j*B + b)*N
+    // mixed precision (allocated on first use)
+    float *basis32;     // FP32 basis, same layout
+    double *R;          // FP64 true residual of the refinement loop
     // reductions
-    double *partial;    // kReduceBlocks * (kMaxRestart+2)
-    double *dscalar;    // device scalars
-    double *npart;      // density partial sums
-    vv::GmresState *gs;
-    vv::GmresStatus *host_status;   // pinned, 2 slots
+    double *partial;    // per instance: part_stride doubles
+    int64_t part_stride;
+    double *dscalar;    // device scalars, 16 per instance
+    double *npart;      // density partial sums, 64 nx per instance
+    vv::GmresState *gs;             // B states
+    vv::GmresStatus *host_status;   // pinned, 2 slots of B statuses
+    double *host_scalars;           // pinned staging of device scalars
     hipEvent_t ev[2];
     // profiling (per-kernel HIP event pairs)
     int prof;           // bit k set: time kernel kind k
     void *prof_state;
     hipEvent_t prof_a, prof_b;   // events of the next main-kernel launch (null: none)
+    // linearised / adjoint operators (scratch allocated on first use):
+    // operator of the GMRES cycle (0: P^-1 J of the Picard step; 1: P^-T A^T,
+    // the transposed full linearisation), the midpoint state f_bar of its
+    // field coupling (null: no coupling)
+    int op_mode;
+    const double *op_fbar;
+    double *T2;         // B*N
+    double *xs;         // 4 regions of B*nx: n | phi | h1 | spare
+    // launch-shape tuning (0: built-in heuristic): v-rows per FFT workgroup,
+    // 16-byte vectors per thread of the CGS kernels
+    int tune_fft_rows, tune_cgs_ept;
+    // captured first GMRES cycle of SNESSolve_KSPONLY (graph replay mode)
+    bool use_graph, capturing;
+    hipGraphExec_t graph_exec;
+    const double *graph_f, *graph_fh;
+    double graph_kp[4];
     std::string err;
 };
 
@@ -96,18 +130,26 @@ hipError_t launch_residual(vv_ctx *c, const double *f, const double *fh, double 
 hipError_t launch_jacobian(vv_ctx *c, const double *u, const double *uscale, double *y);
 hipError_t launch_bracket_add(vv_ctx *c, const double *x, double *y, double factor);
 hipError_t launch_jacobian32(vv_ctx *c, const float *u, const double *uscale, float *y);
+// y = alpha u + factor {u~, H} on all rows (u~: u on rows 2..nv-3, else 0);
+// H = h2d (2D) or hxv (x only) when given, else H_ave
+hipError_t launch_bracket_t(vv_ctx *c, const double *u, const double *uscale, const double *h2d, const double *hxv,
+                            double *y, double alpha, double factor);
+// y += factor {x, H} on rows 2..nv-3, H = h2d / hxv / H_ave as above
+hipError_t launch_bracket_gen(vv_ctx *c, const double *x, const double *h2d, const double *hxv, double *y,
+                              double factor);
 // spectral.hip
 int fft_blocks(vv_ctx *c);
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done);
 hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done);
 hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done);
+hipError_t launch_tridiag_t(vv_ctx *c, double2 *spec, const int32_t *done);   // T_k^-H (transposed P)
 hipError_t launch_rfft32(vv_ctx *c, const float *x, float2 *spec, const int32_t *done);
 hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done);
 hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t *done);
 // krylov.hip
 hipError_t launch_gmres_init(vv_ctx *c, double rtol, double atol, double dtol, int max_it);
 hipError_t launch_norm_start(vv_ctx *c);                  // cycle start on basis[0]
-int cgs_blocks(int64_t N);
+int cgs_blocks_max(int64_t N);                            // partial rows the CGS kernels may use
 hipError_t launch_mdot(vv_ctx *c, int col);               // CGS dots of basis[col+1] vs 0..col
 hipError_t launch_maxpy_norm(vv_ctx *c, int it);          // CGS update of basis[it+1] + Givens
 hipError_t launch_build_solution(vv_ctx *c, double *x);  // x += V y
@@ -115,12 +157,15 @@ hipError_t launch_start32(vv_ctx *c, const double *r);    // FP32 cycle start: V
 hipError_t launch_mdot32(vv_ctx *c, int col);
 hipError_t launch_maxpy_norm32(vv_ctx *c, int it);
 hipError_t launch_build_solution32(vv_ctx *c, double *x);  // x (FP64) += V y (FP32 basis)
-hipError_t launch_residual_restart(vv_ctx *c, const double *b, double *y);  // y = b - y
+hipError_t launch_residual_restart(vv_ctx *c, const double *b, double *y);  // y = b - y (B fields)
+hipError_t launch_norms(vv_ctx *c, const double *x, double *out_dev);       // ||x_b|| of the B fields
 hipError_t launch_norm2(vv_ctx *c, const double *x, int64_t n, double *out_dev);
 hipError_t launch_dot(vv_ctx *c, const double *x, const double *y, int64_t n, double *out_dev);
 hipError_t launch_axpy(vv_ctx *c, double alpha, const double *x, double *y, int64_t n);
-hipError_t launch_axpy_cond(vv_ctx *c, double alpha, const double *x, double *y, int64_t n);
+hipError_t launch_axpy_cond(vv_ctx *c, double alpha, const double *x, double *y);  // per instance, if reason > 0
 hipError_t launch_copy(vv_ctx *c, const double *x, double *y, int64_t n);
+hipError_t launch_zero(vv_ctx *c, double *y, int64_t n);
+hipError_t launch_xadd(vv_ctx *c, double *y, const double *w, double alpha);  // y[b][j][i] += alpha w[b][i]
 // moments.hip
 hipError_t launch_density_partial(vv_ctx *c, const double *f);
 hipError_t launch_density(vv_ctx *c, const double *f, double *n);
@@ -144,6 +189,9 @@ inline void launch_timed(vv_ctx *c, K kernel, dim3 grid, dim3 block, uint32_t sh
         hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, args...);
     }
 }
+
+// element stride between the spectra of consecutive instances
+inline int64_t spec_stride(const vv_ctx *c) { return (int64_t)c->nmodes * c->p.nv; }
 
 // XCD-aware block order: spread consecutive tiles so that each of the 8 XCDs
 // works on one contiguous range of tiles (halo lines stay in that XCD's L2).

@@ -26,7 +26,7 @@ class ResidualTooLarge(RuntimeError):
 
 class Runscript:
     def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False, precision='double',
-                 inner_rtol=1e-5):
+                 inner_rtol=1e-5, normalise_f0=True):
         if not isinstance(cfg, Config):
             cfg = Config(cfg)
         self.cfg = cfg
@@ -52,7 +52,9 @@ class Runscript:
             if profile is None:
                 raise ValueError("give an initial profile(x, v) or f0")
             f0 = initial_distribution(grid, profile)
-        f0 = normalise(np.asarray(f0, dtype=np.float64).reshape(grid.nv, grid.nx), grid)
+        f0 = np.asarray(f0, dtype=np.float64).reshape(grid.nv, grid.nx)
+        if normalise_f0:
+            f0 = normalise(f0, grid)
         self.fc.copy_(torch.from_numpy(f0))
         self.pnorm = 0.0
         self.calculate_moments()
@@ -133,6 +135,15 @@ class Runscript:
         if pred > 10.:
             raise ResidualTooLarge("ERROR: Residual of nonlinear solver too large.")
         return rec
+
+    def set_state(self, f):
+        """Replace the distribution function (a (nv, nx) float64 CUDA tensor
+        or array) and recompute its moments, as after initialisation."""
+        if isinstance(f, torch.Tensor):
+            self.fc.copy_(f.reshape(self.grid.nv, self.grid.nx))
+        else:
+            self.fc.copy_(torch.from_numpy(np.asarray(f, dtype=np.float64).reshape(self.grid.nv, self.grid.nx)))
+        self.calculate_moments()
 
     def run(self, nt=None):
         nt = self.grid.nt if nt is None else nt

@@ -43,10 +43,15 @@ def _dev_ptr(t, numel=None, name="vector", dtype=DOUBLE):
 
 
 class Context:
-    """One libvv1d context: grid, operator choice and device scratch."""
+    """One libvv1d context: grid, operator choice and device scratch.
+
+    With ``hx`` (a sequence of x-spacings) the context is batched: it holds
+    len(hx) instances that share the grid's nx, nv, v-grid and dt, each with
+    its own x-period.  Fields are then (B, nv, nx) tensors, x-vectors (B, nx),
+    and the scalar-returning methods return one value per instance."""
 
     def __init__(self, grid, bracket='arakawaj4', preconditioner='tensor_kinetic', laplace='cfd4',
-                 charge=-1.0, restart=30, stream=None):
+                 charge=-1.0, restart=30, stream=None, hx=None):
         lib = _lib.load()
         if bracket not in _lib.BRACKETS:
             raise _lib.VVError("Poisson bracket %r is not provided (arakawaj1/2/4)" % (bracket,))
@@ -73,9 +78,21 @@ class Context:
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         self._ptr = ctypes.c_void_p()
-        ret = lib.vv_ctx_create(ctypes.byref(p), ctypes.c_void_p(stream), ctypes.byref(self._ptr))
+        if hx is None:
+            self.B = 1
+            self.batched = False
+            ret = lib.vv_ctx_create(ctypes.byref(p), ctypes.c_void_p(stream), ctypes.byref(self._ptr))
+        else:
+            self._hx = np.ascontiguousarray(hx, dtype=np.float64)
+            self.B = int(self._hx.size)
+            self.batched = True
+            p.hx = float(self._hx[0])
+            ret = lib.vv_ctx_create_batched(ctypes.byref(p), ctypes.c_int32(self.B),
+                                            self._hx.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            ctypes.c_void_p(stream), ctypes.byref(self._ptr))
         _lib.check(ret, None)
         self.lib = lib
+        self.stream = stream
         self.N = grid.nx * grid.nv
 
     @property
@@ -98,19 +115,54 @@ class Context:
 
     # fields ----------------------------------------------------------------
     def F(self, t, name="field"):
-        return _dev_ptr(t, self.N, name)
+        return _dev_ptr(t, self.N * self.B, name)
 
     def X(self, t, name="x-vector"):
-        return _dev_ptr(t, self.grid.nx, name)
+        return _dev_ptr(t, self.grid.nx * self.B, name)
 
     def V(self, t, name="v-vector"):
         return _dev_ptr(t, self.grid.nv, name)
 
     def new_field(self):
-        return torch.zeros((self.grid.nv, self.grid.nx), dtype=DOUBLE, device='cuda')
+        shape = (self.B, self.grid.nv, self.grid.nx) if self.batched else (self.grid.nv, self.grid.nx)
+        return torch.zeros(shape, dtype=DOUBLE, device='cuda')
 
     def new_x(self):
-        return torch.zeros(self.grid.nx, dtype=DOUBLE, device='cuda')
+        shape = (self.B, self.grid.nx) if self.batched else (self.grid.nx,)
+        return torch.zeros(shape, dtype=DOUBLE, device='cuda')
+
+    def _per_instance(self, arr):
+        return np.array(arr[:self.B], dtype=np.float64) if self.batched else float(arr[0])
+
+    # batching, graphs, launch shapes ------------------------------------------
+    def set_active(self, active=None):
+        if active is None:
+            self.call("vv_set_active", None)
+        else:
+            a = (ctypes.c_int32 * self.B)(*[int(bool(v)) for v in active])
+            self.call("vv_set_active", a)
+
+    def set_stream(self, handle):
+        """Enqueue on another HIP stream (a hipStream_t as an int; 0 = null stream)."""
+        self.call("vv_set_stream", ctypes.c_void_p(handle))
+        self.stream = handle
+
+    def set_graph(self, enable=True):
+        self.call("vv_set_graph", ctypes.c_int(1 if enable else 0))
+
+    def set_tuning(self, fft_rows=0, cgs_ept=0):
+        self.call("vv_set_tuning", ctypes.c_int32(fft_rows), ctypes.c_int32(cgs_ept))
+
+    def get_tuning(self):
+        r, e = ctypes.c_int32(), ctypes.c_int32()
+        self.call("vv_get_tuning", ctypes.byref(r), ctypes.byref(e))
+        return r.value, e.value
+
+    def autotune(self):
+        """Time the launch-shape candidates; returns (fft_rows, cgs_ept) kept."""
+        r, e = ctypes.c_int32(), ctypes.c_int32()
+        self.call("vv_autotune", ctypes.byref(r), ctypes.byref(e))
+        return r.value, e.value
 
     # operators ---------------------------------------------------------------
     def set_hamiltonian(self, h0, h1p, h1h, h2p, h2h):
@@ -136,9 +188,9 @@ class Context:
         self.call("vv_jacobian", self.F(u), self.F(y))
 
     def function_norm(self, f, fh, y):
-        out = ctypes.c_double()
-        self.call("vv_function_norm", self.F(f), self.F(fh), self.F(y), ctypes.byref(out))
-        return out.value
+        out = (ctypes.c_double * self.B)()
+        self.call("vv_function_norm", self.F(f), self.F(fh), self.F(y), out)
+        return self._per_instance(out)
 
     def density(self, f, n):
         self.call("vv_density", self.F(f), self.X(n))
@@ -156,10 +208,9 @@ class Context:
         self.call("vv_potential_to_hamiltonian", self.X(phi), self.X(h1))
 
     def moments(self, f, n, phi, h1, want_norm=True):
-        out = ctypes.c_double()
-        self.call("vv_moments", self.F(f), self.X(n), self.X(phi), self.X(h1),
-                  ctypes.byref(out) if want_norm else None)
-        return out.value if want_norm else None
+        out = (ctypes.c_double * self.B)()
+        self.call("vv_moments", self.F(f), self.X(n), self.X(phi), self.X(h1), out if want_norm else None)
+        return self._per_instance(out) if want_norm else None
 
     def norm2(self, x):
         out = ctypes.c_double()
@@ -179,16 +230,17 @@ class Context:
         self.call("vv_axpy", ctypes.c_double(alpha), _dev_ptr(x), _dev_ptr(y), ctypes.c_int64(x.numel()))
 
     def ksp_solve(self, b, x, rtol, atol, dtol, max_it):
+        """KspResult (one instance) or a list of them (batched)."""
         kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
-        res = _lib.KspResult()
-        self.call("vv_ksp_solve", self.F(b), self.F(x), ctypes.byref(kp), ctypes.byref(res))
-        return res
+        res = (_lib.KspResult * self.B)()
+        self.call("vv_ksp_solve", self.F(b), self.F(x), ctypes.byref(kp), res)
+        return list(res) if self.batched else res[0]
 
     def snes_ksponly(self, f, fh, rtol, atol, dtol, max_it):
         kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
-        res = _lib.KspResult()
-        self.call("vv_snes_ksponly", self.F(f), self.F(fh), ctypes.byref(kp), ctypes.byref(res))
-        return res
+        res = (_lib.KspResult * self.B)()
+        self.call("vv_snes_ksponly", self.F(f), self.F(fh), ctypes.byref(kp), res)
+        return list(res) if self.batched else res[0]
 
     # mixed precision ----------------------------------------------------------
     def F32(self, t, name="field"):
@@ -219,11 +271,38 @@ class Context:
                   ctypes.byref(res), ctypes.byref(cyc))
         return res, cyc.value
 
+    # linearisation and adjoint ------------------------------------------------
+    def linear_apply(self, u, y, which, fbar=None):
+        """y = A u, A0 u, A^T u or A0^T u (which = 'A', 'A0', 'AT', 'A0T') of
+        the step linearised about the H_ave set last and f_bar (None: no
+        field coupling)."""
+        self.call("vv_linear_apply", self.F(u), self.F(fbar) if fbar is not None else None, self.F(y),
+                  ctypes.c_int32(_lib.LINEAR[which]))
+
+    def precond_t(self, x, y):
+        self.call("vv_precond_apply_t", self.F(x), self.F(y))
+
+    def adjoint_solve(self, lam, mu, rtol, atol, dtol, max_it, fbar=None):
+        kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
+        res = (_lib.KspResult * self.B)()
+        self.call("vv_adjoint_solve", self.F(lam), self.F(fbar) if fbar is not None else None, self.F(mu),
+                  ctypes.byref(kp), res)
+        return list(res) if self.batched else res[0]
+
+    def adjoint_step(self, lam1, lam0, mu, rtol, atol, dtol, max_it, fbar=None):
+        kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
+        res = (_lib.KspResult * self.B)()
+        self.call("vv_adjoint_step", self.F(lam1), self.F(fbar) if fbar is not None else None, self.F(lam0),
+                  self.F(mu), ctypes.byref(kp), res)
+        return list(res) if self.batched else res[0]
+
     def invariants(self, f, h0, h1, h2, phi):
-        out = (ctypes.c_double * 6)()
+        out = (ctypes.c_double * (6 * self.B))()
         self.call("vv_invariants", self.F(f), self.V(h0), self.X(h1), self.X(h2), self.X(phi), out)
         keys = ("mass", "l2", "energy", "ekin", "epot", "efield")
-        return dict(zip(keys, list(out)))
+        vals = list(out)
+        res = [dict(zip(keys, vals[6 * b:6 * b + 6])) for b in range(self.B)]
+        return res if self.batched else res[0]
 
     PROFILE_KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
                        "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32")
