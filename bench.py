@@ -184,7 +184,22 @@ def main():
     ap.add_argument("--batched-instances", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--batched-warmup", type=int, default=1)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n,
+                          "copies": 5}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the timestep is not sharded across GPUs in this build "

@@ -676,6 +676,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
               "pinned status");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_scalars), ds_count(B) * sizeof(double)),
               "pinned scalars");
+    VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_active), Bz * sizeof(int32_t)), "pinned mask");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming), "event");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming), "event");
     VV_CCHECK(hipMemset(c->gs, 0, Bz * sizeof(GmresState)), "zero gmres state");
@@ -781,6 +782,7 @@ int vv_ctx_destroy(vv_ctx *c)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);
     if (c->host_scalars) (void)hipHostFree(c->host_scalars);
+    if (c->host_active) (void)hipHostFree(c->host_active);
     if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
     if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
     Prof *pr = prof_of(c);
@@ -829,11 +831,13 @@ int64_t vv_scratch_bytes(const vv_ctx *c)
 int vv_set_active(vv_ctx *c, const int32_t *active)
 {
     if (int r = check_ctx(c)) return r;
-    std::vector<int32_t> a((size_t)c->B, 1);
-    if (active)
-        for (int b = 0; b < c->B; ++b) a[b] = active[b] ? 1 : 0;
+    // stream-ordered copy from pinned memory: no legacy-stream operation, so
+    // other streams may be capturing graphs meanwhile
     VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
-    VV_CHECK(hipMemcpy(c->active, a.data(), (size_t)c->B * sizeof(int32_t), hipMemcpyHostToDevice), "upload mask");
+    for (int b = 0; b < c->B; ++b) c->host_active[b] = active ? (active[b] ? 1 : 0) : 1;
+    VV_CHECK(hipMemcpyAsync(c->active, c->host_active, (size_t)c->B * sizeof(int32_t), hipMemcpyHostToDevice,
+                            c->stream),
+             "upload mask");
     return VV_OK;
 }
 

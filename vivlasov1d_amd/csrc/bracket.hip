@@ -21,8 +21,14 @@
 namespace vv {
 namespace {
 
+#ifndef VV_BRACKET_SLIDE
+#define VV_BRACKET_SLIDE 1
+#endif
+#ifndef VV_BRACKET_TY
+#define VV_BRACKET_TY 16
+#endif
 constexpr int TX = 64;
-constexpr int TY = 28;      // 4 workgroups per CU fit in LDS
+constexpr int TY = VV_BRACKET_TY;   // 28: 4 FP64 workgroups per CU fit in LDS
 constexpr int HX = TX + 4;
 constexpr int HY = TY + 4;
 constexpr int NT = 256;
@@ -63,13 +69,10 @@ template <int KIND, int MODE, typename T>
 __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 {
     const int b = blockIdx.y;
-    if (a.done != nullptr && *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(a.done) + b * a.gstride))
-        return;
     __shared__ T sF[HY][HX];
     __shared__ T sH[HY][HX];
-    __shared__ double sX[4][HX];      // h1p, h1h, h2p, h2h of the tile's columns
+    __shared__ double sX[4][HX];      // h1p, h1h, h2p, h2h (or the x-only H) of the tile's columns
     __shared__ double sV[HY];         // h0 of the tile's rows
-    __shared__ int sI[HX];            // periodic column index of the tile's columns
 
     const int nx = a.nx, nv = a.nv;
     const int ntx = (nx + TX - 1) / TX;
@@ -81,63 +84,82 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const T *__restrict__ fin = a.f + fo;
     const T *__restrict__ fhin = (MODE == MODE_RESIDUAL) ? a.fh + fo : nullptr;
     T *__restrict__ yout = a.y + fo;
-    const T us = ((MODE == MODE_JACOBIAN || MODE == MODE_TRANSPOSED) && a.uscale != nullptr)
-                     ? (T)*reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.uscale) + b * a.gstride)
-                     : T(1);
     const T half = T(0.5);
+    constexpr bool SCALED = (MODE == MODE_JACOBIAN || MODE == MODE_TRANSPOSED);
+    const bool factors = (a.h2d == nullptr && a.hxv == nullptr);
 
-    // stage the 1D Hamiltonian factors of the tile (H_ave never touches HBM as a 2D field)
+    // Phase 1: every global load of the block is issued before anything waits
+    // on one (the early-exit flag, the scale, the 1D factors and the tile), so
+    // a block pays one memory latency, not one per dependent step.
+    const int32_t dflag =
+        (a.done != nullptr) ? *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(a.done) + b * a.gstride)
+                            : 0;
+    const double usd = (SCALED && a.uscale != nullptr)
+                           ? *reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.uscale) + b * a.gstride)
+                           : 1.0;
+    double xf0 = 0.0, xf1 = 0.0, xf2 = 0.0, xf3 = 0.0, vf = 0.0;
     if (threadIdx.x < HX) {
-        const int c = threadIdx.x;
-        int gi = (i0 - 2 + c) % nx;
+        int gi = i0 - 2 + (int)threadIdx.x;
         if (gi < 0) gi += nx;
-        sI[c] = gi;
-        sX[0][c] = a.h1p[xo + gi];
-        sX[1][c] = a.h1h[xo + gi];
-        sX[2][c] = a.h2p[xo + gi];
-        sX[3][c] = a.h2h[xo + gi];
+        if (gi >= nx) gi -= nx;
+        if (a.hxv != nullptr) {
+            xf0 = a.hxv[xo + gi];
+        } else if (factors) {
+            xf0 = a.h1p[xo + gi];
+            xf1 = a.h1h[xo + gi];
+            xf2 = a.h2p[xo + gi];
+            xf3 = a.h2h[xo + gi];
+        }
     } else if (threadIdx.x < HX + HY) {
-        const int r = threadIdx.x - HX;
-        const int gj = j0 - 2 + r;
-        sV[r] = (gj >= 0 && gj < nv) ? a.h0[gj] : 0.0;
+        const int gj = j0 - 2 + ((int)threadIdx.x - HX);
+        if (factors && gj >= 0 && gj < nv) vf = a.h0[gj];
+    }
+    constexpr int NQ = (HY * HX + NT - 1) / NT;
+    T fa[NQ], fb[NQ];
+    double hq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        fa[q] = T(0);
+        fb[q] = T(0);
+        hq[q] = 0.0;
+        if (idx < HY * HX) {
+            const int r = idx / HX;
+            const int c = idx - r * HX;
+            const int gj = j0 - 2 + r;
+            int gi = i0 - 2 + c;
+            if (gi < 0) gi += nx;
+            if (gi >= nx) gi -= nx;
+            if (gj >= 0 && gj < nv) {
+                const size_t o = (size_t)gj * nx + gi;
+                fa[q] = fin[o];
+                if (MODE == MODE_RESIDUAL) fb[q] = fhin[o];
+                if (a.h2d != nullptr) hq[q] = a.h2d[fo + o];
+            }
+        }
+    }
+
+    // Phase 2: stage the 1D factors, then the tile of the first argument and of H
+    if (dflag) return;                 // uniform over the block, before any barrier
+    if (threadIdx.x < HX) {
+        sX[0][threadIdx.x] = xf0;
+        sX[1][threadIdx.x] = xf1;
+        sX[2][threadIdx.x] = xf2;
+        sX[3][threadIdx.x] = xf3;
+    } else if (threadIdx.x < HX + HY) {
+        sV[threadIdx.x - HX] = vf;
     }
     __syncthreads();
 
+    const T us = (T)usd;
     // FP32: H_ave enters every Arakawa term only through differences, so the
     // float tile holds H_ave - H_ave(tile centre), formed in FP64 and rounded
     // once; the rounding error is then relative to H's variation over the
     // tile, not to |H| (which would cost ~|H|/(hv dH/dv) ulps on fine v grids)
     double href = 0.0;
-    if (sizeof(T) == 4) {
+    if (sizeof(T) == 4 && factors) {
         const int rc = HY / 2, cc = HX / 2;
-        const int gj = min(max(j0 - 2 + rc, 0), nv - 1);
-        href = (((a.h0[gj] + 0.5 * sX[0][cc]) + 0.5 * sX[1][cc]) + 0.5 * sX[2][cc]) + 0.5 * sX[3][cc];
-    }
-
-    // all global loads of the tile are issued before any LDS store
-    constexpr int NQ = (HY * HX + NT - 1) / NT;
-    T fv[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int idx = threadIdx.x + q * NT;
-        fv[q] = T(0);
-        if (idx < HY * HX) {
-            const int r = idx / HX;
-            const int c = idx - r * HX;
-            const int gj = j0 - 2 + r;
-            if (gj >= 0 && gj < nv) {
-                const size_t o = (size_t)gj * nx + sI[c];
-                if (MODE == MODE_RESIDUAL) {
-                    fv[q] = half * fhin[o] + half * fin[o];      // Fave (PETScNLVlasovMP.pyx:44-46)
-                } else if (MODE == MODE_TRANSPOSED) {
-                    fv[q] = (gj >= 2 && gj < nv - 2) ? fin[o] * us : T(0);
-                } else if (MODE == MODE_JACOBIAN) {
-                    fv[q] = fin[o] * us;                         // == VecScale'd Krylov vector
-                } else {
-                    fv[q] = fin[o];
-                }
-            }
-        }
+        href = (((sV[rc] + 0.5 * sX[0][cc]) + 0.5 * sX[1][cc]) + 0.5 * sX[2][cc]) + 0.5 * sX[3][cc];
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -146,23 +168,32 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             const int r = idx / HX;
             const int c = idx - r * HX;
             const int gj = j0 - 2 + r;
-            sF[r][c] = fv[q];
-            // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i)  (PETScVlasovSolver.pyx:145-149)
-            if (a.h2d != nullptr) {
-                sH[r][c] = (gj >= 0 && gj < nv) ? (T)a.h2d[fo + (size_t)gj * nx + sI[c]] : T(0);
-            } else if (a.hxv != nullptr) {
-                sH[r][c] = (gj >= 0 && gj < nv) ? (T)a.hxv[xo + sI[c]] : T(0);
-            } else if (sizeof(T) == 8) {
-                sH[r][c] = (gj >= 0 && gj < nv)
-                               ? (T)((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
-                               : T(0);
+            const bool in = (gj >= 0 && gj < nv);
+            T fv;
+            if (MODE == MODE_RESIDUAL) {
+                fv = half * fb[q] + half * fa[q];            // Fave (PETScNLVlasovMP.pyx:44-46)
+            } else if (MODE == MODE_TRANSPOSED) {
+                fv = (gj >= 2 && gj < nv - 2) ? fa[q] * us : T(0);
+            } else if (MODE == MODE_JACOBIAN) {
+                fv = fa[q] * us;                             // == VecScale'd Krylov vector
             } else {
-                // outside the grid H is 0 in the reference; those cells only meet
-                // rows whose result is not formed (j < 2 or j >= nv - 2)
-                sH[r][c] = (gj >= 0 && gj < nv)
-                               ? (T)(((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
-                                     - href)
-                               : T(0);
+                fv = fa[q];
+            }
+            sF[r][c] = fv;
+            // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i)  (PETScVlasovSolver.pyx:145-149);
+            // outside the grid H is 0: those cells only meet rows whose result
+            // is not formed (j < 2 or j >= nv - 2) except in the transposed mode
+            if (a.h2d != nullptr) {
+                sH[r][c] = in ? (T)hq[q] : T(0);
+            } else if (a.hxv != nullptr) {
+                sH[r][c] = in ? (T)sX[0][c] : T(0);
+            } else if (sizeof(T) == 8) {
+                sH[r][c] = in ? (T)((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
+                              : T(0);
+            } else {
+                sH[r][c] = in ? (T)(((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
+                                    - href)
+                              : T(0);
             }
         }
     }
@@ -175,6 +206,130 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const int cx = lane + 2;
     const T ht_inv = (T)a.ht_inv, af1 = (T)a.inst[b].af1, af2 = (T)a.inst[b].af2, factor = (T)a.factor;
 
+#if VV_BRACKET_SLIDE
+    // Each lane walks down its column: the stencil values of F and H live in
+    // a register window (column 0: rows -2..2, columns +-1: rows -1..1,
+    // columns +-2: row 0) that moves one row per step, so a row costs 5 + 5
+    // LDS reads instead of 25.  Same values, same expressions.
+    const int rr0 = wave * ROWS_PER_WAVE;
+    T f0[5], fm[3], fp[3], fm2, fp2, g0[5], gm[3], gp[3], gm2, gp2;
+    {
+        const int cy = rr0 + 2;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            f0[k] = sF[cy - 2 + k][cx];
+            g0[k] = sH[cy - 2 + k][cx];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            fm[k] = sF[cy - 1 + k][cx - 1];
+            fp[k] = sF[cy - 1 + k][cx + 1];
+            gm[k] = sH[cy - 1 + k][cx - 1];
+            gp[k] = sH[cy - 1 + k][cx + 1];
+        }
+        fm2 = sF[cy][cx - 2];
+        fp2 = sF[cy][cx + 2];
+        gm2 = sH[cy][cx - 2];
+        gp2 = sH[cy][cx + 2];
+    }
+#define F(di, dj) ((di) == 0 ? f0[(dj) + 2] : (di) == -1 ? fm[(dj) + 1] : (di) == 1 ? fp[(dj) + 1] : (di) == -2 ? fm2 : fp2)
+#define H(di, dj) ((di) == 0 ? g0[(dj) + 2] : (di) == -1 ? gm[(dj) + 1] : (di) == 1 ? gp[(dj) + 1] : (di) == -2 ? gm2 : gp2)
+#pragma unroll
+    for (int q = 0; q < ROWS_PER_WAVE; ++q) {
+        const int rr = rr0 + q;
+        const int j = j0 + rr;
+        if (j < nv) {
+            const size_t o = (size_t)j * nx + i;
+            T base;
+            if (MODE == MODE_RESIDUAL) {
+                const T fder = fin[o] - fhin[o];                 // Fder = (0 + F) - Fh
+                base = T(0) + ht_inv * fder;                     // Y = 0; Y += Fder/dt
+            } else if (MODE == MODE_JACOBIAN) {
+                base = T(0) + ht_inv * F(0, 0);                  // Y = 0; Y += u/dt
+            } else if (MODE == MODE_TRANSPOSED) {
+                base = (T)a.alpha * (fin[o] * us);
+            } else {
+                base = yout[o];
+            }
+            T out = base;
+            if (MODE == MODE_TRANSPOSED || (j >= 2 && j < nv - 2)) {
+                if (KIND == VV_BRACKET_J1 || KIND == VV_BRACKET_J4) {
+                    // PoissonBracket.pyx:100-111 (J1 / second order)
+                    const T jpp = (F(1, 0) - F(-1, 0)) * (H(0, 1) - H(0, -1))
+                                - (F(0, 1) - F(0, -1)) * (H(1, 0) - H(-1, 0));
+                    const T jpc = F(1, 0) * (H(1, 1) - H(1, -1))
+                                - F(-1, 0) * (H(-1, 1) - H(-1, -1))
+                                - F(0, 1) * (H(1, 1) - H(-1, 1))
+                                + F(0, -1) * (H(1, -1) - H(-1, -1));
+                    const T jcp = F(1, 1) * (H(0, 1) - H(1, 0))
+                                - F(-1, -1) * (H(-1, 0) - H(0, -1))
+                                - F(-1, 1) * (H(0, 1) - H(-1, 0))
+                                + F(1, -1) * (H(1, 0) - H(0, -1));
+                    if (KIND == VV_BRACKET_J1) {
+                        out = base + factor * (jpp + jpc + jcp) * af1;
+                    } else {
+                        // PoissonBracket.pyx:373-389 (J4 = 2 J1 - J2)
+                        const T jcc2 = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
+                                     - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
+                        const T jpc2 = F(2, 0) * (H(1, 1) - H(1, -1))
+                                     - F(-2, 0) * (H(-1, 1) - H(-1, -1))
+                                     - F(0, 2) * (H(1, 1) - H(-1, 1))
+                                     + F(0, -2) * (H(1, -1) - H(-1, -1));
+                        const T jcp2 = F(1, 1) * (H(0, 2) - H(2, 0))
+                                     - F(-1, -1) * (H(-2, 0) - H(0, -2))
+                                     - F(-1, 1) * (H(0, 2) - H(-2, 0))
+                                     + F(1, -1) * (H(2, 0) - H(0, -2));
+                        const T r1 = (jpp + jpc + jcp) * af1;
+                        const T r2 = (jcc2 + jpc2 + jcp2) * af2;
+                        out = base + factor * (T(2) * r1 - r2);
+                    }
+                } else {
+                    // PoissonBracket.pyx:221-234 (J2)
+                    const T jcc = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
+                                - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
+                    const T jpc = F(2, 0) * (H(1, 1) - H(1, -1))
+                                - F(-2, 0) * (H(-1, 1) - H(-1, -1))
+                                - F(0, 2) * (H(1, 1) - H(-1, 1))
+                                + F(0, -2) * (H(1, -1) - H(-1, -1));
+                    const T jcp = F(1, 1) * (H(0, 2) - H(2, 0))
+                                - F(-1, -1) * (H(-2, 0) - H(0, -2))
+                                - F(-1, 1) * (H(0, 2) - H(-2, 0))
+                                + F(1, -1) * (H(2, 0) - H(0, -2));
+                    out = base + factor * (jcc + jpc + jcp) * af2;
+                }
+            }
+            yout[o] = out;
+        }
+        if (q + 1 < ROWS_PER_WAVE) {
+            // move the window one row down (rows stay inside the staged tile)
+            const int cn = rr + 3;    // LDS row of the next output row
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                f0[k] = f0[k + 1];
+                g0[k] = g0[k + 1];
+            }
+            f0[4] = sF[cn + 2][cx];
+            g0[4] = sH[cn + 2][cx];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                fm[k] = fm[k + 1];
+                fp[k] = fp[k + 1];
+                gm[k] = gm[k + 1];
+                gp[k] = gp[k + 1];
+            }
+            fm[2] = sF[cn + 1][cx - 1];
+            fp[2] = sF[cn + 1][cx + 1];
+            gm[2] = sH[cn + 1][cx - 1];
+            gp[2] = sH[cn + 1][cx + 1];
+            fm2 = sF[cn][cx - 2];
+            fp2 = sF[cn][cx + 2];
+            gm2 = sH[cn][cx - 2];
+            gp2 = sH[cn][cx + 2];
+        }
+    }
+#undef F
+#undef H
+#else
 #define F(di, dj) sF[cy + (dj)][cx + (di)]
 #define H(di, dj) sH[cy + (dj)][cx + (di)]
     for (int q = 0; q < ROWS_PER_WAVE; ++q) {
@@ -246,6 +401,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     }
 #undef F
 #undef H
+#endif
 }
 
 template <int MODE, typename T>

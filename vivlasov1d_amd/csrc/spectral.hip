@@ -22,6 +22,16 @@
 // and rounded).
 #include "vv1d_internal.h"
 
+#ifndef VV_FFT_LTW
+#define VV_FFT_LTW 0
+#endif
+#ifndef VV_FFT_XCD
+#define VV_FFT_XCD 1
+#endif
+#ifndef VV_TRI_PREFETCH
+#define VV_TRI_PREFETCH 0
+#endif
+
 namespace vv {
 namespace {
 
@@ -43,16 +53,30 @@ template <typename C> __device__ __forceinline__ C cmul(C a, C b)
     return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename C> __device__ __forceinline__ C cneg(C a) { return cmk<C>(-a.x, -a.y); }
+// 1/d for the complex divisions of the tridiagonal solve.  With VV_FAST_RCP
+// the hardware reciprocal estimate is refined by two Newton steps instead of
+// the IEEE division sequence (shorter dependent chain; a few ulp).
+__device__ __forceinline__ double rcp_of(double d)
+{
+#if defined(VV_FAST_RCP) && VV_FAST_RCP
+    double r = __builtin_amdgcn_rcp(d);
+    r = r * (2.0 - d * r);
+    r = r * (2.0 - d * r);
+    return r;
+#else
+    return 1.0 / d;
+#endif
+}
+__device__ __forceinline__ float rcp_of(float d) { return 1.0f / d; }
+
 template <typename C> __device__ __forceinline__ C cdiv(C z, C w)
 {
-    using R = typename C::value_type;
-    const R inv = R(1) / (w.x * w.x + w.y * w.y);
+    const auto inv = rcp_of(w.x * w.x + w.y * w.y);
     return cmk<C>((z.x * w.x + z.y * w.y) * inv, (z.y * w.x - z.x * w.y) * inv);
 }
 template <typename C> __device__ __forceinline__ C crecip(C w)
 {
-    using R = typename C::value_type;
-    const R inv = R(1) / (w.x * w.x + w.y * w.y);
+    const auto inv = rcp_of(w.x * w.x + w.y * w.y);
     return cmk<C>(w.x * inv, -w.y * inv);
 }
 
@@ -165,23 +189,52 @@ struct FftArgs {
     const int32_t *done;
 };
 
-__device__ __forceinline__ bool cycle_done(const int32_t *done, int64_t gstride)
+__device__ __forceinline__ int32_t cycle_done(const int32_t *done, int64_t gstride)
 {
-    return done != nullptr &&
-           *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(done) + blockIdx.y * gstride) != 0;
+    return done != nullptr ? *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(done) + blockIdx.y * gstride)
+                           : 0;
 }
 
-template <typename T>
+// The twiddle table exp(-2 pi i m/nx), m < nx, copied into LDS at tw_lds in
+// chunks of TWC complex per thread; called after the tile's own global loads
+// are in flight, so both latencies overlap, and the FFT stages then read
+// their twiddles from LDS instead of waiting on global memory after every
+// barrier.
+template <typename C>
+__device__ __forceinline__ void stage_twiddles(const C *__restrict__ tw, C *tw_lds, int nx)
+{
+    constexpr int TWC = 4;
+    for (int m0 = 0; m0 < nx; m0 += TWC * NT) {
+        C t[TWC];
+#pragma unroll
+        for (int q = 0; q < TWC; ++q) {
+            const int m = m0 + threadIdx.x + q * NT;
+            if (m < nx) t[q] = tw[m];
+        }
+#pragma unroll
+        for (int q = 0; q < TWC; ++q) {
+            const int m = m0 + threadIdx.x + q * NT;
+            if (m < nx) tw_lds[m] = t[q];
+        }
+    }
+}
+
+// LTW: the twiddle table is staged in LDS after the packed rows (when both fit
+// in 64 KiB: (R/2 + 1) nx <= 4096), else read from global memory.
+template <typename T, bool LTW>
 __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
-    if (cycle_done(a.done, a.gstride)) return;
+    const int32_t dflag = cycle_done(a.done, a.gstride);
     x += blockIdx.y * a.N;
     spec += blockIdx.y * a.S;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows;
-    const int j0 = blockIdx.x * R;
+    C *stw = buf + (R >> 1) * nx;      // twiddles after the R/2 packed rows (LTW)
+    // XCD-aware row-block order: the mode-major segments of neighbouring row
+    // blocks share L2 lines, so they are kept on one XCD
+    const int j0 = (VV_FFT_XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * R;
     // every global load of the thread is issued before the LDS stores
     constexpr int PER = 2 * MAXE / NT;   // R * nx <= 2 * MAXE real inputs
     T val[PER];
@@ -192,6 +245,8 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         const int i = idx & (nx - 1);
         val[q] = (idx < R * nx && j0 + r < a.nv) ? x[(size_t)(j0 + r) * nx + i] : T(0);
     }
+    if (dflag) return;                 // uniform over the block, before any barrier
+    if (LTW) stage_twiddles(a.tw, stw, nx);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
@@ -203,7 +258,8 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         }
     }
     __syncthreads();
-    lds_fft<false>(buf, nx, a.logn, R >> 1, a.tw);
+    if (LTW) lds_fft<false>(buf, nx, a.logn, R >> 1, stw);
+    else lds_fft<false>(buf, nx, a.logn, R >> 1, a.tw);
     const int nk = (nx >> 1) + 1;
     const T half = T(0.5);
     for (int idx = threadIdx.x; idx < nk * R; idx += NT) {
@@ -220,17 +276,18 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
     }
 }
 
-template <typename T>
+template <typename T, bool LTW>
 __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
 {
     using C = cx_t<T>;
-    if (cycle_done(a.done, a.gstride)) return;
+    const int32_t dflag = cycle_done(a.done, a.gstride);
     spec += blockIdx.y * a.S;
     y += blockIdx.y * a.N;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows, nh = nx >> 1;
-    const int j0 = blockIdx.x * R;
+    C *stw = buf + (R >> 1) * nx;
+    const int j0 = (VV_FFT_XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * R;   // as in rfft_kernel
     const int npair = R >> 1;
     // Z_k = A_k + i B_k (k <= nx/2) and its Hermitian extension for k > nx/2;
     // every global load of the thread is issued before the LDS stores.
@@ -246,6 +303,8 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         va[q] = (idx < total && ja < a.nv) ? spec[(size_t)k * a.nv + ja] : zero;
         vb[q] = (idx < total && ja + 1 < a.nv) ? spec[(size_t)k * a.nv + ja + 1] : zero;
     }
+    if (dflag) return;                 // uniform over the block, before any barrier
+    if (LTW) stage_twiddles(a.tw, stw, nx);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
@@ -263,7 +322,8 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         }
     }
     __syncthreads();
-    lds_fft<true>(buf, nx, a.logn, npair, a.tw);
+    if (LTW) lds_fft<true>(buf, nx, a.logn, npair, stw);
+    else lds_fft<true>(buf, nx, a.logn, npair, a.tw);
     const T inv_n = T(1) / (T)nx;   // exact: nx is a power of two
     for (int idx = threadIdx.x; idx < R * nx; idx += NT) {
         const int r = idx >> a.logn;
@@ -294,28 +354,34 @@ struct TriArgs {
 // conj(b_j), super conj(a_{j+1})), the mode-k block of the transposed real
 // operator P^T (P^-T x = irfft(T_k^-H rfft(x)): transposing a real
 // x-circulant operator maps its symbol at k to the conjugate at -k).
-template <bool TR, typename C>
-__device__ __forceinline__ void tri_row(const TriArgs &a, double af, double2 e, int j, C &aj, C &bj, C &cj)
+// the three real per-row factors row j of T_k (or of T_k^H, TR) uses
+template <bool TR>
+__device__ __forceinline__ void tri_load(const TriArgs &a, int j, double &lo, double &mid, double &hi)
 {
-    using R = typename C::value_type;
+    lo = mid = hi = 0.0;
     if (j < a.nv) {
         if (!TR) {
-            const double g = a.gm[j];
-            aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * g) * af), (R)(((e.y * 0.5) * g) * af)) : cmk<C>(R(0), R(0));
-            const double g0 = a.g0[j];
-            bj = cmk<C>((R)(((e.x * 1.0) * g0) * af + a.ht_inv), (R)(((e.y * 1.0) * g0) * af));
-            const double gp = a.gp[j];
-            cj = (j < a.nv - 1) ? cmk<C>((R)(((e.x * 0.5) * gp) * af), (R)(((e.y * 0.5) * gp) * af))
-                                : cmk<C>(R(0), R(0));
+            lo = a.gm[j];
+            hi = a.gp[j];
         } else {
-            aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * a.gp[j - 1]) * af), (R)(-(((e.y * 0.5) * a.gp[j - 1]) * af)))
-                         : cmk<C>(R(0), R(0));
-            const double g0 = a.g0[j];
-            bj = cmk<C>((R)(((e.x * 1.0) * g0) * af + a.ht_inv), (R)(-(((e.y * 1.0) * g0) * af)));
-            cj = (j < a.nv - 1)
-                     ? cmk<C>((R)(((e.x * 0.5) * a.gm[j + 1]) * af), (R)(-(((e.y * 0.5) * a.gm[j + 1]) * af)))
-                     : cmk<C>(R(0), R(0));
+            lo = (j > 0) ? a.gp[j - 1] : 0.0;
+            hi = (j < a.nv - 1) ? a.gm[j + 1] : 0.0;
         }
+        mid = a.g0[j];
+    }
+}
+
+template <bool TR, typename C>
+__device__ __forceinline__ void tri_row(const TriArgs &a, double af, double2 e, int j, double lo, double mid,
+                                        double hi, C &aj, C &bj, C &cj)
+{
+    using R = typename C::value_type;
+    const double ey = TR ? -e.y : e.y;
+    if (j < a.nv) {
+        aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * lo) * af), (R)(((ey * 0.5) * lo) * af)) : cmk<C>(R(0), R(0));
+        bj = cmk<C>((R)(((e.x * 1.0) * mid) * af + a.ht_inv), (R)(((ey * 1.0) * mid) * af));
+        cj = (j < a.nv - 1) ? cmk<C>((R)(((e.x * 0.5) * hi) * af), (R)(((ey * 0.5) * hi) * af))
+                            : cmk<C>(R(0), R(0));
     } else {
         aj = cmk<C>(R(0), R(0));
         bj = cmk<C>(R(1), R(0));
@@ -327,7 +393,7 @@ template <int M, typename T, bool TR>
 __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
-    if (cycle_done(a.done, a.gstride)) return;
+    const int32_t dflag = cycle_done(a.done, a.gstride);
     spec += blockIdx.y * a.S;
     const double af = a.inst[blockIdx.y].afpc;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -338,6 +404,15 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     const int P = a.P, n = P * M;
     const C zero = cmk<C>(T(0), T(0));
     C *S = spec + (size_t)k * a.nv;
+    const int c = threadIdx.x;
+    // the per-row factors of the thread's chunk, loaded with the spectrum so
+    // the sequential sweep below never waits on memory
+    double clo[M], cmid[M], chi[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        clo[q] = cmid[q] = chi[q] = 0.0;
+        if (VV_TRI_PREFETCH && c < P) tri_load<TR>(a, c * M + q, clo[q], cmid[q], chi[q]);
+    }
     {
         // n = P*M <= NT*M: every load of the thread in flight before the LDS stores
         C v[M];
@@ -346,6 +421,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
             const int j = threadIdx.x + q * NT;
             v[q] = (j < a.nv) ? S[j] : zero;
         }
+        if (dflag) return;             // uniform over the block, before any barrier
 #pragma unroll
         for (int q = 0; q < M; ++q) {
             const int j = threadIdx.x + q * NT;
@@ -354,21 +430,22 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     }
     __syncthreads();
 
-    const int c = threadIdx.x;
     C Am[M], Cm[M], Dm[M];
     C ae, be, ce;          // coefficients of the chunk's last row
     if (c < P) {
         const int s = c * M, ls = c * (M + 1);
         // forward sweep: y_j + c'_j y_{j+1} = d'_j - a'_j y_{s-1}
         C aj, bj, cj;
-        tri_row<TR>(a, af, e, s, aj, bj, cj);
+        if (!VV_TRI_PREFETCH) tri_load<TR>(a, s, clo[0], cmid[0], chi[0]);
+        tri_row<TR>(a, af, e, s, clo[0], cmid[0], chi[0], aj, bj, cj);
         C r = crecip(bj);
         Cm[0] = cmul(cj, r);
         Dm[0] = cmul(sd[ls], r);
         Am[0] = cmul(aj, r);
 #pragma unroll
         for (int q = 1; q < M; ++q) {
-            tri_row<TR>(a, af, e, s + q, aj, bj, cj);
+            if (!VV_TRI_PREFETCH) tri_load<TR>(a, s + q, clo[q], cmid[q], chi[q]);
+            tri_row<TR>(a, af, e, s + q, clo[q], cmid[q], chi[q], aj, bj, cj);
             if (q == M - 1) {
                 ae = aj;
                 be = bj;
@@ -455,13 +532,18 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     for (int j = threadIdx.x; j < a.nv; j += NT) S[j] = sd[j + j / M];
 }
 
+// twiddles fit in LDS beside R/2 packed complex rows (64 KiB per workgroup)
+bool fft_lds_twiddles(int nx, int R) { return VV_FFT_LTW && ((R >> 1) + 1) * nx <= 4096; }
+
 // v-rows per FFT workgroup: the context's tuned value, else the largest that
-// fits LDS while the launch keeps >= 512 workgroups
+// fits LDS with the twiddles (or, for nx = 4096, without) while the launch
+// keeps >= 512 workgroups
 int fft_rows(const vv_ctx *c)
 {
     if (c->tune_fft_rows > 0) return c->tune_fft_rows;
     const int nx = c->p.nx, nv = c->p.nv;
-    int R = 8192 / nx;                 // LDS: R/2 complex rows of nx = R*nx*8 bytes <= 64 KiB
+    int R = 2;
+    while (((R + 1) * nx <= 4096 || !VV_FFT_LTW) && 2 * R * nx <= 8192) R *= 2;
     if (R < 2) R = 2;
     while (R > 2 && (nv + R - 1) / R < 512) R >>= 1;
     return R;
@@ -473,8 +555,13 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
     FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
-    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    launch_timed(c, rfft_kernel<T>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    if (fft_lds_twiddles(a.nx, a.rows)) {
+        const size_t lds = (size_t)((a.rows >> 1) + 1) * a.nx * sizeof(cx_t<T>);
+        launch_timed(c, rfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    } else {
+        const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
+        launch_timed(c, rfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    }
     return hipGetLastError();
 }
 
@@ -484,8 +571,13 @@ hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, cons
     FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
-    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    launch_timed(c, irfft_kernel<T>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    if (fft_lds_twiddles(a.nx, a.rows)) {
+        const size_t lds = (size_t)((a.rows >> 1) + 1) * a.nx * sizeof(cx_t<T>);
+        launch_timed(c, irfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    } else {
+        const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
+        launch_timed(c, irfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    }
     return hipGetLastError();
 }
 

@@ -98,6 +98,7 @@ struct vv_ctx {
     vv::GmresState *gs;             // B states
     vv::GmresStatus *host_status;   // pinned, 2 slots of B statuses
     double *host_scalars;           // pinned staging of device scalars
+    int32_t *host_active;           // pinned staging of the instance mask
     hipEvent_t ev[2];
     // profiling (per-kernel HIP event pairs)
     int prof;           // bit k set: time kernel kind k

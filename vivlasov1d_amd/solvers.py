@@ -244,7 +244,7 @@ class Context:
 
     # mixed precision ----------------------------------------------------------
     def F32(self, t, name="field"):
-        return _dev_ptr(t, self.N, name, torch.float32)
+        return _dev_ptr(t, self.N * self.B, name, torch.float32)
 
     def jacobian_apply32(self, u, y):
         self.call("vv_jacobian_apply32", self.F32(u), self.F32(y))
