@@ -88,12 +88,16 @@ def cpu_baseline(w, sample_steps, threads):
     return cbaseline.time_sample(w, sample_steps, threads)
 
 
-def timed_run(w, args, precision, kernel):
+def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None):
     """W warm-up steps, then K timed steps; returns the run and its figures."""
     import torch
     from vivlasov1d_amd.runner import Runscript
     nt = args.warmup + args.steps
-    run = Runscript(make_cfg(w, nt), profile=w["profile"], precision=precision, inner_rtol=args.inner_rtol)
+    run = Runscript(make_cfg(w, nt), profile=w["profile"], precision=precision, inner_rtol=args.inner_rtol,
+                    refine=refine)
+    if ksp_rtol is not None:
+        run.snes.getKSP().setTolerances(rtol=ksp_rtol)
+    run.inv0 = run.invariants()
     ctx = run.ctx
     for _ in range(args.warmup):
         run.step()
@@ -115,6 +119,72 @@ def timed_run(w, args, precision, kernel):
         if n:
             kstats[k] = dict(ms=ms, launches=n, bytes=by, avg_us=1e3 * ms / n, gbs=by / (ms * 1e-3) / 1e9)
     return run, t1 - t0, run.history[nrec0:], kstats
+
+
+def _its(recs):
+    import numpy as np
+    gm = [p["gmres"] for r in recs for p in r["picard"]]
+    return {"picard_per_step": float(np.mean([len(r["picard"]) for r in recs])),
+            "gmres_per_solve": float(np.mean(gm)), "gmres_per_step": float(sum(gm)) / max(1, len(recs))}
+
+
+def mixed_figures(w, args, run64, value64):
+    """The mixed-precision form (FP32 GMRES cycles and P^-1 inside the FP64
+    Picard residual / update / Poisson loop, which refines to snes_atol),
+    timed like the FP64 run, with two references beside it: FP64 with the same
+    linear tolerance (separates the precision from the tolerance) and the
+    KSP-refined mixed form (each solve refined in FP64 to ksp_rtol).  Both the
+    FP64 and the mixed run then continue untimed to --compare-steps, where
+    f, phi and the invariant drift are compared."""
+    import numpy as np
+    import torch
+    mrun, melapsed, mrecs, mk = timed_run(w, args, "mixed", args.mixed_kernel)
+    mval = args.steps / melapsed
+    mcy = [p["cycles"] for r in mrecs for p in r["picard"]]
+    out = {
+        "value": round(mval, 3), "unit": "timesteps/s", "ms_per_step": round(1e3 * melapsed / args.steps, 3),
+        "speedup_vs_fp64": round(mval / value64, 3),
+        "dtype": "f32 GMRES/P^-1/J.u (f64 dots, Hessenberg) inside the f64 Picard residual, update and phi",
+        "refine": "newton", "inner_rtol": args.inner_rtol, "snes_atol": 1e-10,
+        "iterations": dict(_its(mrecs), cycles_per_solve=float(np.mean(mcy))),
+        "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
+                        "GB/s": round(v["gbs"], 1), "ms": round(v["ms"], 3)} for k, v in mk.items()},
+    }
+    if not args.no_mixed_refs:
+        srun, sel, srecs, _ = timed_run(w, args, "double", None, ksp_rtol=args.inner_rtol)
+        out["fp64_same_linear_rtol"] = {"value": round(args.steps / sel, 3), "ksp_rtol": args.inner_rtol,
+                                        "iterations": _its(srecs)}
+        del srun
+        krun, kel, krecs, _ = timed_run(w, args, "mixed", None, refine="ksp")
+        kcy = [p["cycles"] for r in krecs for p in r["picard"]]
+        out["ksp_refined"] = {"value": round(args.steps / kel, 3), "speedup_vs_fp64": round(args.steps / kel / value64, 3),
+                              "ksp_rtol": 1e-10, "iterations": dict(_its(krecs), cycles_per_solve=float(np.mean(kcy)))}
+        del krun
+        torch.cuda.empty_cache()
+    # continue both runs (untimed) to the comparison step
+    done = args.warmup + args.steps
+    for _ in range(max(0, args.compare_steps - done)):
+        run64.step()
+        mrun.step()
+    torch.cuda.synchronize()
+    n = max(done, args.compare_steps)
+    phid = run64.pc_int - run64.pc_int.mean()
+    phim = mrun.pc_int - mrun.pc_int.mean()
+    i64, imx = run64.invariants(), mrun.invariants()
+    drift = lambda r, i: {k: (i[k] - r.inv0[k]) / abs(r.inv0[k]) for k in ("mass", "l2", "energy")}
+    out["compare"] = {
+        "steps": n,
+        "linf_f_vs_fp64": float((mrun.fc - run64.fc).abs().max() / run64.fc.abs().max()),
+        "linf_phi_vs_fp64": float((phim - phid).abs().max() / max(phid.abs().max().item(), 1e-300)),
+        "drift_fp64": drift(run64, i64), "drift_mixed": drift(mrun, imx),
+        "picard_per_step_fp64": float(np.mean([len(r["picard"]) for r in run64.history])),
+        "picard_per_step_mixed": float(np.mean([len(r["picard"]) for r in mrun.history])),
+        "reverted_fp64": sum(1 for r in run64.history if r.get("reverted")),
+        "reverted_mixed": sum(1 for r in mrun.history if r.get("reverted")),
+    }
+    del mrun
+    torch.cuda.empty_cache()
+    return out
 
 
 def batched_figures(args):
@@ -179,6 +249,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
     ap.add_argument("--mixed-kernel", default="none", help="kernel timed in the mixed run (e.g. mdot32)")
     ap.add_argument("--inner-rtol", type=float, default=1e-5, help="FP32 cycle reduction of the mixed form")
+    ap.add_argument("--no-mixed-refs", action="store_true",
+                    help="skip the FP64-at-the-same-tolerance and KSP-refined mixed references")
+    ap.add_argument("--compare-steps", type=int, default=1000,
+                    help="timestep at which the mixed and FP64 runs are compared (f, phi, invariant drift)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble timing")
     ap.add_argument("--no-sequential", action="store_true", help="skip the sequential run of the ensemble")
     ap.add_argument("--batched-instances", type=int, default=256)
@@ -236,30 +310,9 @@ def main():
 
     mixed = None
     if not args.no_mixed:
-        f64 = run.fc.clone()
-        phi64 = run.pc_int.clone()
+        mixed = mixed_figures(w, args, run, value)
         del run
         torch.cuda.empty_cache()
-        mrun, melapsed, mrecs, mk = timed_run(w, args, "mixed", args.mixed_kernel)
-        minv = mrun.invariants()
-        phim = mrun.pc_int - mrun.pc_int.mean()
-        phid = phi64 - phi64.mean()
-        mgm = [p["gmres"] for r in mrecs for p in r["picard"]]
-        mcy = [p["cycles"] for r in mrecs for p in r["picard"]]
-        mval = args.steps / melapsed
-        mixed = {
-            "value": round(mval, 3), "unit": "timesteps/s", "ms_per_step": round(1e3 * melapsed / args.steps, 3),
-            "speedup_vs_fp64": round(mval / value, 3), "dtype": "f32 Krylov/P^-1/J.u in f64 refinement",
-            "inner_rtol": args.inner_rtol,
-            "linf_f_vs_fp64": float((mrun.fc - f64).abs().max() / f64.abs().max()),
-            "linf_phi_vs_fp64": float((phim - phid).abs().max() / max(phid.abs().max().item(), 1e-300)),
-            "steps_compared": args.warmup + args.steps,
-            "invariants": {k: minv[k] for k in ("mass", "l2", "energy")},
-            "iterations": {"picard_per_step": float(np.mean([len(r["picard"]) for r in mrecs])),
-                           "fp32_its_per_solve": float(np.mean(mgm)), "cycles_per_solve": float(np.mean(mcy))},
-            "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
-                            "GB/s": round(v["gbs"], 1), "ms": round(v["ms"], 3)} for k, v in mk.items()},
-        }
     batched = None
     if not args.no_batched:
         batched = batched_figures(args)

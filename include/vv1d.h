@@ -245,10 +245,19 @@ int vv_snes_ksponly(vv_ctx *ctx, double *f, const double *fh,
  * r = b - P^-1 J x, the solution x and the stopping test
  * ||r|| <= max(rtol ||b||, atol) stay FP64, on the true preconditioned
  * residual.  res->its counts FP32 iterations, res->rnorm is the true final
- * residual norm; *cycles (may be NULL) the FP32 cycles run. */
+ * residual norm; *cycles (may be NULL) the FP32 cycles run.
+ *
+ * fp64_check = 0 is the form for an outer loop that evaluates the FP64
+ * residual itself (the Picard iteration of the timestep, which then acts as
+ * the refinement): the first FP32 cycle runs with the KSP's own rtol / atol
+ * / dtol / max_it and, when it ends with a reason, that reason, its count
+ * and its (FP64-accumulated) residual estimate are the result, with no FP64
+ * operator applied.  Only a cycle that exhausts the restart length without
+ * a reason continues with the FP64-checked refinement above. */
 typedef struct {
     double  inner_rtol;    /* relative reduction per FP32 cycle, e.g. 1e-5  */
     int32_t max_cycles;    /* cap on FP32 cycles per solve                  */
+    int32_t fp64_check;    /* 1: FP64 residual after every cycle; 0: see above */
 } vv_mixed_params;
 
 int vv_ksp_solve_mixed(vv_ctx *ctx, const double *b, double *x, const vv_ksp_params *kp,

@@ -65,3 +65,32 @@ def test_create_rejects_bad_parameters(lib, field, value, msg):
 def test_null_context_is_an_error(lib):
     assert lib.vv_residual(None, None, None, None) == -1
     assert lib.vv_synchronize(None) == -1
+
+
+def test_mixed_params_layout():
+    """vv_mixed_params: double inner_rtol; int32 max_cycles; int32 fp64_check."""
+    from vivlasov1d_amd import _lib
+    M = _lib.MixedParams
+    assert ctypes.sizeof(M) == 16
+    assert (M.inner_rtol.offset, M.max_cycles.offset, M.fp64_check.offset) == (0, 8, 12)
+
+
+def test_ksp_mixed_refinement_arguments():
+    """refine='newton' solves each system by one FP32 cycle to max(rtol,
+    inner_rtol) with no FP64 check (the Picard loop refines); refine='ksp'
+    keeps the KSP rtol and refines on the FP64 residual."""
+    from vivlasov1d_amd import _lib
+    from vivlasov1d_amd.solvers import KSP
+    k = KSP()
+    k.setTolerances(rtol=1e-10, atol=1e-50, max_it=1000)
+    k.setPrecision('mixed', inner_rtol=1e-5)
+    assert k.mixed_args() == (1e-5, 1e-50, 1e5, 1000, 1e-5, 50, False)
+    k.setTolerances(rtol=1e-3)
+    assert k.mixed_args()[0] == 1e-3
+    k.setTolerances(rtol=1e-10)
+    k.setPrecision('mixed', refine='ksp', max_cycles=7)
+    assert k.mixed_args() == (1e-10, 1e-50, 1e5, 1000, 1e-5, 7, True)
+    with pytest.raises(_lib.VVError):
+        k.setPrecision('mixed', refine='sometimes')
+    with pytest.raises(_lib.VVError):
+        k.setPrecision('half')

@@ -9,8 +9,15 @@ Bars (each written in the test):
   * Mixed KSP: converged reason, true FP64 preconditioned residual
     <= rtol ||b||, and the solution within 1e-8 (relative, max norm) of the
     oracle's FP64 GMRES solution.
-  * Mixed Picard steps: f within 1e-9 relative of the FP64 oracle run and
-    the invariants within 1e-9 relative, at ksp_rtol = snes_atol = 1e-10.
+  * Single FP32 cycle (fp64_check off, the Picard-refined form): converged
+    reason, one cycle, the FP64 GMRES iteration count at the same rtol
+    within one, true FP64 residual <= 1.1 rtol ||b|| (the FP32 operator's
+    error is ~2e-7 of ||b||); a cycle that exhausts the restart length
+    continues with FP64-checked refinement to the full rtol.
+  * Mixed Picard steps, both refinement forms: f within 1e-9 relative of the
+    FP64 oracle run and the invariants within 1e-9 relative, at
+    ksp_rtol = snes_atol = 1e-10 (the Picard-refined form solves each system
+    to inner_rtol = 1e-5 and reaches the same snes_atol).
 """
 
 import math
@@ -105,8 +112,41 @@ def test_mixed_gmres_limits(cuda):
     assert torch.count_nonzero(xd).item() == 0
 
 
+@pytest.mark.parametrize("nx,nv,kind,restart,rtol", [(64, 128, 'arakawaj1', 30, 1e-5),
+                                                      (256, 512, 'arakawaj4', 30, 1e-5),
+                                                      (256, 512, 'arakawaj4', 30, 1e-3),
+                                                      (64, 128, 'arakawaj4', 3, 1e-10)])
+def test_mixed_single_cycle(nx, nv, kind, restart, rtol, cuda):
+    print("case", nx, nv, kind, restart, rtol, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, kind, 17, cuda, restart=restart)
+    pc = O.Preconditioner(g, 'lapack')
+    f = rng.random((nx, nv))
+    f[:, :2] = 0
+    f[:, -2:] = 0
+    fh = f + 1e-3 * rng.standard_normal((nx, nv))
+    b = pc.apply(O.residual(kind, f, fh, H, g))
+    A = lambda v: pc.apply(O.jacobian(kind, v, H, g))
+    bd = T(_nat(b))
+    x64 = ctx.new_field()
+    r64 = ctx.ksp_solve(bd, x64, rtol, 1e-50, 1e5, 1000)
+    assert r64.reason > 0
+    xd = ctx.new_field()
+    res, cycles = ctx.ksp_solve_mixed(bd, xd, rtol, 1e-50, 1e5, 1000, inner_rtol=1e-5, fp64_check=False)
+    print("its", res.its, "cycles", cycles, "fp64 its", r64.its, flush=True)
+    assert res.reason in (O.KSP_CONVERGED_RTOL, O.KSP_CONVERGED_ATOL), res.reason
+    assert abs(res.rnorm0 - np.linalg.norm(b)) <= 1e-13 * np.linalg.norm(b)
+    r = b - A(xd.cpu().numpy().T)
+    if r64.its < restart:
+        assert cycles == 1 and abs(res.its - r64.its) <= 1
+        assert np.linalg.norm(r) <= 1.1 * rtol * np.linalg.norm(b)
+    else:
+        assert cycles >= 2
+        assert np.linalg.norm(r) <= rtol * np.linalg.norm(b) * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("refine", ['newton', 'ksp'])
 @pytest.mark.parametrize("kind", KINDS)
-def test_mixed_picard_steps_match_oracle(kind, cuda):
+def test_mixed_picard_steps_match_oracle(kind, refine, cuda):
     """Three implicit timesteps of the Landau problem on 32x64 with the mixed
     solver against the FP64 oracle run."""
     from vivlasov1d_amd.core import Config
@@ -120,7 +160,7 @@ def test_mixed_picard_steps_match_oracle(kind, cuda):
                              'preconditioner_scheme': 'tensor_kinetic', 'petsc_ksp_rtol': 1e-10,
                              'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
                   'initial_data': {'charge': -1.0}})
-    run = Runscript(cfg, profile=prof, precision='mixed')
+    run = Runscript(cfg, profile=prof, precision='mixed', refine=refine)
     for s in range(3):
         eo = orun.step()
         eg = run.step()
@@ -137,7 +177,8 @@ def test_mixed_picard_steps_match_oracle(kind, cuda):
         assert abs(inv[k] - ref[k]) <= 1e-9 * abs(ref[k]) + 1e-14, k
 
 
-def test_mixed_twostream_step_matches_fp64(cuda):
+@pytest.mark.parametrize("refine", ['newton', 'ksp'])
+def test_mixed_twostream_step_matches_fp64(refine, cuda):
     """One timestep of the two-stream workload at 256x512 J4: mixed against
     the FP64 GPU path (itself pinned to the oracle above)."""
     from vivlasov1d_amd.core import Config
@@ -154,7 +195,7 @@ def test_mixed_twostream_step_matches_fp64(cuda):
                                      'petsc_ksp_rtol': 1e-10, 'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
                           'initial_data': {'charge': -1.0}})
     r64 = Runscript(cfg(), profile=prof)
-    rmx = Runscript(cfg(), profile=prof, precision='mixed')
+    rmx = Runscript(cfg(), profile=prof, precision='mixed', refine=refine)
     for s in range(2):
         e64 = r64.step()
         emx = rmx.step()

@@ -43,7 +43,7 @@ class KspResult(ctypes.Structure):
 
 
 class MixedParams(ctypes.Structure):
-    _fields_ = [("inner_rtol", ctypes.c_double), ("max_cycles", ctypes.c_int32)]
+    _fields_ = [("inner_rtol", ctypes.c_double), ("max_cycles", ctypes.c_int32), ("fp64_check", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p

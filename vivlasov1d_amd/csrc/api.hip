@@ -520,13 +520,37 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
     if (int r = ensure_mixed(c)) return r;
     const int64_t N = c->N;
     VV_CHECK(launch_zero(c, x, N), "zero x");
-    VV_CHECK(launch_norm2(c, b, N, ds_norms(c)), "norm");
-    double rnorm0 = 0.0;
-    if (int r = read_scalars(c, ds_norms(c), 1, &rnorm0)) return r;
-    const double ttol = std::fmax(kp->rtol * rnorm0, kp->atol);
-    double rnorm = rnorm0;
+    double rnorm0 = 0.0, rnorm = 0.0;
     const double *rsrc = b;
     int its = 0, rounds = 0, reason = 0;
+    if (!mp->fp64_check) {
+        // one FP32 cycle with the KSP's own tolerances; its reason stands
+        VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
+        VV_CHECK(launch_start32(c, b), "cycle start");
+        if (int r = run_cycle(c, true)) return r;
+        VV_CHECK(launch_build_solution32(c, x), "build solution");
+        const GmresStatus st = status_slot(c, 0)[0];
+        if (st.reason != 0) {
+            fill_results(c, res);
+            if (cycles) *cycles = st.it > 0 ? 1 : 0;
+            return VV_OK;
+        }
+        // the restart length ran out: FP64-checked refinement from here
+        its = st.it;
+        rounds = 1;
+        rnorm0 = st.rnorm0;
+        VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
+        if (int r = precond(c, c->X, c->R, nullptr)) return r;
+        VV_CHECK(launch_residual_restart(c, b, c->R), "residual");
+        VV_CHECK(launch_norm2(c, c->R, N, ds_norms(c)), "norm");
+        if (int r = read_scalars(c, ds_norms(c), 1, &rnorm)) return r;
+        rsrc = c->R;
+    } else {
+        VV_CHECK(launch_norm2(c, b, N, ds_norms(c)), "norm");
+        if (int r = read_scalars(c, ds_norms(c), 1, &rnorm0)) return r;
+        rnorm = rnorm0;
+    }
+    const double ttol = std::fmax(kp->rtol * rnorm0, kp->atol);
     for (;;) {
         if (!std::isfinite(rnorm)) {
             reason = VV_KSP_DIVERGED_NANORINF;

@@ -574,9 +574,9 @@ __global__ __launch_bounds__(NT) void cvt_partial_kernel(const double *__restric
     partial += blockIdx.y * pstride;
     double acc = 0.0;
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < len; e += (int64_t)gridDim.x * NT) {
-        const float f = (float)r[e];
-        v0[e] = f;
-        acc += (double)f * (double)f;
+        const double d = r[e];
+        v0[e] = (float)d;
+        acc += d * d;          // ||r|| of the FP64 right-hand side
     }
     const double s = block_sum_all(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;

@@ -26,7 +26,7 @@ class ResidualTooLarge(RuntimeError):
 
 class Runscript:
     def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False, precision='double',
-                 inner_rtol=1e-5, normalise_f0=True):
+                 inner_rtol=1e-5, normalise_f0=True, refine='newton'):
         if not isinstance(cfg, Config):
             cfg = Config(cfg)
         self.cfg = cfg
@@ -68,9 +68,12 @@ class Runscript:
         self.snes.setFunction(self.vlasov_solver.function_snes_mult, self.fb)
         self.snes.getKSP().setTolerances(rtol=sol['petsc_ksp_rtol'], atol=sol['petsc_ksp_atol'],
                                          max_it=sol['petsc_ksp_max_iter'])
-        # 'mixed': FP32 GMRES cycles inside FP64 iterative refinement; the
-        # residual, update, moments and Poisson solve stay FP64
-        self.snes.getKSP().setPrecision(precision, inner_rtol=inner_rtol)
+        # 'mixed': FP32 GMRES cycles; the residual, update, moments and
+        # Poisson solve stay FP64.  refine='newton': the Picard loop below is
+        # the refinement (one FP32 cycle per solve, to inner_rtol, until the
+        # FP64 residual meets snes_atol); 'ksp': each solve refined in FP64 to
+        # petsc_ksp_rtol
+        self.snes.getKSP().setPrecision(precision, inner_rtol=inner_rtol, refine=refine)
         self.precision = precision
         self.snes_atol = sol['petsc_snes_atol']
         self.snes_max_iter = sol['petsc_snes_max_iter']

@@ -252,19 +252,21 @@ class Context:
     def precond32(self, x, y):
         self.call("vv_precond_apply32", self.F32(x), self.F32(y))
 
-    def ksp_solve_mixed(self, b, x, rtol, atol, dtol, max_it, inner_rtol=1e-5, max_cycles=50):
-        """Iterative refinement with FP32 GMRES cycles; returns (KspResult, cycles)."""
+    def ksp_solve_mixed(self, b, x, rtol, atol, dtol, max_it, inner_rtol=1e-5, max_cycles=50, fp64_check=True):
+        """FP32 GMRES cycles; returns (KspResult, cycles).  fp64_check: refine
+        on the true FP64 residual after every cycle; False: one FP32 cycle with
+        the given tolerances decides (for an outer loop that refines)."""
         kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
-        mp = _lib.MixedParams(inner_rtol, int(max_cycles))
+        mp = _lib.MixedParams(inner_rtol, int(max_cycles), 1 if fp64_check else 0)
         res = _lib.KspResult()
         cyc = ctypes.c_int32()
         self.call("vv_ksp_solve_mixed", self.F(b), self.F(x), ctypes.byref(kp), ctypes.byref(mp),
                   ctypes.byref(res), ctypes.byref(cyc))
         return res, cyc.value
 
-    def snes_ksponly_mixed(self, f, fh, rtol, atol, dtol, max_it, inner_rtol=1e-5, max_cycles=50):
+    def snes_ksponly_mixed(self, f, fh, rtol, atol, dtol, max_it, inner_rtol=1e-5, max_cycles=50, fp64_check=True):
         kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
-        mp = _lib.MixedParams(inner_rtol, int(max_cycles))
+        mp = _lib.MixedParams(inner_rtol, int(max_cycles), 1 if fp64_check else 0)
         res = _lib.KspResult()
         cyc = ctypes.c_int32()
         self.call("vv_snes_ksponly_mixed", self.F(f), self.F(fh), ctypes.byref(kp), ctypes.byref(mp),
@@ -561,21 +563,38 @@ class KSP:
         self.ctx = None
         self.type = 'gmres'
         self._res = None
-        # 'double': the reference's FP64 GMRES; 'mixed': FP64 iterative
-        # refinement around FP32 GMRES cycles (same stopping test)
+        # 'double': the reference's FP64 GMRES.  'mixed': FP32 GMRES cycles,
+        # refined either by the nonlinear (Picard) iteration around the solve
+        # (refine='newton': each solve is one FP32 cycle to the relative
+        # tolerance max(rtol, inner_rtol); the FP64 residual F, the update and
+        # the Poisson solve of the outer loop drive the error to snes_atol) or
+        # inside the solve (refine='ksp': FP64 iterative refinement on the true
+        # preconditioned residual to the KSP's own rtol).
         self.precision = 'double'
+        self.refine = 'newton'
         self.inner_rtol = 1e-5
         self.max_cycles = 50
         self.cycles = 0
 
-    def setPrecision(self, precision, inner_rtol=None, max_cycles=None):
+    def setPrecision(self, precision, inner_rtol=None, max_cycles=None, refine=None):
         if precision not in ('double', 'mixed'):
             raise _lib.VVError("precision must be 'double' or 'mixed'")
+        if refine is not None and refine not in ('newton', 'ksp'):
+            raise _lib.VVError("refine must be 'newton' or 'ksp'")
         self.precision = precision
+        if refine is not None:
+            self.refine = refine
         if inner_rtol is not None:
             self.inner_rtol = float(inner_rtol)
         if max_cycles is not None:
             self.max_cycles = int(max_cycles)
+
+    def mixed_args(self):
+        """(rtol, atol, dtol, max_it, inner_rtol, max_cycles, fp64_check) of a mixed solve."""
+        if self.refine == 'newton':
+            return (max(self.rtol, self.inner_rtol), self.atol, self.dtol, self.max_it, self.inner_rtol,
+                    self.max_cycles, False)
+        return (self.rtol, self.atol, self.dtol, self.max_it, self.inner_rtol, self.max_cycles, True)
 
     def setType(self, t):
         if t != 'gmres':
@@ -600,8 +619,7 @@ class KSP:
 
     def solve(self, b, x):
         if self.precision == 'mixed':
-            self._res, self.cycles = self.ctx.ksp_solve_mixed(b, x, self.rtol, self.atol, self.dtol, self.max_it,
-                                                              self.inner_rtol, self.max_cycles)
+            self._res, self.cycles = self.ctx.ksp_solve_mixed(b, x, *self.mixed_args())
         else:
             self._res = self.ctx.ksp_solve(b, x, self.rtol, self.atol, self.dtol, self.max_it)
 
@@ -654,8 +672,7 @@ class SNES:
             raise _lib.VVError("the native SNES path needs the tensor_kinetic preconditioner")
         k = self.ksp
         if k.precision == 'mixed':
-            self.last, k.cycles = s.ctx.snes_ksponly_mixed(x, s.Fh, k.rtol, k.atol, k.dtol, k.max_it,
-                                                           k.inner_rtol, k.max_cycles)
+            self.last, k.cycles = s.ctx.snes_ksponly_mixed(x, s.Fh, *k.mixed_args())
         else:
             self.last = s.ctx.snes_ksponly(x, s.Fh, k.rtol, k.atol, k.dtol, k.max_it)
         k._res = self.last
