@@ -80,6 +80,15 @@ template <typename C> __device__ __forceinline__ C crecip(C w)
     return cmk<C>(w.x * inv, -w.y * inv);
 }
 
+// p[i] when ok, else zero.  The address is clamped to p[0] instead of
+// selecting between p + i and a local zero, so the load stays an
+// unconditional global load that can be in flight with the others.
+template <typename C> __device__ __forceinline__ C load_or_zero(const C *__restrict__ p, int64_t i, bool ok)
+{
+    const C v = p[ok ? i : 0];
+    return ok ? v : cmk<C>(typename C::value_type(0), typename C::value_type(0));
+}
+
 // complex type of a real type
 template <typename T> struct CxOf;
 template <> struct CxOf<double> { using type = double2; };
@@ -98,8 +107,9 @@ __device__ __forceinline__ C twiddle(const C *__restrict__ tw, int m)
 // (unnormalised).  Each stage loads every butterfly of the thread into
 // registers, synchronises, then stores, so one buffer suffices.  C is
 // double2 (FP64 path) or float2 (FP32 preconditioner of the mixed solver).
+// (inlined so that buf is known to be LDS: ds_read/ds_write, not flat accesses)
 template <bool INV, typename C>
-__device__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw)
+__device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw)
 {
     int Ns = 1;
     const int nq = n >> 2;
@@ -300,8 +310,8 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         const int idx = threadIdx.x + q * NT;
         const int k = idx / npair;
         const int ja = j0 + 2 * (idx - k * npair);
-        va[q] = (idx < total && ja < a.nv) ? spec[(size_t)k * a.nv + ja] : zero;
-        vb[q] = (idx < total && ja + 1 < a.nv) ? spec[(size_t)k * a.nv + ja + 1] : zero;
+        va[q] = load_or_zero(spec, (int64_t)k * a.nv + ja, idx < total && ja < a.nv);
+        vb[q] = load_or_zero(spec, (int64_t)k * a.nv + ja + 1, idx < total && ja + 1 < a.nv);
     }
     if (dflag) return;                 // uniform over the block, before any barrier
     if (LTW) stage_twiddles(a.tw, stw, nx);
@@ -358,17 +368,24 @@ struct TriArgs {
 template <bool TR>
 __device__ __forceinline__ void tri_load(const TriArgs &a, int j, double &lo, double &mid, double &hi)
 {
-    lo = mid = hi = 0.0;
-    if (j < a.nv) {
-        if (!TR) {
-            lo = a.gm[j];
-            hi = a.gp[j];
-        } else {
-            lo = (j > 0) ? a.gp[j - 1] : 0.0;
-            hi = (j < a.nv - 1) ? a.gm[j + 1] : 0.0;
-        }
-        mid = a.g0[j];
+    // clamped addresses, no branches: the loads of several rows can be in
+    // flight together
+    const bool ok = j < a.nv;
+    const int jc = ok ? j : 0;
+    double l, m, h;
+    if (!TR) {
+        l = a.gm[jc];
+        h = a.gp[jc];
+    } else {
+        l = a.gp[jc > 0 ? jc - 1 : 0];
+        h = a.gm[jc < a.nv - 1 ? jc + 1 : jc];
+        l = jc > 0 ? l : 0.0;
+        h = jc < a.nv - 1 ? h : 0.0;
     }
+    m = a.g0[jc];
+    lo = ok ? l : 0.0;
+    mid = ok ? m : 0.0;
+    hi = ok ? h : 0.0;
 }
 
 template <bool TR, typename C>
@@ -419,7 +436,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
 #pragma unroll
         for (int q = 0; q < M; ++q) {
             const int j = threadIdx.x + q * NT;
-            v[q] = (j < a.nv) ? S[j] : zero;
+            v[q] = load_or_zero(S, j, j < a.nv);
         }
         if (dflag) return;             // uniform over the block, before any barrier
 #pragma unroll
