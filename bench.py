@@ -105,7 +105,7 @@ def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None):
     if kernel == "all":
         ctx.profile(None)
     elif kernel not in (None, "none"):
-        ctx.profile((kernel,))
+        ctx.profile(tuple(kernel.split(",")))
     nrec0 = len(run.history)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -241,9 +241,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="twostream_1024x2048_J4")
-    ap.add_argument("--kernel", default="mdot",
-                    help="kernel timed with HIP events for the roofline ('all', 'none' or one of %s)" % (KERNELS,))
-    ap.add_argument("--cpu-steps", type=int, default=1, help="timesteps of the CPU baseline sample")
+    ap.add_argument("--kernel", default="maxpy_norm",
+                    help="kernels timed with HIP events in the timed FP64 run ('all', 'none' or a comma list of "
+                         "%s); the roofline is the one with the largest total time.  Each timed dispatch costs "
+                         "~2%% of throughput, so the default times the dominant kernel only" % (KERNELS,))
+    ap.add_argument("--bracket-kernels", default="jacobian,residual",
+                    help="kernels timed in a second, untimed FP64 run for the bracket HBM GB/s ('none' skips)")
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="timesteps of the CPU baseline sample (about 10-20 s on 16 host threads)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
@@ -294,6 +299,21 @@ def main():
                 "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": d["bytes"] / d["launches"], "avg_launch_us": round(d["avg_us"], 3),
                 "launches": d["launches"]}
+        # HBM bytes per launch measured by the PMC runs summarised in profiles/
+        from vivlasov1d_amd.profiles import read_summary
+        src = os.path.join("profiles", "%s_%s.txt" % (dom, args.workload))
+        tk = read_summary(os.path.join(os.path.dirname(os.path.abspath(__file__)), src))
+        if "traffic_bytes_per_launch" in tk:
+            roof["traffic"] = tk["traffic_bytes_per_launch"]
+            roof["traffic_source"] = src
+            roof["traffic_over_algorithmic"] = round(tk["traffic_bytes_per_launch"] / roof["bytes_per_launch"], 3)
+    bracket = None
+    if args.bracket_kernels not in ("", "none"):
+        brun, _, _, bk = timed_run(w, args, "double", args.bracket_kernels)
+        del brun
+        bracket = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
+                       "frac_hbm_peak": round(v["gbs"] / HBM_PEAK_GBS, 4),
+                       "bytes_per_launch": v["bytes"] / v["launches"]} for k, v in bk.items()}
     sb = sum(step_bytes(N, r) for r in recs)
     picard = [len(r["picard"]) for r in recs]
     gm = [p["gmres"] for r in recs for p in r["picard"]]
@@ -337,6 +357,7 @@ def main():
                         "ms": round(v["ms"], 3)} for k, v in kstats.items()},
         "cpu_baseline": cpu,
         "invariants": {k: inv[k] for k in ("mass", "l2", "energy")},
+        "bracket": bracket,
         "mixed": mixed,
         "batched": batched,
     }

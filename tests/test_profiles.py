@@ -1,0 +1,23 @@
+"""The committed rocprofv3 summaries parse, and what they record is consistent:
+the HIP-event average of the roofline kernel (bench.py without the profiler)
+agrees with rocprofv3's average over the same timed region, and the
+counter-measured HBM traffic is within the algorithmic bytes' range."""
+
+import os
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("name", ["maxpy_norm_twostream_1024x2048_J4.txt", "jacobian_twostream_1024x2048_J4.txt"])
+def test_profile_summary(name):
+    from vivlasov1d_amd.profiles import read_summary
+    k = read_summary(os.path.join(ROOT, "profiles", name))
+    for key in ("kernel", "rocprof_avg_us_timed_region", "bench_avg_us_unprofiled", "traffic_bytes_per_launch",
+                "fetch_factor", "write_factor"):
+        assert key in k, key
+    assert abs(k["bench_avg_us_unprofiled"] - k["rocprof_avg_us_timed_region"]) <= 0.1 * k["rocprof_avg_us_timed_region"]
+    assert 1.9 < k["fetch_factor"] < 2.1 and 0.95 < k["write_factor"] < 1.05
+    assert k["traffic_bytes_per_launch"] > 0
+    assert "command: cd /tmp && export TMPDIR=/tmp && rocprofv3" in open(os.path.join(ROOT, "profiles", name)).read()
