@@ -8,13 +8,16 @@
 // result is bit-identical to the reference's unfused double arithmetic.
 //
 // Layout: natural order, element (i, j) at i + nx*j (x contiguous).
-// Tiling: one workgroup computes a 64 (x) x 28 (v) tile.  The tile of the
-// bracket's first argument and of H_ave, both with a 2-cell halo, is staged
-// in LDS; H_ave is composed on the fly from its 1D factors
+// Tiling: one workgroup computes a 64 (x) x TY (v) tile (TY = 16).  The tile
+// of the bracket's first argument and of H_ave, both with a 2-cell halo, is
+// staged in LDS; every global load of a thread is issued before the first
+// LDS store.  H_ave is composed on the fly from its 1D factors
 // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i), exactly the
 // value the reference's 2D Have vector holds (PETScVlasovSolver.pyx:145-149),
-// so it costs no HBM traffic.  Each wavefront owns 7 rows of the tile; lane
-// l computes column i0 + l, so global loads and stores are 512-byte rows.
+// so it costs no HBM traffic.  Each wavefront owns TY/4 consecutive rows;
+// lane l computes column i0 + l (global loads and stores are 512-byte rows)
+// and walks down its rows with the stencil values in a sliding register
+// window.
 
 #include "vv1d_internal.h"
 
@@ -28,7 +31,7 @@ namespace {
 #define VV_BRACKET_TY 16
 #endif
 constexpr int TX = 64;
-constexpr int TY = VV_BRACKET_TY;   // 28: 4 FP64 workgroups per CU fit in LDS
+constexpr int TY = VV_BRACKET_TY;
 constexpr int HX = TX + 4;
 constexpr int HY = TY + 4;
 constexpr int NT = 256;
