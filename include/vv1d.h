@@ -179,6 +179,12 @@ int vv_function_norm(vv_ctx *ctx, const double *f, const double *fh, double *y,
  * n_i = hv * sum_j f_ij. */
 int vv_density(vv_ctx *ctx, const double *f, double *n);
 
+/* Toolbox.compute_velocity_density / compute_energy_density
+ * (Toolbox.pyx:103-148): u_i = hv sum_j v_j f_ij, e_i = hv sum_j v_j^2 f_ij
+ * (x-vectors, per instance).  Output moments of the HDF5 schema; not on the
+ * timestep's path. */
+int vv_velocity_moments(vv_ctx *ctx, const double *f, double *u, double *e);
+
 /* PETScPoissonSolver.formRHS + the Poisson KSPSolve
  * (PETScPoissonSolver.pyx:58-74; run_base.py:689-690): solves A phi = b,
  * b_i = -(n_i - mean n) q, exactly on the periodic grid; phi is returned

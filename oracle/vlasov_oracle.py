@@ -323,6 +323,22 @@ def density(f, grid):
     return n * grid.hv
 
 
+def velocity_density(f, grid):
+    """u_i = (sum_j v_j f_ij) * hv, summed in j (Toolbox.pyx:103-123)."""
+    u = np.zeros(f.shape[0])
+    for j in range(f.shape[1]):
+        u += grid.v[j] * f[:, j]
+    return u * grid.hv
+
+
+def energy_density(f, grid):
+    """e_i = (sum_j v_j**2 f_ij) * hv, summed in j (Toolbox.pyx:128-148)."""
+    e = np.zeros(f.shape[0])
+    for j in range(f.shape[1]):
+        e += grid.v2[j] * f[:, j]
+    return e * grid.hv
+
+
 def potential_to_hamiltonian(phi, nx):
     """h1_i = phi_i - sum(phi)/nx  (Toolbox.pyx:38-74), kept 1D."""
     return phi - phi.sum() / nx

@@ -1,0 +1,76 @@
+"""A GPU run writing the reference's HDF5 schema: the file holds, step by
+step, exactly the state the run held (f bitwise), the moments and potential
+of that state (against the oracle), and the invariants the reference's
+diagnostics would compute from it."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import vlasov_oracle as O
+from vivlasov1d_amd import hdf5 as H
+
+try:
+    H.library()
+    HAVE = True
+except H.HDF5Error:
+    HAVE = False
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not HAVE, reason="no HDF5 C library in this image")]
+
+
+def _cfg(nx, nv, nt, nsave):
+    from vivlasov1d_amd.core import Config
+    return Config({'grid': {'nx': nx, 'nv': nv, 'L': 4 * math.pi, 'vmax': 8.0, 'ht': 0.05, 'nt': nt},
+                   'solver': {'poisson_bracket': 'arakawaj4', 'laplace_operator': 'CFD4',
+                              'preconditioner_type': 'tensor', 'preconditioner_scheme': 'tensor_kinetic',
+                              'petsc_ksp_rtol': 1e-10, 'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
+                   'io': {'nsave': nsave},
+                   'initial_data': {'charge': -1.0}})
+
+
+def _prof(x, v):
+    return (1 + 0.5 * np.cos(0.5 * x)) * np.exp(-0.5 * v ** 2) / math.sqrt(2 * math.pi)
+
+
+@pytest.mark.parametrize("nsave", [1, 2])
+def test_run_writes_its_states(tmp_path, nsave, cuda):
+    from vivlasov1d_amd.runner import Runscript
+    nx, nv, nt = 32, 64, 4
+    path = str(tmp_path / "landau.hdf5")
+    run = Runscript(_cfg(nx, nv, nt, nsave), profile=_prof, hdf5_out=path)
+    states = {0: run.fc.cpu().numpy().copy()}
+    for s in range(1, nt + 1):
+        run.step()
+        states[s] = run.fc.cpu().numpy().copy()
+    inv_last = run.invariants()
+    run.close()
+    d = H.read(path)
+    saved = [s for s in range(nt + 1) if s % nsave == 0]
+    assert np.array_equal(d["t"][:, 0, 0], np.array([0.05 * s for s in saved]))
+    g = O.Grid(nx, nv, 4 * math.pi, 8.0, ht=0.05)
+    assert np.array_equal(d["x"], g.x) and np.array_equal(d["v"], g.v)
+    assert d["attrs"]["charge"] == -1.0
+    for k, s in enumerate(saved):
+        f = d["f"][k]
+        assert np.array_equal(f, states[s]), s
+        fo = f.T                                        # oracle layout (nx, nv)
+        n_ref = O.density(fo, g)
+        assert np.abs(d["n"][k] - n_ref).max() <= 1e-13 * np.abs(n_ref).max()
+        e_ref = O.energy_density(fo, g)
+        assert np.abs(d["u"][k] - O.velocity_density(fo, g)).max() <= 1e-13 * np.abs(e_ref).max()
+        assert np.abs(d["e"][k] - e_ref).max() <= 1e-13 * np.abs(e_ref).max()
+        phi_ref = O.poisson_solve(O.poisson_rhs(n_ref, -1.0, nx), g.hx, 'cfd4')
+        phi = d["phi_int"][k]
+        assert np.abs((phi - phi.mean()) - (phi_ref - phi_ref.mean())).max() <= 1e-11 * np.abs(phi_ref).max()
+        # the Hamiltonian parts as the reference's 2D fields
+        assert np.array_equal(d["h0"][k].T, np.broadcast_to(O.kinetic_hamiltonian(g)[None, :], (nx, nv)))
+        assert np.allclose(d["h1"][k].T, np.broadcast_to(O.potential_to_hamiltonian(phi, nx)[:, None], (nx, nv)),
+                           rtol=0, atol=1e-14 * np.abs(phi).max())
+        assert not d["h2"][k].any() and not d["phi_ext"][k].any()
+    # the invariants a reader of the file computes equal the run's own
+    k = len(saved) - 1
+    ref = O.invariants(d["f"][k].T, O.kinetic_hamiltonian(g), d["h1"][k][0], np.zeros(nx), d["phi_int"][k], g, -1.0)
+    for key in ("mass", "l2", "energy"):
+        assert abs(ref[key] - inv_last[key]) <= 1e-12 * abs(ref[key]), key

@@ -124,6 +124,12 @@ def test_moments_and_poisson(nx, nv, laplace, cuda):
     n = ctx.new_x()
     ctx.density(T(_nat(f)), n)
     assert np.abs(n.cpu().numpy() - n_ref).max() <= 1e-13 * np.abs(n_ref).max()
+    # output moments u, e (same sequential sums in j)
+    u, e = ctx.new_x(), ctx.new_x()
+    ctx.velocity_moments(T(_nat(f)), u, e)
+    u_ref, e_ref = O.velocity_density(f, g), O.energy_density(f, g)
+    assert np.abs(u.cpu().numpy() - u_ref).max() <= 1e-13 * np.abs(e_ref).max()
+    assert np.abs(e.cpu().numpy() - e_ref).max() <= 1e-13 * np.abs(e_ref).max()
     phi_ref = O.poisson_solve(O.poisson_rhs(n_ref, -1.0, nx), g.hx, laplace)
     phi = ctx.new_x()
     ctx.poisson_solve(T(n_ref), phi)

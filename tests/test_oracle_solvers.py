@@ -104,6 +104,21 @@ def test_density_and_normalisation_of_maxwellian():
     assert np.all(f2[:, :2] == 0) and np.all(f2[:, -2:] == 0)
 
 
+def test_velocity_and_energy_density_of_drifting_maxwellian():
+    """u = n u0 and e = n (1 + u0^2) for f = n(x) M(v - u0) (the rectangle rule
+    is spectrally accurate for a Gaussian that vanishes at the v-boundary); a
+    wrong power of v or a missing hv fails by O(1)."""
+    g = O.Grid(16, 257, 4 * np.pi, 10.0)
+    u0 = 0.7
+    nx_prof = 1 + 0.1 * np.cos(0.5 * g.x)
+    f = O.initial_distribution(g, lambda x, v: O.maxwellian(v, vd=u0) * (1 + 0.1 * np.cos(0.5 * x)))
+    np.testing.assert_allclose(O.velocity_density(f, g), nx_prof * u0, rtol=1e-12)
+    np.testing.assert_allclose(O.energy_density(f, g), nx_prof * (1 + u0 ** 2), rtol=1e-12)
+    # symmetric in v: u vanishes to rounding
+    fs = O.initial_distribution(g, lambda x, v: O.maxwellian(v) * (1 + 0.1 * np.cos(0.5 * x)))
+    assert np.abs(O.velocity_density(fs, g)).max() <= 1e-14
+
+
 def test_invariants_closed_form():
     g = O.Grid(32, 401, 2 * np.pi, 10.0)
     f = np.broadcast_to(O.maxwellian(g.v)[None, :], (g.nx, g.nv)).copy()

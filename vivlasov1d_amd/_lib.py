@@ -76,6 +76,7 @@ SIGNATURES = {
     "vv_jacobian": (ctypes.c_int, [_P, _P, _P]),
     "vv_function_norm": (ctypes.c_int, [_P, _P, _P, _P, _PD]),
     "vv_density": (ctypes.c_int, [_P, _P, _P]),
+    "vv_velocity_moments": (ctypes.c_int, [_P, _P, _P, _P]),
     "vv_poisson_solve": (ctypes.c_int, [_P, _P, _P]),
     "vv_poisson_matrix_mult": (ctypes.c_int, [_P, _P, _P]),
     "vv_poisson_function_mult": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -683,6 +683,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(dalloc(&c->gm, nv), "alloc gm");
     VV_CCHECK(dalloc(&c->g0, nv), "alloc g0");
     VV_CCHECK(dalloc(&c->gp, nv), "alloc gp");
+    VV_CCHECK(dalloc(&c->vgrid, nv), "alloc v grid");
     VV_CCHECK(dalloc(&c->twiddle, nx), "alloc twiddle");
     VV_CCHECK(dalloc(&c->twiddle32, nx), "alloc twiddle32");
     VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * nv * Bz), "alloc spectrum");
@@ -761,6 +762,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(hipMemcpy(c->gm, gm.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->g0, g0.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->gp, gp.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->vgrid, p->v, nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->eigen, eig.data(), c->nmodes * sizeof(double2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->twiddle, tw.data(), nx * sizeof(double2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->twiddle32, tw32.data(), nx * sizeof(float2), hipMemcpyHostToDevice), "upload");
@@ -801,7 +803,7 @@ int vv_ctx_destroy(vv_ctx *c)
     void *bufs[] = {c->inst,   c->active, c->h0,      c->h1p,     c->h1h,  c->h2p,   c->h2h,    c->eigen,
                     c->gm,     c->g0,     c->gp,      c->twiddle, c->twiddle32, c->spec, c->pinv, c->pwork,
                     c->X,      c->rhs,    c->D,       c->basis,   c->basis32, c->R,   c->partial, c->dscalar,
-                    c->npart,  c->gs,     c->T2,      c->xs};
+                    c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);
@@ -1060,6 +1062,14 @@ int vv_density(vv_ctx *c, const double *f, double *n)
     if (int r = check_ctx(c)) return r;
     if (!f || !n) return fail(c, VV_ERR_INVALID, "null argument");
     VV_CHECK(launch_density(c, f, n), "density");
+    return VV_OK;
+}
+
+int vv_velocity_moments(vv_ctx *c, const double *f, double *u, double *e)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!f || !u || !e) return fail(c, VV_ERR_INVALID, "null argument");
+    VV_CHECK(launch_velocity_moments(c, f, u, e), "velocity moments");
     return VV_OK;
 }
 

@@ -61,30 +61,35 @@ __device__ __forceinline__ double final_sum_1(const double *part, int nb, double
     return block_sum_all(a, red);
 }
 
-// Column sums of part[b*PV + v] (b < nb) for v < PV, by the whole block:
-// thread t owns column v = t % PV and rows b = t / PV (mod NT / PV), loads
-// issued 16 at a time.  Result in out[v].
+// Column sums of part[b*PV + v] (b < nb) for v < ncols (<= PV), by the whole
+// block: the columns are padded to cols = max(8, next power of two), thread t
+// owns column t % cols and rows b = t / cols (mod NT / cols), loads issued 16
+// at a time (one round for nb <= 16 NT / cols).  Result in out[v].
 template <int PV>
-__device__ void final_sum_cols(const double *part, int nb, double *out, double *red /*[NT]*/)
+__device__ void final_sum_cols(const double *part, int nb, int ncols, double *out, double *red /*[NT]*/)
 {
-    constexpr int RS = NT / PV;      // row streams
-    const int v = threadIdx.x % PV, r0 = threadIdx.x / PV;
+    int cols = 8;
+    while (cols < ncols) cols <<= 1;
+    const int RS = NT / cols;        // row streams
+    const int v = threadIdx.x & (cols - 1), r0 = threadIdx.x / cols;
     double acc = 0.0;
-    for (int b0 = r0; b0 < nb; b0 += RS * 16) {
-        double x[16];
+    if (v < ncols) {
+        for (int b0 = r0; b0 < nb; b0 += RS * 16) {
+            double x[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int b = b0 + q * RS;
-            x[q] = (b < nb) ? part[(size_t)b * PV + v] : 0.0;
+            for (int q = 0; q < 16; ++q) {
+                const int b = b0 + q * RS;
+                x[q] = (b < nb) ? part[(size_t)b * PV + v] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc += x[q];
         }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) acc += x[q];
     }
     red[threadIdx.x] = acc;
     __syncthreads();
-    if (threadIdx.x < PV) {
+    if (threadIdx.x < ncols) {
         double s = 0.0;
-        for (int r = 0; r < RS; ++r) s += red[r * PV + threadIdx.x];
+        for (int r = 0; r < RS; ++r) s += red[r * cols + threadIdx.x];
         out[threadIdx.x] = s;
     }
     __syncthreads();
@@ -323,7 +328,7 @@ __global__ __launch_bounds__(NT) void mdot_finish_kernel(const double *part, int
     if (g->st.done) return;
     __shared__ double s_red[NT];
     __shared__ double s_h[PV];
-    final_sum_cols<PV>(part, nb, s_h, s_red);
+    final_sum_cols<PV>(part, nb, col + 1, s_h, s_red);
     if (threadIdx.x < col + 1) {
         const double h = s_h[threadIdx.x];
         g->hh[col * (kMaxRestart + 1) + threadIdx.x] = h;
@@ -407,7 +412,10 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 
 // VecNormalize of V_{it+1} from the maxpy partials, then
 // KSPGMRESUpdateHessenberg (rotations, happy breakdown, residual estimate)
-// and the convergence test; sets done when the cycle ends.
+// and the convergence test; sets done when the cycle ends.  The Hessenberg
+// column and the previous rotations are staged in LDS by the whole block
+// (loads in flight together with the norm's partials), so the sequential
+// rotation chain of thread 0 reads LDS, not dependent global loads.
 __global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, int nb, int64_t pstride, int it,
                                                           GmresState *g)
 {
@@ -415,40 +423,49 @@ __global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, in
     part += blockIdx.y * pstride;
     if (g->st.done) return;
     __shared__ double red[NW + 1];
-    const double tot = final_sum_1(part, nb, red);
+    __shared__ double s_h[kMaxRestart + 2], s_c[kMaxRestart], s_s[kMaxRestart];
+    double *hh = g->hh + (size_t)it * (kMaxRestart + 1);
+    if (threadIdx.x <= it) s_h[threadIdx.x] = hh[threadIdx.x];
+    if (threadIdx.x < it) {
+        s_c[threadIdx.x] = g->cc[threadIdx.x];
+        s_s[threadIdx.x] = g->ss[threadIdx.x];
+    }
+    const double grs_it = g->grs[it];
+    const double tot = final_sum_1(part, nb, red);     // barriers: the LDS stores above are visible
     if (threadIdx.x != 0) return;
     const double tt = sqrt(tot);
-    double *hh = g->hh + (size_t)it * (kMaxRestart + 1);
-    hh[it + 1] = tt;
+    s_h[it + 1] = tt;
     g->scale[it + 1] = tt != 0.0 ? 1.0 / tt : 1.0;
     // happy breakdown
-    double hapbnd = fabs(tt / g->grs[it]);
+    double hapbnd = fabs(tt / grs_it);
     if (hapbnd > g->haptol) hapbnd = g->haptol;
     const bool hapend = tt < hapbnd;
     // apply the previous rotations to the new column
     for (int q = 1; q <= it; ++q) {
-        const double t0 = hh[q - 1];
-        hh[q - 1] = g->cc[q - 1] * t0 + g->ss[q - 1] * hh[q];
-        hh[q] = g->cc[q - 1] * hh[q] - g->ss[q - 1] * t0;
+        const double t0 = s_h[q - 1];
+        s_h[q - 1] = s_c[q - 1] * t0 + s_s[q - 1] * s_h[q];
+        s_h[q] = s_c[q - 1] * s_h[q] - s_s[q - 1] * t0;
     }
     double res;
     int reason = 0;
     if (!hapend) {
-        const double d = sqrt(hh[it] * hh[it] + hh[it + 1] * hh[it + 1]);
+        const double d = sqrt(s_h[it] * s_h[it] + s_h[it + 1] * s_h[it + 1]);
         if (d == 0.0) {
             reason = VV_KSP_DIVERGED_BREAKDOWN;
             res = g->st.res;
         } else {
-            g->cc[it] = hh[it] / d;
-            g->ss[it] = hh[it + 1] / d;
-            g->grs[it + 1] = -(g->ss[it] * g->grs[it]);
-            g->grs[it] = g->cc[it] * g->grs[it];
-            hh[it] = g->cc[it] * hh[it] + g->ss[it] * hh[it + 1];
-            res = fabs(g->grs[it + 1]);
+            const double cc = s_h[it] / d, ss = s_h[it + 1] / d;
+            g->cc[it] = cc;
+            g->ss[it] = ss;
+            g->grs[it + 1] = -(ss * grs_it);
+            g->grs[it] = cc * grs_it;
+            s_h[it] = cc * s_h[it] + ss * s_h[it + 1];
+            res = fabs(-(ss * grs_it));
         }
     } else {
         res = 0.0;
     }
+    for (int q = 0; q <= it + 1; ++q) hh[q] = s_h[q];
     g->st.it = it + 1;
     g->st.its += 1;
     g->st.res = res;

@@ -11,6 +11,10 @@
 //                    post (one workgroup): h1 = phi - mean phi
 //                    (Toolbox.pyx:38-74) and the Poisson residual norm
 //                    ||A phi + (n - mean n) q|| of run_base.py:631-637.
+//  velocity_moments_kernel
+//                    u_i = hv sum_j v_j f_ij and e_i = hv sum_j v_j^2 f_ij
+//                    (Toolbox.pyx:103-148), the output moments: one thread
+//                    per column, rows in order, as the reference's loop.
 //  invariants_kernel mass, L2, kinetic / potential energy over f, plus the
 //                    field energy over phi (distribution_function.pyx:130-199,
 //                    hamiltonian.pyx:67-100, potential.pyx:48-59).
@@ -39,6 +43,25 @@ __global__ __launch_bounds__(NT) void density_kernel(const double *__restrict__ 
     double s = 0.0;
     for (int j = j0; j < j1; ++j) s += f[(size_t)j * nx + i];
     npart[(size_t)blockIdx.y * nx + i] = s;
+}
+
+__global__ __launch_bounds__(NT) void velocity_moments_kernel(const double *__restrict__ f,
+                                                              const double *__restrict__ v, int nx, int nv,
+                                                              double hv, double *__restrict__ u, double *__restrict__ e)
+{
+    f += (size_t)blockIdx.y * nx * nv;
+    u += (size_t)blockIdx.y * nx;
+    e += (size_t)blockIdx.y * nx;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= nx) return;
+    double su = 0.0, se = 0.0;
+    for (int j = 0; j < nv; ++j) {
+        const double fv = f[(size_t)j * nx + i];
+        su += v[j] * fv;
+        se += (v[j] * v[j]) * fv;          // v2 = v**2 (Grid), then v2_j * f_ij
+    }
+    u[i] = su * hv;
+    e[i] = se * hv;
 }
 
 __device__ double block_sum_big(double v, double *red)
@@ -318,6 +341,13 @@ hipError_t launch_density(vv_ctx *c, const double *f, double *n)
     hipLaunchKernelGGL(density_finish_kernel, dim3((c->p.nx + NT - 1) / NT, c->B), dim3(NT), 0, c->stream, c->npart,
                        DCH,
                        c->p.nx, c->p.hv, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity_moments(vv_ctx *c, const double *f, double *u, double *e)
+{
+    hipLaunchKernelGGL(velocity_moments_kernel, dim3((c->p.nx + NT - 1) / NT, c->B), dim3(NT), 0, c->stream, f,
+                       c->vgrid, c->p.nx, c->p.nv, c->p.hv, u, e);
     return hipGetLastError();
 }
 

@@ -22,9 +22,6 @@
 // and rounded).
 #include "vv1d_internal.h"
 
-#ifndef VV_FFT_LTW
-#define VV_FFT_LTW 0
-#endif
 #ifndef VV_FFT_XCD
 #define VV_FFT_XCD 1
 #endif
@@ -102,36 +99,79 @@ __device__ __forceinline__ C twiddle(const C *__restrict__ tw, int m)
     return INV ? cmk<C>(t.x, -t.y) : t;
 }
 
+constexpr int MAXS = 6;     // radix-4 stages of the longest transform (MAXE = 4^6)
+
+// Twiddles of one thread's butterflies, loaded once before a transform of
+// length n <= 4 NT.  There every butterfly the thread owns, in any transform
+// of the block and any stage, has the index j = threadIdx.x mod n/4 (mod n/2
+// in the radix-2 stage of odd logn, where n <= 2 NT), so 3 twiddles per stage
+// cover them; their loads are in flight with the tile's own loads instead of
+// being waited on after every barrier of the transform.
+template <bool INV, typename C>
+struct FftTw {
+    C w[MAXS][3];
+    C w2;
+    __device__ __forceinline__ void load(const C *__restrict__ tw, int n, int logn)
+    {
+        const int ns = logn >> 1;
+        const int j = threadIdx.x & ((n >> 2) - 1);
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s) {
+            if (s < ns) {
+                const int st = (j & ((1 << (2 * s)) - 1)) << (logn - 2 - 2 * s);
+                w[s][0] = twiddle<INV>(tw, st);
+                w[s][1] = twiddle<INV>(tw, 2 * st);
+                w[s][2] = twiddle<INV>(tw, 3 * st);
+            }
+        }
+        if (logn & 1) {
+            const int jj = threadIdx.x & ((n >> 1) - 1);
+            w2 = twiddle<INV>(tw, (jj & ((1 << (2 * ns)) - 1)) << (logn - 1 - 2 * ns));
+        }
+    }
+};
+
 // Batched in-LDS Stockham FFT of `nb` transforms of length n = 2^logn stored
 // back to back in buf.  Forward uses exp(-2 pi i m/n), inverse the conjugate
 // (unnormalised).  Each stage loads every butterfly of the thread into
 // registers, synchronises, then stores, so one buffer suffices.  C is
 // double2 (FP64 path) or float2 (FP32 preconditioner of the mixed solver).
-// (inlined so that buf is known to be LDS: ds_read/ds_write, not flat accesses)
-template <bool INV, typename C>
-__device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw)
+// PRE: twiddles from pt (n <= 4 NT), else from the global table tw.
+// Inlined so that buf is known to be LDS (ds_read/ds_write, not flat
+// accesses); index arithmetic by shifts (n is a power of two).
+template <bool INV, bool PRE, typename C>
+__device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw,
+                                        const FftTw<INV, C> &pt)
 {
-    int Ns = 1;
-    const int nq = n >> 2;
-    for (int s = 0; s < (logn >> 1); ++s) {
-        const int total = nb * nq;
+    const int nq = n >> 2, lq = logn - 2, ns = logn >> 1;
+    const int total4 = nb * nq;
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s) {
+        if (s >= ns) break;                // uniform over the block
+        const int Ns = 1 << (2 * s);
         C v[MAXE / 4 / NT][4];
         int dst[MAXE / 4 / NT];
 #pragma unroll
         for (int q = 0; q < MAXE / 4 / NT; ++q) {
             const int t = threadIdx.x + q * NT;
             dst[q] = -1;
-            if (t < total) {
-                const int b = t / nq;
-                const int j = t - b * nq;
+            if (t < total4) {
+                const int b = t >> lq;
+                const int j = t & (nq - 1);
                 const int k = j & (Ns - 1);
-                const C *src = buf + b * n;
+                const C *src = buf + (b << logn);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) v[q][r] = src[j + r * nq];
-                const int st = (n / (Ns * 4)) * k;
-                v[q][1] = cmul(v[q][1], twiddle<INV>(tw, st));
-                v[q][2] = cmul(v[q][2], twiddle<INV>(tw, 2 * st));
-                v[q][3] = cmul(v[q][3], twiddle<INV>(tw, 3 * st));
+                if (PRE) {
+                    v[q][1] = cmul(v[q][1], pt.w[s][0]);
+                    v[q][2] = cmul(v[q][2], pt.w[s][1]);
+                    v[q][3] = cmul(v[q][3], pt.w[s][2]);
+                } else {
+                    const int st = k << (lq - 2 * s);
+                    v[q][1] = cmul(v[q][1], twiddle<INV>(tw, st));
+                    v[q][2] = cmul(v[q][2], twiddle<INV>(tw, 2 * st));
+                    v[q][3] = cmul(v[q][3], twiddle<INV>(tw, 3 * st));
+                }
                 const C a0 = cadd(v[q][0], v[q][2]);
                 const C a1 = csub(v[q][0], v[q][2]);
                 const C a2 = cadd(v[q][1], v[q][3]);
@@ -142,7 +182,7 @@ __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C
                 v[q][1] = cadd(a1, a3);
                 v[q][2] = csub(a0, a2);
                 v[q][3] = csub(a1, a3);
-                dst[q] = b * n + (j - k) * 4 + k;
+                dst[q] = (b << logn) + ((j - k) << 2) + k;
             }
         }
         __syncthreads();
@@ -154,9 +194,9 @@ __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C
             }
         }
         __syncthreads();
-        Ns <<= 2;
     }
     if (logn & 1) {
+        const int Ns = 1 << (2 * ns);
         const int nh = n >> 1;
         const int total = nb * nh;
         C v[MAXE / 2 / NT][2];
@@ -166,15 +206,16 @@ __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C
             const int t = threadIdx.x + q * NT;
             dst[q] = -1;
             if (t < total) {
-                const int b = t / nh;
-                const int j = t - b * nh;
+                const int b = t >> (logn - 1);
+                const int j = t & (nh - 1);
                 const int k = j & (Ns - 1);
-                const C *src = buf + b * n;
+                const C *src = buf + (b << logn);
                 const C x0 = src[j];
-                const C x1 = cmul(src[j + nh], twiddle<INV>(tw, (n / (Ns * 2)) * k));
+                const C w = PRE ? pt.w2 : twiddle<INV>(tw, k << (logn - 1 - 2 * ns));
+                const C x1 = cmul(src[j + nh], w);
                 v[q][0] = cadd(x0, x1);
                 v[q][1] = csub(x0, x1);
-                dst[q] = b * n + (j - k) * 2 + k;
+                dst[q] = (b << logn) + ((j - k) << 1) + k;
             }
         }
         __syncthreads();
@@ -205,33 +246,8 @@ __device__ __forceinline__ int32_t cycle_done(const int32_t *done, int64_t gstri
                            : 0;
 }
 
-// The twiddle table exp(-2 pi i m/nx), m < nx, copied into LDS at tw_lds in
-// chunks of TWC complex per thread; called after the tile's own global loads
-// are in flight, so both latencies overlap, and the FFT stages then read
-// their twiddles from LDS instead of waiting on global memory after every
-// barrier.
-template <typename C>
-__device__ __forceinline__ void stage_twiddles(const C *__restrict__ tw, C *tw_lds, int nx)
-{
-    constexpr int TWC = 4;
-    for (int m0 = 0; m0 < nx; m0 += TWC * NT) {
-        C t[TWC];
-#pragma unroll
-        for (int q = 0; q < TWC; ++q) {
-            const int m = m0 + threadIdx.x + q * NT;
-            if (m < nx) t[q] = tw[m];
-        }
-#pragma unroll
-        for (int q = 0; q < TWC; ++q) {
-            const int m = m0 + threadIdx.x + q * NT;
-            if (m < nx) tw_lds[m] = t[q];
-        }
-    }
-}
-
-// LTW: the twiddle table is staged in LDS after the packed rows (when both fit
-// in 64 KiB: (R/2 + 1) nx <= 4096), else read from global memory.
-template <typename T, bool LTW>
+// PRE (nx <= 4 NT): the thread's twiddles are loaded with its tile.
+template <typename T, bool PRE>
 __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
@@ -241,7 +257,6 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows;
-    C *stw = buf + (R >> 1) * nx;      // twiddles after the R/2 packed rows (LTW)
     // XCD-aware row-block order: the mode-major segments of neighbouring row
     // blocks share L2 lines, so they are kept on one XCD
     const int j0 = (VV_FFT_XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * R;
@@ -255,8 +270,9 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         const int i = idx & (nx - 1);
         val[q] = (idx < R * nx && j0 + r < a.nv) ? x[(size_t)(j0 + r) * nx + i] : T(0);
     }
+    FftTw<false, C> pt;
+    if (PRE) pt.load(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
-    if (LTW) stage_twiddles(a.tw, stw, nx);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
@@ -268,8 +284,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         }
     }
     __syncthreads();
-    if (LTW) lds_fft<false>(buf, nx, a.logn, R >> 1, stw);
-    else lds_fft<false>(buf, nx, a.logn, R >> 1, a.tw);
+    lds_fft<false, PRE>(buf, nx, a.logn, R >> 1, a.tw, pt);
     const int nk = (nx >> 1) + 1;
     const T half = T(0.5);
     for (int idx = threadIdx.x; idx < nk * R; idx += NT) {
@@ -286,7 +301,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
     }
 }
 
-template <typename T, bool LTW>
+template <typename T, bool PRE>
 __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
 {
     using C = cx_t<T>;
@@ -296,7 +311,6 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows, nh = nx >> 1;
-    C *stw = buf + (R >> 1) * nx;
     const int j0 = (VV_FFT_XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * R;   // as in rfft_kernel
     const int npair = R >> 1;
     // Z_k = A_k + i B_k (k <= nx/2) and its Hermitian extension for k > nx/2;
@@ -304,7 +318,6 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
     const int total = (nh + 1) * npair;
     constexpr int PER = (MAXE / 2 + 64 + NT - 1) / NT;   // (nx/2 + 1) R/2 <= MAXE/2 + 64
     C va[PER], vb[PER];
-    const C zero = cmk<C>(T(0), T(0));
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
@@ -313,8 +326,9 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         va[q] = load_or_zero(spec, (int64_t)k * a.nv + ja, idx < total && ja < a.nv);
         vb[q] = load_or_zero(spec, (int64_t)k * a.nv + ja + 1, idx < total && ja + 1 < a.nv);
     }
+    FftTw<true, C> pt;
+    if (PRE) pt.load(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
-    if (LTW) stage_twiddles(a.tw, stw, nx);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
@@ -332,8 +346,7 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         }
     }
     __syncthreads();
-    if (LTW) lds_fft<true>(buf, nx, a.logn, npair, stw);
-    else lds_fft<true>(buf, nx, a.logn, npair, a.tw);
+    lds_fft<true, PRE>(buf, nx, a.logn, npair, a.tw, pt);
     const T inv_n = T(1) / (T)nx;   // exact: nx is a power of two
     for (int idx = threadIdx.x; idx < R * nx; idx += NT) {
         const int r = idx >> a.logn;
@@ -368,24 +381,20 @@ struct TriArgs {
 template <bool TR>
 __device__ __forceinline__ void tri_load(const TriArgs &a, int j, double &lo, double &mid, double &hi)
 {
-    // clamped addresses, no branches: the loads of several rows can be in
-    // flight together
-    const bool ok = j < a.nv;
-    const int jc = ok ? j : 0;
-    double l, m, h;
+    // Clamped addresses and no selects on the loaded values: nothing here
+    // waits on a load, so the loads of all rows of a chunk are in flight
+    // together.  Values loaded for rows outside the system are never used:
+    // tri_row makes those rows identity rows and drops the sub-diagonal of
+    // row 0 and the super-diagonal of row nv-1.
+    const int jc = j < a.nv ? j : a.nv - 1;
     if (!TR) {
-        l = a.gm[jc];
-        h = a.gp[jc];
+        lo = a.gm[jc];
+        hi = a.gp[jc];
     } else {
-        l = a.gp[jc > 0 ? jc - 1 : 0];
-        h = a.gm[jc < a.nv - 1 ? jc + 1 : jc];
-        l = jc > 0 ? l : 0.0;
-        h = jc < a.nv - 1 ? h : 0.0;
+        lo = a.gp[jc > 0 ? jc - 1 : 0];
+        hi = a.gm[jc < a.nv - 1 ? jc + 1 : jc];
     }
-    m = a.g0[jc];
-    lo = ok ? l : 0.0;
-    mid = ok ? m : 0.0;
-    hi = ok ? h : 0.0;
+    mid = a.g0[jc];
 }
 
 template <bool TR, typename C>
@@ -549,18 +558,14 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     for (int j = threadIdx.x; j < a.nv; j += NT) S[j] = sd[j + j / M];
 }
 
-// twiddles fit in LDS beside R/2 packed complex rows (64 KiB per workgroup)
-bool fft_lds_twiddles(int nx, int R) { return VV_FFT_LTW && ((R >> 1) + 1) * nx <= 4096; }
-
 // v-rows per FFT workgroup: the context's tuned value, else the largest that
-// fits LDS with the twiddles (or, for nx = 4096, without) while the launch
-// keeps >= 512 workgroups
+// fits 64 KiB of LDS while the launch keeps >= 512 workgroups
 int fft_rows(const vv_ctx *c)
 {
     if (c->tune_fft_rows > 0) return c->tune_fft_rows;
     const int nx = c->p.nx, nv = c->p.nv;
     int R = 2;
-    while (((R + 1) * nx <= 4096 || !VV_FFT_LTW) && 2 * R * nx <= 8192) R *= 2;
+    while (2 * R * nx <= 8192) R *= 2;
     if (R < 2) R = 2;
     while (R > 2 && (nv + R - 1) / R < 512) R >>= 1;
     return R;
@@ -572,13 +577,9 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
     FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
-    if (fft_lds_twiddles(a.nx, a.rows)) {
-        const size_t lds = (size_t)((a.rows >> 1) + 1) * a.nx * sizeof(cx_t<T>);
-        launch_timed(c, rfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
-    } else {
-        const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-        launch_timed(c, rfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
-    }
+    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
+    if (a.nx <= 4 * NT) launch_timed(c, rfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    else launch_timed(c, rfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
     return hipGetLastError();
 }
 
@@ -588,13 +589,9 @@ hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, cons
     FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
-    if (fft_lds_twiddles(a.nx, a.rows)) {
-        const size_t lds = (size_t)((a.rows >> 1) + 1) * a.nx * sizeof(cx_t<T>);
-        launch_timed(c, irfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
-    } else {
-        const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-        launch_timed(c, irfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
-    }
+    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
+    if (a.nx <= 4 * NT) launch_timed(c, irfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    else launch_timed(c, irfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
     return hipGetLastError();
 }
 

@@ -76,6 +76,7 @@ struct vv_ctx {
     // preconditioner data (device)
     double2 *eigen;     // nmodes used (k = 0..nx/2)
     double *gm, *g0, *gp;   // per-row real factors of the bands
+    double *vgrid;          // v_j (velocity moments)
     double2 *twiddle;   // exp(-2 pi i m / nx), m = 0..nx-1
     float2 *twiddle32;  // the same, rounded to float (FP32 preconditioner)
     double2 *spec;      // nmodes * nv complex spectrum, mode-major
@@ -170,6 +171,7 @@ hipError_t launch_xadd(vv_ctx *c, double *y, const double *w, double alpha);  //
 // moments.hip
 hipError_t launch_density_partial(vv_ctx *c, const double *f);
 hipError_t launch_density(vv_ctx *c, const double *f, double *n);
+hipError_t launch_velocity_moments(vv_ctx *c, const double *f, double *u, double *e);
 hipError_t launch_poisson(vv_ctx *c, const double *n_in, double *n_out, double *phi, double *h1, double *pres,
                           int from_part);
 hipError_t launch_poisson_mult(vv_ctx *c, const double *x, const double *n, double *y);

@@ -26,7 +26,7 @@ class ResidualTooLarge(RuntimeError):
 
 class Runscript:
     def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False, precision='double',
-                 inner_rtol=1e-5, normalise_f0=True, refine='newton'):
+                 inner_rtol=1e-5, normalise_f0=True, refine='newton', hdf5_out=None):
         if not isinstance(cfg, Config):
             cfg = Config(cfg)
         self.cfg = cfg
@@ -80,6 +80,16 @@ class Runscript:
         self.itime = 0
         self.history = []
         self.log = log
+        # HDF5 output in the reference's schema (run_base.py:528-562, 766-783):
+        # the initial state as timestep 0, then every nsave'th step
+        self.nsave = max(1, int(cfg['io'].get('nsave', 1) or 1))
+        path = hdf5_out if hdf5_out is not None else cfg['io'].get('hdf5_output')
+        self.output = None
+        if path:
+            from .hdf5 import HDF5Output
+            self.U, self.E = ctx.new_x(), ctx.new_x()
+            self.output = HDF5Output(path, grid.x, grid.v, self.charge)
+            self.save_hdf5_vectors()
 
     # run_base.py:631-637 --------------------------------------------------
     def calculate_residual(self):
@@ -137,7 +147,29 @@ class Runscript:
             self.log.write(json.dumps(rec) + "\n")
         if pred > 10.:
             raise ResidualTooLarge("ERROR: Residual of nonlinear solver too large.")
+        self.save_to_hdf5(self.itime)
         return rec
+
+    # run_base.py:766-783 ----------------------------------------------------
+    def save_to_hdf5(self, itime):
+        if self.output is not None and itime % self.nsave == 0:
+            self.save_hdf5_vectors()
+
+    def save_hdf5_vectors(self):
+        """t, h0, h1, h2, f, phi_int, phi_ext, n, u, e of the current state; the
+        Hamiltonian parts are written as the reference's 2D fields (h0 v-only,
+        h1 and h2 x-only, broadcast)."""
+        g = self.grid
+        self.ctx.velocity_moments(self.fc, self.U, self.E)
+        h0 = np.broadcast_to(self.h0.cpu().numpy()[:, None], (g.nv, g.nx))
+        h1 = np.broadcast_to(self.h1c.cpu().numpy()[None, :], (g.nv, g.nx))
+        h2 = np.broadcast_to(self.h2c.cpu().numpy()[None, :], (g.nv, g.nx))
+        self.output.save(self.itime * g.ht, h0, h1, h2, self.fc, self.pc_int, self.pc_ext, self.N, self.U, self.E)
+
+    def close(self):
+        if self.output is not None:
+            self.output.close()
+            self.output = None
 
     def set_state(self, f):
         """Replace the distribution function (a (nv, nx) float64 CUDA tensor
@@ -161,3 +193,4 @@ class Runscript:
 
     def __exit__(self, *exc):
         self.ctx.synchronize()
+        self.close()

@@ -195,6 +195,10 @@ class Context:
     def density(self, f, n):
         self.call("vv_density", self.F(f), self.X(n))
 
+    def velocity_moments(self, f, u, e):
+        """u = hv sum_j v_j f_ij, e = hv sum_j v_j^2 f_ij (Toolbox.pyx:103-148)."""
+        self.call("vv_velocity_moments", self.F(f), self.X(u), self.X(e))
+
     def poisson_solve(self, n, phi):
         self.call("vv_poisson_solve", self.X(n), self.X(phi))
 
