@@ -49,6 +49,7 @@ typedef struct vv_ctx vv_ctx;
  * (vlasov/solvers/preconditioner/TensorProduct.pyx:66-86) */
 #define VV_PRECOND_NONE           0
 #define VV_PRECOND_TENSOR_KINETIC 1
+#define VV_PRECOND_TENSOR_POTENTIAL 2   /* FFT along v x tridiagonal along x; nv a power of two */
 
 /* Laplacian: poisson module PETScPoissonCFD2 | PETScPoissonCFD4
  * (vlasov/run/run_base.py:93-94, 115) */
@@ -77,10 +78,11 @@ typedef struct {
     int32_t laplace;       /* VV_LAPLACE_*                                    */
     int32_t restart;       /* GMRES restart length (PETSc default 30), <= 64  */
     const double *v;       /* HOST, nv: velocity grid v_j                     */
-    const double *eigen;   /* HOST, 2*nx (re, im interleaved): the kinetic
-                              eigenvalues of TensorProduct.compute_eigenvalues
-                              (TensorProduct.pyx:109-116); required when
-                              precond = VV_PRECOND_TENSOR_KINETIC             */
+    const double *eigen;   /* HOST, re/im interleaved: the eigenvalues of
+                              TensorProduct.compute_eigenvalues
+                              (TensorProduct.pyx:109-116) -- of length nx for
+                              VV_PRECOND_TENSOR_KINETIC, of length nv for
+                              VV_PRECOND_TENSOR_POTENTIAL                     */
 } vv_params;
 
 /* Solver construction: PETScVlasovSolverBase.__init__
@@ -161,6 +163,17 @@ int vv_bracket_add(vv_ctx *ctx, const double *x, double *y, double factor);
  * (TensorProductKinetic.pyx:146-160, KineticFast.pyx:102-224):
  * y = irfft_x(T_k^-1 rfft_x(x)).  x and y may alias. */
 int vv_precond_apply(vv_ctx *ctx, const double *x, double *y);
+
+/* TensorProductPreconditioner.update_matrices, called from update_history at
+ * every timestep (PETScVlasovSolver.pyx:135-139): for
+ * VV_PRECOND_TENSOR_POTENTIAL the x-tridiagonal of every v-mode is rebuilt
+ * from phi (an x-vector per instance; TensorProductPotentialFast.pyx:205-264):
+ * sub e*1.0*(phi_i - phi_{i-1})*a, diagonal e*2.0*(phi_{i+1} - phi_{i-1})*a +
+ * 1/dt, super e*1.0*(phi_{i+1} - phi_i)*a, a = 0.5/(12 hx hv), without the
+ * periodic corners.  Until the first call the potential preconditioner is
+ * dt * identity.  A no-op for VV_PRECOND_TENSOR_KINETIC, whose matrices do
+ * not change. */
+int vv_precond_update(vv_ctx *ctx, const double *phi);
 
 /* PETScVlasovSolverBase.function / function_snes_mult / function_mult
  * (PETScVlasovSolver.pyx:168-173, 188-193): y = P^-1 F(f). */

@@ -310,6 +310,107 @@ class Preconditioner:
         return P
 
 
+def potential_bands(grid, phi, eigen_l):
+    """Sub-, main and super-diagonal of T_l along x for v-mode l
+    (TensorProductPotentialFast.pyx:229-264; PotentialSciPy.pyx:126-164):
+    diagm_i = e*1.0*(phi_i - phi_{i-1})*a, diag_i = e*2.0*(phi_{i+1} -
+    phi_{i-1})*a + 1/dt, diagp_i = e*1.0*(phi_{i+1} - phi_i)*a, indices mod nx,
+    a = 0.5*hx_inv*hv_inv/12.  The band matrix takes diagp[:-1] and diagm[1:],
+    so the periodic corners are dropped.
+    """
+    a = 0.5 * grid.hx_inv * grid.hv_inv / 12.
+    er, ei = float(eigen_l.real), float(eigen_l.imag)
+    phi = np.asarray(phi, dtype=np.float64)
+    phim, phip = np.roll(phi, 1), np.roll(phi, -1)
+    gm, g0, gp = phi - phim, phip - phim, phip - phi
+    diagm = ((er * 1.0) * gm * a) + 1j * ((ei * 1.0) * gm * a)
+    diag = ((er * 2.0) * g0 * a + grid.ht_inv) + 1j * ((ei * 2.0) * g0 * a)
+    diagp = ((er * 1.0) * gp * a) + 1j * ((ei * 1.0) * gp * a)
+    return diagm, diag, diagp
+
+
+class PotentialPreconditioner:
+    """P^-1 x = irfft_v( T_l^-1 rfft_v(x) )  (TensorProductPotential.pyx:157-173):
+    a real FFT along v for every x, then for every v-mode l = 0..nv/2 the
+    tridiagonal system T_l along x, whose coefficients depend on phi and are
+    rebuilt by update() (update_matrices, called from update_history at every
+    timestep, PETScVlasovSolver.pyx:135-139).  ``variant`` as in
+    Preconditioner: 'lapack' (zgbtrf/zgbtrs, PotentialFast.pyx:230-264) or
+    'scipy' (solve_banded, PotentialSciPy.pyx:96-104).
+    """
+
+    def __init__(self, grid, phi, variant='lapack'):
+        self.grid = grid
+        self.variant = variant
+        self.eigen = kinetic_eigenvalues(grid.nv)        # compute_eigenvalues(nv)
+        self.nmodes = grid.nv // 2 + 1
+        self.update(phi)
+
+    def update(self, phi):
+        nx = self.grid.nx
+        self.bands = [potential_bands(self.grid, phi, self.eigen[l]) for l in range(self.nmodes)]
+        self.info = []
+        self.lu, self.ab3 = [], []
+        for l in range(self.nmodes):
+            dm, d, dp = self.bands[l]
+            if self.variant == 'lapack':
+                ab = np.zeros((4, nx), dtype=np.complex128, order='F')
+                ab[1, 1:] = dp[:-1]
+                ab[2, :] = d
+                ab[3, :-1] = dm[1:]
+                lu, piv, info = lapack.zgbtrf(ab, 1, 1)
+                if info != 0:
+                    print("   ERROR in LU Decomposition.")
+                self.info.append(info)
+                self.lu.append((lu, piv))
+            elif self.variant == 'scipy':
+                ab = np.zeros((3, nx), dtype=np.complex128)
+                ab[0, 1:] = dp[:-1]
+                ab[1, :] = d
+                ab[2, :-1] = dm[1:]
+                self.ab3.append(ab)
+            else:
+                raise ValueError(self.variant)
+
+    def pivots_are_identity(self):
+        if self.variant != 'lapack':
+            return True
+        return all(np.array_equal(piv, np.arange(self.grid.nx)) for (_, piv) in self.lu)
+
+    def apply(self, x):
+        """x in the oracle layout (nx, nv)."""
+        X = np.fft.rfft(x, axis=1)               # (nx, nv//2+1)
+        C = np.empty_like(X)
+        for l in range(self.nmodes):
+            if self.variant == 'lapack':
+                lu, piv = self.lu[l]
+                C[:, l], info = lapack.zgbtrs(lu, 1, 1, X[:, l].copy(), piv)
+            else:
+                C[:, l] = solve_banded((1, 1), self.ab3[l], X[:, l].copy())
+        return np.fft.irfft(C, n=self.grid.nv, axis=1)
+
+    def dense_matrix(self):
+        """The operator P (not P^-1), assembled densely for tiny grids
+        (natural order i + nx*j)."""
+        nx, nv = self.grid.nx, self.grid.nv
+        N = nx * nv
+        P = np.zeros((N, N))
+        Ts = []
+        for l in range(self.nmodes):
+            dm, d, dp = self.bands[l]
+            Ts.append(np.diag(d) + np.diag(dp[:-1], 1) + np.diag(dm[1:], -1))
+        for col in range(N):
+            e = np.zeros(N)
+            e[col] = 1.0
+            x = e.reshape((nv, nx)).T
+            X = np.fft.rfft(x, axis=1)
+            Y = np.empty_like(X)
+            for l in range(self.nmodes):
+                Y[:, l] = Ts[l] @ X[:, l]
+            P[:, col] = np.fft.irfft(Y, n=nv, axis=1).T.reshape(N)
+        return P
+
+
 # --------------------------------------------------------------------------
 # moments and Poisson
 #   toolbox/Toolbox.pyx:38-98; poisson/PETScPoissonSolver.pyx:58-74;
@@ -558,7 +659,8 @@ class OracleRun:
 
     def __init__(self, grid, f0, bracket='arakawaj4', laplace='cfd4', charge=-1.0,
                  mass=1.0, ksp_rtol=1e-10, ksp_atol=1e-50, ksp_max_it=1000,
-                 snes_atol=1e-10, snes_max_iter=20, restart=30, precond_variant='lapack'):
+                 snes_atol=1e-10, snes_max_iter=20, restart=30, precond_variant='lapack',
+                 preconditioner='tensor_kinetic'):
         self.grid = grid
         self.kind = bracket
         self.laplace = laplace
@@ -566,12 +668,19 @@ class OracleRun:
         self.ksp = dict(rtol=ksp_rtol, atol=ksp_atol, max_it=ksp_max_it, restart=restart)
         self.snes_atol = snes_atol
         self.snes_max_iter = snes_max_iter
-        self.pc = Preconditioner(grid, precond_variant)
         self.h0 = kinetic_hamiltonian(grid, mass)
         self.fc = normalise(np.array(f0, dtype=np.float64), grid)
         self.h2c = np.zeros(grid.nx)
         self.h2h = np.zeros(grid.nx)
         self.calculate_moments()
+        # TensorProductPreconditioner.create (TensorProduct.pyx:66-86), built
+        # after the initial moments as in run_base.py (its phi is pc_int)
+        if preconditioner in ('tensor_kinetic', 'tensor_kinetic_scipy'):
+            self.pc = Preconditioner(grid, precond_variant)
+        elif preconditioner in ('tensor_potential', 'tensor_potential_scipy'):
+            self.pc = PotentialPreconditioner(grid, self.phi, precond_variant)
+        else:
+            raise ValueError(preconditioner)
         self.log = []
 
     # --- moments / Poisson (run_base.py:640-704, 741-742)
@@ -612,6 +721,9 @@ class OracleRun:
         self.fh = self.fc.copy()
         self.h1h = self.h1c.copy()
         self.h2h = self.h2c.copy()
+        # update_history -> preconditioner.update_matrices (PETScVlasovSolver.pyx:135-139)
+        if isinstance(self.pc, PotentialPreconditioner):
+            self.pc.update(self.phi)
         self.update_previous()
         pred = self.calculate_residual()
         entry = dict(initial=pred, picard=[])

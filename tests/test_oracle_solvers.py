@@ -218,3 +218,69 @@ def test_driver_abort_branch(monkeypatch):
     monkeypatch.setattr(run, 'calculate_residual', lambda: 100.0)
     with pytest.raises(RuntimeError):
         run.step()
+
+
+def _potential_case(nx=8, nv=16, seed=0):
+    g = O.Grid(nx, nv, 2 * np.pi, 5.0, ht=0.1)
+    phi = np.random.default_rng(seed).standard_normal(nx) * 0.3
+    return g, phi
+
+
+def test_potential_preconditioner_is_minus_half_j1_of_phi_plus_identity_over_dt():
+    """Interior rows of P (1 <= i <= nx-2, 2 <= j <= nv-3) equal
+    I/dt - 1/2 J1(., phi) from the pinned bracket oracle; the other rows differ
+    (dropped periodic corners in x, periodic v in the FFT, masked bracket
+    rows).  The sign is the reference's: its bands put +e*(phi differences)
+    where the bracket {u, phi} = -u_v phi_x carries a minus."""
+    g, phi = _potential_case()
+    pc = O.PotentialPreconditioner(g, phi)
+    P = pc.dense_matrix()
+    nx, nv = g.nx, g.nv
+    N = nx * nv
+    h = np.broadcast_to(phi[:, None], (nx, nv)).copy()
+    J = np.zeros((N, N))
+    for col in range(N):
+        e = np.zeros(N)
+        e[col] = 1.0
+        u = e.reshape((nv, nx)).T
+        y = g.ht_inv * u
+        O.bracket_add('arakawaj1', u, h, y, -0.5, g)
+        J[:, col] = y.T.reshape(N)
+    d = np.abs(P - J).max(axis=1).reshape(nv, nx)
+    interior = np.zeros((nv, nx), dtype=bool)
+    interior[2:nv - 2, 1:nx - 1] = True
+    assert d[interior].max() <= 1e-14 * np.abs(P).max()
+    assert d[~interior].min() > 1e-6        # every other row differs
+
+
+def test_potential_preconditioner_variants_and_inverse():
+    g, phi = _potential_case(16, 32, 1)
+    x = np.random.default_rng(2).standard_normal((16, 32))
+    a = O.PotentialPreconditioner(g, phi, 'lapack')
+    b = O.PotentialPreconditioner(g, phi, 'scipy')
+    assert a.pivots_are_identity()
+    np.testing.assert_allclose(a.apply(x), b.apply(x), rtol=0, atol=1e-13 * np.abs(a.apply(x)).max())
+    g8, phi8 = _potential_case(8, 16, 3)
+    c = O.PotentialPreconditioner(g8, phi8)
+    x8 = np.random.default_rng(4).standard_normal((8, 16))
+    y = c.dense_matrix() @ c.apply(x8).T.reshape(-1)
+    assert np.abs(y - x8.T.reshape(-1)).max() <= 1e-13 * np.abs(x8).max()
+    # update() rebuilds the bands from a new phi
+    c.update(2 * phi8)
+    assert np.allclose(c.bands[1][0], O.potential_bands(g8, 2 * phi8, c.eigen[1])[0])
+
+
+def test_driver_with_potential_preconditioner():
+    """The Picard driver with tensor_potential converges to the same step as
+    with tensor_kinetic (the preconditioner changes the iterates, not the
+    converged solution)."""
+    g = O.Grid(16, 32, 4 * np.pi, 8.0, ht=0.05)
+    prof = lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * O.maxwellian(v)
+    f0 = O.initial_distribution(g, prof)
+    rk = O.OracleRun(g, f0, ksp_rtol=1e-13, snes_atol=1e-13)
+    rp = O.OracleRun(g, f0, ksp_rtol=1e-13, snes_atol=1e-13, preconditioner='tensor_potential')
+    for _ in range(2):
+        rk.step()
+        ep = rp.step()
+        assert all(p['reason'] > 0 for p in ep['picard'])
+    assert np.abs(rk.fc - rp.fc).max() <= 1e-10 * np.abs(rk.fc).max()

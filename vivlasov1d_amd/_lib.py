@@ -15,7 +15,10 @@ VV_OK = 0
 ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "unsupported option", -4: "non-finite value"}
 
 BRACKETS = {"arakawaj1": 1, "arakawaj2": 2, "arakawaj4": 4}
-PRECONDITIONERS = {"": 0, None: 0, "none": 0, "tensor_kinetic": 1}
+# the _scipy schemes of the reference state the same operators with SciPy's
+# banded solver (TensorProduct.pyx:66-86)
+PRECONDITIONERS = {"": 0, None: 0, "none": 0, "tensor_kinetic": 1, "tensor_kinetic_scipy": 1,
+                   "tensor_potential": 2, "tensor_potential_scipy": 2}
 LAPLACE = {"cfd2": 2, "cfd4": 4}
 LINEAR = {"A": 0, "A0": 1, "AT": 2, "A0T": 3}
 
@@ -72,6 +75,7 @@ SIGNATURES = {
     "vv_jacobian_apply": (ctypes.c_int, [_P, _P, _P]),
     "vv_bracket_add": (ctypes.c_int, [_P, _P, _P, _D]),
     "vv_precond_apply": (ctypes.c_int, [_P, _P, _P]),
+    "vv_precond_update": (ctypes.c_int, [_P, _P]),
     "vv_function": (ctypes.c_int, [_P, _P, _P, _P]),
     "vv_jacobian": (ctypes.c_int, [_P, _P, _P]),
     "vv_function_norm": (ctypes.c_int, [_P, _P, _P, _P, _PD]),

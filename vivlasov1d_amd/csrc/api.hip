@@ -140,7 +140,7 @@ int check_ctx(vv_ctx *c)
 // algorithmic HBM bytes per launch (compulsory reads + writes of each kernel,
 // all instances of the context)
 double b_field(vv_ctx *c) { return 8.0 * (double)c->N * c->B; }
-double b_spec(vv_ctx *c) { return 16.0 * (double)c->nmodes * (double)c->p.nv * c->B; }
+double b_spec(vv_ctx *c) { return 16.0 * (double)c->nmodes * (double)c->fft_rows * c->B; }
 
 // Arnoldi iterations enqueued past the end of the cycle return at once: their
 // launches keep their (short) time but move no algorithmic bytes.
@@ -153,25 +153,45 @@ void prof_mark_idle(vv_ctx *c, size_t from, int done_its)
 }
 
 // y = P^-1 x (spectrum in c->spec); done: optional early-exit flag
+// y = P^-1 x (tr: P^-T x, the mode blocks T_k^H) of the tensor-product
+// preconditioner: rFFT, one tridiagonal system per mode, irFFT.  The
+// potential form transforms along v, so its field is transposed first and
+// back after (x and y may alias: the transforms go through the spectrum and
+// the transposed scratch).  it: the Arnoldi step (profile record) or -1.
+int pc_apply(vv_ctx *c, const double *x, double *y, const int32_t *done, int it, bool tr)
+{
+    const double F = b_field(c), S = b_spec(c);
+    const bool pot = c->tb != nullptr;
+    const double *src = x;
+    double *dst = y;
+    if (pot) {
+        VV_CHECK(launch_transpose(c, x, c->tb, false, done), "transpose");
+        src = dst = c->tb;
+    }
+    VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, src, c->spec, done), "rfft");
+    if (tr) {
+        VV_CHECK(launch_tridiag_t(c, c->spec, done), "tridiag (transposed)");
+    } else {
+        VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
+    }
+    VV_PLAUNCH(P_IRFFT, S + F, it, launch_irfft(c, c->spec, dst, done), "irfft");
+    if (pot) VV_CHECK(launch_transpose(c, c->tb, y, true, done), "transpose");
+    return VV_OK;
+}
+
 int precond(vv_ctx *c, const double *x, double *y, const int32_t *done)
 {
     if (c->p.precond == VV_PRECOND_NONE) {
         if (x != y) VV_CHECK(launch_copy(c, x, y, c->N * c->B), "copy");
         return VV_OK;
     }
-    VV_PLAUNCH(P_RFFT, b_field(c) + b_spec(c), -1, launch_rfft(c, x, c->spec, done), "rfft");
-    VV_PLAUNCH(P_TRI, 2 * b_spec(c), -1, launch_tridiag_k(c, c->spec, done), "tridiag");
-    VV_PLAUNCH(P_IRFFT, b_field(c) + b_spec(c), -1, launch_irfft(c, c->spec, y, done), "irfft");
-    return VV_OK;
+    return pc_apply(c, x, y, done, -1, false);
 }
 
 // y = P^-T x: the transposed preconditioner (mode blocks T_k^H)
 int precond_t(vv_ctx *c, const double *x, double *y, const int32_t *done)
 {
-    VV_CHECK(launch_rfft(c, x, c->spec, done), "rfft");
-    VV_CHECK(launch_tridiag_t(c, c->spec, done), "tridiag (transposed)");
-    VV_CHECK(launch_irfft(c, c->spec, y, done), "irfft");
-    return VV_OK;
+    return pc_apply(c, x, y, done, -1, true);
 }
 
 int ensure_linear(vv_ctx *c)
@@ -256,9 +276,7 @@ int arnoldi_step(vv_ctx *c, int it)
         if (int r = apply_pc(c, c->X, w, done)) return r;
     } else {
         VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
-        VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, c->X, c->spec, done), "rfft");
-        VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
-        VV_PLAUNCH(P_IRFFT, S + F, it, launch_irfft(c, c->spec, w, done), "irfft");
+        if (int r = pc_apply(c, c->X, w, done, it, false)) return r;
     }
     VV_PLAUNCH(P_MDOT, F * (it + 2), it, launch_mdot(c, it), "mdot");
     VV_PLAUNCH(P_MAXPY, F * (it + 3), it, launch_maxpy_norm(c, it), "maxpy_norm");
@@ -266,14 +284,29 @@ int arnoldi_step(vv_ctx *c, int it)
 }
 
 // y = P^-1 x in FP32 (complex64 spectrum in c->spec)
-int precond32(vv_ctx *c, const float *x, float *y, const int32_t *done)
+// FP32 P^-1 (the mixed solver's cycle), as pc_apply
+int pc_apply32(vv_ctx *c, const float *x, float *y, const int32_t *done, int it)
 {
     float2 *spec = reinterpret_cast<float2 *>(c->spec);
     const double F = 4.0 * (double)c->N * c->B, S = 0.5 * b_spec(c);
-    VV_PLAUNCH(P_RFFT32, F + S, -1, launch_rfft32(c, x, spec, done), "rfft32");
-    VV_PLAUNCH(P_TRI32, 2 * S, -1, launch_tridiag32(c, spec, done), "tridiag32");
-    VV_PLAUNCH(P_IRFFT32, S + F, -1, launch_irfft32(c, spec, y, done), "irfft32");
+    const bool pot = c->tb != nullptr;
+    float *tb = reinterpret_cast<float *>(c->tb);
+    const float *src = x;
+    float *dst = y;
+    if (pot) {
+        VV_CHECK(launch_transpose32(c, x, tb, false, done), "transpose32");
+        src = dst = tb;
+    }
+    VV_PLAUNCH(P_RFFT32, F + S, it, launch_rfft32(c, src, spec, done), "rfft32");
+    VV_PLAUNCH(P_TRI32, 2 * S, it, launch_tridiag32(c, spec, done), "tridiag32");
+    VV_PLAUNCH(P_IRFFT32, S + F, it, launch_irfft32(c, spec, dst, done), "irfft32");
+    if (pot) VV_CHECK(launch_transpose32(c, tb, y, true, done), "transpose32");
     return VV_OK;
+}
+
+int precond32(vv_ctx *c, const float *x, float *y, const int32_t *done)
+{
+    return pc_apply32(c, x, y, done, -1);
 }
 
 // one Arnoldi step of the FP32 cycle: V_{it+1} = P32^-1 J32 v_it, CGS in FP64
@@ -285,12 +318,9 @@ int arnoldi_step32(vv_ctx *c, int it)
     float *v = c->basis32 + (size_t)it * vs;
     float *w = c->basis32 + (size_t)(it + 1) * vs;
     float *X = reinterpret_cast<float *>(c->X);
-    float2 *spec = reinterpret_cast<float2 *>(c->spec);
-    const double F = 4.0 * (double)c->N * c->B, S = 0.5 * b_spec(c);
+    const double F = 4.0 * (double)c->N * c->B;
     VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], X), "jacobian32");
-    VV_PLAUNCH(P_RFFT32, F + S, it, launch_rfft32(c, X, spec, done), "rfft32");
-    VV_PLAUNCH(P_TRI32, 2 * S, it, launch_tridiag32(c, spec, done), "tridiag32");
-    VV_PLAUNCH(P_IRFFT32, S + F, it, launch_irfft32(c, spec, w, done), "irfft32");
+    if (int r = pc_apply32(c, X, w, done, it)) return r;
     VV_PLAUNCH(P_MDOT32, F * (it + 2), it, launch_mdot32(c, it), "mdot32");
     VV_PLAUNCH(P_MAXPY32, F * (it + 3), it, launch_maxpy_norm32(c, it), "maxpy_norm32");
     return VV_OK;
@@ -401,8 +431,8 @@ int restart_cycles(vv_ctx *c, const double *b, double *x)
 
 int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, vv_ksp_result *res)
 {
-    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve needs the tensor_kinetic preconditioner");
+    if (c->p.precond == VV_PRECOND_NONE)
+        return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve needs a tensor-product preconditioner");
     VV_CHECK(launch_zero(c, x, c->N * c->B), "zero x");
     VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
     if (int r = cycle_start_from(c, b)) return r;
@@ -474,8 +504,8 @@ int snes_graph(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp)
 
 int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, vv_ksp_result *res)
 {
-    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(c, VV_ERR_UNSUPPORTED, "vv_snes_ksponly needs the tensor_kinetic preconditioner");
+    if (c->p.precond == VV_PRECOND_NONE)
+        return fail(c, VV_ERR_UNSUPPORTED, "vv_snes_ksponly needs a tensor-product preconditioner");
     if (c->use_graph) {
         if (int r = snes_graph(c, f, fh, kp)) return r;
     } else {
@@ -514,8 +544,8 @@ int ensure_mixed(vv_ctx *c)
 int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, const vv_mixed_params *mp,
                          vv_ksp_result *res, int32_t *cycles)
 {
-    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve_mixed needs the tensor_kinetic preconditioner");
+    if (c->p.precond == VV_PRECOND_NONE)
+        return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve_mixed needs a tensor-product preconditioner");
     if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the mixed-precision solve takes one instance");
     if (int r = ensure_mixed(c)) return r;
     const int64_t N = c->N;
@@ -603,15 +633,18 @@ int check_params(const vv_params *p)
     if (p->nv < 8 || p->nv > 4096) return fail(nullptr, VV_ERR_INVALID, "nv must be in [8, 4096]");
     if (p->bracket != VV_BRACKET_J1 && p->bracket != VV_BRACKET_J2 && p->bracket != VV_BRACKET_J4)
         return fail(nullptr, VV_ERR_UNSUPPORTED, "bracket must be arakawaj1, arakawaj2 or arakawaj4");
-    if (p->precond != VV_PRECOND_NONE && p->precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(nullptr, VV_ERR_UNSUPPORTED, "preconditioner must be none or tensor_kinetic");
+    if (p->precond != VV_PRECOND_NONE && p->precond != VV_PRECOND_TENSOR_KINETIC &&
+        p->precond != VV_PRECOND_TENSOR_POTENTIAL)
+        return fail(nullptr, VV_ERR_UNSUPPORTED, "preconditioner must be none, tensor_kinetic or tensor_potential");
+    if (p->precond == VV_PRECOND_TENSOR_POTENTIAL && !is_pow2(p->nv))
+        return fail(nullptr, VV_ERR_UNSUPPORTED, "tensor_potential needs nv a power of two (FFT along v)");
     if (p->laplace != VV_LAPLACE_CFD2 && p->laplace != VV_LAPLACE_CFD4)
         return fail(nullptr, VV_ERR_UNSUPPORTED, "laplace operator must be cfd2 or cfd4");
     if (p->restart < 1 || p->restart > kMaxRestart) return fail(nullptr, VV_ERR_INVALID, "restart must be in [1, 64]");
     if (!(p->hv > 0) || !(p->ht > 0)) return fail(nullptr, VV_ERR_INVALID, "hx, hv, ht must be > 0");
     if (!p->v) return fail(nullptr, VV_ERR_INVALID, "v grid missing");
-    if (p->precond == VV_PRECOND_TENSOR_KINETIC && !p->eigen)
-        return fail(nullptr, VV_ERR_INVALID, "kinetic eigenvalues missing");
+    if (p->precond != VV_PRECOND_NONE && !p->eigen)
+        return fail(nullptr, VV_ERR_INVALID, "preconditioner eigenvalues missing");
     return VV_OK;
 }
 
@@ -635,9 +668,16 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     c->stream = reinterpret_cast<hipStream_t>(stream);
     const int nx = p->nx, nv = p->nv;
     c->N = (int64_t)nx * nv;
-    c->nmodes = nx / 2 + 1;
+    const bool pot = p->precond == VV_PRECOND_TENSOR_POTENTIAL;
+    c->fft_n = pot ? nv : nx;
+    c->fft_rows = pot ? nx : nv;
+    c->nmodes = c->fft_n / 2 + 1;
     c->logn = 0;
-    while ((1 << c->logn) < nx) ++c->logn;
+    while ((1 << c->logn) < c->fft_n) ++c->logn;
+    c->coef_stride = pot ? nx : 0;
+    c->cf_lo = pot ? 1.0 : 0.5;     // PotentialFast.pyx:250-252 / KineticFast.pyx:258-260
+    c->cf_mid = pot ? 2.0 : 1.0;
+    c->cf_hi = pot ? 1.0 : 0.5;
     // Grid.create (Grid.pyx:40-48) and the bracket / band factors
     const double hv = p->hv;
     c->ht_inv = 1. / p->ht;
@@ -680,13 +720,15 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(dalloc(&c->h2p, nx * Bz), "alloc h2p");
     VV_CCHECK(dalloc(&c->h2h, nx * Bz), "alloc h2h");
     VV_CCHECK(dalloc(&c->eigen, c->nmodes), "alloc eigen");
-    VV_CCHECK(dalloc(&c->gm, nv), "alloc gm");
-    VV_CCHECK(dalloc(&c->g0, nv), "alloc g0");
-    VV_CCHECK(dalloc(&c->gp, nv), "alloc gp");
+    const size_t ncoef = pot ? (size_t)nx * Bz : (size_t)nv;
+    VV_CCHECK(dalloc(&c->gm, ncoef), "alloc gm");
+    VV_CCHECK(dalloc(&c->g0, ncoef), "alloc g0");
+    VV_CCHECK(dalloc(&c->gp, ncoef), "alloc gp");
+    if (pot) VV_CCHECK(dalloc(&c->tb, c->N * Bz), "alloc transposed field");
     VV_CCHECK(dalloc(&c->vgrid, nv), "alloc v grid");
-    VV_CCHECK(dalloc(&c->twiddle, nx), "alloc twiddle");
-    VV_CCHECK(dalloc(&c->twiddle32, nx), "alloc twiddle32");
-    VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * nv * Bz), "alloc spectrum");
+    VV_CCHECK(dalloc(&c->twiddle, c->fft_n), "alloc twiddle");
+    VV_CCHECK(dalloc(&c->twiddle32, c->fft_n), "alloc twiddle32");
+    VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * c->fft_rows * Bz), "alloc spectrum");
     VV_CCHECK(dalloc(&c->pinv, nx * Bz), "alloc pinv");
     VV_CCHECK(dalloc(&c->pwork, (3 * (size_t)nx + 1) * Bz), "alloc Poisson scratch");
     VV_CCHECK(dalloc(&c->X, c->N * Bz), "alloc X");
@@ -713,8 +755,10 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(hipMemset(c->h0, 0, nv * sizeof(double)), "zero h0");
 
     // host tables
-    std::vector<double> gm(nv), g0(nv), gp(nv);
-    for (int j = 0; j < nv; ++j) {
+    // kinetic band factors of v (KineticFast.pyx:258-260); the potential
+    // factors are phi differences, zero until vv_precond_update
+    std::vector<double> gm(ncoef, 0.0), g0(ncoef, 0.0), gp(ncoef, 0.0);
+    for (int j = 0; j < nv && !pot; ++j) {
         const double v = p->v[j];
         gm[j] = 2. * hv * v - hv2;     // KineticFast.pyx:258
         g0[j] = 4. * hv * v;           // :259
@@ -725,10 +769,11 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
         eig[k].x = p->eigen ? p->eigen[2 * k] : 0.0;
         eig[k].y = p->eigen ? p->eigen[2 * k + 1] : 0.0;
     }
-    std::vector<double2> tw(nx);
-    std::vector<float2> tw32(nx);
-    for (int m = 0; m < nx; ++m) {
-        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)nx;
+    const int nf = c->fft_n;
+    std::vector<double2> tw(nf);
+    std::vector<float2> tw32(nf);
+    for (int m = 0; m < nf; ++m) {
+        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)nf;
         tw[m].x = (double)cosl(ang);
         tw[m].y = (double)sinl(ang);
         tw32[m].x = (float)cosl(ang);
@@ -759,13 +804,13 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     std::vector<int32_t> act(Bz, 1);
     VV_CCHECK(hipMemcpy(c->inst, c->inst_host.data(), Bz * sizeof(InstConst), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->active, act.data(), Bz * sizeof(int32_t), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->gm, gm.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->g0, g0.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->gp, gp.data(), nv * sizeof(double), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->gm, gm.data(), ncoef * sizeof(double), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->g0, g0.data(), ncoef * sizeof(double), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->gp, gp.data(), ncoef * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->vgrid, p->v, nv * sizeof(double), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->eigen, eig.data(), c->nmodes * sizeof(double2), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->twiddle, tw.data(), nx * sizeof(double2), hipMemcpyHostToDevice), "upload");
-    VV_CCHECK(hipMemcpy(c->twiddle32, tw32.data(), nx * sizeof(float2), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->twiddle, tw.data(), nf * sizeof(double2), hipMemcpyHostToDevice), "upload");
+    VV_CCHECK(hipMemcpy(c->twiddle32, tw32.data(), nf * sizeof(float2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->pinv, pinv.data(), nx * Bz * sizeof(double), hipMemcpyHostToDevice), "upload");
 #undef VV_CCHECK
     *out = c;
@@ -803,7 +848,7 @@ int vv_ctx_destroy(vv_ctx *c)
     void *bufs[] = {c->inst,   c->active, c->h0,      c->h1p,     c->h1h,  c->h2p,   c->h2h,    c->eigen,
                     c->gm,     c->g0,     c->gp,      c->twiddle, c->twiddle32, c->spec, c->pinv, c->pwork,
                     c->X,      c->rhs,    c->D,       c->basis,   c->basis32, c->R,   c->partial, c->dscalar,
-                    c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid};
+                    c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid, c->tb};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);
@@ -848,8 +893,9 @@ int64_t vv_scratch_bytes(const vv_ctx *c)
     const int64_t nx = c->p.nx, nv = c->p.nv, B = c->B;
     int64_t bytes = (int64_t)sizeof(double) * B * ((c->p.restart + 4) * c->N + 64 * nx + 8 * nx + 1 + c->part_stride) +
                     (int64_t)sizeof(double) * (4 * nv + (int64_t)ds_count(c->B)) +
-                    (int64_t)sizeof(double2) * (B * (int64_t)c->nmodes * nv + nx + c->nmodes) +
-                    (int64_t)sizeof(float2) * nx + (int64_t)(sizeof(GmresState) + sizeof(InstConst) + 4) * B;
+                    (int64_t)sizeof(double2) * (B * (int64_t)c->nmodes * c->fft_rows + c->fft_n + c->nmodes) +
+                    (int64_t)sizeof(float2) * c->fft_n + (int64_t)(sizeof(GmresState) + sizeof(InstConst) + 4) * B;
+    if (c->tb) bytes += (int64_t)sizeof(double) * B * (c->N + 2 * nx);
     if (c->basis32) bytes += (int64_t)sizeof(float) * B * (c->p.restart + 1) * c->N + 8 * B * c->N;
     return bytes;
 }
@@ -882,8 +928,8 @@ int vv_set_graph(vv_ctx *c, int enable)
 int vv_set_tuning(vv_ctx *c, int32_t fft_rows, int32_t cgs_ept)
 {
     if (int r = check_ctx(c)) return r;
-    if (fft_rows != 0 && (fft_rows < 2 || (fft_rows & 1) || fft_rows * c->p.nx > 8192))
-        return fail(c, VV_ERR_INVALID, "fft_rows must be 0 or even, >= 2, with fft_rows * nx <= 8192");
+    if (fft_rows != 0 && (fft_rows < 2 || (fft_rows & 1) || fft_rows * c->fft_n > 8192))
+        return fail(c, VV_ERR_INVALID, "fft_rows must be 0 or even, >= 2, with fft_rows * (FFT length) <= 8192");
     if (cgs_ept != 0 && cgs_ept != 1 && cgs_ept != 2 && cgs_ept != 4 && cgs_ept != 8)
         return fail(c, VV_ERR_INVALID, "cgs_ept must be 0, 1, 2, 4 or 8");
     if (c->graph_exec) {
@@ -910,8 +956,8 @@ int vv_get_tuning(vv_ctx *c, int32_t *fft_rows, int32_t *cgs_ept)
 int vv_autotune(vv_ctx *c, int32_t *fft_rows, int32_t *cgs_ept)
 {
     if (int r = check_ctx(c)) return r;
-    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(c, VV_ERR_UNSUPPORTED, "autotune needs the tensor_kinetic preconditioner");
+    if (c->p.precond == VV_PRECOND_NONE)
+        return fail(c, VV_ERR_UNSUPPORTED, "autotune needs a tensor-product preconditioner");
     if (c->graph_exec) {
         (void)hipGraphExecDestroy(c->graph_exec);
         c->graph_exec = nullptr;
@@ -938,7 +984,7 @@ int vv_autotune(vv_ctx *c, int32_t *fft_rows, int32_t *cgs_ept)
         if ((rc = launch_zero(c, c->X, c->N * c->B) == hipSuccess ? VV_OK : VV_ERR_HIP)) break;
         int bestR = 0;
         float bestT = 0.f;
-        for (int R = 2; R * c->p.nx <= 8192; R <<= 1) {
+        for (int R = 2; R * c->fft_n <= 8192; R <<= 1) {
             c->tune_fft_rows = R;
             float ms = 0.f;
             if ((rc = timed([&]() { return precond(c, c->X, c->D, nullptr); }, 5, &ms))) break;
@@ -1021,6 +1067,15 @@ int vv_bracket_add(vv_ctx *c, const double *x, double *y, double factor)
     if (!x || !y) return fail(c, VV_ERR_INVALID, "null field");
     if (x == y) return fail(c, VV_ERR_INVALID, "bracket_add: x and y must not alias");
     VV_CHECK(launch_bracket_add(c, x, y, factor), "bracket");
+    return VV_OK;
+}
+
+int vv_precond_update(vv_ctx *c, const double *phi)
+{
+    if (int r = check_ctx(c)) return r;
+    if (c->p.precond != VV_PRECOND_TENSOR_POTENTIAL) return VV_OK;
+    if (!phi) return fail(c, VV_ERR_INVALID, "null phi");
+    VV_CHECK(launch_potential_bands(c, phi), "potential bands");
     return VV_OK;
 }
 
@@ -1200,8 +1255,8 @@ int vv_precond_apply32(vv_ctx *c, const float *x, float *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!x || !y) return fail(c, VV_ERR_INVALID, "null field");
-    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(c, VV_ERR_UNSUPPORTED, "precond_apply32 needs the tensor_kinetic preconditioner");
+    if (c->p.precond == VV_PRECOND_NONE)
+        return fail(c, VV_ERR_UNSUPPORTED, "precond_apply32 needs a tensor-product preconditioner");
     return precond32(c, x, y, nullptr);
 }
 
@@ -1209,8 +1264,8 @@ int vv_precond_apply_t(vv_ctx *c, const double *x, double *y)
 {
     if (int r = check_ctx(c)) return r;
     if (!x || !y) return fail(c, VV_ERR_INVALID, "null field");
-    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(c, VV_ERR_UNSUPPORTED, "the transposed preconditioner needs tensor_kinetic");
+    if (c->p.precond == VV_PRECOND_NONE)
+        return fail(c, VV_ERR_UNSUPPORTED, "the transposed preconditioner needs a tensor-product preconditioner");
     return precond_t(c, x, y, nullptr);
 }
 
@@ -1235,8 +1290,8 @@ int vv_adjoint_solve(vv_ctx *c, const double *lam, const double *fbar, double *m
     if (int r = check_ctx(c)) return r;
     if (!lam || !mu || !kp) return fail(c, VV_ERR_INVALID, "null argument");
     if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
-    if (c->p.precond != VV_PRECOND_TENSOR_KINETIC)
-        return fail(c, VV_ERR_UNSUPPORTED, "the adjoint solve needs the tensor_kinetic preconditioner");
+    if (c->p.precond == VV_PRECOND_NONE)
+        return fail(c, VV_ERR_UNSUPPORTED, "the adjoint solve needs a tensor-product preconditioner");
     if (c->use_graph) return fail(c, VV_ERR_UNSUPPORTED, "switch graph replay off for the adjoint solve");
     if (int r = ensure_linear(c)) return r;
     if (int r = precond_t(c, lam, c->rhs, nullptr)) return r;

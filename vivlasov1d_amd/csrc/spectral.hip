@@ -16,6 +16,13 @@
 //                 numpy.fft.irfft / pyFFTW (imaginary parts of the k = 0 and
 //                 k = nx/2 coefficients are ignored, as in a c2r transform).
 //
+// The potential form (TensorProductPotential.pyx:157-173) runs the same three
+// kernels with the roles of x and v exchanged: the field is transposed
+// (transpose_kernel), rFFT along v, one tridiagonal system along x per
+// v-mode with bands from phi differences (potential_bands_kernel), irFFT,
+// transposed back.  The transform geometry (length, rows, modes, band
+// constants) comes from the context.
+//
 // Every kernel is a template on the real type: double for the FP64 path,
 // float for the FP32 preconditioner inside the mixed-precision GMRES cycle
 // (complex64 spectrum, float twiddles; T_k's coefficients are formed in FP64
@@ -364,6 +371,8 @@ struct TriArgs {
     int64_t S, gstride;
     const double2 *eigen;
     const double *gm, *g0, *gp;
+    int64_t cstride;         // instance stride of gm/g0/gp (0: shared)
+    double clo, cmid, chi;   // band constants (kinetic 0.5, 1.0, 0.5; potential 1.0, 2.0, 1.0)
     const InstConst *inst;   // band factor afpc of instance blockIdx.y
     double ht_inv;
     const int32_t *done;
@@ -404,9 +413,9 @@ __device__ __forceinline__ void tri_row(const TriArgs &a, double af, double2 e, 
     using R = typename C::value_type;
     const double ey = TR ? -e.y : e.y;
     if (j < a.nv) {
-        aj = (j > 0) ? cmk<C>((R)(((e.x * 0.5) * lo) * af), (R)(((ey * 0.5) * lo) * af)) : cmk<C>(R(0), R(0));
-        bj = cmk<C>((R)(((e.x * 1.0) * mid) * af + a.ht_inv), (R)(((ey * 1.0) * mid) * af));
-        cj = (j < a.nv - 1) ? cmk<C>((R)(((e.x * 0.5) * hi) * af), (R)(((ey * 0.5) * hi) * af))
+        aj = (j > 0) ? cmk<C>((R)(((e.x * a.clo) * lo) * af), (R)(((ey * a.clo) * lo) * af)) : cmk<C>(R(0), R(0));
+        bj = cmk<C>((R)(((e.x * a.cmid) * mid) * af + a.ht_inv), (R)(((ey * a.cmid) * mid) * af));
+        cj = (j < a.nv - 1) ? cmk<C>((R)(((e.x * a.chi) * hi) * af), (R)(((ey * a.chi) * hi) * af))
                             : cmk<C>(R(0), R(0));
     } else {
         aj = cmk<C>(R(0), R(0));
@@ -421,6 +430,9 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     using C = cx_t<T>;
     const int32_t dflag = cycle_done(a.done, a.gstride);
     spec += blockIdx.y * a.S;
+    a.gm += blockIdx.y * a.cstride;
+    a.g0 += blockIdx.y * a.cstride;
+    a.gp += blockIdx.y * a.cstride;
     const double af = a.inst[blockIdx.y].afpc;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *sd = reinterpret_cast<C *>(lds_raw);   // P*(M+1): right-hand sides / solution, chunk c at c*(M+1)
@@ -558,12 +570,73 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     for (int j = threadIdx.x; j < a.nv; j += NT) S[j] = sd[j + j / M];
 }
 
+// out = in^T for every instance (blockIdx.z): in holds R rows of C, out C
+// rows of R.  32x32 tiles through LDS (one pad column), loads and stores
+// coalesced along the contiguous axis of each side; clamped addresses for the
+// edge tiles.  The potential preconditioner transforms along v, so it works on
+// the transposed field.
+template <typename T>
+__global__ __launch_bounds__(NT) void transpose_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int C,
+                                                       int64_t N, const int32_t *done, int64_t gstride)
+{
+    const int b = blockIdx.z;
+    if (done != nullptr && *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(done) + b * gstride))
+        return;
+    in += (size_t)b * N;
+    out += (size_t)b * N;
+    __shared__ T tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;      // 32 x 8
+    T v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = r0 + ty + 8 * q, cc = c0 + tx;
+        const bool ok = r < R && cc < C;
+        const T t = in[(size_t)(ok ? r : 0) * C + (ok ? cc : 0)];
+        v[q] = ok ? t : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tile[ty + 8 * q][tx] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int oc = c0 + ty + 8 * q, orr = r0 + tx;
+        if (oc < C && orr < R) out[(size_t)oc * R + orr] = tile[tx][ty + 8 * q];
+    }
+}
+
+// The phi differences of the potential bands (TensorProductPotentialFast.pyx:
+// 243-252): gm_i = phi_i - phi_{i-1}, g0_i = phi_{i+1} - phi_{i-1},
+// gp_i = phi_{i+1} - phi_i, indices mod nx, per instance (blockIdx.y).
+__global__ __launch_bounds__(NT) void potential_bands_kernel(const double *__restrict__ phi, int nx,
+                                                             double *__restrict__ gm, double *__restrict__ g0,
+                                                             double *__restrict__ gp)
+{
+    const size_t o = (size_t)blockIdx.y * nx;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= nx) return;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    gm[o + i] = phi[o + i] - phi[o + im];
+    g0[o + i] = phi[o + ip] - phi[o + im];
+    gp[o + i] = phi[o + ip] - phi[o + i];
+}
+
+template <typename T>
+hipError_t transpose_t(vv_ctx *c, const T *in, T *out, bool back, const int32_t *done)
+{
+    const int R = back ? c->p.nx : c->p.nv, C = back ? c->p.nv : c->p.nx;
+    dim3 grid((C + 31) / 32, (R + 31) / 32, c->B);
+    hipLaunchKernelGGL(transpose_kernel<T>, grid, dim3(NT), 0, c->stream, in, out, R, C, c->N, done,
+                       (int64_t)sizeof(GmresState));
+    return hipGetLastError();
+}
+
 // v-rows per FFT workgroup: the context's tuned value, else the largest that
 // fits 64 KiB of LDS while the launch keeps >= 512 workgroups
 int fft_rows(const vv_ctx *c)
 {
     if (c->tune_fft_rows > 0) return c->tune_fft_rows;
-    const int nx = c->p.nx, nv = c->p.nv;
+    const int nx = c->fft_n, nv = c->fft_rows;
     int R = 2;
     while (2 * R * nx <= 8192) R *= 2;
     if (R < 2) R = 2;
@@ -574,7 +647,7 @@ int fft_rows(const vv_ctx *c)
 template <typename T>
 hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const int32_t *done)
 {
-    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
+    FftArgs<T> a{c->fft_n, c->logn, c->fft_rows, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
@@ -586,7 +659,7 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
 template <typename T>
 hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, const int32_t *done)
 {
-    FftArgs<T> a{c->p.nx, c->logn, c->p.nv, fft_rows(c), c->N, spec_stride(c),
+    FftArgs<T> a{c->fft_n, c->logn, c->fft_rows, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
@@ -613,9 +686,9 @@ template <typename T, bool TR = false>
 hipError_t tridiag_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
 {
     int P, M;
-    tridiag_shape(c->p.nv, &P, &M);
-    TriArgs a{c->p.nv, P, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp, c->inst,
-              c->ht_inv, done};
+    tridiag_shape(c->fft_rows, &P, &M);
+    TriArgs a{c->fft_rows, P, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp,
+              c->coef_stride, c->cf_lo, c->cf_mid, c->cf_hi, c->inst, c->ht_inv, done};
     const size_t lds = (size_t)P * (M + 1) * sizeof(cx_t<T>);
     dim3 grid(c->nmodes, c->B), block(NT);
     switch (M) {
@@ -632,7 +705,7 @@ hipError_t tridiag_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
 int fft_blocks(vv_ctx *c)
 {
     const int R = fft_rows(c);
-    return (c->p.nv + R - 1) / R;
+    return (c->fft_rows + R - 1) / R;
 }
 
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done)
@@ -668,6 +741,23 @@ hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t
 hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done)
 {
     return tridiag_t<float>(c, spec, done);
+}
+
+hipError_t launch_transpose(vv_ctx *c, const double *in, double *out, bool back, const int32_t *done)
+{
+    return transpose_t<double>(c, in, out, back, done);
+}
+
+hipError_t launch_transpose32(vv_ctx *c, const float *in, float *out, bool back, const int32_t *done)
+{
+    return transpose_t<float>(c, in, out, back, done);
+}
+
+hipError_t launch_potential_bands(vv_ctx *c, const double *phi)
+{
+    hipLaunchKernelGGL(potential_bands_kernel, dim3((c->p.nx + NT - 1) / NT, c->B), dim3(NT), 0, c->stream, phi,
+                       c->p.nx, c->gm, c->g0, c->gp);
+    return hipGetLastError();
 }
 
 }  // namespace vv

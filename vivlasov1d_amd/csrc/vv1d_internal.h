@@ -67,19 +67,28 @@ struct vv_ctx {
     std::vector<vv::InstConst> inst_host;
     int32_t *active;            // device, B flags: instance takes part in the next solve
     int64_t N;          // nx * nv (per instance)
-    int nmodes;         // nx/2 + 1
-    int logn;           // log2(nx)
+    // transform geometry of the tensor-product preconditioner: the real FFT
+    // has length fft_n over fft_rows rows and nmodes = fft_n/2 + 1 modes, each
+    // mode a tridiagonal system of length fft_rows.  kinetic: FFT along x
+    // (fft_n = nx, fft_rows = nv); potential: along v on the transposed field
+    // (fft_n = nv, fft_rows = nx)
+    int nmodes;         // fft_n/2 + 1
+    int logn;           // log2(fft_n)
+    int fft_n, fft_rows;
+    int64_t coef_stride;        // instance stride of gm/g0/gp (0: shared)
+    double cf_lo, cf_mid, cf_hi;  // band constants: kinetic 0.5/1.0/0.5, potential 1.0/2.0/1.0
+    double *tb;                 // potential: the transposed field (B N doubles)
     // grid-derived constants (x-dependent ones per instance in inst)
     double ht_inv, hv_inv;
     // Hamiltonian factors (device, ctx-owned copies): h0 shared, h1*/h2* per instance
     double *h0, *h1p, *h1h, *h2p, *h2h;
     // preconditioner data (device)
-    double2 *eigen;     // nmodes used (k = 0..nx/2)
-    double *gm, *g0, *gp;   // per-row real factors of the bands
+    double2 *eigen;     // nmodes used (mode k = 0..fft_n/2)
+    double *gm, *g0, *gp;   // per-row real factors of the bands (kinetic: of v; potential: phi differences per instance)
     double *vgrid;          // v_j (velocity moments)
-    double2 *twiddle;   // exp(-2 pi i m / nx), m = 0..nx-1
+    double2 *twiddle;   // exp(-2 pi i m / fft_n), m = 0..fft_n-1
     float2 *twiddle32;  // the same, rounded to float (FP32 preconditioner)
-    double2 *spec;      // nmodes * nv complex spectrum, mode-major
+    double2 *spec;      // nmodes * fft_rows complex spectrum per instance, mode-major
     // Poisson
     double *pinv;       // first row of the mean-free circulant inverse (nx per instance)
     double *pwork;      // Poisson scratch (3*nx + 1 per instance)
@@ -148,6 +157,10 @@ hipError_t launch_tridiag_t(vv_ctx *c, double2 *spec, const int32_t *done);   //
 hipError_t launch_rfft32(vv_ctx *c, const float *x, float2 *spec, const int32_t *done);
 hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done);
 hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t *done);
+// potential preconditioner: field (nv rows of nx) <-> transposed (nx rows of nv)
+hipError_t launch_transpose(vv_ctx *c, const double *in, double *out, bool back, const int32_t *done);
+hipError_t launch_transpose32(vv_ctx *c, const float *in, float *out, bool back, const int32_t *done);
+hipError_t launch_potential_bands(vv_ctx *c, const double *phi);
 // krylov.hip
 hipError_t launch_gmres_init(vv_ctx *c, double rtol, double atol, double dtol, int max_it);
 hipError_t launch_norm_start(vv_ctx *c);                  // cycle start on basis[0]
@@ -194,7 +207,7 @@ inline void launch_timed(vv_ctx *c, K kernel, dim3 grid, dim3 block, uint32_t sh
 }
 
 // element stride between the spectra of consecutive instances
-inline int64_t spec_stride(const vv_ctx *c) { return (int64_t)c->nmodes * c->p.nv; }
+inline int64_t spec_stride(const vv_ctx *c) { return (int64_t)c->nmodes * c->fft_rows; }
 
 // XCD-aware block order: spread consecutive tiles so that each of the 8 XCDs
 // works on one contiguous range of tiles (halo lines stay in that XCD's L2).

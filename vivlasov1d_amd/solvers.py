@@ -56,14 +56,18 @@ class Context:
         if bracket not in _lib.BRACKETS:
             raise _lib.VVError("Poisson bracket %r is not provided (arakawaj1/2/4)" % (bracket,))
         if preconditioner not in _lib.PRECONDITIONERS:
-            raise _lib.VVError("preconditioner %r is not provided (tensor_kinetic or none)" % (preconditioner,))
+            raise _lib.VVError("preconditioner %r is not provided (tensor_kinetic, tensor_potential or none)"
+                               % (preconditioner,))
         if laplace not in _lib.LAPLACE:
             raise _lib.VVError("laplace operator %r is not provided (cfd2/cfd4)" % (laplace,))
         self.grid = grid
         self.bracket, self.preconditioner, self.laplace = bracket, preconditioner, laplace
         self.charge = float(charge)
         self._v = np.ascontiguousarray(grid.v, dtype=np.float64)
-        eig = kinetic_eigenvalues(grid.nx)
+        # TensorProduct.compute_eigenvalues of the transformed direction: x for
+        # tensor_kinetic, v for tensor_potential
+        self.potential = _lib.PRECONDITIONERS[preconditioner] == 2
+        eig = kinetic_eigenvalues(grid.nv if self.potential else grid.nx)
         self._eig = np.ascontiguousarray(np.stack([eig.real, eig.imag], axis=1).reshape(-1))
         p = _lib.VVParams()
         p.nx, p.nv = grid.nx, grid.nv
@@ -177,6 +181,11 @@ class Context:
 
     def bracket_add(self, x, y, factor):
         self.call("vv_bracket_add", self.F(x), self.F(y), ctypes.c_double(factor))
+
+    def precond_update(self, phi):
+        """update_matrices: the potential preconditioner's bands from phi (an
+        x-vector per instance); a no-op for tensor_kinetic."""
+        self.call("vv_precond_update", self.X(phi))
 
     def precond(self, x, y):
         self.call("vv_precond_apply", self.F(x), self.F(y))
@@ -383,26 +392,36 @@ class TimeDerivative:
 
 
 class TensorProductPreconditioner:
-    """TensorProductPreconditioner.create (TensorProduct.pyx:66-86)."""
+    """TensorProductPreconditioner.create (TensorProduct.pyx:66-86): the
+    kinetic (FFT along x, tridiagonal along v) and the potential (FFT along v,
+    tridiagonal along x with phi-dependent bands) forms; the _scipy schemes
+    state the same operators.  ``phi`` is the potential the potential form's
+    bands follow (refreshed by update_matrices)."""
 
-    def __init__(self, context):
+    def __init__(self, context, phi=None):
         self.ctx = context
+        self.phi = phi
 
     @staticmethod
     def create(type, da1, daphi, grid, phi, context=None):
-        if type == 'tensor_kinetic':
-            ctx = context if context is not None else Context(grid)
-            return TensorProductPreconditioner(ctx)
-        if type in ('tensor_kinetic_scipy', 'tensor_potential', 'tensor_potential_scipy', 'tensor_diagonal'):
-            raise _lib.VVError("preconditioner %r is not provided by this library" % type)
+        if type in ('tensor_kinetic', 'tensor_kinetic_scipy', 'tensor_potential', 'tensor_potential_scipy'):
+            ctx = context if context is not None else Context(grid, preconditioner=type)
+            pc = TensorProductPreconditioner(ctx, phi)
+            pc.update_matrices()
+            return pc
+        if type == 'tensor_diagonal':
+            raise _lib.VVError("preconditioner %r is not provided (the reference has no implementation)" % type)
         return None
 
     def tensorProduct(self, X, Y):
-        """Y = irfft_x(T_k^-1 rfft_x X) (TensorProductKinetic.pyx:146-160)."""
+        """Y = P^-1 X (TensorProductKinetic.pyx:146-160, TensorProductPotential.pyx:157-173)."""
         self.ctx.precond(X, Y)
 
     def update_matrices(self):
-        pass       # the kinetic preconditioner depends on the grid only (KineticFast.pyx:244-245)
+        """The potential form's bands from phi (PotentialFast.pyx:205-264); the
+        kinetic form depends on the grid only (KineticFast.pyx:244-245)."""
+        if self.ctx.potential and self.phi is not None:
+            self.ctx.precond_update(self.phi)
 
 
 class PETScVlasovSolver:
@@ -431,7 +450,9 @@ class PETScVlasovSolver:
             laplace=config.get_laplace_operator(), charge=charge, restart=restart)
         self.poisson_bracket = PoissonBracket(config.get_poisson_bracket(), self.ctx)
         self.time_derivative = TimeDerivative(grid)
-        self.preconditioner = TensorProductPreconditioner(self.ctx) if pc else None
+        self.preconditioner = TensorProductPreconditioner(self.ctx, phi) if pc else None
+        if self.preconditioner is not None:
+            self.preconditioner.update_matrices()       # built from the initial phi (:104)
         self.Fp = self.ctx.new_field()
         self.Fh = self.ctx.new_field()
         self.Np = self.Up = self.Ep = self.Ap = None
@@ -443,7 +464,7 @@ class PETScVlasovSolver:
         self.Nh, self.Uh, self.Eh, self.Ah = Nh, Uh, Eh, Ah
 
     def update_history(self, F):
-        """Fh = F; the kinetic preconditioner needs no update (:135-139)."""
+        """Fh = F; the preconditioner's matrices follow phi (:135-139)."""
         self.Fh.copy_(F)
         if self.preconditioner is not None:
             self.preconditioner.update_matrices()
