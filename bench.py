@@ -187,6 +187,63 @@ def mixed_figures(w, args, run64, value64):
     return out
 
 
+def adjoint_figures(args):
+    """The adjoint workload (bump-on-tail 256x512 J4, dt = 0.05, K steps,
+    checkpoint every 20): loss = field energy of the final state, gradient in
+    the amplitudes a_m of cos(m k x) M(v), m = 1..16, a_m ~ N(0, 1e-3) from
+    default_rng(0), by the discrete adjoint behind torch.autograd.Function;
+    checked against central differences (h = 1e-6) along random directions of
+    a.  Tolerances are tightened (snes_atol 1e-14, ksp_rtol 1e-13) so that the
+    forward map is smooth to well below the difference step."""
+    import numpy as np
+    import torch
+    from vivlasov1d_amd.adjoint import FieldEnergy, ImplicitTimesteps, Stepper, perturbation_basis
+    from vivlasov1d_amd.core import Config, bump_on_tail, initial_distribution, normalise
+    nx, nv, k, K = 256, 512, 0.3, args.adjoint_steps
+    cfg = Config({"grid": {"nx": nx, "nv": nv, "L": 2 * np.pi / k, "vmax": 10.0, "ht": 0.05, "nt": K},
+                  "solver": {"poisson_bracket": "arakawaj4", "laplace_operator": "CFD4",
+                             "preconditioner_type": "tensor", "preconditioner_scheme": "tensor_kinetic",
+                             "petsc_ksp_rtol": 1e-13, "petsc_snes_atol": 1e-14, "petsc_snes_max_iter": 30},
+                  "initial_data": {"charge": -1.0}})
+    st = Stepper(cfg, ksp_rtol_adj=1e-13, checkpoint=args.adjoint_checkpoint)
+    base = torch.from_numpy(normalise(initial_distribution(st.grid, bump_on_tail(k, 0.0)), st.grid)).cuda()
+    basis = torch.from_numpy(perturbation_basis(st.grid, k, 16)).cuda()
+    rng = np.random.default_rng(0)
+    a0 = torch.from_numpy(rng.normal(0.0, 1e-3, 16)).cuda()
+
+    def energy(a):
+        return FieldEnergy.apply(ImplicitTimesteps.apply(base + (a[:, None, None] * basis).sum(0), st, K), st)
+
+    a = a0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E = energy(a)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    E.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    grad = a.grad.detach().clone()
+    its = list(st.adjoint_its)
+    h = 1e-6
+    errs = []
+    with torch.no_grad():
+        for _ in range(args.adjoint_fd_dirs):
+            d = torch.from_numpy(rng.standard_normal(16)).cuda()
+            d /= d.norm()
+            fd = (energy(a0 + h * d).item() - energy(a0 - h * d).item()) / (2 * h)
+            got = float((grad * d).sum())
+            errs.append(abs(fd - got) / abs(fd))
+    del st
+    torch.cuda.empty_cache()
+    return {"workload": "bump-on-tail k=0.3 256x512 J4 dt=0.05, %d steps, checkpoint %d, 16 modes a_m~N(0,1e-3)"
+                        % (K, args.adjoint_checkpoint),
+            "loss": E.item(), "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+            "adjoint_gmres_per_step": float(np.mean(its)) if its else None,
+            "grad_norm": float(grad.norm()), "fd_h": h, "fd_rel_err": errs,
+            "tolerances": {"snes_atol": 1e-14, "ksp_rtol": 1e-13, "adjoint_ksp_rtol": 1e-13}}
+
+
 def batched_figures(args):
     """Aggregate timesteps/s of the uneven-grid ensemble, batched and sequential."""
     import numpy as np
@@ -244,7 +301,7 @@ def main():
     ap.add_argument("--kernel", default="maxpy_norm",
                     help="kernels timed with HIP events in the timed FP64 run ('all', 'none' or a comma list of "
                          "%s); the roofline is the one with the largest total time.  Each timed dispatch costs "
-                         "~2%% of throughput, so the default times the dominant kernel only" % (KERNELS,))
+                         "about 2 percent of throughput, so the default times the dominant kernel only" % (KERNELS,))
     ap.add_argument("--bracket-kernels", default="jacobian,residual",
                     help="kernels timed in a second, untimed FP64 run for the bracket HBM GB/s ('none' skips)")
     ap.add_argument("--cpu-steps", type=int, default=20,
@@ -259,6 +316,10 @@ def main():
     ap.add_argument("--compare-steps", type=int, default=1000,
                     help="timestep at which the mixed and FP64 runs are compared (f, phi, invariant drift)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=20)
+    ap.add_argument("--adjoint-fd-dirs", type=int, default=2, help="finite-difference directions of the check")
     ap.add_argument("--no-sequential", action="store_true", help="skip the sequential run of the ensemble")
     ap.add_argument("--batched-instances", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=3)
@@ -336,6 +397,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_figures(args)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_figures(args)
 
     line = {
         "metric": "timesteps/s at 1024x2048 J4 (plain FP64)" if args.workload.startswith("twostream_1024")
@@ -360,6 +424,7 @@ def main():
         "bracket": bracket,
         "mixed": mixed,
         "batched": batched,
+        "adjoint": adjoint,
     }
     if cpu and "value" in cpu and cpu["value"]:
         line["gpu_over_cpu"] = round(value / cpu["value"], 2)
