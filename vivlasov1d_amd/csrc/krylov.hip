@@ -234,6 +234,8 @@ struct CgsArgs {
     double *part;
     int64_t pstride;
     GmresState *g;
+    int64_t noff;      // maxpy: offset of its norm partials in part (the dot partials stay at 0)
+    int nbd;           // maxpy: rows of dot partials (the mdot grid)
 };
 
 template <int EPT, int G, typename S>
@@ -317,28 +319,9 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
     }
 }
 
-// Column sums of the mdot partials: HH(:, col) = <w, v_v> and the VecMAXPY
-// coefficients lhh = -HH(:, col).
-template <int PV>
-__global__ __launch_bounds__(NT) void mdot_finish_kernel(const double *part, int nb, int64_t pstride, int col,
-                                                         GmresState *g)
-{
-    g += blockIdx.y;
-    part += blockIdx.y * pstride;
-    if (g->st.done) return;
-    __shared__ double s_red[NT];
-    __shared__ double s_h[PV];
-    final_sum_cols<PV>(part, nb, col + 1, s_h, s_red);
-    if (threadIdx.x < col + 1) {
-        const double h = s_h[threadIdx.x];
-        g->hh[col * (kMaxRestart + 1) + threadIdx.x] = h;
-        g->coef[threadIdx.x] = -h;
-    }
-}
-
 // CGS update  V_{it+1} <- w - sum_j h_j v_j  (VecMAXPY with lhh = -h) and
 // per-block partial sums of its squared norm (of the stored values).
-template <int EPT, typename S>
+template <int EPT, int PV, typename S>
 __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 {
     using V = typename V16<S>::type;
@@ -349,22 +332,36 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
     if (g->st.done) return;
     __shared__ double red[NW + 1];
     __shared__ double s_coef[kMaxRestart + 1], s_scale[kMaxRestart + 1];
+    __shared__ double s_red[NT], s_h[PV];
     const int nvec = it + 1;
-    if (threadIdx.x < nvec) {
-        s_coef[threadIdx.x] = g->coef[threadIdx.x];
-        s_scale[threadIdx.x] = g->scale[threadIdx.x];
-    }
-    __syncthreads();
     const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
     V *W = reinterpret_cast<V *>(const_cast<S *>(a.basis) + (size_t)nvec * a.vstride);
-    double w[EPT][L];
+    // w's loads are issued first, in flight while the dot partials are summed
+    V wl[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int64_t e = e0 + (int64_t)q * NT;
-        const V t = (e < nvv) ? W[e] : V{};
+        wl[q] = (e < nvv) ? W[e] : V{};
+    }
+    if (threadIdx.x < nvec) s_scale[threadIdx.x] = g->scale[threadIdx.x];
+    // HH(:, it) = <w, v_j>: every workgroup sums the mdot partials in the same
+    // order (so all hold the same values); workgroup 0 records the column
+    final_sum_cols<PV>(a.part, a.nbd, nvec, s_h, s_red);
+    if (threadIdx.x < nvec) {
+        const double h = s_h[threadIdx.x];
+        s_coef[threadIdx.x] = -h;            // VecMAXPY coefficients lhh = -HH(:, it)
+        if (blockIdx.x == 0) {
+            g->hh[it * (kMaxRestart + 1) + threadIdx.x] = h;
+            g->coef[threadIdx.x] = -h;
+        }
+    }
+    __syncthreads();
+    double w[EPT][L];
 #pragma unroll
-        for (int l = 0; l < L; ++l) w[q][l] = vget(t, l);
+    for (int q = 0; q < EPT; ++q) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) w[q][l] = vget(wl[q], l);
     }
     int j = 0;
     for (; j + 2 <= nvec; j += 2) {
@@ -407,7 +404,7 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
         }
     }
     const double s = block_sum_all(acc, red);
-    if (threadIdx.x == 0) a.part[blockIdx.x] = s;
+    if (threadIdx.x == 0) a.part[a.noff + blockIdx.x] = s;
 }
 
 // VecNormalize of V_{it+1} from the maxpy partials, then
@@ -666,6 +663,10 @@ int cgs_blocks_v(const vv_ctx *c, int64_t nvv)
     return (int)((nvv + (int64_t)NT * ept - 1) / ((int64_t)NT * ept));
 }
 
+// the maxpy norm partials sit after the dot partials (rows x kMaxRestart) in
+// each instance's part_stride = rows x (kMaxRestart + 2) doubles
+int64_t norm_partial_offset(const vv_ctx *c) { return c->part_stride / (kMaxRestart + 2) * kMaxRestart; }
+
 }  // namespace
 
 // partial-sum rows the CGS kernels can need (one vector per thread)
@@ -693,13 +694,15 @@ namespace {
 template <int E, typename S>
 void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 {
-    CgsArgs<S> a{basis, c->N, c->N * c->B, c->partial, c->part_stride, c->gs};
-    const dim3 grid(cgs_blocks_v(c, c->N / V16<S>::L), c->B), block(NT);
+    const int nb = cgs_blocks_v(c, c->N / V16<S>::L);
+    CgsArgs<S> a{basis, c->N, c->N * c->B, c->partial, c->part_stride, c->gs, norm_partial_offset(c), nb};
+    const dim3 grid(nb, c->B), block(NT);
     if (which == 0) {
         if (c->p.restart < 32) launch_timed(c, mdot_kernel<E, 32, S>, grid, block, 0, a, arg);
         else launch_timed(c, mdot_kernel<E, 64, S>, grid, block, 0, a, arg);
     } else if (which == 1) {
-        launch_timed(c, maxpy_norm_kernel<E, S>, grid, block, 0, a, arg);
+        if (c->p.restart < 32) launch_timed(c, maxpy_norm_kernel<E, 32, S>, grid, block, 0, a, arg);
+        else launch_timed(c, maxpy_norm_kernel<E, 64, S>, grid, block, 0, a, arg);
     } else {
         launch_timed(c, update_kernel<E, S>, grid, block, 0, x, a);
     }
@@ -717,19 +720,12 @@ hipError_t cgs_dispatch(vv_ctx *c, const S *basis, int which, int arg, double *x
     return hipGetLastError();
 }
 
+// CGS dots of w with V_0..V_col: per-workgroup partials only; their column
+// sums are formed by every workgroup of the following maxpy_norm launch
 template <typename S>
 hipError_t mdot_t(vv_ctx *c, const S *basis, int col)
 {
-    hipError_t e = cgs_dispatch(c, basis, 0, col, nullptr);
-    if (e != hipSuccess) return e;
-    const int nb = cgs_blocks_v(c, c->N / V16<S>::L);
-    if (c->p.restart < 32)
-        hipLaunchKernelGGL((mdot_finish_kernel<32>), dim3(1, c->B), dim3(NT), 0, c->stream, c->partial, nb,
-                           c->part_stride, col, c->gs);
-    else
-        hipLaunchKernelGGL((mdot_finish_kernel<64>), dim3(1, c->B), dim3(NT), 0, c->stream, c->partial, nb,
-                           c->part_stride, col, c->gs);
-    return hipGetLastError();
+    return cgs_dispatch(c, basis, 0, col, nullptr);
 }
 
 template <typename S>
@@ -737,7 +733,7 @@ hipError_t maxpy_norm_t(vv_ctx *c, const S *basis, int it)
 {
     hipError_t e = cgs_dispatch(c, basis, 1, it, nullptr);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(gmres_update_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial,
+    hipLaunchKernelGGL(gmres_update_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial + norm_partial_offset(c),
                        cgs_blocks_v(c, c->N / V16<S>::L), c->part_stride, it, c->gs);
     return hipGetLastError();
 }
