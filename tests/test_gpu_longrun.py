@@ -58,3 +58,33 @@ def test_many_steps_match_c_oracle(kind, steps, cuda):
     for k in ('mass', 'l2'):
         assert abs(inv[k] - ref[k]) <= 1e-12 * abs(ref[k]), k
     crun.close()
+
+
+def test_linear_landau_100_steps_match_c_oracle(cuda):
+    """The CPU-runnable workload at full length: linear Landau damping, x in
+    [0, 4 pi) nx = 64, v in [-10, 10] nv = 128, f0 = (1 + 0.01 cos 0.5x)
+    exp(-v^2/2)/sqrt(2 pi), J4, dt = 0.1, 100 steps: f within 1e-12 of the C
+    restatement after every step, the same Picard counts, and the relative
+    drift of mass and L2 over the run within 1e-12 on both sides alike."""
+    from oracle import cbaseline
+    from vivlasov1d_amd.core import Config
+    from vivlasov1d_amd.runner import Runscript
+    prof = lambda x, v: (1 + 0.01 * np.cos(0.5 * x)) * np.exp(-0.5 * v ** 2) / math.sqrt(2 * math.pi)
+    g = O.Grid(64, 128, 4 * math.pi, 10.0, ht=0.1)
+    cfg = Config({'grid': {'nx': 64, 'nv': 128, 'L': 4 * math.pi, 'vmax': 10.0, 'ht': 0.1, 'nt': 100},
+                  'solver': {'poisson_bracket': 'arakawaj4', 'laplace_operator': 'CFD4',
+                             'preconditioner_type': 'tensor', 'preconditioner_scheme': 'tensor_kinetic',
+                             'petsc_ksp_rtol': 1e-10, 'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
+                  'initial_data': {'charge': -1.0}})
+    run = Runscript(cfg, profile=prof)
+    crun = cbaseline.CRun(g, O.normalise(O.initial_distribution(g, prof), g).T, bracket='arakawaj4')
+    inv0 = run.invariants()
+    for s in range(100):
+        ec, eg = crun.step(), run.step()
+        assert ec['picard'] == len(eg['picard']), s
+        fc = crun.f()
+        assert np.abs(run.fc.cpu().numpy() - fc).max() <= 1e-12 * np.abs(fc).max(), s
+    inv = run.invariants()
+    for k in ('mass', 'l2'):
+        assert abs(inv[k] - inv0[k]) <= 1e-12 * abs(inv0[k]), k
+    crun.close()
