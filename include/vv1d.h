@@ -198,6 +198,61 @@ int vv_density(vv_ctx *ctx, const double *f, double *n);
  * timestep's path. */
 int vv_velocity_moments(vv_ctx *ctx, const double *f, double *u, double *e);
 
+/* Dissipation term of the solver (PETScVlasovSolver.pyx:39-42, 101-102;
+ * chosen by config.py:148-163: dissipation_type 'collisions' with
+ * collision_operator collT / collE, or 'double_bracket' with bracket_operator
+ * ffh / fhh; nothing when coll_freq == 0).  Once set, every residual
+ * (vv_residual, vv_function, vv_function_norm, vv_snes_ksponly) adds the
+ * term function_solver adds (PETScNLVlasovMP.pyx:56-58) and every Jacobian
+ * apply (vv_jacobian_apply, vv_jacobian, the GMRES operator) the term
+ * jacobian_solver adds (:38-39).  The double bracket's Jacobian uses the
+ * average .5 fh + .5 f of the last residual evaluation, as the reference's
+ * jacobian_solver uses self.Fave.  Single-instance FP64 contexts only: the
+ * mixed-precision, adjoint and batched entry points return
+ * VV_ERR_UNSUPPORTED while a dissipation term is set. */
+#define VV_DISSIPATION_NONE  0
+#define VV_DISSIPATION_COLLT 1   /* CollisionOperatorT (CollisionOperator.pyx:68-156) */
+#define VV_DISSIPATION_COLLE 2   /* CollisionOperatorE (CollisionOperator.pyx:160-250) */
+#define VV_DISSIPATION_FFH   3   /* DoubleBracketFFH (DoubleBracket.pyx:67-82) */
+#define VV_DISSIPATION_FHH   4   /* DoubleBracketFHH (DoubleBracket.pyx:86-97) */
+
+typedef struct {
+    int32_t kind;          /* VV_DISSIPATION_*                                 */
+    int32_t pad;
+    double  coll_freq;     /* nu (solver.coll_freq)                            */
+    double  coll_diff;     /* weight of the diffusion term (default 1)         */
+    double  coll_drag;     /* weight of the drag term (default 1)              */
+} vv_dissipation;
+
+int vv_set_dissipation(vv_ctx *ctx, const vv_dissipation *d);
+
+/* PETScVlasovSolverBase.set_moments (PETScVlasovSolver.pyx:123-132): device
+ * x-vectors (nx each) of the current (p) and history (h) density N, velocity
+ * density U, energy density E and collision factor A.  The context keeps the
+ * pointers, not copies, as the reference keeps the Vec objects: the caller
+ * refreshes their contents (vv_velocity_moments, vv_collision_factor) and
+ * keeps them alive.  E may be NULL (no operator reads it). */
+int vv_set_moments(vv_ctx *ctx, const double *np, const double *up, const double *ep,
+                   const double *ap, const double *nh, const double *uh,
+                   const double *eh, const double *ah);
+
+/* Toolbox.compute_collision_factor (Toolbox.pyx:153-165):
+ * a_i = n_i n_i / (n_i e_i - u_i u_i). */
+int vv_collision_factor(vv_ctx *ctx, const double *n, const double *u, const double *e,
+                        double *a);
+
+/* CollisionOperator{T,E}.function (CollisionOperator.pyx:72-110, 164-202) of
+ * the context's collision operator: y += C(f) on rows 2..nv-3 with
+ * u = U/n.  e is unused (signature parity) and may be NULL. */
+int vv_collision_add(vv_ctx *ctx, const double *f, double *y, const double *n,
+                     const double *u, const double *e, const double *a, double factor);
+
+/* DoubleBracket{FFH,FHH}.function / .jacobian (DoubleBracket.pyx:69-97) of the
+ * context's double bracket with H = H_ave: y += D(fave) resp. y += D'(fave) u. */
+int vv_double_bracket_function(vv_ctx *ctx, const double *fave, double *y, double factor);
+int vv_double_bracket_jacobian(vv_ctx *ctx, const double *u, const double *fave, double *y,
+                               double factor);
+
 /* PETScPoissonSolver.formRHS + the Poisson KSPSolve
  * (PETScPoissonSolver.pyx:58-74; run_base.py:689-690): solves A phi = b,
  * b_i = -(n_i - mean n) q, exactly on the periodic grid; phi is returned

@@ -29,6 +29,17 @@ How each function is pinned (the tests in ``tests/test_oracle_*.py``):
 * ``density`` / invariants -- closed-form Maxwellian moments.
 * ``gmres`` -- dense ``numpy.linalg.solve`` on small systems, restart
   equivalence, monotone residual history.
+* ``collision_add`` / ``collision_factor`` -- (i) against ``collision_stencil``,
+  a transcription of the reference's assembled ``jacobian()`` of the same
+  operators; (ii) exact discrete conservation of mass, momentum and energy
+  when the moments are those of f and f vanishes near the v-boundaries;
+  (iii) a drifting Maxwellian lies in the null space to O(hv^2) (halving hv
+  quarters the remainder); the diffusion part alone has sum f C(f) > 0;
+  a = 1/T for a Maxwellian.
+* ``double_bracket_*`` -- the Arakawa trilinear antisymmetry gives
+  sum H D_ffh = nu sum {F,H}^2, sum F D_ffh = 0, sum F D_fhh = nu sum {F,H}^2,
+  sum H D_fhh = 0; the Jacobian terms equal central differences of the
+  function terms (quadratic / linear in f, so exact up to rounding).
 * ``OracleRun`` (Picard driver) -- each stop / revert / abort branch forced
   from the host; invariant drift bounds.
 """
@@ -187,6 +198,150 @@ def jacobian(kind, u, H, grid):
     """jacobian_solver (PETScNLVlasovMP.pyx:33-40): Y = u/dt + .5 B(u, H)."""
     y = grid.ht_inv * u
     return bracket_add(kind, u, H, y, 0.5, grid)
+
+
+# --------------------------------------------------------------------------
+# dissipation                 components/CollisionOperator.pyx:68-250
+#                             components/DoubleBracket.pyx:67-97
+#                             toolbox/Toolbox.pyx:153-165
+# --------------------------------------------------------------------------
+def collision_factor(n, u, e):
+    """a_i = n_i n_i / (n_i e_i - u_i u_i)  (Toolbox.pyx:153-165); u, e are the
+    raw first and second velocity moments."""
+    return n * n / (n * e - u * u)
+
+
+def collision_add(kind, f, y, n, U, a, factor, grid, coll_freq, coll_diff=1., coll_drag=1.):
+    """``y += C(f)`` on rows 2..nv-3, in place (CollisionOperatorT.function,
+    CollisionOperator.pyx:72-110; CollisionOperatorE.function, :164-202).
+
+    drag_fac = -factor*nu*drag*hv_inv*0.5, diff_fac = -factor*nu*diff*hv2_inv
+    (left to right), u = U/n, and with f+, f0, f- the rows j+1, j, j-1:
+      collt: drag = ((v+ - u) f+ - (v- - u) f-) * a;  diff = (f+ - 2 f0) + f-
+      colle: drag = (v+ - u) f+ - (v- - u) f-;        diff = ((f+ - 2 f0) + f-) / a
+      y += drag_fac*drag + diff_fac*diff
+    Nothing is added when nu == 0 or factor == 0 (:87).
+    """
+    if coll_freq == 0. or factor == 0.:
+        return y
+    drag_fac = -factor * coll_freq * coll_drag * grid.hv_inv * 0.5
+    diff_fac = -factor * coll_freq * coll_diff * grid.hv2_inv
+    s, nv = STENCIL, grid.nv
+    u = (U / n)[:, None]
+    aa = a[:, None]
+    vp = grid.v[s + 1:nv - s + 1][None, :]
+    vm = grid.v[s - 1:nv - s - 1][None, :]
+    fp, f0, fm = f[:, s + 1:nv - s + 1], f[:, s:nv - s], f[:, s - 1:nv - s - 1]
+    if kind == 'collt':
+        drag = ((vp - u) * fp - (vm - u) * fm) * aa
+        diff = (fp - 2. * f0) + fm
+    elif kind == 'colle':
+        drag = (vp - u) * fp - (vm - u) * fm
+        diff = ((fp - 2. * f0) + fm) / aa
+    else:
+        raise ValueError("unknown collision operator %r" % kind)
+    y[:, s:nv - s] += drag_fac * drag + diff_fac * diff
+    return y
+
+
+def collision_stencil(kind, n, U, a, factor, grid, coll_freq, coll_diff=1., coll_drag=1.):
+    """Second statement of the collision operators: the (j-1, j, j+1) entries
+    the reference's assembled ``jacobian()`` adds for row j
+    (CollisionOperatorT.jacobian :147-153; CollisionOperatorE.jacobian
+    :240-246), as three (nx, nv) arrays (zero outside rows 2..nv-3)."""
+    drag_fac = -factor * coll_freq * coll_drag * grid.hv_inv * 0.5
+    diff_fac = -factor * coll_freq * coll_diff * grid.hv2_inv
+    nx, nv, s = grid.nx, grid.nv, STENCIL
+    lo, mid, hi = np.zeros((nx, nv)), np.zeros((nx, nv)), np.zeros((nx, nv))
+    u = U / n
+    for j in range(s, nv - s):
+        if kind == 'collt':
+            lo[:, j] = -drag_fac * (grid.v[j - 1] - u) * a + diff_fac
+            mid[:, j] = -2. * diff_fac
+            hi[:, j] = +drag_fac * (grid.v[j + 1] - u) * a + diff_fac
+        else:
+            lo[:, j] = -drag_fac * (grid.v[j - 1] - u) + diff_fac / a
+            mid[:, j] = -2. * diff_fac / a
+            hi[:, j] = +drag_fac * (grid.v[j + 1] - u) + diff_fac / a
+    return lo, mid, hi
+
+
+def double_bracket_function(kind, bkind, fave, H, y, factor, grid, coll_freq):
+    """``y += D(Fave)`` in place, with b = 0 + {Fave, H} (factor 1.0):
+      ffh: y += (-nu*factor) {Fave, b}   (DoubleBracketFFH.function, DoubleBracket.pyx:79-82)
+      fhh: y += (-nu*factor) {b, H}      (DoubleBracketFHH.function, :94-97)
+    ``bkind`` is the Poisson bracket (J1/J2/J4) the solver was built with."""
+    b = bracket(bkind, fave, H, grid)
+    if kind == 'ffh':
+        return bracket_add(bkind, fave, b, y, -coll_freq * factor, grid)
+    if kind == 'fhh':
+        return bracket_add(bkind, b, H, y, -coll_freq * factor, grid)
+    raise ValueError("unknown double bracket %r" % kind)
+
+
+def double_bracket_jacobian(kind, bkind, u, fave, H, y, factor, grid, coll_freq):
+    """``y += D'(Fave) u`` in place:
+      ffh: b = {Fave, H}; y += (-nu f) {u, b}; b = {u, H}; y += (-nu f) {Fave, b}
+           (DoubleBracketFFH.jacobian, DoubleBracket.pyx:69-76)
+      fhh: b = {u, H}; y += (-nu f) {b, H}   (DoubleBracketFHH.jacobian, :88-91)"""
+    if kind == 'ffh':
+        b = bracket(bkind, fave, H, grid)
+        bracket_add(bkind, u, b, y, -coll_freq * factor, grid)
+        b = bracket(bkind, u, H, grid)
+        return bracket_add(bkind, fave, b, y, -coll_freq * factor, grid)
+    if kind == 'fhh':
+        b = bracket(bkind, u, H, grid)
+        return bracket_add(bkind, b, H, y, -coll_freq * factor, grid)
+    raise ValueError("unknown double bracket %r" % kind)
+
+
+class Dissipation:
+    """The dissipation term of the solver (PETScVlasovSolver.pyx:101-102,
+    chosen by config.py:148-163): a collision operator ('collt' | 'colle',
+    needing the moments set by set_moments, PETScVlasovSolver.pyx:123-132) or
+    a double bracket ('ffh' | 'fhh').  ``mom`` holds the x-vectors
+    np, up, ap (current) and nh, uh, ah (history): density, raw velocity
+    density and collision factor."""
+
+    def __init__(self, kind, coll_freq, coll_diff=1., coll_drag=1.):
+        if kind not in ('collt', 'colle', 'ffh', 'fhh'):
+            raise ValueError("unknown dissipation %r" % kind)
+        self.kind = kind
+        self.coll_freq, self.coll_diff, self.coll_drag = float(coll_freq), float(coll_diff), float(coll_drag)
+
+    @property
+    def collisions(self):
+        return self.kind in ('collt', 'colle')
+
+    def _coll(self, f, y, n, U, a, factor, grid):
+        return collision_add(self.kind, f, y, n, U, a, factor, grid, self.coll_freq, self.coll_diff, self.coll_drag)
+
+    def function(self, bkind, f, fh, fave, H, y, grid, mom):
+        """The terms function_solver adds after B(Fave, H) (PETScNLVlasovMP.pyx:56-58)."""
+        if self.collisions:
+            self._coll(f, y, mom['np'], mom['up'], mom['ap'], 0.5, grid)
+            self._coll(fh, y, mom['nh'], mom['uh'], mom['ah'], 0.5, grid)
+            return y
+        return double_bracket_function(self.kind, bkind, fave, H, y, 1.0, grid, self.coll_freq)
+
+    def jacobian(self, bkind, u, fave, H, y, grid, mom):
+        """The terms jacobian_solver adds after .5 B(u, H) (PETScNLVlasovMP.pyx:38-39)."""
+        if self.collisions:
+            return self._coll(u, y, mom['np'], mom['up'], mom['ap'], 0.5, grid)
+        return double_bracket_jacobian(self.kind, bkind, u, fave, H, y, 0.5, grid, self.coll_freq)
+
+
+def residual_diss(kind, f, fh, H, grid, diss, mom=None):
+    """function_solver with its dissipation term (PETScNLVlasovMP.pyx:43-58)."""
+    y = residual(kind, f, fh, H, grid)
+    return diss.function(kind, f, fh, 0.5 * fh + 0.5 * f, H, y, grid, mom)
+
+
+def jacobian_diss(kind, u, fave, H, grid, diss, mom=None):
+    """jacobian_solver with its dissipation term (PETScNLVlasovMP.pyx:33-40);
+    ``fave`` is the average the last function_solver call formed."""
+    y = jacobian(kind, u, H, grid)
+    return diss.jacobian(kind, u, fave, H, y, grid, mom)
 
 
 # --------------------------------------------------------------------------
@@ -660,9 +815,10 @@ class OracleRun:
     def __init__(self, grid, f0, bracket='arakawaj4', laplace='cfd4', charge=-1.0,
                  mass=1.0, ksp_rtol=1e-10, ksp_atol=1e-50, ksp_max_it=1000,
                  snes_atol=1e-10, snes_max_iter=20, restart=30, precond_variant='lapack',
-                 preconditioner='tensor_kinetic'):
+                 preconditioner='tensor_kinetic', dissipation=None):
         self.grid = grid
         self.kind = bracket
+        self.diss = dissipation
         self.laplace = laplace
         self.charge = charge
         self.ksp = dict(rtol=ksp_rtol, atol=ksp_atol, max_it=ksp_max_it, restart=restart)
@@ -685,8 +841,13 @@ class OracleRun:
 
     # --- moments / Poisson (run_base.py:640-704, 741-742)
     def calculate_moments(self, f=None):
+        """n, u, e, a of f (run_base.py:640-651), then Poisson -> phi, h1."""
         f = self.fc if f is None else f
         self.n = density(f, self.grid)
+        if self.diss is not None and self.diss.collisions:
+            self.u = velocity_density(f, self.grid)
+            self.e = energy_density(f, self.grid)
+            self.a = collision_factor(self.n, self.u, self.e)
         b = poisson_rhs(self.n, self.charge, self.grid.nx)
         self.phi = poisson_solve(b, self.grid.hx, self.laplace)
         self.h1c = potential_to_hamiltonian(self.phi, self.grid.nx)
@@ -694,15 +855,29 @@ class OracleRun:
     def update_previous(self):
         self.H = h_average(self.h0, self.h1c, self.h1h, self.h2c, self.h2h)
 
+    def _mom(self):
+        if not self.diss.collisions:
+            return None
+        return dict(np=self.n, up=self.u, ap=self.a, nh=self.nh, uh=self.uh, ah=self.ah)
+
+    def _residual(self, f):
+        if self.diss is None:
+            return residual(self.kind, f, self.fh, self.H, self.grid)
+        # function_solver also sets the average the Jacobian's double bracket uses
+        self.fave = 0.5 * self.fh + 0.5 * f
+        return residual_diss(self.kind, f, self.fh, self.H, self.grid, self.diss, self._mom())
+
     def function(self, f):
-        return self.pc.apply(residual(self.kind, f, self.fh, self.H, self.grid))
+        return self.pc.apply(self._residual(f))
 
     def jacobian_apply(self, u):
-        return self.pc.apply(jacobian(self.kind, u, self.H, self.grid))
+        if self.diss is None:
+            return self.pc.apply(jacobian(self.kind, u, self.H, self.grid))
+        return self.pc.apply(jacobian_diss(self.kind, u, self.fave, self.H, self.grid, self.diss, self._mom()))
 
     def calculate_residual(self):
         """||F(fc)||_2 + ||A phi + (n - n_mean) q||_2  (run_base.py:631-637)."""
-        fn = np.linalg.norm(residual(self.kind, self.fc, self.fh, self.H, self.grid))
+        fn = np.linalg.norm(self._residual(self.fc))
         pr = poisson_function_mult(self.phi, self.n, self.charge, self.grid.hx2_inv, self.laplace)
         return fn + np.linalg.norm(pr)
 
@@ -721,6 +896,8 @@ class OracleRun:
         self.fh = self.fc.copy()
         self.h1h = self.h1c.copy()
         self.h2h = self.h2c.copy()
+        if self.diss is not None and self.diss.collisions:
+            self.nh, self.uh, self.ah = self.n.copy(), self.u.copy(), self.a.copy()
         # update_history -> preconditioner.update_matrices (PETScVlasovSolver.pyx:135-139)
         if isinstance(self.pc, PotentialPreconditioner):
             self.pc.update(self.phi)

@@ -94,3 +94,29 @@ def test_ksp_mixed_refinement_arguments():
         k.setPrecision('mixed', refine='sometimes')
     with pytest.raises(_lib.VVError):
         k.setPrecision('half')
+
+
+def test_dissipation_layout_and_kinds():
+    """vv_dissipation: int32 kind, int32 pad, double coll_freq, coll_diff,
+    coll_drag; the kind codes match include/vv1d.h and the reference's
+    create() names (CollisionOperator.pyx:52-57, DoubleBracket.pyx:51-56)."""
+    from vivlasov1d_amd import _lib
+    D = _lib.Dissipation
+    assert ctypes.sizeof(D) == 32
+    assert (D.kind.offset, D.coll_freq.offset, D.coll_diff.offset, D.coll_drag.offset) == (0, 8, 16, 24)
+    text = open(HEADER).read()
+    for name, code in (("COLLT", 1), ("COLLE", 2), ("FFH", 3), ("FHH", 4)):
+        assert "#define VV_DISSIPATION_%s %d" % (name, code) in " ".join(text.split())
+        assert _lib.DISSIPATION[name.lower()] == code
+
+
+def test_config_selects_dissipation():
+    """config.py:148-163: collisions only with dissipation_type 'collisions'
+    and coll_freq != 0; double brackets only with 'double_bracket'."""
+    from vivlasov1d_amd.core import Config
+    cfg = Config({'solver': {'dissipation_type': 'collisions', 'collision_operator': 'collT', 'coll_freq': 1e-3}})
+    assert cfg.get_collision_operator() == 'collt' and cfg.get_double_bracket() == ''
+    cfg = Config({'solver': {'dissipation_type': 'double_bracket', 'bracket_operator': 'FHH', 'coll_freq': 1e-3}})
+    assert cfg.get_double_bracket() == 'fhh' and cfg.get_collision_operator() == ''
+    cfg = Config({'solver': {'dissipation_type': 'collisions', 'collision_operator': 'collE', 'coll_freq': 0.0}})
+    assert cfg.get_collision_operator() == '' and cfg.is_dissipation_none()

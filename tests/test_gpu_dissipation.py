@@ -1,0 +1,213 @@
+"""GPU parity of the dissipation terms (collision operators, double brackets)
+against the oracle.
+
+Bars:
+  * collision factor, collision operators, double brackets, and the residual
+    and Jacobian with a dissipation term: bit-identical to the oracle (the
+    kernels keep the reference's unfused expression order);
+  * GMRES on P^-1 J with the term: iteration count within 1 of the oracle's,
+    solution within 1e-8 relative;
+  * Picard timesteps of a collisional / double-bracket Landau run: Picard
+    counts equal, f within 1e-9 relative after each step.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import vlasov_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BRACKETS = ['arakawaj1', 'arakawaj2', 'arakawaj4']
+GRIDS = [(16, 32), (64, 128), (128, 50)]
+
+
+def _nat(a):
+    return np.ascontiguousarray(a.T)
+
+
+def _setup(nx, nv, kind, seed, cuda, vmax=6.0):
+    import torch
+    from vivlasov1d_amd.core import Grid as PGrid
+    from vivlasov1d_amd.solvers import Context
+    rng = np.random.default_rng(seed)
+    g = O.Grid(nx, nv, 4 * np.pi, vmax, ht=0.05)
+    pg = PGrid(nx, nv, 4 * np.pi, vmax, ht=0.05)
+    assert np.array_equal(pg.v, g.v) and pg.hv == g.hv and pg.hv2_inv == g.hv2_inv
+    ctx = Context(pg, bracket=kind, preconditioner='tensor_kinetic', laplace='cfd4', charge=-1.0)
+    h0 = O.kinetic_hamiltonian(g)
+    h1p, h1h, h2p, h2h = (0.1 * rng.standard_normal(nx) for _ in range(4))
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    ctx.set_hamiltonian(T(h0), T(h1p), T(h1h), T(h2p), T(h2h))
+    return g, ctx, O.h_average(h0, h1p, h1h, h2p, h2h), rng, T
+
+
+def _moments(f, g):
+    n, u, e = O.density(f, g), O.velocity_density(f, g), O.energy_density(f, g)
+    return n, u, e, O.collision_factor(n, u, e)
+
+
+def test_collision_factor_bitwise(cuda):
+    g, ctx, H, rng, T = _setup(64, 128, 'arakawaj4', 1, cuda)
+    f = 0.5 + rng.random((g.nx, g.nv))
+    n, u, e, a = _moments(f, g)
+    ad = ctx.new_x()
+    ctx.collision_factor(T(n), T(u), T(e), ad)
+    assert np.array_equal(ad.cpu().numpy(), a)
+
+
+@pytest.mark.parametrize("nx,nv", GRIDS)
+@pytest.mark.parametrize("kind", ['collt', 'colle'])
+def test_collision_operator_bitwise(nx, nv, kind, cuda):
+    print("case", nx, nv, kind, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, 'arakawaj4', 2 + nx, cuda)
+    nu, diff, drag = 0.37, 1.3, 0.6
+    ctx.set_dissipation(kind, nu, diff, drag)
+    f = rng.random((nx, nv))
+    n, U, e, a = _moments(f, g)
+    y0 = rng.standard_normal((nx, nv))
+    yd = T(_nat(y0))
+    ctx.collision_add(T(_nat(f)), yd, T(n), T(U), T(e), T(a), 0.5)
+    ref = O.collision_add(kind, f, y0.copy(), n, U, a, 0.5, g, nu, diff, drag)
+    assert np.array_equal(yd.cpu().numpy().T, ref)
+
+
+@pytest.mark.parametrize("bkind", BRACKETS)
+@pytest.mark.parametrize("kind", ['ffh', 'fhh'])
+def test_double_bracket_bitwise(kind, bkind, cuda):
+    print("case", kind, bkind, flush=True)
+    g, ctx, H, rng, T = _setup(64, 128, bkind, 3, cuda)
+    nu = 0.25
+    ctx.set_dissipation(kind, nu)
+    fave, u = rng.standard_normal((2, 64, 128))
+    y0 = rng.standard_normal((64, 128))
+    yd = T(_nat(y0))
+    ctx.double_bracket_function(T(_nat(fave)), yd, 1.0)
+    ref = O.double_bracket_function(kind, bkind, fave, H, y0.copy(), 1.0, g, nu)
+    assert np.array_equal(yd.cpu().numpy().T, ref)
+    yd = T(_nat(y0))
+    ctx.double_bracket_jacobian(T(_nat(u)), T(_nat(fave)), yd, 0.5)
+    ref = O.double_bracket_jacobian(kind, bkind, u, fave, H, y0.copy(), 0.5, g, nu)
+    assert np.array_equal(yd.cpu().numpy().T, ref)
+
+
+def _diss_state(g, ctx, rng, T, dkind):
+    nx, nv = g.nx, g.nv
+    f = rng.random((nx, nv))
+    fh = f + 0.01 * rng.standard_normal((nx, nv))
+    n, U, e, a = _moments(f, g)
+    nh, Uh, eh, ah = _moments(fh, g)
+    mom = dict(np=n, up=U, ap=a, nh=nh, uh=Uh, ah=ah)
+    dev = [T(x) for x in (n, U, e, a, nh, Uh, eh, ah)]
+    ctx.set_moments(*dev)
+    diss = O.Dissipation(dkind, 0.2, 1.1, 0.9)
+    ctx.set_dissipation(dkind, 0.2, 1.1, 0.9)
+    return f, fh, mom, diss, dev
+
+
+@pytest.mark.parametrize("bkind", BRACKETS)
+@pytest.mark.parametrize("dkind", ['collt', 'colle', 'ffh', 'fhh'])
+def test_residual_and_jacobian_with_dissipation_bitwise(dkind, bkind, cuda):
+    print("case", dkind, bkind, flush=True)
+    g, ctx, H, rng, T = _setup(64, 128, bkind, 4, cuda)
+    f, fh, mom, diss, dev = _diss_state(g, ctx, rng, T, dkind)
+    u = rng.standard_normal((64, 128))
+    y = ctx.new_field()
+    ctx.residual(T(_nat(f)), T(_nat(fh)), y)
+    ref = O.residual_diss(bkind, f, fh, H, g, diss, mom)
+    assert np.array_equal(y.cpu().numpy().T, ref), np.abs(y.cpu().numpy().T - ref).max()
+    ctx.jacobian_apply(T(_nat(u)), y)
+    ref = O.jacobian_diss(bkind, u, 0.5 * fh + 0.5 * f, H, g, diss, mom)
+    assert np.array_equal(y.cpu().numpy().T, ref), np.abs(y.cpu().numpy().T - ref).max()
+    # the unpreconditioned norm includes the term
+    out = ctx.function_norm(T(_nat(f)), T(_nat(fh)), y)
+    assert abs(out - np.linalg.norm(O.residual_diss(bkind, f, fh, H, g, diss, mom))) <= 1e-13 * out
+
+
+@pytest.mark.parametrize("dkind", ['collt', 'ffh', 'fhh'])
+def test_gmres_with_dissipation_matches_oracle(dkind, cuda):
+    print("case", dkind, flush=True)
+    g, ctx, H, rng, T = _setup(64, 128, 'arakawaj4', 5, cuda)
+    f, fh, mom, diss, dev = _diss_state(g, ctx, rng, T, dkind)
+    pc = O.Preconditioner(g, 'lapack')
+    fave = 0.5 * fh + 0.5 * f
+    b = pc.apply(O.residual_diss('arakawaj4', f, fh, H, g, diss, mom))
+    A = lambda v: pc.apply(O.jacobian_diss('arakawaj4', v, fave, H, g, diss, mom))
+    x_ref, its_ref, reason_ref = O.gmres(A, b, rtol=1e-10, atol=1e-50, max_it=1000, restart=30)
+    # the residual evaluation sets the average the double bracket's Jacobian uses
+    y = ctx.new_field()
+    ctx.residual(T(_nat(f)), T(_nat(fh)), y)
+    xd = ctx.new_field()
+    res = ctx.ksp_solve(T(_nat(b)), xd, 1e-10, 1e-50, 1e5, 1000)
+    assert res.reason == reason_ref, (res.reason, reason_ref)
+    assert abs(res.its - its_ref) <= 1, (res.its, its_ref)
+    x = xd.cpu().numpy().T
+    assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+
+
+@pytest.mark.parametrize("dkind,nu", [('collt', 0.05), ('colle', 0.05), ('ffh', 0.01), ('fhh', 0.01)])
+def test_picard_with_dissipation_matches_oracle(dkind, nu, cuda):
+    """Three implicit timesteps of a Landau run with the dissipation term the
+    configuration selects (config.py:148-163), against the oracle's run."""
+    from vivlasov1d_amd.core import Config
+    from vivlasov1d_amd.runner import Runscript
+    print("case", dkind, nu, flush=True)
+    nx, nv, L, vmax, ht = 32, 64, 4 * np.pi, 8.0, 0.05
+    prof = lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * np.exp(-0.5 * (v - 0.5) ** 2) / math.sqrt(2 * np.pi)
+    g = O.Grid(nx, nv, L, vmax, ht=ht)
+    orun = O.OracleRun(g, O.initial_distribution(g, prof), ksp_rtol=1e-10, snes_atol=1e-10,
+                       dissipation=O.Dissipation(dkind, nu, 1.0, 1.0))
+    coll = dkind in ('collt', 'colle')
+    cfg = Config({'grid': {'nx': nx, 'nv': nv, 'L': L, 'vmax': vmax, 'ht': ht, 'nt': 3},
+                  'solver': {'poisson_bracket': 'ArakawaJ4', 'laplace_operator': 'CFD4',
+                             'preconditioner_type': 'tensor', 'preconditioner_scheme': 'tensor_kinetic',
+                             'petsc_ksp_rtol': 1e-10, 'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20,
+                             'dissipation_type': 'collisions' if coll else 'double_bracket',
+                             'collision_operator': dkind if coll else None,
+                             'bracket_operator': None if coll else dkind, 'coll_freq': nu},
+                  'initial_data': {'charge': -1.0}})
+    run = Runscript(cfg, profile=prof)
+    assert run.ctx.dissipation == dkind
+    for s in range(3):
+        eo = orun.step()
+        eg = run.step()
+        print(s, eo['initial'], eg['initial'], len(eo['picard']), len(eg['picard']), flush=True)
+        assert abs(eg['initial'] - eo['initial']) <= 1e-9 * eo['initial']
+        assert len(eg['picard']) == len(eo['picard'])
+        for po, pg in zip(eo['picard'], eg['picard']):
+            assert abs(pg['gmres'] - po['gmres_its']) <= 1
+        f = run.fc.cpu().numpy().T
+        assert np.abs(f - orun.fc).max() <= 1e-9 * np.abs(orun.fc).max()
+    if coll:
+        # the collision moments the runner keeps are those of f
+        n, U, e, a = _moments(orun.fc, g)
+        assert np.abs(run.A.cpu().numpy() - a).max() <= 1e-9 * np.abs(a).max()
+
+
+def test_dissipation_rejections(cuda):
+    from vivlasov1d_amd import _lib
+    g, ctx, H, rng, T = _setup(16, 32, 'arakawaj4', 6, cuda)
+    ctx.set_dissipation('ffh', 0.1)
+    f = ctx.new_field()
+    f32 = f.float()
+    with pytest.raises(_lib.VVError, match="FP32 Jacobian"):
+        ctx.jacobian_apply32(f32, f32.clone())
+    with pytest.raises(_lib.VVError, match="mixed-precision"):
+        ctx.ksp_solve_mixed(f, ctx.new_field(), 1e-8, 1e-50, 1e5, 100)
+    with pytest.raises(_lib.VVError, match="adjoint"):
+        ctx.adjoint_solve(f, ctx.new_field(), 1e-8, 1e-50, 1e5, 100)
+    with pytest.raises(_lib.VVError, match="graph"):
+        ctx.set_graph(True)
+    with pytest.raises(_lib.VVError, match="collision operator"):
+        ctx.collision_add(f, ctx.new_field(), ctx.new_x(), ctx.new_x(), None, ctx.new_x(), 0.5)
+    # a collision operator without moments is an error, not a silent no-op
+    ctx.set_dissipation('collt', 0.1)
+    ctx.set_moments(None, None, None, None, None, None, None, None)
+    with pytest.raises(_lib.VVError, match="set_moments"):
+        ctx.residual(f, f.clone(), ctx.new_field())
+    # coll_freq == 0 is no dissipation (config.py:148-163)
+    ctx.set_dissipation('collt', 0.0)
+    assert ctx.dissipation is None
+    ctx.residual(f, f.clone(), ctx.new_field())

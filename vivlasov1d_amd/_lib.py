@@ -21,6 +21,10 @@ PRECONDITIONERS = {"": 0, None: 0, "none": 0, "tensor_kinetic": 1, "tensor_kinet
                    "tensor_potential": 2, "tensor_potential_scipy": 2}
 LAPLACE = {"cfd2": 2, "cfd4": 4}
 LINEAR = {"A": 0, "A0": 1, "AT": 2, "A0T": 3}
+# dissipation terms: collision operators (collision_operator collT / collE) and
+# double brackets (bracket_operator ffh / fhh), CollisionOperator.pyx:43-57,
+# DoubleBracket.pyx:43-56
+DISSIPATION = {"": 0, None: 0, "none": 0, "collt": 1, "colle": 2, "ffh": 3, "fhh": 4}
 
 KSP_REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", -3: "DIVERGED_ITS", -4: "DIVERGED_DTOL",
                -5: "DIVERGED_BREAKDOWN", -9: "DIVERGED_NANORINF", 0: "ITERATING"}
@@ -47,6 +51,11 @@ class KspResult(ctypes.Structure):
 
 class MixedParams(ctypes.Structure):
     _fields_ = [("inner_rtol", ctypes.c_double), ("max_cycles", ctypes.c_int32), ("fp64_check", ctypes.c_int32)]
+
+
+class Dissipation(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("coll_freq", ctypes.c_double),
+                ("coll_diff", ctypes.c_double), ("coll_drag", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p
@@ -81,6 +90,12 @@ SIGNATURES = {
     "vv_function_norm": (ctypes.c_int, [_P, _P, _P, _P, _PD]),
     "vv_density": (ctypes.c_int, [_P, _P, _P]),
     "vv_velocity_moments": (ctypes.c_int, [_P, _P, _P, _P]),
+    "vv_set_dissipation": (ctypes.c_int, [_P, ctypes.POINTER(Dissipation)]),
+    "vv_set_moments": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vv_collision_factor": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "vv_collision_add": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _D]),
+    "vv_double_bracket_function": (ctypes.c_int, [_P, _P, _P, _D]),
+    "vv_double_bracket_jacobian": (ctypes.c_int, [_P, _P, _P, _P, _D]),
     "vv_poisson_solve": (ctypes.c_int, [_P, _P, _P]),
     "vv_poisson_matrix_mult": (ctypes.c_int, [_P, _P, _P]),
     "vv_poisson_function_mult": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -27,6 +27,7 @@ UNITS = {
     "spectral.hip": ["-ffp-contract=fast"],
     "krylov.hip": ["-ffp-contract=off"],
     "moments.hip": ["-ffp-contract=off"],
+    "dissipation.hip": ["-ffp-contract=off"],
     "api.hip": [],
 }
 

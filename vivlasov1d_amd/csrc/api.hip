@@ -256,6 +256,81 @@ int linear_apply(vv_ctx *c, const double *u, const double *uscale, const double 
     return VV_OK;
 }
 
+// ---- dissipation terms (PETScNLVlasovMP.pyx:38-39, 56-58) ------------------
+bool diss_on(const vv_ctx *c) { return c->diss_kind != VV_DISSIPATION_NONE; }
+bool diss_coll(const vv_ctx *c)
+{
+    return c->diss_kind == VV_DISSIPATION_COLLT || c->diss_kind == VV_DISSIPATION_COLLE;
+}
+
+int need_moments(vv_ctx *c)
+{
+    const double *const *m = c->mom;
+    if (!m[0] || !m[1] || !m[3] || !m[4] || !m[5] || !m[7])
+        return fail(c, VV_ERR_INVALID, "collision operator: set_moments first");
+    return VV_OK;
+}
+
+// The terms function_solver adds after B(Fave, H): the collision operator at
+// f with the current moments and at fh with the history moments (each with
+// factor 1/2), or the double bracket of Fave = .5 fh + .5 f with factor 1.
+// Fave and b = {Fave, H_ave} stay in the context for the Jacobian's double
+// bracket, as the reference's self.Fave does.
+int diss_function(vv_ctx *c, const double *f, const double *fh, double *y)
+{
+    if (!diss_on(c)) return VV_OK;
+    if (diss_coll(c)) {
+        if (int r = need_moments(c)) return r;
+        const double *const *m = c->mom;
+        VV_CHECK(launch_collision(c, f, nullptr, y, m[0], m[1], m[3], 0.5, nullptr), "collision");
+        VV_CHECK(launch_collision(c, fh, nullptr, y, m[4], m[5], m[7], 0.5, nullptr), "collision");
+        return VV_OK;
+    }
+    VV_CHECK(launch_average(c, f, fh, c->fave), "average");
+    VV_CHECK(launch_bracket_acc(c, c->fave, nullptr, nullptr, c->db1, 1.0, true, nullptr), "double bracket");
+    c->db1_valid = true;
+    const double fac = -c->coll_freq * 1.0;
+    if (c->diss_kind == VV_DISSIPATION_FFH)        // {Fave, b}  (DoubleBracket.pyx:80-82)
+        VV_CHECK(launch_bracket_acc(c, c->fave, nullptr, c->db1, y, fac, false, nullptr), "double bracket");
+    else                                            // {b, H_ave}  (DoubleBracket.pyx:95-97)
+        VV_CHECK(launch_bracket_acc(c, c->db1, nullptr, nullptr, y, fac, false, nullptr), "double bracket");
+    return VV_OK;
+}
+
+// The terms jacobian_solver adds after .5 B(u, H) for u = *us * x (us may be
+// null): the collision operator with the current moments and factor 1/2, or
+// the double bracket's Jacobian with factor 1/2 (DoubleBracket.pyx:69-76,
+// 88-91).  done: early exit of kernels enqueued past a cycle's convergence.
+int diss_jacobian(vv_ctx *c, const double *x, const double *us, double *y, const int32_t *done)
+{
+    if (!diss_on(c)) return VV_OK;
+    if (diss_coll(c)) {
+        if (int r = need_moments(c)) return r;
+        VV_CHECK(launch_collision(c, x, us, y, c->mom[0], c->mom[1], c->mom[3], 0.5, done), "collision");
+        return VV_OK;
+    }
+    const double fac = -c->coll_freq * 0.5;
+    if (c->diss_kind == VV_DISSIPATION_FFH) {
+        if (!c->db1_valid) {
+            VV_CHECK(launch_bracket_acc(c, c->fave, nullptr, nullptr, c->db1, 1.0, true, nullptr), "double bracket");
+            c->db1_valid = true;
+        }
+        VV_CHECK(launch_bracket_acc(c, x, us, c->db1, y, fac, false, done), "double bracket");      // {u, {Fave, H}}
+        VV_CHECK(launch_bracket_acc(c, x, us, nullptr, c->db2, 1.0, true, done), "double bracket");  // {u, H}
+        VV_CHECK(launch_bracket_acc(c, c->fave, nullptr, c->db2, y, fac, false, done), "double bracket");
+    } else {
+        VV_CHECK(launch_bracket_acc(c, x, us, nullptr, c->db2, 1.0, true, done), "double bracket");  // {u, H}
+        VV_CHECK(launch_bracket_acc(c, c->db2, nullptr, nullptr, y, fac, false, done), "double bracket");
+    }
+    return VV_OK;
+}
+
+int no_diss(vv_ctx *c, const char *what)
+{
+    if (!diss_on(c)) return VV_OK;
+    return fail(c, VV_ERR_UNSUPPORTED, std::string(what) + " does not take a dissipation term");
+}
+
 // the preconditioner of the GMRES cycle: P^-1, or P^-T for the adjoint
 int apply_pc(vv_ctx *c, const double *x, double *y, const int32_t *done)
 {
@@ -276,6 +351,7 @@ int arnoldi_step(vv_ctx *c, int it)
         if (int r = apply_pc(c, c->X, w, done)) return r;
     } else {
         VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
+        if (int r = diss_jacobian(c, v, &c->gs->scale[it], c->X, done)) return r;
         if (int r = pc_apply(c, c->X, w, done, it, false)) return r;
     }
     VV_PLAUNCH(P_MDOT, F * (it + 2), it, launch_mdot(c, it), "mdot");
@@ -418,6 +494,7 @@ int restart_cycles(vv_ctx *c, const double *b, double *x)
             if (int r = linear_apply(c, x, nullptr, c->op_fbar, c->X, c->ht_inv, -0.5, 0.5, true)) return r;
         } else {
             VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
+            if (int r = diss_jacobian(c, x, nullptr, c->X, nullptr)) return r;
         }
         if (int r = apply_pc(c, c->X, c->basis, nullptr)) return r;
         VV_CHECK(launch_residual_restart(c, b, c->basis), "restart residual");
@@ -507,9 +584,11 @@ int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, v
     if (c->p.precond == VV_PRECOND_NONE)
         return fail(c, VV_ERR_UNSUPPORTED, "vv_snes_ksponly needs a tensor-product preconditioner");
     if (c->use_graph) {
+        if (int r = no_diss(c, "graph replay")) return r;
         if (int r = snes_graph(c, f, fh, kp)) return r;
     } else {
         VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, c->X), "residual");
+        if (int r = diss_function(c, f, fh, c->X)) return r;
         if (int r = precond(c, c->X, c->rhs, nullptr)) return r;
         VV_CHECK(launch_zero(c, c->D, c->N * c->B), "zero x");
         VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
@@ -683,6 +762,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     c->ht_inv = 1. / p->ht;
     c->hv_inv = 1. / hv;
     const double hv2 = hv * hv;
+    c->hv2_inv = 1. / hv2;
     c->inst_host.resize(B);
     for (int b = 0; b < B; ++b) {
         InstConst &ic = c->inst_host[b];
@@ -848,7 +928,8 @@ int vv_ctx_destroy(vv_ctx *c)
     void *bufs[] = {c->inst,   c->active, c->h0,      c->h1p,     c->h1h,  c->h2p,   c->h2h,    c->eigen,
                     c->gm,     c->g0,     c->gp,      c->twiddle, c->twiddle32, c->spec, c->pinv, c->pwork,
                     c->X,      c->rhs,    c->D,       c->basis,   c->basis32, c->R,   c->partial, c->dscalar,
-                    c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid, c->tb};
+                    c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid, c->tb,     c->fave,    c->db1,
+                    c->db2};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);
@@ -916,6 +997,8 @@ int vv_set_active(vv_ctx *c, const int32_t *active)
 int vv_set_graph(vv_ctx *c, int enable)
 {
     if (int r = check_ctx(c)) return r;
+    if (enable)
+        if (int r = no_diss(c, "graph replay")) return r;
     if (enable && c->stream == nullptr) return fail(c, VV_ERR_INVALID, "graph replay needs a non-null stream");
     if (!enable && c->graph_exec) {
         (void)hipGraphExecDestroy(c->graph_exec);
@@ -1040,6 +1123,7 @@ int vv_set_hamiltonian(vv_ctx *c, const double *h0, const double *h1p, const dou
     VV_CHECK(hipMemcpyAsync(c->h1h, h1h, bx, hipMemcpyDeviceToDevice, c->stream), "copy h1h");
     VV_CHECK(hipMemcpyAsync(c->h2p, h2p, bx, hipMemcpyDeviceToDevice, c->stream), "copy h2p");
     VV_CHECK(hipMemcpyAsync(c->h2h, h2h, bx, hipMemcpyDeviceToDevice, c->stream), "copy h2h");
+    c->db1_valid = false;
     return VV_OK;
 }
 
@@ -1049,7 +1133,7 @@ int vv_residual(vv_ctx *c, const double *f, const double *fh, double *y)
     if (!f || !fh || !y) return fail(c, VV_ERR_INVALID, "null field");
     if (y == f || y == fh) return fail(c, VV_ERR_INVALID, "residual: y must not alias f or fh");
     VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, y), "residual");
-    return VV_OK;
+    return diss_function(c, f, fh, y);
 }
 
 int vv_jacobian_apply(vv_ctx *c, const double *u, double *y)
@@ -1058,7 +1142,7 @@ int vv_jacobian_apply(vv_ctx *c, const double *u, double *y)
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
     if (u == y) return fail(c, VV_ERR_INVALID, "jacobian_apply: u and y must not alias");
     VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, u, nullptr, y), "jacobian");
-    return VV_OK;
+    return diss_jacobian(c, u, nullptr, y, nullptr);
 }
 
 int vv_bracket_add(vv_ctx *c, const double *x, double *y, double factor)
@@ -1091,6 +1175,7 @@ int vv_function(vv_ctx *c, const double *f, const double *fh, double *y)
     if (int r = check_ctx(c)) return r;
     if (!f || !fh || !y) return fail(c, VV_ERR_INVALID, "null field");
     VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, c->X), "residual");
+    if (int r = diss_function(c, f, fh, c->X)) return r;
     return precond(c, c->X, y, nullptr);
 }
 
@@ -1099,6 +1184,7 @@ int vv_jacobian(vv_ctx *c, const double *u, double *y)
     if (int r = check_ctx(c)) return r;
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
     VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, u, nullptr, c->X), "jacobian");
+    if (int r = diss_jacobian(c, u, nullptr, c->X, nullptr)) return r;
     return precond(c, c->X, y, nullptr);
 }
 
@@ -1108,6 +1194,7 @@ int vv_function_norm(vv_ctx *c, const double *f, const double *fh, double *y, do
     if (!f || !fh || !y || !out) return fail(c, VV_ERR_INVALID, "null argument");
     if (y == f || y == fh) return fail(c, VV_ERR_INVALID, "function_norm: y must not alias f or fh");
     VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, y), "residual");
+    if (int r = diss_function(c, f, fh, y)) return r;
     VV_CHECK(launch_norms(c, y, ds_norms(c)), "norm");
     return read_scalars(c, ds_norms(c), (size_t)c->B, out);
 }
@@ -1125,6 +1212,97 @@ int vv_velocity_moments(vv_ctx *c, const double *f, double *u, double *e)
     if (int r = check_ctx(c)) return r;
     if (!f || !u || !e) return fail(c, VV_ERR_INVALID, "null argument");
     VV_CHECK(launch_velocity_moments(c, f, u, e), "velocity moments");
+    return VV_OK;
+}
+
+int vv_set_dissipation(vv_ctx *c, const vv_dissipation *d)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!d) return fail(c, VV_ERR_INVALID, "null argument");
+    if (d->kind < VV_DISSIPATION_NONE || d->kind > VV_DISSIPATION_FHH)
+        return fail(c, VV_ERR_UNSUPPORTED, "dissipation must be none, collt, colle, ffh or fhh");
+    // config.py:148-163: no dissipation object when coll_freq == 0
+    const int kind = d->coll_freq == 0. ? VV_DISSIPATION_NONE : d->kind;
+    if (kind != VV_DISSIPATION_NONE && c->B != 1)
+        return fail(c, VV_ERR_UNSUPPORTED, "dissipation terms take a single-instance context");
+    if (kind != VV_DISSIPATION_NONE && c->use_graph)
+        return fail(c, VV_ERR_UNSUPPORTED, "switch graph replay off before setting a dissipation term");
+    if ((kind == VV_DISSIPATION_FFH || kind == VV_DISSIPATION_FHH) && !c->fave) {
+        VV_CHECK(dalloc(&c->fave, (size_t)c->N), "alloc double-bracket average");
+        VV_CHECK(dalloc(&c->db1, (size_t)c->N), "alloc double-bracket scratch");
+        VV_CHECK(dalloc(&c->db2, (size_t)c->N), "alloc double-bracket scratch");
+        VV_CHECK(launch_zero(c, c->fave, c->N), "zero");    // the solver's Fave starts as a zero Vec
+    }
+    c->diss_kind = kind;
+    c->coll_freq = d->coll_freq;
+    c->coll_diff = d->coll_diff;
+    c->coll_drag = d->coll_drag;
+    c->db1_valid = false;
+    return VV_OK;
+}
+
+int vv_set_moments(vv_ctx *c, const double *np, const double *up, const double *ep, const double *ap,
+                   const double *nh, const double *uh, const double *eh, const double *ah)
+{
+    if (int r = check_ctx(c)) return r;
+    const double *m[8] = {np, up, ep, ap, nh, uh, eh, ah};
+    for (int k = 0; k < 8; ++k) c->mom[k] = m[k];
+    return VV_OK;
+}
+
+int vv_collision_factor(vv_ctx *c, const double *n, const double *u, const double *e, double *a)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!n || !u || !e || !a) return fail(c, VV_ERR_INVALID, "null argument");
+    VV_CHECK(launch_collision_factor(c, n, u, e, a), "collision factor");
+    return VV_OK;
+}
+
+int vv_collision_add(vv_ctx *c, const double *f, double *y, const double *n, const double *u, const double *e,
+                     const double *a, double factor)
+{
+    (void)e;
+    if (int r = check_ctx(c)) return r;
+    if (!f || !y || !n || !u || !a) return fail(c, VV_ERR_INVALID, "null argument");
+    if (f == y) return fail(c, VV_ERR_INVALID, "collision_add: f and y must not alias");
+    if (!diss_coll(c)) return fail(c, VV_ERR_UNSUPPORTED, "the context has no collision operator");
+    VV_CHECK(launch_collision(c, f, nullptr, y, n, u, a, factor, nullptr), "collision");
+    return VV_OK;
+}
+
+int vv_double_bracket_function(vv_ctx *c, const double *fave, double *y, double factor)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!fave || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (fave == y) return fail(c, VV_ERR_INVALID, "double_bracket_function: fave and y must not alias");
+    if (c->diss_kind != VV_DISSIPATION_FFH && c->diss_kind != VV_DISSIPATION_FHH)
+        return fail(c, VV_ERR_UNSUPPORTED, "the context has no double bracket");
+    const double fac = -c->coll_freq * factor;
+    VV_CHECK(launch_bracket_acc(c, fave, nullptr, nullptr, c->db2, 1.0, true, nullptr), "double bracket");
+    if (c->diss_kind == VV_DISSIPATION_FFH)
+        VV_CHECK(launch_bracket_acc(c, fave, nullptr, c->db2, y, fac, false, nullptr), "double bracket");
+    else
+        VV_CHECK(launch_bracket_acc(c, c->db2, nullptr, nullptr, y, fac, false, nullptr), "double bracket");
+    return VV_OK;
+}
+
+int vv_double_bracket_jacobian(vv_ctx *c, const double *u, const double *fave, double *y, double factor)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!u || !fave || !y) return fail(c, VV_ERR_INVALID, "null field");
+    if (u == y || fave == y) return fail(c, VV_ERR_INVALID, "double_bracket_jacobian: y must not alias u or fave");
+    if (c->diss_kind != VV_DISSIPATION_FFH && c->diss_kind != VV_DISSIPATION_FHH)
+        return fail(c, VV_ERR_UNSUPPORTED, "the context has no double bracket");
+    const double fac = -c->coll_freq * factor;
+    if (c->diss_kind == VV_DISSIPATION_FFH) {
+        VV_CHECK(launch_bracket_acc(c, fave, nullptr, nullptr, c->db2, 1.0, true, nullptr), "double bracket");
+        VV_CHECK(launch_bracket_acc(c, u, nullptr, c->db2, y, fac, false, nullptr), "double bracket");
+        VV_CHECK(launch_bracket_acc(c, u, nullptr, nullptr, c->db2, 1.0, true, nullptr), "double bracket");
+        VV_CHECK(launch_bracket_acc(c, fave, nullptr, c->db2, y, fac, false, nullptr), "double bracket");
+    } else {
+        VV_CHECK(launch_bracket_acc(c, u, nullptr, nullptr, c->db2, 1.0, true, nullptr), "double bracket");
+        VV_CHECK(launch_bracket_acc(c, c->db2, nullptr, nullptr, y, fac, false, nullptr), "double bracket");
+    }
     return VV_OK;
 }
 
@@ -1221,6 +1399,7 @@ int vv_ksp_solve_mixed(vv_ctx *c, const double *b, double *x, const vv_ksp_param
     if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
     if (!(mp->inner_rtol > 0.0 && mp->inner_rtol < 1.0) || mp->max_cycles < 1)
         return fail(c, VV_ERR_INVALID, "inner_rtol must be in (0, 1) and max_cycles >= 1");
+    if (int r = no_diss(c, "the mixed-precision solve")) return r;
     return ksp_solve_mixed_impl(c, b, x, kp, mp, res, cycles);
 }
 
@@ -1233,6 +1412,7 @@ int vv_snes_ksponly_mixed(vv_ctx *c, double *f, const double *fh, const vv_ksp_p
     if (!(mp->inner_rtol > 0.0 && mp->inner_rtol < 1.0) || mp->max_cycles < 1)
         return fail(c, VV_ERR_INVALID, "inner_rtol must be in (0, 1) and max_cycles >= 1");
     if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the mixed-precision solve takes one instance");
+    if (int r = no_diss(c, "the mixed-precision solve")) return r;
     if (int r = vv_function(c, f, fh, c->rhs)) return r;
     vv_ksp_result kr{};
     if (int r = ksp_solve_mixed_impl(c, c->rhs, c->D, kp, mp, &kr, cycles)) return r;
@@ -1247,6 +1427,7 @@ int vv_jacobian_apply32(vv_ctx *c, const float *u, float *y)
     if (int r = check_ctx(c)) return r;
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
     if (u == y) return fail(c, VV_ERR_INVALID, "jacobian_apply32: u and y must not alias");
+    if (int r = no_diss(c, "the FP32 Jacobian")) return r;
     VV_PLAUNCH(P_JAC32, 8.0 * (double)c->N * c->B, -1, launch_jacobian32(c, u, nullptr, y), "jacobian32");
     return VV_OK;
 }
@@ -1274,6 +1455,7 @@ int vv_linear_apply(vv_ctx *c, const double *u, const double *fbar, double *y, i
     if (int r = check_ctx(c)) return r;
     if (!u || !y) return fail(c, VV_ERR_INVALID, "null field");
     if (u == y || (fbar && (fbar == y))) return fail(c, VV_ERR_INVALID, "linear_apply: y must not alias u or fbar");
+    if (int r = no_diss(c, "the linearised operator")) return r;
     const double it = c->ht_inv;
     switch (which) {
     case VV_LIN_A: return linear_apply(c, u, nullptr, fbar, y, it, 0.5, 0.5, false);
@@ -1293,6 +1475,7 @@ int vv_adjoint_solve(vv_ctx *c, const double *lam, const double *fbar, double *m
     if (c->p.precond == VV_PRECOND_NONE)
         return fail(c, VV_ERR_UNSUPPORTED, "the adjoint solve needs a tensor-product preconditioner");
     if (c->use_graph) return fail(c, VV_ERR_UNSUPPORTED, "switch graph replay off for the adjoint solve");
+    if (int r = no_diss(c, "the adjoint solve")) return r;
     if (int r = ensure_linear(c)) return r;
     if (int r = precond_t(c, lam, c->rhs, nullptr)) return r;
     c->op_mode = 1;

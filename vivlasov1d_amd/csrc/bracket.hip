@@ -1,6 +1,7 @@
 // Arakawa J1 / J2 / J4 Poisson-bracket kernels: the implicit-midpoint
 // residual F(f), the Jacobian-vector product J u and the plain accumulation
-// y += factor {x, H}.
+// y += factor {x, H} (or y = 0 + factor {x, H}; the double-bracket
+// dissipation terms are chains of these).
 //
 // This translation unit is compiled with -ffp-contract=off: every stencil
 // expression below keeps the reference's IEEE operation order
@@ -57,7 +58,7 @@ struct BrArgs {
     int64_t N;
     const T *f;            // residual: f; jacobian: u; accumulate: x
     const T *fh;           // residual: f at the previous time level
-    const double *uscale;  // jacobian: optional device scale of u (unnormalised Krylov vector)
+    const double *uscale;  // optional device scale of the first argument (unnormalised Krylov vector)
     T *y;
     const double *h0, *h1p, *h1h, *h2p, *h2h;
     const double *h2d;     // optional second argument as a 2D field (instead of H_ave)
@@ -66,6 +67,8 @@ struct BrArgs {
     double ht_inv, factor, alpha;
     const int32_t *done;   // optional early-exit flag (GMRES cycle finished)
     int64_t gstride;
+    int store;             // accumulate mode: y = 0 + factor {x, H} (the reference's
+                           // Vec.set(0.) followed by PoissonBracket.function)
 };
 
 template <int KIND, int MODE, typename T>
@@ -88,7 +91,6 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const T *__restrict__ fhin = (MODE == MODE_RESIDUAL) ? a.fh + fo : nullptr;
     T *__restrict__ yout = a.y + fo;
     const T half = T(0.5);
-    constexpr bool SCALED = (MODE == MODE_JACOBIAN || MODE == MODE_TRANSPOSED);
     const bool factors = (a.h2d == nullptr && a.hxv == nullptr);
 
     // Phase 1: every global load of the block is issued before anything waits
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const int32_t dflag =
         (a.done != nullptr) ? *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(a.done) + b * a.gstride)
                             : 0;
-    const double usd = (SCALED && a.uscale != nullptr)
+    const double usd = (a.uscale != nullptr)
                            ? *reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.uscale) + b * a.gstride)
                            : 1.0;
     double xf0 = 0.0, xf1 = 0.0, xf2 = 0.0, xf3 = 0.0, vf = 0.0;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             } else if (MODE == MODE_JACOBIAN) {
                 fv = fa[q] * us;                             // == VecScale'd Krylov vector
             } else {
-                fv = fa[q];
+                fv = (a.uscale != nullptr) ? fa[q] * us : fa[q];
             }
             sF[r][c] = fv;
             // ((((h0_j + .5 h1p_i) + .5 h1h_i) + .5 h2p_i) + .5 h2h_i)  (PETScVlasovSolver.pyx:145-149);
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             } else if (MODE == MODE_TRANSPOSED) {
                 base = (T)a.alpha * (fin[o] * us);
             } else {
-                base = yout[o];
+                base = a.store ? T(0) : yout[o];
             }
             T out = base;
             if (MODE == MODE_TRANSPOSED || (j >= 2 && j < nv - 2)) {
@@ -351,7 +353,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
         } else if (MODE == MODE_TRANSPOSED) {
             base = (T)a.alpha * (fin[o] * us);
         } else {
-            base = yout[o];
+            base = a.store ? T(0) : yout[o];
         }
         T out = base;
         if (MODE == MODE_TRANSPOSED || (j >= 2 && j < nv - 2)) {
@@ -501,6 +503,20 @@ hipError_t launch_bracket_gen(vv_ctx *c, const double *x, const double *h2d, con
     a.hxv = hxv;
     a.y = y;
     a.factor = factor;
+    return launch_mode<MODE_ACCUMULATE>(c, a);
+}
+
+hipError_t launch_bracket_acc(vv_ctx *c, const double *x, const double *xscale, const double *h2d, double *y,
+                              double factor, bool store, const int32_t *done)
+{
+    BrArgs<double> a = base_args<double>(c);
+    a.f = x;
+    a.uscale = xscale;
+    a.h2d = h2d;
+    a.y = y;
+    a.factor = factor;
+    a.store = store ? 1 : 0;
+    a.done = done;
     return launch_mode<MODE_ACCUMULATE>(c, a);
 }
 

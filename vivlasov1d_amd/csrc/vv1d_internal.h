@@ -130,6 +130,15 @@ struct vv_ctx {
     hipGraphExec_t graph_exec;
     const double *graph_f, *graph_fh;
     double graph_kp[4];
+    // dissipation term (VV_DISSIPATION_*) and its parameters; moment vectors
+    // are caller-owned device x-vectors: N, U, E, A current then history
+    int diss_kind;
+    double coll_freq, coll_diff, coll_drag, hv2_inv;
+    const double *mom[8];
+    double *fave;       // .5 fh + .5 f of the last residual (double brackets)
+    double *db1;        // {fave, H_ave}, valid while db1_valid
+    double *db2;        // scratch bracket of the Jacobian
+    bool db1_valid;
     std::string err;
 };
 
@@ -148,6 +157,15 @@ hipError_t launch_bracket_t(vv_ctx *c, const double *u, const double *uscale, co
 // y += factor {x, H} on rows 2..nv-3, H = h2d / hxv / H_ave as above
 hipError_t launch_bracket_gen(vv_ctx *c, const double *x, const double *h2d, const double *hxv, double *y,
                               double factor);
+// y (+)= factor {s x, H} on rows 2..nv-3 (store: y = 0 + ..., other rows 0),
+// s = *xscale when given, H = h2d or H_ave
+hipError_t launch_bracket_acc(vv_ctx *c, const double *x, const double *xscale, const double *h2d, double *y,
+                              double factor, bool store, const int32_t *done);
+// dissipation.hip: y += C(s f) of the context's collision operator; a = n n / (n e - u u)
+hipError_t launch_collision(vv_ctx *c, const double *f, const double *fscale, double *y, const double *n,
+                            const double *U, const double *a, double factor, const int32_t *done);
+hipError_t launch_collision_factor(vv_ctx *c, const double *n, const double *u, const double *e, double *a);
+hipError_t launch_average(vv_ctx *c, const double *f, const double *fh, double *out);   // .5 fh + .5 f
 // spectral.hip
 int fft_blocks(vv_ctx *c);
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done);

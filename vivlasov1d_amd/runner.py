@@ -4,8 +4,10 @@
 vlasov/run/run_nonlinear_matrixfree_split_mp.py:15-174 together with the
 parts of ``viVlasov1Dbase`` (vlasov/run/run_base.py:35-807) the timestep
 uses: initial data and normalisation, moments + Poisson, residual, history,
-and the Picard loop with its stop / revert / abort rules.  HDF5 output and
-the other run modes are not part of this library.
+and the Picard loop with its stop / revert / abort rules, the collision /
+double-bracket dissipation terms the configuration selects, and (optional)
+HDF5 output in the reference's schema.  The other run modes are not part of
+this library.
 """
 
 import json
@@ -44,6 +46,12 @@ class Runscript:
         self.h2c, self.h2h = ctx.new_x(), ctx.new_x()
         self.fc, self.fh, self.fl, self.fb = ctx.new_field(), ctx.new_field(), ctx.new_field(), ctx.new_field()
         self.N = ctx.new_x()
+        # velocity / energy density and collision factor, current and history
+        # (run_base.py:313-320); refreshed each Picard iteration only when a
+        # collision operator reads them
+        self.U, self.E, self.A = ctx.new_x(), ctx.new_x(), ctx.new_x()
+        self.Nh, self.Uh, self.Eh, self.Ah = ctx.new_x(), ctx.new_x(), ctx.new_x(), ctx.new_x()
+        self.collisions = bool(cfg.get_collision_operator())
         self.pc_int, self.ph_int = ctx.new_x(), ctx.new_x()
         self.pc_ext = ctx.new_x()
         self.toolbox = Toolbox(None, None, grid, ctx)
@@ -60,7 +68,10 @@ class Runscript:
         self.calculate_moments()
         # solver objects (run_nonlinear_matrixfree_split_mp.py:26-80)
         self.vlasov_solver = PETScVlasovSolver(cfg, None, None, grid, self.h0, self.h1c, self.h1h, self.h2c,
-                                               self.h2h, self.pc_int, self.charge, context=ctx)
+                                               self.h2h, self.pc_int, self.charge, coll_freq=sol['coll_freq'],
+                                               coll_diff=sol['coll_diff'], coll_drag=sol['coll_drag'],
+                                               regularisation=sol.get('regularisation', 0.0), context=ctx)
+        self.vlasov_solver.set_moments(self.N, self.U, self.E, self.A, self.Nh, self.Uh, self.Eh, self.Ah)
         self.poisson_solver = PETScPoissonSolver(None, grid.nx, grid.hx, self.charge,
                                                  laplace=cfg.get_laplace_operator(), context=ctx)
         self.snes = SNES()
@@ -87,7 +98,6 @@ class Runscript:
         self.output = None
         if path:
             from .hdf5 import HDF5Output
-            self.U, self.E = ctx.new_x(), ctx.new_x()
             self.output = HDF5Output(path, grid.x, grid.v, self.charge)
             self.save_hdf5_vectors()
 
@@ -102,6 +112,9 @@ class Runscript:
     # run_base.py:640-704, 741-742 -------------------------------------------
     def calculate_moments(self):
         self.pnorm = self.ctx.moments(self.fc, self.N, self.pc_int, self.h1c, want_norm=True)
+        if self.collisions:
+            self.ctx.velocity_moments(self.fc, self.U, self.E)
+            self.ctx.collision_factor(self.N, self.U, self.E, self.A)
 
     # run_base.py:749-763 ----------------------------------------------------
     def make_history(self):
@@ -109,6 +122,9 @@ class Runscript:
         self.h1h.copy_(self.h1c)
         self.h2h.copy_(self.h2c)
         self.ph_int.copy_(self.pc_int)
+        if self.collisions:
+            for cur, his in ((self.N, self.Nh), (self.U, self.Uh), (self.E, self.Eh), (self.A, self.Ah)):
+                his.copy_(cur)
         self.vlasov_solver.update_history(self.fc)
 
     def step(self):

@@ -98,6 +98,7 @@ class Context:
         self.lib = lib
         self.stream = stream
         self.N = grid.nx * grid.nv
+        self.dissipation = None
 
     @property
     def handle(self):
@@ -207,6 +208,39 @@ class Context:
     def velocity_moments(self, f, u, e):
         """u = hv sum_j v_j f_ij, e = hv sum_j v_j^2 f_ij (Toolbox.pyx:103-148)."""
         self.call("vv_velocity_moments", self.F(f), self.X(u), self.X(e))
+
+    # dissipation (CollisionOperator.pyx, DoubleBracket.pyx) ------------------
+    def set_dissipation(self, kind, coll_freq=0., coll_diff=1., coll_drag=1.):
+        """kind: 'collt' | 'colle' | 'ffh' | 'fhh' | None; nothing is added
+        when coll_freq == 0 (config.py:148-163)."""
+        if kind not in _lib.DISSIPATION:
+            raise _lib.VVError("dissipation %r is not provided (collt, colle, ffh, fhh)" % (kind,))
+        d = _lib.Dissipation()
+        d.kind = _lib.DISSIPATION[kind]
+        d.coll_freq, d.coll_diff, d.coll_drag = float(coll_freq), float(coll_diff), float(coll_drag)
+        self.call("vv_set_dissipation", ctypes.byref(d))
+        self.dissipation = kind if (d.kind and coll_freq != 0.) else None
+
+    def set_moments(self, np_, up, ep, ap, nh, uh, eh, ah):
+        """The moment x-vectors the collision operators read; the tensors must
+        stay alive (the context keeps their addresses, PETScVlasovSolver.pyx:123-132)."""
+        ptr = lambda t: None if t is None else self.X(t)
+        self._moments = (np_, up, ep, ap, nh, uh, eh, ah)
+        self.call("vv_set_moments", *[ptr(t) for t in self._moments])
+
+    def collision_factor(self, n, u, e, a):
+        """a = n n / (n e - u u) (Toolbox.pyx:153-165)."""
+        self.call("vv_collision_factor", self.X(n), self.X(u), self.X(e), self.X(a))
+
+    def collision_add(self, f, y, n, u, e, a, factor):
+        self.call("vv_collision_add", self.F(f), self.F(y), self.X(n), self.X(u),
+                  None if e is None else self.X(e), self.X(a), ctypes.c_double(factor))
+
+    def double_bracket_function(self, fave, y, factor):
+        self.call("vv_double_bracket_function", self.F(fave), self.F(y), ctypes.c_double(factor))
+
+    def double_bracket_jacobian(self, u, fave, y, factor):
+        self.call("vv_double_bracket_jacobian", self.F(u), self.F(fave), self.F(y), ctypes.c_double(factor))
 
     def poisson_solve(self, n, phi):
         self.call("vv_poisson_solve", self.X(n), self.X(phi))
@@ -372,6 +406,62 @@ class PoissonBracket:
         self.ctx.bracket_add(F, Y, factor)
 
 
+class CollisionOperator:
+    """CollisionOperator.create (CollisionOperator.pyx:43-57): 'collt' |
+    'colle' build the operator on the context, anything else the no-op base."""
+
+    def __init__(self, kind, context, coll_freq=0., coll_diff=1., coll_drag=1.):
+        self.kind = kind
+        self.ctx = context
+        self.coll_freq, self.coll_diff, self.coll_drag = coll_freq, coll_diff, coll_drag
+
+    @staticmethod
+    def create(type, da1, grid, coll_freq=0., coll_diff=1., coll_drag=1., context=None):
+        kind = type if type in ('collt', 'colle') else None
+        ctx = context if context is not None else Context(grid, preconditioner='')
+        if kind is not None:
+            ctx.set_dissipation(kind, coll_freq, coll_diff, coll_drag)
+        return CollisionOperator(kind, ctx, coll_freq, coll_diff, coll_drag)
+
+    def function(self, F, Y, N, U, E, A, factor):
+        """Y += C(F) with u = U/N (:72-110, :164-202); a no-op for the base
+        class and when coll_freq or factor is 0."""
+        if self.kind is None or self.coll_freq == 0. or factor == 0.:
+            return
+        self.ctx.collision_add(F, Y, N, U, E, A, factor)
+
+    def jacobian(self, J, N, U, E, A, factor):
+        raise _lib.VVError("the assembled Jacobian (direct-solver path) is not provided")
+
+
+class DoubleBracket:
+    """DoubleBracket.create (DoubleBracket.pyx:43-56): 'ffh' | 'fhh' build the
+    term on the context, anything else the no-op base.  H must be the
+    context's H_ave (set through update_previous / set_hamiltonian); the Have
+    arguments are accepted for signature parity."""
+
+    def __init__(self, kind, context, coll_freq=0.):
+        self.kind = kind
+        self.ctx = context
+        self.coll_freq = coll_freq
+
+    @staticmethod
+    def create(type, da1, grid, poisson_bracket, coll_freq=0., context=None):
+        kind = type if type in ('ffh', 'fhh') else None
+        ctx = context if context is not None else poisson_bracket.ctx
+        if kind is not None:
+            ctx.set_dissipation(kind, coll_freq)
+        return DoubleBracket(kind, ctx, coll_freq)
+
+    def function(self, Fave, Have, Y, factor):
+        if self.kind is not None:
+            self.ctx.double_bracket_function(Fave, Y, factor)
+
+    def jacobian(self, F, Fave, Have, Y, factor):
+        if self.kind is not None:
+            self.ctx.double_bracket_jacobian(F, Fave, Y, factor)
+
+
 class TimeDerivative:
     """TimeDerivative.create (TimeDerivative.pyx:36-53): only 'point'."""
 
@@ -436,9 +526,9 @@ class PETScVlasovSolver:
                  coll_diff=1., coll_drag=1., regularisation=0., restart=30, context=None):
         if config.get_averaging_operator() != 'point':
             raise _lib.VVError("averaging operator %r is not provided" % config.get_averaging_operator())
-        if config.get_collision_operator() or config.get_double_bracket():
-            raise _lib.VVError("collision / double-bracket dissipation is not provided")
         if regularisation != 0.:
+            # Regularisation.function calls a method the reference does not define
+            # (Regularisation.pyx:41-48): a non-zero epsilon fails there too
             raise _lib.VVError("regularisation is not provided")
         self.grid = grid
         self.charge = charge
@@ -450,6 +540,12 @@ class PETScVlasovSolver:
             laplace=config.get_laplace_operator(), charge=charge, restart=restart)
         self.poisson_bracket = PoissonBracket(config.get_poisson_bracket(), self.ctx)
         self.time_derivative = TimeDerivative(grid)
+        # dissipation (:101-102): the context adds it to every residual and
+        # Jacobian apply once set
+        self.double_bracket = DoubleBracket.create(config.get_double_bracket(), None, grid, self.poisson_bracket,
+                                                   coll_freq, context=self.ctx)
+        self.collision_operator = CollisionOperator.create(config.get_collision_operator(), None, grid, coll_freq,
+                                                           coll_diff, coll_drag, context=self.ctx)
         self.preconditioner = TensorProductPreconditioner(self.ctx, phi) if pc else None
         if self.preconditioner is not None:
             self.preconditioner.update_matrices()       # built from the initial phi (:104)
@@ -459,9 +555,11 @@ class PETScVlasovSolver:
         self.Nh = self.Uh = self.Eh = self.Ah = None
 
     def set_moments(self, Np, Up, Ep, Ap, Nh, Uh, Eh, Ah):
-        """PETScVlasovSolver.pyx:123-132 (used only by collision operators)."""
+        """PETScVlasovSolver.pyx:123-132: the moment vectors the collision
+        operator reads (their addresses; the caller refreshes the contents)."""
         self.Np, self.Up, self.Ep, self.Ap = Np, Up, Ep, Ap
         self.Nh, self.Uh, self.Eh, self.Ah = Nh, Uh, Eh, Ah
+        self.ctx.set_moments(Np, Up, Ep, Ap, Nh, Uh, Eh, Ah)
 
     def update_history(self, F):
         """Fh = F; the preconditioner's matrices follow phi (:135-139)."""
@@ -564,6 +662,15 @@ class Toolbox:
 
     def compute_density(self, F, N):
         self.ctx.density(F, N)
+
+    def compute_velocity_density(self, F, U):
+        self.ctx.velocity_moments(F, U, self.ctx.new_x())
+
+    def compute_energy_density(self, F, E):
+        self.ctx.velocity_moments(F, self.ctx.new_x(), E)
+
+    def compute_collision_factor(self, N, U, E, A):
+        self.ctx.collision_factor(N, U, E, A)
 
     def potential_to_hamiltonian(self, P, H):
         self.ctx.potential_to_hamiltonian(P, H)
