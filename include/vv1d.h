@@ -253,6 +253,34 @@ int vv_double_bracket_function(vv_ctx *ctx, const double *fave, double *y, doubl
 int vv_double_bracket_jacobian(vv_ctx *ctx, const double *u, const double *fave, double *y,
                                double factor);
 
+/* Explicit Arakawa steppers of the initial guess (run_base.py:786-927,
+ * chosen by solver.initial_guess, iterated solver.initial_iter = niter times
+ * per step with dt/niter each).  The reference's PETScArakawaSymplectic and
+ * PETScArakawaRungeKutta cannot run as written (bracket arguments in the
+ * wrong order, an accumulator never cleared, an undefined rk18_J4); these
+ * are the schemes they state, with the tendency of the implicit residual,
+ * df/dt = -{f, H} on rows 2..nv-3.  x is updated in place; h0 is the v-vector
+ * of the kinetic Hamiltonian, h1 and h2 x-vectors (the run script passes its
+ * history potentials h1h, h2h).  Single-instance contexts. */
+
+/* PETScArakawaSymplectic.kinetic (PETScArakawaSymplectic.pyx:62-77):
+ * x += factor*ht/niter * (-{x, H0}). */
+int vv_symplectic_kinetic(vv_ctx *ctx, double *x, const double *h0, double factor,
+                          int32_t niter);
+
+/* PETScArakawaSymplectic.potential (PETScArakawaSymplectic.pyx:80-97):
+ * x += factor*ht/niter * (-{x, H1 + H2}). */
+int vv_symplectic_potential(vv_ctx *ctx, double *x, const double *h1, const double *h2,
+                            double factor, int32_t niter);
+
+/* PETScArakawaRungeKutta.rk4_16 / rk4_18 (PETScArakawaRungeKutta.pyx:20-98):
+ * one RK4 step (classical 1/6 rule or the 3/8 rule) of dt/niter with
+ * H = H0 + H1 + H2. */
+#define VV_RK4_16 1
+#define VV_RK4_18 2
+int vv_rk4(vv_ctx *ctx, double *x, const double *h0, const double *h1, const double *h2,
+           int32_t rule, int32_t niter);
+
 /* PETScPoissonSolver.formRHS + the Poisson KSPSolve
  * (PETScPoissonSolver.pyx:58-74; run_base.py:689-690): solves A phi = b,
  * b_i = -(n_i - mean n) q, exactly on the periodic grid; phi is returned

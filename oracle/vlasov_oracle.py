@@ -331,6 +331,63 @@ class Dissipation:
         return double_bracket_jacobian(self.kind, bkind, u, fave, H, y, 0.5, grid, self.coll_freq)
 
 
+# --------------------------------------------------------------------------
+# explicit initial guess        run_base.py:786-927,
+#                               solvers/explicit/PETScArakawaSymplectic.pyx:62-97,
+#                               solvers/explicit/PETScArakawaRungeKutta.pyx:20-98
+# The reference's steppers cannot run as written (bracket arguments passed in
+# the wrong order, an accumulator never cleared, an undefined rk18_J4); these
+# are the schemes they state, with the time derivative of the implicit
+# residual, df/dt = -{f, H}, and each tendency formed afresh.
+# --------------------------------------------------------------------------
+def tendency(kind, x, h, grid):
+    """-{x, h} on rows 2..nv-3 (0 + (-1.0) * bracket), zero elsewhere."""
+    return bracket(kind, x, h, grid, factor=-1.0)
+
+
+def symplectic_kinetic(kind, x, h0, factor, grid, niter=1):
+    """PETScArakawaSymplectic.kinetic (:62-77): x += factor*ht/niter * (-{x, H0})."""
+    return x + factor * grid.ht / float(niter) * tendency(kind, x, h0, grid)
+
+
+def symplectic_potential(kind, x, h12, factor, grid, niter=1):
+    """PETScArakawaSymplectic.potential (:80-97): x += factor*ht/niter * (-{x, H1 + H2})."""
+    return x + factor * grid.ht / float(niter) * tendency(kind, x, h12, grid)
+
+
+def symplectic4_coefficients():
+    """run_base.py:892-901 (fourth-order Forest-Ruth / Yoshida composition)."""
+    fac2 = 2. ** (1. / 3.)
+    c1 = 0.5 / (2. - fac2)
+    c2 = c1 * (1. - fac2)
+    d1 = 1. / (2. - fac2)
+    d2 = -d1 * fac2
+    return (c1, c2, c2, c1), (d1, d2, d1)
+
+
+def rk4_16(kind, x, h, grid, niter=1):
+    """PETScArakawaRungeKutta.rk4_16 (:20-57), classical RK4, h = h0 + h1 + h2."""
+    s = grid.ht / float(niter)
+    t1 = tendency(kind, x, h, grid)
+    t2 = tendency(kind, x + 0.5 * grid.ht / float(niter) * t1, h, grid)
+    t3 = tendency(kind, x + 0.5 * grid.ht / float(niter) * t2, h, grid)
+    t4 = tendency(kind, x + 1.0 * grid.ht / float(niter) * t3, h, grid)
+    return x + s * (t1 + 2. * t2 + 2. * t3 + t4) / 6.
+
+
+def rk4_18(kind, x, h, grid, niter=1):
+    """PETScArakawaRungeKutta.rk4_18 (:61-98), the 3/8 rule, h = h0 + h1 + h2."""
+    s = grid.ht / float(niter)
+    t1 = tendency(kind, x, h, grid)
+    t2 = tendency(kind, x + 1. / 3. * grid.ht / float(niter) * t1, h, grid)
+    t3 = tendency(kind, x - 1. / 3. * grid.ht / float(niter) * t1 + s * t2, h, grid)
+    t4 = tendency(kind, x + s * t1 - s * t2 + s * t3, h, grid)
+    return x + s * (t1 + 3. * t2 + 3. * t3 + t4) / 8.
+
+
+GUESSES = ('symplectic2', 'symplectic4', 'rk4', 'rk4_16', 'rk4_18')
+
+
 def residual_diss(kind, f, fh, H, grid, diss, mom=None):
     """function_solver with its dissipation term (PETScNLVlasovMP.pyx:43-58)."""
     y = residual(kind, f, fh, H, grid)
@@ -815,10 +872,15 @@ class OracleRun:
     def __init__(self, grid, f0, bracket='arakawaj4', laplace='cfd4', charge=-1.0,
                  mass=1.0, ksp_rtol=1e-10, ksp_atol=1e-50, ksp_max_it=1000,
                  snes_atol=1e-10, snes_max_iter=20, restart=30, precond_variant='lapack',
-                 preconditioner='tensor_kinetic', dissipation=None):
+                 preconditioner='tensor_kinetic', dissipation=None, initial_guess=None, initial_iter=1):
         self.grid = grid
         self.kind = bracket
         self.diss = dissipation
+        if initial_guess not in (None, 'None', '') + GUESSES:
+            raise ValueError("initial guess %r" % (initial_guess,))
+        self.guess = initial_guess if initial_guess not in (None, 'None', '') else None
+        self.niter = int(initial_iter)
+        self.H = np.zeros((grid.nx, grid.nv))      # the solver's Have starts as a zero Vec
         self.laplace = laplace
         self.charge = charge
         self.ksp = dict(rtol=ksp_rtol, atol=ksp_atol, max_it=ksp_max_it, restart=restart)
@@ -891,6 +953,41 @@ class OracleRun:
             self.fc = self.fc - delta
         return its, reason
 
+    # --- explicit initial guess (run_base.py:786-927)
+    def _guess_iteration(self):
+        g, k = self.grid, self.kind
+        h0 = np.broadcast_to(self.h0[None, :], (g.nx, g.nv))
+        h12 = np.broadcast_to((self.h1h + self.h2h)[:, None], (g.nx, g.nv))
+        if self.guess == 'symplectic2':
+            self.fc = symplectic_kinetic(k, self.fc, h0, 0.5, g, self.niter)
+            self.fc = symplectic_potential(k, self.fc, h12, 1.0, g, self.niter)
+            self.fc = symplectic_kinetic(k, self.fc, h0, 0.5, g, self.niter)
+        elif self.guess == 'symplectic4':
+            c, d = symplectic4_coefficients()
+            for q in range(3):
+                self.fc = symplectic_kinetic(k, self.fc, h0, c[q], g, self.niter)
+                self.fc = symplectic_potential(k, self.fc, h12, d[q], g, self.niter)
+            self.fc = symplectic_kinetic(k, self.fc, h0, c[3], g, self.niter)
+        else:
+            # the solver's H = H0 + H1 + H2 of 2D Vecs (PETScArakawaRungeKutta.pyx:28-32)
+            h = (h0 + self.h1h[:, None]) + self.h2h[:, None]
+            rk = rk4_16 if self.guess == 'rk4_16' else rk4_18   # 'rk4' -> rk18 (run_base.py:825)
+            self.fc = rk(k, self.fc, h, g, self.niter)
+        self.calculate_moments()
+
+    def initial_guess(self):
+        """run_base.py:786-805: back up f, take niter explicit iterations
+        (moments after each), keep them only if the residual went down.  As in
+        the reference, a reverted guess leaves the guessed moments in place."""
+        fl = self.fc.copy()
+        prev = self.calculate_residual()
+        for _ in range(self.niter):
+            self._guess_iteration()
+        ig = self.calculate_residual()
+        if ig > prev:
+            self.fc = fl
+        return prev, ig
+
     def step(self):
         # make_history (run_base.py:749-763) + update_history (PETScVlasovSolver.pyx:135-139)
         self.fh = self.fc.copy()
@@ -901,9 +998,12 @@ class OracleRun:
         # update_history -> preconditioner.update_matrices (PETScVlasovSolver.pyx:135-139)
         if isinstance(self.pc, PotentialPreconditioner):
             self.pc.update(self.phi)
+        guess = self.initial_guess() if self.guess else None
         self.update_previous()
         pred = self.calculate_residual()
         entry = dict(initial=pred, picard=[])
+        if guess is not None:
+            entry['guess'] = guess
         i = 0
         while True:
             i += 1

@@ -25,6 +25,7 @@ LINEAR = {"A": 0, "A0": 1, "AT": 2, "A0T": 3}
 # double brackets (bracket_operator ffh / fhh), CollisionOperator.pyx:43-57,
 # DoubleBracket.pyx:43-56
 DISSIPATION = {"": 0, None: 0, "none": 0, "collt": 1, "colle": 2, "ffh": 3, "fhh": 4}
+RK4_RULES = {"rk4_16": 1, "rk4_18": 2}
 
 KSP_REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", -3: "DIVERGED_ITS", -4: "DIVERGED_DTOL",
                -5: "DIVERGED_BREAKDOWN", -9: "DIVERGED_NANORINF", 0: "ITERATING"}
@@ -96,6 +97,9 @@ SIGNATURES = {
     "vv_collision_add": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _D]),
     "vv_double_bracket_function": (ctypes.c_int, [_P, _P, _P, _D]),
     "vv_double_bracket_jacobian": (ctypes.c_int, [_P, _P, _P, _P, _D]),
+    "vv_symplectic_kinetic": (ctypes.c_int, [_P, _P, _P, _D, ctypes.c_int32]),
+    "vv_symplectic_potential": (ctypes.c_int, [_P, _P, _P, _P, _D, ctypes.c_int32]),
+    "vv_rk4": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
     "vv_poisson_solve": (ctypes.c_int, [_P, _P, _P]),
     "vv_poisson_matrix_mult": (ctypes.c_int, [_P, _P, _P]),
     "vv_poisson_function_mult": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -28,6 +28,7 @@ UNITS = {
     "krylov.hip": ["-ffp-contract=off"],
     "moments.hip": ["-ffp-contract=off"],
     "dissipation.hip": ["-ffp-contract=off"],
+    "explicit.hip": ["-ffp-contract=off"],
     "api.hip": [],
 }
 

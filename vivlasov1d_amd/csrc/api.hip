@@ -929,7 +929,7 @@ int vv_ctx_destroy(vv_ctx *c)
                     c->gm,     c->g0,     c->gp,      c->twiddle, c->twiddle32, c->spec, c->pinv, c->pwork,
                     c->X,      c->rhs,    c->D,       c->basis,   c->basis32, c->R,   c->partial, c->dscalar,
                     c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid, c->tb,     c->fave,    c->db1,
-                    c->db2};
+                    c->db2,    c->ex};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);
@@ -1212,6 +1212,80 @@ int vv_velocity_moments(vv_ctx *c, const double *f, double *u, double *e)
     if (int r = check_ctx(c)) return r;
     if (!f || !u || !e) return fail(c, VV_ERR_INVALID, "null argument");
     VV_CHECK(launch_velocity_moments(c, f, u, e), "velocity moments");
+    return VV_OK;
+}
+
+// ---- explicit steppers of the initial guess ---------------------------------
+namespace {
+int explicit_setup(vv_ctx *c, double *x, int32_t niter)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!x) return fail(c, VV_ERR_INVALID, "null field");
+    if (niter < 1) return fail(c, VV_ERR_INVALID, "niter must be >= 1");
+    if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the explicit steppers take a single-instance context");
+    if (!c->ex) VV_CHECK(dalloc(&c->ex, 6 * (size_t)c->N), "alloc explicit-stepper scratch");
+    return VV_OK;
+}
+
+// t = -{x, H} on rows 2..nv-3, 0 elsewhere
+hipError_t tendency(vv_ctx *c, const double *x, const double *H, double *t)
+{
+    return launch_bracket_acc(c, x, nullptr, H, t, -1.0, true, nullptr);
+}
+}  // namespace
+
+int vv_symplectic_kinetic(vv_ctx *c, double *x, const double *h0, double factor, int32_t niter)
+{
+    if (int r = explicit_setup(c, x, niter)) return r;
+    if (!h0) return fail(c, VV_ERR_INVALID, "null h0");
+    double *H = c->ex, *t = c->ex + c->N;
+    VV_CHECK(launch_compose_h(c, h0, nullptr, nullptr, H), "compose H");
+    VV_CHECK(tendency(c, x, H, t), "tendency");
+    VV_CHECK(launch_stage(c, x, t, factor * c->p.ht / (double)niter, x), "update");
+    return VV_OK;
+}
+
+int vv_symplectic_potential(vv_ctx *c, double *x, const double *h1, const double *h2, double factor, int32_t niter)
+{
+    if (int r = explicit_setup(c, x, niter)) return r;
+    if (!h1 || !h2) return fail(c, VV_ERR_INVALID, "null h1 / h2");
+    double *H = c->ex, *t = c->ex + c->N;
+    VV_CHECK(launch_compose_h(c, nullptr, h1, h2, H), "compose H");
+    VV_CHECK(tendency(c, x, H, t), "tendency");
+    VV_CHECK(launch_stage(c, x, t, factor * c->p.ht / (double)niter, x), "update");
+    return VV_OK;
+}
+
+int vv_rk4(vv_ctx *c, double *x, const double *h0, const double *h1, const double *h2, int32_t rule, int32_t niter)
+{
+    if (int r = explicit_setup(c, x, niter)) return r;
+    if (!h0 || !h1 || !h2) return fail(c, VV_ERR_INVALID, "null Hamiltonian part");
+    if (rule != VV_RK4_16 && rule != VV_RK4_18) return fail(c, VV_ERR_INVALID, "rule must be VV_RK4_16 or VV_RK4_18");
+    const size_t N = (size_t)c->N;
+    double *H = c->ex, *t1 = H + N, *t2 = t1 + N, *t3 = t2 + N, *t4 = t3 + N, *st = t4 + N;
+    const double ht = c->p.ht, ni = (double)niter, s = ht / ni;
+    VV_CHECK(launch_compose_h(c, h0, h1, h2, H), "compose H");
+    VV_CHECK(tendency(c, x, H, t1), "tendency");
+    if (rule == VV_RK4_16) {
+        // PETScArakawaRungeKutta.pyx:41-57
+        VV_CHECK(launch_stage(c, x, t1, 0.5 * ht / ni, st), "stage");
+        VV_CHECK(tendency(c, st, H, t2), "tendency");
+        VV_CHECK(launch_stage(c, x, t2, 0.5 * ht / ni, st), "stage");
+        VV_CHECK(tendency(c, st, H, t3), "tendency");
+        VV_CHECK(launch_stage(c, x, t3, 1.0 * ht / ni, st), "stage");
+        VV_CHECK(tendency(c, st, H, t4), "tendency");
+        VV_CHECK(launch_rk4_end(c, false, x, t1, t2, t3, t4, s), "update");
+    } else {
+        // PETScArakawaRungeKutta.pyx:82-98
+        const double c13 = 1. / 3. * ht / ni;
+        VV_CHECK(launch_stage(c, x, t1, c13, st), "stage");
+        VV_CHECK(tendency(c, st, H, t2), "tendency");
+        VV_CHECK(launch_rk18_stage3(c, x, t1, t2, c13, s, st), "stage");
+        VV_CHECK(tendency(c, st, H, t3), "tendency");
+        VV_CHECK(launch_rk18_stage4(c, x, t1, t2, t3, s, st), "stage");
+        VV_CHECK(tendency(c, st, H, t4), "tendency");
+        VV_CHECK(launch_rk4_end(c, true, x, t1, t2, t3, t4, s), "update");
+    }
     return VV_OK;
 }
 

@@ -139,6 +139,7 @@ struct vv_ctx {
     double *db1;        // {fave, H_ave}, valid while db1_valid
     double *db2;        // scratch bracket of the Jacobian
     bool db1_valid;
+    double *ex;         // explicit steppers: H | t1 | t2 | t3 | t4 | stage (6 N)
     std::string err;
 };
 
@@ -166,6 +167,15 @@ hipError_t launch_collision(vv_ctx *c, const double *f, const double *fscale, do
                             const double *U, const double *a, double factor, const int32_t *done);
 hipError_t launch_collision_factor(vv_ctx *c, const double *n, const double *u, const double *e, double *a);
 hipError_t launch_average(vv_ctx *c, const double *f, const double *fh, double *out);   // .5 fh + .5 f
+// explicit.hip: H = h0 | h1 + h2 | (h0 + h1) + h2 as a 2D field; stage combinations
+hipError_t launch_compose_h(vv_ctx *c, const double *h0, const double *h1, const double *h2, double *H);
+hipError_t launch_stage(vv_ctx *c, const double *x, const double *t, double s, double *out);   // x + s t
+hipError_t launch_rk18_stage3(vv_ctx *c, const double *x, const double *t1, const double *t2, double cc, double s,
+                              double *out);
+hipError_t launch_rk18_stage4(vv_ctx *c, const double *x, const double *t1, const double *t2, const double *t3,
+                              double s, double *out);
+hipError_t launch_rk4_end(vv_ctx *c, bool three_eighths, double *x, const double *t1, const double *t2,
+                          const double *t3, const double *t4, double s);
 // spectral.hip
 int fft_blocks(vv_ctx *c);
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done);

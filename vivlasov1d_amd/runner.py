@@ -5,8 +5,8 @@ vlasov/run/run_nonlinear_matrixfree_split_mp.py:15-174 together with the
 parts of ``viVlasov1Dbase`` (vlasov/run/run_base.py:35-807) the timestep
 uses: initial data and normalisation, moments + Poisson, residual, history,
 and the Picard loop with its stop / revert / abort rules, the collision /
-double-bracket dissipation terms the configuration selects, and (optional)
-HDF5 output in the reference's schema.  The other run modes are not part of
+double-bracket dissipation terms and the explicit initial guess the
+configuration selects, and (optional) HDF5 output in the reference's schema.  The other run modes are not part of
 this library.
 """
 
@@ -19,7 +19,8 @@ import torch
 
 from . import _lib
 from .core import Config, Grid, initial_distribution, kinetic_hamiltonian, normalise
-from .solvers import (Context, PETScPoissonSolver, PETScVlasovSolver, SNES, Toolbox)
+from .solvers import (Context, PETScArakawaRungeKutta, PETScArakawaSymplectic, PETScPoissonSolver,
+                      PETScVlasovSolver, SNES, Toolbox)
 
 
 class ResidualTooLarge(RuntimeError):
@@ -88,6 +89,22 @@ class Runscript:
         self.precision = precision
         self.snes_atol = sol['petsc_snes_atol']
         self.snes_max_iter = sol['petsc_snes_max_iter']
+        # initial guess (run_base.py:148, 171-183, 368-372): the steppers see the
+        # history potentials h1h, h2h; 'rk4' is the 3/8 rule the reference's
+        # initial_guess_rk4 names (rk18, :825)
+        self.nInitial = int(sol.get('initial_iter', 1) or 1)
+        self.arakawa_rk4 = PETScArakawaRungeKutta(cfg, None, grid, self.h0, self.h1h, self.h2h, self.nInitial,
+                                                  context=ctx)
+        self.arakawa_symplectic = PETScArakawaSymplectic(cfg, None, grid, self.h0, self.h1h, self.h2h,
+                                                         self.nInitial, context=ctx)
+        guesses = {None: None, 'None': None, '': None, 'rk4': self.initial_guess_rk4,
+                   'rk4_18': self.initial_guess_rk4, 'rk4_16': self.initial_guess_rk4_16,
+                   'symplectic2': self.initial_guess_symplectic2, 'symplectic4': self.initial_guess_symplectic4}
+        ig = sol.get('initial_guess')
+        if ig not in guesses:
+            raise _lib.VVError("initial guess %r is not provided (rk4, rk4_16, rk4_18, symplectic2, symplectic4)"
+                               % (ig,))
+        self.initial_guess_method = guesses[ig]
         self.itime = 0
         self.history = []
         self.log = log
@@ -127,14 +144,62 @@ class Runscript:
                 his.copy_(cur)
         self.vlasov_solver.update_history(self.fc)
 
+    # run_base.py:786-927 --------------------------------------------------
+    def initial_guess(self):
+        """Back up f, take nInitial explicit iterations (moments after each)
+        and keep them only if the residual went down.  As in the reference, an
+        undone guess leaves its moments in place until the Picard loop's."""
+        if self.initial_guess_method is None:
+            return None
+        self.fl.copy_(self.fc)
+        prev = self.calculate_residual()
+        self.initial_guess_method()
+        ig = self.calculate_residual()
+        if ig > prev:
+            self.fc.copy_(self.fl)
+        return prev, ig
+
+    def initial_guess_rk4(self):
+        for _ in range(self.nInitial):
+            self.arakawa_rk4.rk4_18(self.fc)
+            self.calculate_moments()
+
+    def initial_guess_rk4_16(self):
+        for _ in range(self.nInitial):
+            self.arakawa_rk4.rk4_16(self.fc)
+            self.calculate_moments()
+
+    def initial_guess_symplectic2(self):
+        for _ in range(self.nInitial):
+            self.arakawa_symplectic.kinetic(self.fc, 0.5)
+            self.arakawa_symplectic.potential(self.fc, 1.0)
+            self.arakawa_symplectic.kinetic(self.fc, 0.5)
+            self.calculate_moments()
+
+    def initial_guess_symplectic4(self):
+        fac2 = 2. ** (1. / 3.)
+        c1 = 0.5 / (2. - fac2)
+        c2 = c1 * (1. - fac2)
+        d1 = 1. / (2. - fac2)
+        d2 = - d1 * fac2
+        for _ in range(self.nInitial):
+            for c, d in ((c1, d1), (c2, d2), (c2, d1)):
+                self.arakawa_symplectic.kinetic(self.fc, c)
+                self.arakawa_symplectic.potential(self.fc, d)
+            self.arakawa_symplectic.kinetic(self.fc, c1)
+            self.calculate_moments()
+
     def step(self):
         """One timestep (run_nonlinear_matrixfree_split_mp.py:112-171)."""
         self.itime += 1
         self.make_history()
         # calculate_external: no external potential -> h2c stays 0 (run_base.py:707-725)
+        guess = self.initial_guess()
         self.vlasov_solver.update_previous(self.fc)
         pred = self.calculate_residual()
         rec = dict(step=self.itime, initial=pred, picard=[])
+        if guess is not None:
+            rec['guess'] = guess
         i = 0
         while True:
             i += 1

@@ -242,6 +242,21 @@ class Context:
     def double_bracket_jacobian(self, u, fave, y, factor):
         self.call("vv_double_bracket_jacobian", self.F(u), self.F(fave), self.F(y), ctypes.c_double(factor))
 
+    # explicit steppers of the initial guess (run_base.py:786-927) ----------
+    def symplectic_kinetic(self, x, h0, factor, niter=1):
+        """x += factor*ht/niter * (-{x, H0})  (PETScArakawaSymplectic.pyx:62-77)."""
+        self.call("vv_symplectic_kinetic", self.F(x), self.V(h0), ctypes.c_double(factor), ctypes.c_int32(niter))
+
+    def symplectic_potential(self, x, h1, h2, factor, niter=1):
+        """x += factor*ht/niter * (-{x, H1 + H2})  (PETScArakawaSymplectic.pyx:80-97)."""
+        self.call("vv_symplectic_potential", self.F(x), self.X(h1), self.X(h2), ctypes.c_double(factor),
+                  ctypes.c_int32(niter))
+
+    def rk4(self, x, h0, h1, h2, rule='rk4_16', niter=1):
+        """One RK4 step of dt/niter with H = H0 + H1 + H2 (PETScArakawaRungeKutta.pyx:20-98)."""
+        self.call("vv_rk4", self.F(x), self.V(h0), self.X(h1), self.X(h2), ctypes.c_int32(_lib.RK4_RULES[rule]),
+                  ctypes.c_int32(niter))
+
     def poisson_solve(self, n, phi):
         self.call("vv_poisson_solve", self.X(n), self.X(phi))
 
@@ -460,6 +475,42 @@ class DoubleBracket:
     def jacobian(self, F, Fave, Have, Y, factor):
         if self.kind is not None:
             self.ctx.double_bracket_jacobian(F, Fave, Y, factor)
+
+
+class PETScArakawaSymplectic:
+    """PETScArakawaSymplectic (solvers/explicit/PETScArakawaSymplectic.pyx:17-97)
+    on the GPU.  H0 is the kinetic Hamiltonian's v-vector, H1 and H2 the
+    x-vectors of the potentials (kept by reference, as the reference keeps
+    its Vecs); niter divides the time step."""
+
+    def __init__(self, config, da1, grid, H0, H1, H2, niter=1, context=None):
+        self.grid, self.niter = grid, int(niter)
+        self.H0, self.H1, self.H2 = H0, H1, H2
+        self.ctx = context if context is not None else Context(grid, bracket=config.get_poisson_bracket(),
+                                                               preconditioner='')
+
+    def kinetic(self, X, factor=0.5):
+        self.ctx.symplectic_kinetic(X, self.H0, factor, self.niter)
+
+    def potential(self, X, factor=1.0):
+        self.ctx.symplectic_potential(X, self.H1, self.H2, factor, self.niter)
+
+
+class PETScArakawaRungeKutta:
+    """PETScArakawaRungeKutta (solvers/explicit/PETScArakawaRungeKutta.pyx:15-98)
+    on the GPU: one RK4 step of dt/niter with H = H0 + H1 + H2."""
+
+    def __init__(self, config, da1, grid, H0, H1, H2, niter=1, context=None):
+        self.grid, self.niter = grid, int(niter)
+        self.H0, self.H1, self.H2 = H0, H1, H2
+        self.ctx = context if context is not None else Context(grid, bracket=config.get_poisson_bracket(),
+                                                               preconditioner='')
+
+    def rk4_16(self, X, H1=None):
+        self.ctx.rk4(X, self.H0, self.H1, self.H2, 'rk4_16', self.niter)
+
+    def rk4_18(self, X, H1=None):
+        self.ctx.rk4(X, self.H0, self.H1, self.H2, 'rk4_18', self.niter)
 
 
 class TimeDerivative:
