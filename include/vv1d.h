@@ -188,6 +188,12 @@ int vv_jacobian(vv_ctx *ctx, const double *u, double *y);
 int vv_function_norm(vv_ctx *ctx, const double *f, const double *fh, double *y,
                      double *out);
 
+/* run_base.calculate_residual (run_base.py:631-637) in one host read:
+ * y = F(f), *fnorm = ||y||_2 and *pnorm = ||A phi + (n - mean n) q||_2 of
+ * the last vv_moments call (per instance in a batched context). */
+int vv_residual_norms(vv_ctx *ctx, const double *f, const double *fh, double *y,
+                      double *fnorm, double *pnorm);
+
 /* Toolbox.compute_density (vlasov/toolbox/Toolbox.pyx:79-98):
  * n_i = hv * sum_j f_ij. */
 int vv_density(vv_ctx *ctx, const double *f, double *n);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "vv_function": (ctypes.c_int, [_P, _P, _P, _P]),
     "vv_jacobian": (ctypes.c_int, [_P, _P, _P]),
     "vv_function_norm": (ctypes.c_int, [_P, _P, _P, _P, _PD]),
+    "vv_residual_norms": (ctypes.c_int, [_P, _P, _P, _P, _PD, _PD]),
     "vv_density": (ctypes.c_int, [_P, _P, _P]),
     "vv_velocity_moments": (ctypes.c_int, [_P, _P, _P, _P]),
     "vv_set_dissipation": (ctypes.c_int, [_P, ctypes.POINTER(Dissipation)]),

@@ -430,8 +430,11 @@ int copy_status(vv_ctx *c, int slot)
 // steps (the status copy of step k is read while steps k+1, k+2 run); returns
 // once the cycle has finished on the device for every instance, with the
 // statuses in host slot 0.  Steps enqueued past the end of an instance's
-// cycle return at once on the device.
-int run_cycle(vv_ctx *c, bool fp32)
+// cycle return at once on the device.  build: x += V y is enqueued before
+// the final drain (FP64 x, from the FP32 basis when fp32); update: then
+// f -= x for instances whose solve converged (SNESSolve_KSPONLY's step), so
+// one drain ends the solve.
+int run_cycle(vv_ctx *c, bool fp32, double *build = nullptr, double *update = nullptr)
 {
     const int max_k = c->p.restart;
     const int chunk = 2;
@@ -441,10 +444,13 @@ int run_cycle(vv_ctx *c, bool fp32)
     while (it < max_k && !stop) {
         const int end = it + chunk < max_k ? it + chunk : max_k;
         for (; it < end; ++it) {
+            // the chunk's last Hessenberg update stores the statuses into the
+            // host slot itself (no copy dispatch between Arnoldi steps)
+            if (it == end - 1) c->status_out = c->host_status_dev + (size_t)slot * c->B;
             int r = fp32 ? arnoldi_step32(c, it) : arnoldi_step(c, it);
+            c->status_out = nullptr;
             if (r) return r;
         }
-        if (int r = copy_status(c, slot)) return r;
         VV_CHECK(hipEventRecord(c->ev[slot], c->stream), "event");
         if (pend >= 0) {
             VV_CHECK(hipEventSynchronize(c->ev[pend]), "event sync");
@@ -452,6 +458,11 @@ int run_cycle(vv_ctx *c, bool fp32)
         }
         pend = slot;
         slot ^= 1;
+    }
+    if (build) {
+        if (fp32) VV_CHECK(launch_build_solution32(c, build), "build solution");
+        else VV_CHECK(launch_build_solution(c, build), "build solution");
+        if (update) VV_CHECK(launch_axpy_cond(c, -1.0, build, update), "update");
     }
     if (int r = copy_status(c, 0)) return r;
     VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
@@ -499,10 +510,7 @@ int restart_cycles(vv_ctx *c, const double *b, double *x)
         if (int r = apply_pc(c, c->X, c->basis, nullptr)) return r;
         VV_CHECK(launch_residual_restart(c, b, c->basis), "restart residual");
         VV_CHECK(launch_norm_start(c), "cycle start");
-        if (int r = run_cycle(c, false)) return r;
-        VV_CHECK(launch_build_solution(c, x), "build solution");
-        if (int r = copy_status(c, 0)) return r;
-        VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+        if (int r = run_cycle(c, false, x)) return r;
     }
 }
 
@@ -513,10 +521,7 @@ int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *k
     VV_CHECK(launch_zero(c, x, c->N * c->B), "zero x");
     VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
     if (int r = cycle_start_from(c, b)) return r;
-    if (int r = run_cycle(c, false)) return r;
-    VV_CHECK(launch_build_solution(c, x), "build solution");
-    if (int r = copy_status(c, 0)) return r;
-    VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+    if (int r = run_cycle(c, false, x)) return r;
     if (int r = restart_cycles(c, b, x)) return r;
     fill_results(c, res);
     return VV_OK;
@@ -593,10 +598,14 @@ int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, v
         VV_CHECK(launch_zero(c, c->D, c->N * c->B), "zero x");
         VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
         if (int r = cycle_start_from(c, c->rhs)) return r;
-        if (int r = run_cycle(c, false)) return r;
-        VV_CHECK(launch_build_solution(c, c->D), "build solution");
-        if (int r = copy_status(c, 0)) return r;
-        VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
+        // one instance: the step f -= D is enqueued with the solution before
+        // the drain (a no-op unless the cycle converged; a restart leaves it
+        // to the update below)
+        if (int r = run_cycle(c, false, c->D, c->B == 1 ? f : nullptr)) return r;
+        if (c->B == 1 && status_slot(c, 0)[0].reason != 0) {
+            fill_results(c, res);
+            return VV_OK;
+        }
     }
     if (int r = restart_cycles(c, c->rhs, c->D)) return r;
     // SNESSolve_KSPONLY takes the step unless the linear solve diverged
@@ -636,8 +645,7 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
         // one FP32 cycle with the KSP's own tolerances; its reason stands
         VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
         VV_CHECK(launch_start32(c, b), "cycle start");
-        if (int r = run_cycle(c, true)) return r;
-        VV_CHECK(launch_build_solution32(c, x), "build solution");
+        if (int r = run_cycle(c, true, x)) return r;
         const GmresStatus st = status_slot(c, 0)[0];
         if (st.reason != 0) {
             fill_results(c, res);
@@ -683,10 +691,9 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
         if (rt < mp->inner_rtol) rt = mp->inner_rtol;
         VV_CHECK(launch_gmres_init(c, rt, 0.0, 1.0e300, kp->max_it - its), "gmres init");
         VV_CHECK(launch_start32(c, rsrc), "cycle start");
-        if (int r = run_cycle(c, true)) return r;
+        if (int r = run_cycle(c, true, x)) return r;
         its += status_slot(c, 0)[0].it;
         ++rounds;
-        VV_CHECK(launch_build_solution32(c, x), "build solution");
         // true FP64 residual r = b - P^-1 J x
         VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
         if (int r = precond(c, c->X, c->R, nullptr)) return r;
@@ -821,6 +828,8 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(dalloc(&c->gs, Bz), "alloc gmres state");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_status), 2 * Bz * sizeof(GmresStatus)),
               "pinned status");
+    VV_CCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->host_status_dev), c->host_status, 0),
+              "pinned status device address");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_scalars), ds_count(B) * sizeof(double)),
               "pinned scalars");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_active), Bz * sizeof(int32_t)), "pinned mask");
@@ -1197,6 +1206,25 @@ int vv_function_norm(vv_ctx *c, const double *f, const double *fh, double *y, do
     if (int r = diss_function(c, f, fh, y)) return r;
     VV_CHECK(launch_norms(c, y, ds_norms(c)), "norm");
     return read_scalars(c, ds_norms(c), (size_t)c->B, out);
+}
+
+int vv_residual_norms(vv_ctx *c, const double *f, const double *fh, double *y, double *fnorm, double *pnorm)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!f || !fh || !y || !fnorm || !pnorm) return fail(c, VV_ERR_INVALID, "null argument");
+    if (y == f || y == fh) return fail(c, VV_ERR_INVALID, "residual_norms: y must not alias f or fh");
+    VV_PLAUNCH(P_RES, 3 * b_field(c), -1, launch_residual(c, f, fh, y), "residual");
+    if (int r = diss_function(c, f, fh, y)) return r;
+    VV_CHECK(launch_norms(c, y, ds_norms(c)), "norm");
+    // ds_norms (B values) is followed by the Poisson residuals ds_pres (stride 16)
+    const size_t B = (size_t)c->B;
+    std::vector<double> tmp(B + 16 * B);
+    if (int r = read_scalars(c, ds_norms(c), B + 16 * (B - 1) + 1, tmp.data())) return r;
+    for (size_t b = 0; b < B; ++b) {
+        fnorm[b] = tmp[b];
+        pnorm[b] = tmp[B + 16 * b];
+    }
+    return VV_OK;
 }
 
 int vv_density(vv_ctx *c, const double *f, double *n)

@@ -413,12 +413,30 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 // column and the previous rotations are staged in LDS by the whole block
 // (loads in flight together with the norm's partials), so the sequential
 // rotation chain of thread 0 reads LDS, not dependent global loads.
+// hout (optional, pinned host memory mapped for the device): the instance's
+// status is also stored there, so the host polls it without a copy.
+__device__ __forceinline__ void put_status(GmresStatus *hout, const GmresStatus &st)
+{
+    // per-field vector stores by one lane
+    hout->it = st.it;
+    hout->its = st.its;
+    hout->reason = st.reason;
+    hout->done = st.done;
+    hout->active = st.active;
+    hout->res = st.res;
+    hout->rnorm0 = st.rnorm0;
+}
+
 __global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, int nb, int64_t pstride, int it,
-                                                          GmresState *g)
+                                                          GmresState *g, GmresStatus *hout)
 {
     g += blockIdx.y;
     part += blockIdx.y * pstride;
-    if (g->st.done) return;
+    if (hout) hout += blockIdx.y;
+    if (g->st.done) {
+        if (hout && threadIdx.x == 0) put_status(hout, g->st);
+        return;
+    }
     __shared__ double red[NW + 1];
     __shared__ double s_h[kMaxRestart + 2], s_c[kMaxRestart], s_s[kMaxRestart];
     double *hh = g->hh + (size_t)it * (kMaxRestart + 1);
@@ -471,6 +489,7 @@ __global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, in
     if (reason == 0 && g->st.its >= g->max_it) reason = VV_KSP_DIVERGED_ITS;
     g->st.reason = reason;
     if (reason != 0 || g->st.it >= g->max_k || g->st.its >= g->max_it) g->st.done = 1;
+    if (hout) put_status(hout, g->st);
 }
 
 // KSPGMRESBuildSoln: back substitution with the rotated Hessenberg matrix;
@@ -734,7 +753,7 @@ hipError_t maxpy_norm_t(vv_ctx *c, const S *basis, int it)
     hipError_t e = cgs_dispatch(c, basis, 1, it, nullptr);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(gmres_update_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial + norm_partial_offset(c),
-                       cgs_blocks_v(c, c->N / V16<S>::L), c->part_stride, it, c->gs);
+                       cgs_blocks_v(c, c->N / V16<S>::L), c->part_stride, it, c->gs, c->status_out);
     return hipGetLastError();
 }
 

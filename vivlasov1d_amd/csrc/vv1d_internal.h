@@ -107,6 +107,8 @@ struct vv_ctx {
     double *npart;      // density partial sums, 64 nx per instance
     vv::GmresState *gs;             // B states
     vv::GmresStatus *host_status;   // pinned, 2 slots of B statuses
+    vv::GmresStatus *host_status_dev;   // the same memory as the device addresses it
+    vv::GmresStatus *status_out;    // set for one launch: the Hessenberg update also stores the statuses there
     double *host_scalars;           // pinned staging of device scalars
     int32_t *host_active;           // pinned staging of the instance mask
     hipEvent_t ev[2];

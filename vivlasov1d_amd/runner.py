@@ -65,7 +65,6 @@ class Runscript:
         if normalise_f0:
             f0 = normalise(f0, grid)
         self.fc.copy_(torch.from_numpy(f0))
-        self.pnorm = 0.0
         self.calculate_moments()
         # solver objects (run_nonlinear_matrixfree_split_mp.py:26-80)
         self.vlasov_solver = PETScVlasovSolver(cfg, None, None, grid, self.h0, self.h1c, self.h1h, self.h2c,
@@ -122,13 +121,14 @@ class Runscript:
     def calculate_residual(self):
         """||F(fc)||_2 + ||A phi + (n - mean n) q||_2.  The Poisson part is the
         residual of the last moments/Poisson solve, which the fused moments
-        kernel returns (the reference recomputes it from the same phi, n)."""
-        fnorm = self.vlasov_solver.function_norm(self.fc, self.fb)
-        return fnorm + self.pnorm
+        kernel leaves on the device (the reference recomputes it from the same
+        phi, n); both norms come back in one read."""
+        fnorm, pnorm = self.ctx.residual_norms(self.fc, self.vlasov_solver.Fh, self.fb)
+        return fnorm + pnorm
 
     # run_base.py:640-704, 741-742 -------------------------------------------
     def calculate_moments(self):
-        self.pnorm = self.ctx.moments(self.fc, self.N, self.pc_int, self.h1c, want_norm=True)
+        self.ctx.moments(self.fc, self.N, self.pc_int, self.h1c, want_norm=False)
         if self.collisions:
             self.ctx.velocity_moments(self.fc, self.U, self.E)
             self.ctx.collision_factor(self.N, self.U, self.E, self.A)

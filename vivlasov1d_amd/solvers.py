@@ -202,6 +202,12 @@ class Context:
         self.call("vv_function_norm", self.F(f), self.F(fh), self.F(y), out)
         return self._per_instance(out)
 
+    def residual_norms(self, f, fh, y):
+        """(||F(f)||_2, Poisson residual norm of the last moments call) in one read."""
+        fo, po = (ctypes.c_double * self.B)(), (ctypes.c_double * self.B)()
+        self.call("vv_residual_norms", self.F(f), self.F(fh), self.F(y), fo, po)
+        return self._per_instance(fo), self._per_instance(po)
+
     def density(self, f, n):
         self.call("vv_density", self.F(f), self.X(n))
 
