@@ -138,23 +138,35 @@ struct FftTw {
     }
 };
 
+// Where the last stage of lds_fft puts its outputs: back into the LDS buffer
+// (the default), or wherever a kernel wants them (the irFFT stores them to
+// its output rows directly).  pos = transform * n + natural index.
+template <typename C>
+struct LdsSink {
+    C *buf;
+    __device__ __forceinline__ void operator()(int pos, C v) const { buf[pos] = v; }
+};
+
 // Batched in-LDS Stockham FFT of `nb` transforms of length n = 2^logn stored
 // back to back in buf.  Forward uses exp(-2 pi i m/n), inverse the conjugate
 // (unnormalised).  Each stage loads every butterfly of the thread into
 // registers, synchronises, then stores, so one buffer suffices.  C is
 // double2 (FP64 path) or float2 (FP32 preconditioner of the mixed solver).
 // PRE: twiddles from pt (n <= 4 NT), else from the global table tw.
+// s0: first radix-4 stage to run (1 when the caller did stage 0 in
+// registers); the last stage's outputs go to `last`.
 // Inlined so that buf is known to be LDS (ds_read/ds_write, not flat
 // accesses); index arithmetic by shifts (n is a power of two).
-template <bool INV, bool PRE, typename C>
+template <bool INV, bool PRE, typename C, typename Sink>
 __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw,
-                                        const FftTw<INV, C> &pt)
+                                        const FftTw<INV, C> &pt, int s0, const Sink &last)
 {
     const int nq = n >> 2, lq = logn - 2, ns = logn >> 1;
     const int total4 = nb * nq;
 #pragma unroll
     for (int s = 0; s < MAXS; ++s) {
         if (s >= ns) break;                // uniform over the block
+        if (s < s0) continue;
         const int Ns = 1 << (2 * s);
         C v[MAXE / 4 / NT][4];
         int dst[MAXE / 4 / NT];
@@ -193,11 +205,15 @@ __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C
             }
         }
         __syncthreads();
+        const bool fin = (s == ns - 1) && !(logn & 1);
 #pragma unroll
         for (int q = 0; q < MAXE / 4 / NT; ++q) {
             if (dst[q] >= 0) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) buf[dst[q] + r * Ns] = v[q][r];
+                for (int r = 0; r < 4; ++r) {
+                    if (fin) last(dst[q] + r * Ns, v[q][r]);
+                    else buf[dst[q] + r * Ns] = v[q][r];
+                }
             }
         }
         __syncthreads();
@@ -229,8 +245,8 @@ __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C
 #pragma unroll
         for (int q = 0; q < MAXE / 2 / NT; ++q) {
             if (dst[q] >= 0) {
-                buf[dst[q]] = v[q][0];
-                buf[dst[q] + Ns] = v[q][1];
+                last(dst[q], v[q][0]);
+                last(dst[q] + Ns, v[q][1]);
             }
         }
         __syncthreads();
@@ -267,31 +283,48 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
     // XCD-aware row-block order: the mode-major segments of neighbouring row
     // blocks share L2 lines, so they are kept on one XCD
     const int j0 = (VV_FFT_XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * R;
-    // every global load of the thread is issued before the LDS stores
-    constexpr int PER = 2 * MAXE / NT;   // R * nx <= 2 * MAXE real inputs
-    T val[PER];
+    // Stage 0 of the FFT runs in registers: butterfly (p, j) of complex row p
+    // (real rows 2p, 2p+1) takes z_p[j + m nx/4], m = 0..3, exactly the
+    // elements the thread loads (coalesced along j); its twiddles are 1.
+    // Every global load of the thread is issued before anything waits.
+    constexpr int QB = MAXE / 4 / NT;    // butterflies per thread
+    const int nq = nx >> 2, lq = a.logn - 2;
+    const int total4 = (R >> 1) * nq;
+    T xr[QB][4], xi[QB][4];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int idx = threadIdx.x + q * NT;
-        const int r = idx >> a.logn;
-        const int i = idx & (nx - 1);
-        val[q] = (idx < R * nx && j0 + r < a.nv) ? x[(size_t)(j0 + r) * nx + i] : T(0);
+    for (int q = 0; q < QB; ++q) {
+        const int bq = threadIdx.x + q * NT;
+        const int p = bq >> lq, j = bq & (nq - 1);
+        const bool ok = bq < total4;
+        const bool r0 = ok && j0 + 2 * p < a.nv, r1 = ok && j0 + 2 * p + 1 < a.nv;
+        const T *x0 = x + (size_t)(j0 + 2 * p) * nx + j;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            xr[q][m] = r0 ? x0[m * nq] : T(0);
+            xi[q][m] = r1 ? x0[nx + m * nq] : T(0);
+        }
     }
     FftTw<false, C> pt;
     if (PRE) pt.load(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int idx = threadIdx.x + q * NT;
-        if (idx < R * nx) {
-            const int r = idx >> a.logn;
-            const int i = idx & (nx - 1);
-            T *slot = reinterpret_cast<T *>(&buf[(r >> 1) * nx + i]);
-            slot[r & 1] = val[q];
+    for (int q = 0; q < QB; ++q) {
+        const int bq = threadIdx.x + q * NT;
+        if (bq < total4) {
+            const int p = bq >> lq, j = bq & (nq - 1);
+            const C v0 = cmk<C>(xr[q][0], xi[q][0]), v1 = cmk<C>(xr[q][1], xi[q][1]);
+            const C v2 = cmk<C>(xr[q][2], xi[q][2]), v3 = cmk<C>(xr[q][3], xi[q][3]);
+            const C a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), d = csub(v1, v3);
+            const C a3 = cmk<C>(d.y, -d.x);             // (v1 - v3) * (-i)
+            C *z = buf + (p << a.logn) + (j << 2);      // stage 0 outputs: 4j + r
+            z[0] = cadd(a0, a2);
+            z[1] = cadd(a1, a3);
+            z[2] = csub(a0, a2);
+            z[3] = csub(a1, a3);
         }
     }
     __syncthreads();
-    lds_fft<false, PRE>(buf, nx, a.logn, R >> 1, a.tw, pt);
+    lds_fft<false, PRE>(buf, nx, a.logn, R >> 1, a.tw, pt, 1, LdsSink<C>{buf});
     const int nk = (nx >> 1) + 1;
     const T half = T(0.5);
     for (int idx = threadIdx.x; idx < nk * R; idx += NT) {
@@ -307,6 +340,22 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         spec[(size_t)k * a.nv + j0 + r] = out;
     }
 }
+
+// Real rows 2b, 2b+1 of an irFFT workgroup from its complex transform b
+// (backward-normalised by 1/nx, exact: nx is a power of two).
+template <typename T>
+struct IrfftSink {
+    T *y;
+    int j0, nx, logn, nv;
+    __device__ __forceinline__ void operator()(int pos, cx_t<T> z) const
+    {
+        const int b = pos >> logn, i = pos & (nx - 1);
+        const T inv_n = T(1) / (T)nx;
+        const int r = j0 + 2 * b;
+        if (r < nv) y[(size_t)r * nx + i] = z.x * inv_n;
+        if (r + 1 < nv) y[(size_t)(r + 1) * nx + i] = z.y * inv_n;
+    }
+};
 
 template <typename T, bool PRE>
 __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
@@ -353,16 +402,8 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         }
     }
     __syncthreads();
-    lds_fft<true, PRE>(buf, nx, a.logn, npair, a.tw, pt);
-    const T inv_n = T(1) / (T)nx;   // exact: nx is a power of two
-    for (int idx = threadIdx.x; idx < R * nx; idx += NT) {
-        const int r = idx >> a.logn;
-        const int i = idx & (nx - 1);
-        if (j0 + r < a.nv) {
-            const C z = buf[(r >> 1) * nx + i];
-            y[(size_t)(j0 + r) * nx + i] = ((r & 1) ? z.y : z.x) * inv_n;
-        }
-    }
+    // the last stage's outputs (natural order) go straight to the output rows
+    lds_fft<true, PRE>(buf, nx, a.logn, npair, a.tw, pt, 0, IrfftSink<T>{y, j0, nx, a.logn, a.nv});
 }
 
 struct TriArgs {
