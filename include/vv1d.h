@@ -49,7 +49,7 @@ typedef struct vv_ctx vv_ctx;
  * (vlasov/solvers/preconditioner/TensorProduct.pyx:66-86) */
 #define VV_PRECOND_NONE           0
 #define VV_PRECOND_TENSOR_KINETIC 1
-#define VV_PRECOND_TENSOR_POTENTIAL 2   /* FFT along v x tridiagonal along x; nv a power of two */
+#define VV_PRECOND_TENSOR_POTENTIAL 2   /* FFT along v x tridiagonal along x */
 
 /* Laplacian: poisson module PETScPoissonCFD2 | PETScPoissonCFD4
  * (vlasov/run/run_base.py:93-94, 115) */
@@ -67,7 +67,8 @@ typedef struct vv_ctx vv_ctx;
 /* Grid and operator selection.  Mirrors Grid.create
  * (vlasov/core/Grid.pyx:14-51) and the solver options of run_base.py:35-196. */
 typedef struct {
-    int32_t nx;            /* points in x (periodic); power of two, 8..4096   */
+    int32_t nx;            /* points in x (periodic); 8..4096 (any length:    */
+                           /* FFT for powers of two, else a direct DFT)      */
     int32_t nv;            /* points in v; 8..4096                            */
     double  hx;            /* L / nx                                          */
     double  hv;            /* (vmax - vmin) / (nv - 1)                        */

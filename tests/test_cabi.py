@@ -44,7 +44,7 @@ def test_binding_covers_the_header():
 
 
 @pytest.mark.parametrize("field,value,msg", [
-    ("nx", 100, "power of two"), ("nv", 4, "nv must be"), ("bracket", 3, "bracket"),
+    ("nx", 5000, "nx must be"), ("nv", 4, "nv must be"), ("bracket", 3, "bracket"),
     ("laplace", 6, "laplace"), ("restart", 0, "restart"), ("precond", 7, "preconditioner")])
 def test_create_rejects_bad_parameters(lib, field, value, msg):
     from vivlasov1d_amd import _lib

@@ -147,14 +147,7 @@ def test_gmres_with_dissipation_matches_oracle(dkind, cuda):
     assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
 
 
-_FFH_FAULT = ("one run of this case ended in an illegal memory access while an experimental fused "
-              "preconditioner (since removed) was in the library; the cause was not identified, so the case "
-              "is not run (the double bracket's residual, Jacobian and GMRES tests above cover ffh)")
-
-
-@pytest.mark.parametrize("dkind,nu", [('collt', 0.05), ('colle', 0.05),
-                                      pytest.param('ffh', 0.01, marks=pytest.mark.skip(reason=_FFH_FAULT)),
-                                      ('fhh', 0.01)])
+@pytest.mark.parametrize("dkind,nu", [('collt', 0.05), ('colle', 0.05), ('ffh', 0.01), ('fhh', 0.01)])
 def test_picard_with_dissipation_matches_oracle(dkind, nu, cuda):
     """Three implicit timesteps of a Landau run with the dissipation term the
     configuration selects (config.py:148-163), against the oracle's run."""

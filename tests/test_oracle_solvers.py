@@ -13,11 +13,16 @@ from oracle import vlasov_oracle as O
 # ---------------------------------------------------------------- preconditioner
 def test_eigenvalues_are_centered_difference_symbol():
     """ifftshift(exp(2 pi i k (n-1)/n) - exp(2 pi i k/n)) = 2 i sin(2 pi k/n)
-    for even n (the symbol of f_{i+1} - f_{i-1})."""
-    for n in (8, 64, 1024):
+    for even n (the symbol of f_{i+1} - f_{i-1}); for odd n ifftshift
+    rotates by (n-1)/2, which gives 2 i sin(pi (2k - 1)/n)."""
+    for n in (8, 10, 64, 100, 1024):
         e = O.kinetic_eigenvalues(n)
         k = np.arange(n)
         np.testing.assert_allclose(e, 2j * np.sin(2 * np.pi * k / n), rtol=0, atol=1e-11)
+    for n in (9, 51, 201):
+        e = O.kinetic_eigenvalues(n)
+        k = np.arange(n)
+        np.testing.assert_allclose(e, 2j * np.sin(np.pi * (2 * k - 1) / n), rtol=0, atol=1e-11)
 
 
 def test_lapack_and_scipy_statements_agree():
@@ -34,8 +39,26 @@ def test_pivots_identity_at_benchmark_sizes():
         assert O.Preconditioner(O.Grid(nx, nv, L, vmax, ht=ht)).pivots_are_identity()
 
 
-def test_p_times_pinv_is_identity():
-    g = O.Grid(8, 16, 2 * np.pi, 4.0, ht=0.1)
+def test_odd_length_preconditioner_is_a_modewise_dense_solve():
+    """Odd nx: the reference's eigenvalues make T_0 complex (2 i sin(-pi/nx)),
+    and the c2r transform drops Im of mode 0, so P^-1 is not the inverse of
+    the forward composition; it is still irfft(T_k^-1 rfft(x)) mode by mode,
+    checked here against dense complex solves of the bands."""
+    for nx, nv in ((9, 10), (51, 12)):
+        g = O.Grid(nx, nv, 2 * np.pi, 4.0, ht=0.1)
+        pc = O.Preconditioner(g)
+        x = np.random.default_rng(nx).standard_normal((nx, nv))
+        X = np.fft.rfft(x, axis=0)
+        for k in range(nx // 2 + 1):
+            dm, d, dp = O.kinetic_bands(g, O.kinetic_eigenvalues(nx)[k])
+            X[k] = np.linalg.solve(np.diag(d) + np.diag(dp[:-1], 1) + np.diag(dm[1:], -1), X[k])
+        ref = np.fft.irfft(X, n=nx, axis=0)
+        assert np.abs(pc.apply(x) - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("nx,nv", [(8, 16), (10, 12)])
+def test_p_times_pinv_is_identity(nx, nv):
+    g = O.Grid(nx, nv, 2 * np.pi, 4.0, ht=0.1)
     pc = O.Preconditioner(g)
     P = pc.dense_matrix()
     x = np.random.default_rng(1).standard_normal((g.nx, g.nv))
@@ -44,12 +67,13 @@ def test_p_times_pinv_is_identity():
     assert np.abs(Py - x).max() <= 1e-12 * np.abs(x).max()
 
 
-def test_preconditioner_is_half_j1_of_h0_plus_identity_over_dt():
+@pytest.mark.parametrize("nx", [8, 10])
+def test_preconditioner_is_half_j1_of_h0_plus_identity_over_dt(nx):
     """On interior rows P u = u/dt + 1/2 B_J1(u, h0): the bands of
     KineticFast.formBandedPreconditionerMatrix are the Fourier symbol of the
     J1 bracket with the kinetic Hamiltonian (checked through the stencil
-    tables, not the oracle's bracket)."""
-    g = O.Grid(8, 16, 2 * np.pi, 4.0, ht=0.1)
+    tables, not the oracle's bracket), for even nx."""
+    g = O.Grid(nx, 16, 2 * np.pi, 4.0, ht=0.1)
     pc = O.Preconditioner(g)
     P = pc.dense_matrix()
     H0 = np.broadcast_to(O.kinetic_hamiltonian(g)[None, :], (g.nx, g.nv)).copy()
@@ -61,9 +85,9 @@ def test_preconditioner_is_half_j1_of_h0_plus_identity_over_dt():
 
 
 # ---------------------------------------------------------------- Poisson
+@pytest.mark.parametrize("nx", [16, 9, 20])
 @pytest.mark.parametrize("laplace", ['cfd4', 'cfd2'])
-def test_poisson_solve_vs_dense_lstsq(laplace):
-    nx = 16
+def test_poisson_solve_vs_dense_lstsq(laplace, nx):
     hx = 0.37
     A = np.zeros((nx, nx))
     for c in range(nx):

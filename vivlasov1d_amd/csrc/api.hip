@@ -113,8 +113,6 @@ int hipfail(vv_ctx *c, hipError_t e, const char *where)
         if (_e != hipSuccess) return hipfail(c, _e, where);    \
     } while (0)
 
-bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-
 template <typename T>
 hipError_t dalloc(T **p, size_t count)
 {
@@ -342,7 +340,7 @@ int apply_pc(vv_ctx *c, const double *x, double *y, const int32_t *done)
 int arnoldi_step(vv_ctx *c, int it)
 {
     const int32_t *done = &c->gs->st.done;
-    const size_t vs = (size_t)c->N * c->B;
+    const size_t vs = (size_t)c->vs;
     double *v = c->basis + (size_t)it * vs;
     double *w = c->basis + (size_t)(it + 1) * vs;
     const double F = b_field(c), S = b_spec(c);
@@ -390,7 +388,7 @@ int precond32(vv_ctx *c, const float *x, float *y, const int32_t *done)
 int arnoldi_step32(vv_ctx *c, int it)
 {
     const int32_t *done = &c->gs->st.done;
-    const size_t vs = (size_t)c->N * c->B;
+    const size_t vs = (size_t)c->vs;
     float *v = c->basis32 + (size_t)it * vs;
     float *w = c->basis32 + (size_t)(it + 1) * vs;
     float *X = reinterpret_cast<float *>(c->X);
@@ -618,7 +616,7 @@ int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, v
 int ensure_mixed(vv_ctx *c)
 {
     if (c->basis32) return VV_OK;
-    VV_CHECK(dalloc(&c->basis32, (size_t)(c->p.restart + 1) * c->N * c->B), "alloc FP32 basis");
+    VV_CHECK(dalloc(&c->basis32, (size_t)(c->p.restart + 1) * c->vs), "alloc FP32 basis");
     VV_CHECK(dalloc(&c->R, (size_t)c->N * c->B), "alloc residual");
     return VV_OK;
 }
@@ -714,16 +712,13 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
 
 int check_params(const vv_params *p)
 {
-    if (!is_pow2(p->nx) || p->nx < 8 || p->nx > kMaxNx)
-        return fail(nullptr, VV_ERR_INVALID, "nx must be a power of two in [8, 4096]");
+    if (p->nx < 8 || p->nx > kMaxNx) return fail(nullptr, VV_ERR_INVALID, "nx must be in [8, 4096]");
     if (p->nv < 8 || p->nv > 4096) return fail(nullptr, VV_ERR_INVALID, "nv must be in [8, 4096]");
     if (p->bracket != VV_BRACKET_J1 && p->bracket != VV_BRACKET_J2 && p->bracket != VV_BRACKET_J4)
         return fail(nullptr, VV_ERR_UNSUPPORTED, "bracket must be arakawaj1, arakawaj2 or arakawaj4");
     if (p->precond != VV_PRECOND_NONE && p->precond != VV_PRECOND_TENSOR_KINETIC &&
         p->precond != VV_PRECOND_TENSOR_POTENTIAL)
         return fail(nullptr, VV_ERR_UNSUPPORTED, "preconditioner must be none, tensor_kinetic or tensor_potential");
-    if (p->precond == VV_PRECOND_TENSOR_POTENTIAL && !is_pow2(p->nv))
-        return fail(nullptr, VV_ERR_UNSUPPORTED, "tensor_potential needs nv a power of two (FFT along v)");
     if (p->laplace != VV_LAPLACE_CFD2 && p->laplace != VV_LAPLACE_CFD4)
         return fail(nullptr, VV_ERR_UNSUPPORTED, "laplace operator must be cfd2 or cfd4");
     if (p->restart < 1 || p->restart > kMaxRestart) return fail(nullptr, VV_ERR_INVALID, "restart must be in [1, 64]");
@@ -741,6 +736,8 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     *out = nullptr;
     if (int r = check_params(p)) return r;
     if (B < 1 || B > 65535) return fail(nullptr, VV_ERR_INVALID, "batch must be in [1, 65535]");
+    if (B > 1 && ((int64_t)p->nx * p->nv) % 4 != 0)
+        return fail(nullptr, VV_ERR_UNSUPPORTED, "a batched context needs nx * nv divisible by 4");
     for (int b = 0; b < B; ++b)
         if (!(hxs[b] > 0)) return fail(nullptr, VV_ERR_INVALID, "hx, hv, ht must be > 0");
 
@@ -754,6 +751,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     c->stream = reinterpret_cast<hipStream_t>(stream);
     const int nx = p->nx, nv = p->nv;
     c->N = (int64_t)nx * nv;
+    c->vs = (c->N * B + 3) / 4 * 4;
     const bool pot = p->precond == VV_PRECOND_TENSOR_POTENTIAL;
     c->fft_n = pot ? nv : nx;
     c->fft_rows = pot ? nx : nv;
@@ -821,7 +819,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(dalloc(&c->X, c->N * Bz), "alloc X");
     VV_CCHECK(dalloc(&c->rhs, c->N * Bz), "alloc rhs");
     VV_CCHECK(dalloc(&c->D, c->N * Bz), "alloc D");
-    VV_CCHECK(dalloc(&c->basis, (size_t)(p->restart + 1) * c->N * Bz), "alloc Krylov basis");
+    VV_CCHECK(dalloc(&c->basis, (size_t)(p->restart + 1) * c->vs), "alloc Krylov basis");
     VV_CCHECK(dalloc(&c->partial, (size_t)c->part_stride * Bz), "alloc partial");
     VV_CCHECK(dalloc(&c->dscalar, ds_count(B)), "alloc scalars");
     VV_CCHECK(dalloc(&c->npart, (size_t)64 * nx * Bz), "alloc density partials");
@@ -986,7 +984,8 @@ int64_t vv_scratch_bytes(const vv_ctx *c)
                     (int64_t)sizeof(double2) * (B * (int64_t)c->nmodes * c->fft_rows + c->fft_n + c->nmodes) +
                     (int64_t)sizeof(float2) * c->fft_n + (int64_t)(sizeof(GmresState) + sizeof(InstConst) + 4) * B;
     if (c->tb) bytes += (int64_t)sizeof(double) * B * (c->N + 2 * nx);
-    if (c->basis32) bytes += (int64_t)sizeof(float) * B * (c->p.restart + 1) * c->N + 8 * B * c->N;
+    if (c->basis32) bytes += (int64_t)sizeof(float) * (c->p.restart + 1) * c->vs + 8 * B * c->N;
+    bytes += (int64_t)sizeof(double) * (c->p.restart + 1) * (c->vs - B * c->N);   // basis stride padding
     return bytes;
 }
 
@@ -1089,7 +1088,7 @@ int vv_autotune(vv_ctx *c, int32_t *fft_rows, int32_t *cgs_ept)
         c->tune_fft_rows = bestR;
         // CGS vectors per thread, at basis size min(restart, 10)
         const int col = (c->p.restart < 10 ? c->p.restart : 10) - 1;
-        if ((rc = launch_zero(c, c->basis, (size_t)(col + 2) * c->N * c->B) == hipSuccess ? VV_OK : VV_ERR_HIP))
+        if ((rc = launch_zero(c, c->basis, (size_t)(col + 2) * c->vs) == hipSuccess ? VV_OK : VV_ERR_HIP))
             break;
         int bestE = 0;
         bestT = 0.f;

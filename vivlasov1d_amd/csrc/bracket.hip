@@ -71,6 +71,18 @@ struct BrArgs {
                            // Vec.set(0.) followed by PoissonBracket.function)
 };
 
+// Periodic x index of a tile column: the tile with its halo spans columns
+// i0-2 .. i0+TX+1, which on grids narrower than TX+2 (nx < 66) reaches past
+// nx + nx - 1; the extra columns feed no stored output but must still load
+// from inside the row.
+__device__ __forceinline__ int wrap_x(int gi, int nx)
+{
+    if (gi < 0) gi += nx;
+    if (gi >= nx) gi -= nx;
+    if (gi >= nx) gi %= nx;
+    return gi;
+}
+
 template <int KIND, int MODE, typename T>
 __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 {
@@ -104,9 +116,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
                            : 1.0;
     double xf0 = 0.0, xf1 = 0.0, xf2 = 0.0, xf3 = 0.0, vf = 0.0;
     if (threadIdx.x < HX) {
-        int gi = i0 - 2 + (int)threadIdx.x;
-        if (gi < 0) gi += nx;
-        if (gi >= nx) gi -= nx;
+        const int gi = wrap_x(i0 - 2 + (int)threadIdx.x, nx);
         if (a.hxv != nullptr) {
             xf0 = a.hxv[xo + gi];
         } else if (factors) {
@@ -132,9 +142,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             const int r = idx / HX;
             const int c = idx - r * HX;
             const int gj = j0 - 2 + r;
-            int gi = i0 - 2 + c;
-            if (gi < 0) gi += nx;
-            if (gi >= nx) gi -= nx;
+            const int gi = wrap_x(i0 - 2 + c, nx);
             if (gj >= 0 && gj < nv) {
                 const size_t o = (size_t)gj * nx + gi;
                 fa[q] = fin[o];

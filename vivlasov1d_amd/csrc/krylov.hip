@@ -224,8 +224,19 @@ __device__ __forceinline__ void vset(float4 &v, int l, double x)
 }
 
 // The basis is vector-major across instances: V_j of instance b at
-// basis + j * vstride + b * N (vstride = B N), so V_j of all instances is one
-// contiguous field and the operator kernels see instance stride N.
+// basis + j * vstride + b * N (vstride = B N rounded up to a multiple of 4
+// elements, so every V_j starts on a 32-byte boundary), so V_j of all
+// instances is one contiguous field and the operator kernels see instance
+// stride N.
+// TAIL: N is not a multiple of the vector width L (odd grids such as 9 x 9;
+// one instance only, a batched context needs N % 4 == 0).  Thread t < N - L
+// nvv of workgroup 0 then also owns the scalar element L nvv + t.
+template <typename S, bool TAIL>
+__device__ __forceinline__ int64_t tail_element(int64_t N, int64_t nvv)
+{
+    constexpr int L = V16<S>::L;
+    return (TAIL && blockIdx.x == 0 && (int64_t)threadIdx.x < N - nvv * L) ? nvv * L + threadIdx.x : -1;
+}
 template <typename S>
 struct CgsArgs {
     const S *basis;
@@ -256,7 +267,7 @@ __device__ __forceinline__ void load_group(const S *basis, int64_t vstride, int 
 
 // VecMDot of w = V_{col+1} against v_0..v_col, written as HH(:, col) and the
 // VecMAXPY coefficients -HH(:, col).
-template <int EPT, int PV, typename S>
+template <int EPT, int PV, typename S, bool TAIL>
 __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 {
     using V = typename V16<S>::type;
@@ -273,6 +284,8 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
     const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
     const V *W = reinterpret_cast<const V *>(a.basis + (size_t)nvec * a.vstride);
+    const int64_t et = tail_element<S, TAIL>(a.N, nvv);
+    const double wt = (TAIL && et >= 0) ? (double)a.basis[(size_t)nvec * a.vstride + et] : 0.0;
     V w[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
@@ -293,6 +306,7 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 #pragma unroll
                 for (int l = 0; l < L; ++l) acc += vget(w[q], l) * (vget(x[gg][q], l) * sv);
             }
+            if (TAIL && et >= 0) acc += wt * ((double)a.basis[(size_t)(v + gg) * a.vstride + et] * sv);
             acc = wave_sum(acc);
             if (lane == 0) red[wave][v + gg] = acc;
         }
@@ -307,6 +321,7 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 #pragma unroll
             for (int l = 0; l < L; ++l) acc += vget(w[q], l) * (vget(x[0][q], l) * sv);
         }
+        if (TAIL && et >= 0) acc += wt * ((double)a.basis[(size_t)v * a.vstride + et] * sv);
         acc = wave_sum(acc);
         if (lane == 0) red[wave][v] = acc;
     }
@@ -321,7 +336,7 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 
 // CGS update  V_{it+1} <- w - sum_j h_j v_j  (VecMAXPY with lhh = -h) and
 // per-block partial sums of its squared norm (of the stored values).
-template <int EPT, int PV, typename S>
+template <int EPT, int PV, typename S, bool TAIL>
 __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 {
     using V = typename V16<S>::type;
@@ -336,7 +351,10 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
     const int nvec = it + 1;
     const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
-    V *W = reinterpret_cast<V *>(const_cast<S *>(a.basis) + (size_t)nvec * a.vstride);
+    S *Ws = const_cast<S *>(a.basis) + (size_t)nvec * a.vstride;
+    V *W = reinterpret_cast<V *>(Ws);
+    const int64_t et = tail_element<S, TAIL>(a.N, nvv);
+    double wt = (TAIL && et >= 0) ? (double)Ws[et] : 0.0;
     // w's loads are issued first, in flight while the dot partials are summed
     V wl[EPT];
 #pragma unroll
@@ -375,6 +393,7 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 #pragma unroll
                 for (int l = 0; l < L; ++l) w[q][l] = w[q][l] + cj * (vget(x[gg][q], l) * sj);
             }
+            if (TAIL && et >= 0) wt = wt + cj * ((double)a.basis[(size_t)(j + gg) * a.vstride + et] * sj);
         }
     }
     if (j < nvec) {
@@ -386,8 +405,14 @@ __global__ __launch_bounds__(NT) void maxpy_norm_kernel(CgsArgs<S> a, int it)
 #pragma unroll
             for (int l = 0; l < L; ++l) w[q][l] = w[q][l] + cj * (vget(x[0][q], l) * sj);
         }
+        if (TAIL && et >= 0) wt = wt + cj * ((double)a.basis[(size_t)j * a.vstride + et] * sj);
     }
     double acc = 0.0;
+    if (TAIL && et >= 0) {
+        const S o = (S)wt;
+        Ws[et] = o;
+        acc += (double)o * (double)o;
+    }
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int64_t e = e0 + (int64_t)q * NT;
@@ -532,7 +557,7 @@ __global__ __launch_bounds__(NT) void backsolve_kernel(GmresState *g)
 
 // x += sum_j coef_j v_j  (VEC_TEMP = 0 + sum_j nrs_j v_j; x = x + VEC_TEMP);
 // x is FP64 for either basis type.
-template <int EPT, typename S>
+template <int EPT, typename S, bool TAIL>
 __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsArgs<S> a)
 {
     using V = typename V16<S>::type;
@@ -550,6 +575,8 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
     __syncthreads();
     const int64_t nvv = a.N / L;
     const int64_t e0 = (int64_t)blockIdx.x * NT * EPT + threadIdx.x;
+    const int64_t et = tail_element<S, TAIL>(a.N, nvv);
+    double tt = 0.0;
     double t[EPT][L];
 #pragma unroll
     for (int q = 0; q < EPT; ++q)
@@ -567,6 +594,7 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
 #pragma unroll
                 for (int l = 0; l < L; ++l) t[q][l] = t[q][l] + cj * (vget(v[gg][q], l) * sj);
             }
+            if (TAIL && et >= 0) tt = tt + cj * ((double)a.basis[(size_t)(j + gg) * a.vstride + et] * sj);
         }
     }
     if (j < n) {
@@ -578,7 +606,9 @@ __global__ __launch_bounds__(NT) void update_kernel(double *__restrict__ x, CgsA
 #pragma unroll
             for (int l = 0; l < L; ++l) t[q][l] = t[q][l] + cj * (vget(v[0][q], l) * sj);
         }
+        if (TAIL && et >= 0) tt = tt + cj * ((double)a.basis[(size_t)j * a.vstride + et] * sj);
     }
+    if (TAIL && et >= 0) x[et] = x[et] + tt;
     double2 *X = reinterpret_cast<double2 *>(x);
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
@@ -650,7 +680,7 @@ __global__ __launch_bounds__(NT) void xadd_kernel(double *__restrict__ y, const 
     y += blockIdx.y * N;
     w += (size_t)blockIdx.y * nx;
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < N; e += (int64_t)gridDim.x * NT)
-        y[e] = y[e] + alpha * w[e & (nx - 1)];
+        y[e] = y[e] + alpha * w[e % nx];
 }
 
 // y = 0 (a kernel rather than a memset node, so captured graphs replay it)
@@ -710,32 +740,39 @@ hipError_t launch_norm_start(vv_ctx *c)
 
 namespace {
 
-template <int E, typename S>
+template <int E, typename S, bool TAIL>
 void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 {
     const int nb = cgs_blocks_v(c, c->N / V16<S>::L);
-    CgsArgs<S> a{basis, c->N, c->N * c->B, c->partial, c->part_stride, c->gs, norm_partial_offset(c), nb};
+    CgsArgs<S> a{basis, c->N, c->vs, c->partial, c->part_stride, c->gs, norm_partial_offset(c), nb};
     const dim3 grid(nb, c->B), block(NT);
     if (which == 0) {
-        if (c->p.restart < 32) launch_timed(c, mdot_kernel<E, 32, S>, grid, block, 0, a, arg);
-        else launch_timed(c, mdot_kernel<E, 64, S>, grid, block, 0, a, arg);
+        if (c->p.restart < 32) launch_timed(c, mdot_kernel<E, 32, S, TAIL>, grid, block, 0, a, arg);
+        else launch_timed(c, mdot_kernel<E, 64, S, TAIL>, grid, block, 0, a, arg);
     } else if (which == 1) {
-        if (c->p.restart < 32) launch_timed(c, maxpy_norm_kernel<E, 32, S>, grid, block, 0, a, arg);
-        else launch_timed(c, maxpy_norm_kernel<E, 64, S>, grid, block, 0, a, arg);
+        if (c->p.restart < 32) launch_timed(c, maxpy_norm_kernel<E, 32, S, TAIL>, grid, block, 0, a, arg);
+        else launch_timed(c, maxpy_norm_kernel<E, 64, S, TAIL>, grid, block, 0, a, arg);
     } else {
-        launch_timed(c, update_kernel<E, S>, grid, block, 0, x, a);
+        launch_timed(c, update_kernel<E, S, TAIL>, grid, block, 0, x, a);
+    }
+}
+
+template <typename S, bool TAIL>
+void cgs_dispatch_e(vv_ctx *c, const S *basis, int which, int arg, double *x)
+{
+    switch (cgs_ept_v(c, c->N / V16<S>::L)) {
+    case 8: cgs_launch<8, S, TAIL>(c, basis, which, arg, x); break;
+    case 4: cgs_launch<4, S, TAIL>(c, basis, which, arg, x); break;
+    case 2: cgs_launch<2, S, TAIL>(c, basis, which, arg, x); break;
+    default: cgs_launch<1, S, TAIL>(c, basis, which, arg, x); break;
     }
 }
 
 template <typename S>
 hipError_t cgs_dispatch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 {
-    switch (cgs_ept_v(c, c->N / V16<S>::L)) {
-    case 8: cgs_launch<8>(c, basis, which, arg, x); break;
-    case 4: cgs_launch<4>(c, basis, which, arg, x); break;
-    case 2: cgs_launch<2>(c, basis, which, arg, x); break;
-    default: cgs_launch<1>(c, basis, which, arg, x); break;
-    }
+    if (c->N % V16<S>::L) cgs_dispatch_e<S, true>(c, basis, which, arg, x);
+    else cgs_dispatch_e<S, false>(c, basis, which, arg, x);
     return hipGetLastError();
 }
 

@@ -95,8 +95,9 @@ struct PoiArgs {
 // (PETScPoissonCFD4.pyx:78, PETScPoissonCFD2.pyx:78).
 __device__ __forceinline__ double lap(const double *x, int i, int nx, int laplace, double hx2_inv)
 {
-    const int m = nx - 1;
-    const double xm2 = x[(i - 2) & m], xm1 = x[(i - 1) & m], x0 = x[i], xp1 = x[(i + 1) & m], xp2 = x[(i + 2) & m];
+    const int im2 = i >= 2 ? i - 2 : i - 2 + nx, im1 = i >= 1 ? i - 1 : i - 1 + nx;
+    const int ip1 = i + 1 < nx ? i + 1 : i + 1 - nx, ip2 = i + 2 < nx ? i + 2 : i + 2 - nx;
+    const double xm2 = x[im2], xm1 = x[im1], x0 = x[i], xp1 = x[ip1], xp2 = x[ip2];
     if (laplace == VV_LAPLACE_CFD4)
         return (5. * xm2 - 32. * xm1 + 54. * x0 - 32. * xp1 + 5. * xp2) * hx2_inv / 12.;
     return (2. * x0 - xm1 - xp1) * hx2_inv;
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(NT) void poisson_conv_kernel(PoiArgs a)
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= nx) return;
     double s = 0.0;
-    for (int m = 0; m < nx; ++m) s += sc[m] * sb[(i - m) & (nx - 1)];
+    for (int m = 0, d = i; m < nx; ++m, --d) s += sc[m] * sb[d < 0 ? d + nx : d];   // sb[(i - m) mod nx]
     a.work[nx + i] = s;
 }
 

@@ -406,6 +406,89 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
     lds_fft<true, PRE>(buf, nx, a.logn, npair, a.tw, pt, 0, IrfftSink<T>{y, j0, nx, a.logn, a.nv});
 }
 
+// Transforms of a length n that is not a power of two (n = 100, 201, ...):
+// a direct real DFT with exact twiddle indices (k m mod n into the table of
+// exp(-2 pi i m/n)), same spectrum layout and c2r semantics as the FFT
+// kernels above.  O(n) work per output; the workgroup's rows and, when they
+// fit 32 KiB, the twiddles are staged in LDS.  Rows are not packed in pairs,
+// so `rows` may be odd.
+template <typename T, bool TWL>
+__global__ __launch_bounds__(NT) void rdft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
+{
+    using C = cx_t<T>;
+    if (cycle_done(a.done, a.gstride)) return;      // uniform over the block, before any barrier
+    x += blockIdx.y * a.N;
+    spec += blockIdx.y * a.S;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int n = a.nx, R = a.rows, ld = n | 1, nk = n / 2 + 1;
+    C *stw = reinterpret_cast<C *>(lds_raw);
+    T *sx = reinterpret_cast<T *>(lds_raw + (TWL ? (size_t)n * sizeof(C) : 0));
+    const int j0 = blockIdx.x * R;
+    const int nr = a.nv - j0 < R ? a.nv - j0 : R;   // rows of this block
+    for (int e = threadIdx.x; e < nr * n; e += NT) {
+        const int r = e / n;
+        sx[r * ld + (e - r * n)] = x[(size_t)j0 * n + e];
+    }
+    if (TWL)
+        for (int e = threadIdx.x; e < n; e += NT) stw[e] = a.tw[e];
+    __syncthreads();
+    const C *tw = TWL ? stw : a.tw;
+    for (int o = threadIdx.x; o < nk * nr; o += NT) {
+        const int k = o / nr, r = o - k * nr;
+        const T *xr = sx + r * ld;
+        T re = T(0), im = T(0);
+        for (int m = 0, t = 0; m < n; ++m) {
+            const C w = tw[t];
+            re += xr[m] * w.x;
+            im += xr[m] * w.y;
+            t += k;
+            if (t >= n) t -= n;
+        }
+        spec[(size_t)k * a.nv + j0 + r] = cmk<C>(re, im);
+    }
+}
+
+// y[j][m] = (Re X_0 + 2 sum_{k=1}^{(n-1)/2} Re(X_k e^{2 pi i k m/n}) [+ Re X_{n/2} (-1)^m]) / n
+template <typename T, bool TWL>
+__global__ __launch_bounds__(NT) void irdft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
+{
+    using C = cx_t<T>;
+    if (cycle_done(a.done, a.gstride)) return;      // uniform over the block, before any barrier
+    spec += blockIdx.y * a.S;
+    y += blockIdx.y * a.N;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int n = a.nx, R = a.rows, nk = n / 2 + 1, K = (n - 1) / 2;
+    C *stw = reinterpret_cast<C *>(lds_raw);
+    C *ss = reinterpret_cast<C *>(lds_raw + (TWL ? (size_t)n * sizeof(C) : 0));
+    const int j0 = blockIdx.x * R;
+    const int nr = a.nv - j0 < R ? a.nv - j0 : R;
+    for (int e = threadIdx.x; e < nk * nr; e += NT) {
+        const int k = e / nr, r = e - k * nr;
+        C v = spec[(size_t)k * a.nv + j0 + r];
+        if (k == 0 || 2 * k == n) v.y = T(0);
+        ss[r * nk + k] = v;
+    }
+    if (TWL)
+        for (int e = threadIdx.x; e < n; e += NT) stw[e] = a.tw[e];
+    __syncthreads();
+    const C *tw = TWL ? stw : a.tw;
+    const T inv_n = T(1) / (T)n;
+    for (int o = threadIdx.x; o < nr * n; o += NT) {
+        const int r = o / n, m = o - r * n;
+        const C *X = ss + r * nk;
+        T acc = T(0);
+        for (int k = 1, t = m; k <= K; ++k) {
+            const C w = tw[t];
+            acc += X[k].x * w.x + X[k].y * w.y;
+            t += m;
+            if (t >= n) t -= n;
+        }
+        T v = X[0].x + T(2) * acc;
+        if (!(n & 1)) v += (m & 1) ? -X[n / 2].x : X[n / 2].x;
+        y[(size_t)j0 * n + o] = v * inv_n;
+    }
+}
+
 struct TriArgs {
     int nv;           // system size
     int P;            // chunks (threads), power of two
@@ -685,9 +768,49 @@ int fft_rows(const vv_ctx *c)
     return R;
 }
 
+// The DFT kernels' launch: rows per workgroup = fft_rows(c) capped so that
+// the rows take <= 32 KiB of LDS; twiddles in LDS when they fit 32 KiB.
+template <typename T>
+FftArgs<T> dft_args(vv_ctx *c, const cx_t<T> *tw, const int32_t *done, bool *twl, size_t *lds)
+{
+    const int n = c->fft_n;
+    const size_t row_bytes = (size_t)(n + 2) * sizeof(T);   // >= (n | 1) reals and >= (n/2 + 1) complex
+    int R = fft_rows(c);
+    const int cap = (int)(32768 / row_bytes);
+    if (R > cap) R = cap > 1 ? cap : 1;
+    *twl = (size_t)n * sizeof(cx_t<T>) <= 32768;
+    *lds = (*twl ? (size_t)n * sizeof(cx_t<T>) : 0) + (size_t)R * row_bytes;
+    return FftArgs<T>{n, 0, c->fft_rows, R, c->N, spec_stride(c), (int64_t)sizeof(GmresState), tw, done};
+}
+
+template <typename T>
+hipError_t rdft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const int32_t *done)
+{
+    bool twl;
+    size_t lds;
+    const FftArgs<T> a = dft_args<T>(c, tw, done, &twl, &lds);
+    const dim3 grid((a.nv + a.rows - 1) / a.rows, c->B);
+    if (twl) launch_timed(c, rdft_kernel<T, true>, grid, dim3(NT), (uint32_t)lds, a, x, spec);
+    else launch_timed(c, rdft_kernel<T, false>, grid, dim3(NT), (uint32_t)lds, a, x, spec);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t irdft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, const int32_t *done)
+{
+    bool twl;
+    size_t lds;
+    const FftArgs<T> a = dft_args<T>(c, tw, done, &twl, &lds);
+    const dim3 grid((a.nv + a.rows - 1) / a.rows, c->B);
+    if (twl) launch_timed(c, irdft_kernel<T, true>, grid, dim3(NT), (uint32_t)lds, a, spec, y);
+    else launch_timed(c, irdft_kernel<T, false>, grid, dim3(NT), (uint32_t)lds, a, spec, y);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const int32_t *done)
 {
+    if (c->fft_n & (c->fft_n - 1)) return rdft_t<T>(c, tw, x, spec, done);
     FftArgs<T> a{c->fft_n, c->logn, c->fft_rows, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
@@ -700,6 +823,7 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
 template <typename T>
 hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, const int32_t *done)
 {
+    if (c->fft_n & (c->fft_n - 1)) return irdft_t<T>(c, tw, spec, y, done);
     FftArgs<T> a{c->fft_n, c->logn, c->fft_rows, fft_rows(c), c->N, spec_stride(c),
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;

@@ -13,7 +13,7 @@
 namespace vv {
 
 constexpr int kMaxRestart = 64;     // GMRES restart limit (basis of restart+1 vectors)
-constexpr int kMaxNx = 4096;        // largest x size (power of two)
+constexpr int kMaxNx = 4096;        // largest x size
 constexpr int kMaxNv = 8192;
 constexpr int kReduceBlocks = 512;  // fixed block count of N-sized reductions
 
@@ -67,13 +67,14 @@ struct vv_ctx {
     std::vector<vv::InstConst> inst_host;
     int32_t *active;            // device, B flags: instance takes part in the next solve
     int64_t N;          // nx * nv (per instance)
+    int64_t vs;         // element stride of the Krylov basis vectors: B N rounded up to a multiple of 4
     // transform geometry of the tensor-product preconditioner: the real FFT
     // has length fft_n over fft_rows rows and nmodes = fft_n/2 + 1 modes, each
     // mode a tridiagonal system of length fft_rows.  kinetic: FFT along x
     // (fft_n = nx, fft_rows = nv); potential: along v on the transposed field
     // (fft_n = nv, fft_rows = nx)
     int nmodes;         // fft_n/2 + 1
-    int logn;           // log2(fft_n)
+    int logn;           // log2(fft_n) when fft_n is a power of two (FFT kernels); else a direct DFT
     int fft_n, fft_rows;
     int64_t coef_stride;        // instance stride of gm/g0/gp (0: shared)
     double cf_lo, cf_mid, cf_hi;  // band constants: kinetic 0.5/1.0/0.5, potential 1.0/2.0/1.0
