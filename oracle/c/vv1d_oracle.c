@@ -11,6 +11,7 @@
  *   preconditioner   vlasov/solvers/preconditioner/TensorProductKinetic.pyx:146-160,
  *                    TensorProductKineticFast.pyx:61-90, 102-272 (FFTW and
  *                    LAPACK zgbtrf/zgbtrs replaced by an iterative radix-2 FFT
+ *                    -- a direct DFT for lengths that are not powers of two --
  *                    and a pivot-free banded LU: the reference's LU never pivots
  *                    for these diagonally dominant T_k)
  *   Poisson          vlasov/solvers/poisson/PETScPoissonSolver.pyx:58-74,
@@ -103,8 +104,30 @@ static void bracket_op(const vvo_run *r, int mode, const double *f, const double
 }
 
 /* ----------------------------------------------------------- preconditioner */
+/* z <- sum_m z_m w^(k m mod n), w = exp(-+2 pi i/n): the textbook DFT, for
+ * lengths that are not powers of two (work: one n-vector of scratch) */
+static void dft_inplace(cplx *z, int n, const cplx *tw, int inverse)
+{
+    cplx *t = malloc(sizeof(cplx) * n);
+    for (int k = 0; k < n; ++k) {
+        cplx s = 0.0;
+        for (int m = 0, q = 0; m < n; ++m) {
+            s += z[m] * (inverse ? conj(tw[q]) : tw[q]);
+            q += k;
+            if (q >= n) q -= n;
+        }
+        t[k] = s;
+    }
+    memcpy(z, t, sizeof(cplx) * n);
+    free(t);
+}
+
 static void fft_inplace(cplx *z, int n, int logn, const cplx *tw, int inverse)
 {
+    if (n & (n - 1)) {
+        dft_inplace(z, n, tw, inverse);
+        return;
+    }
     for (int i = 1, j = 0; i < n; ++i) {          /* bit reversal */
         int bit = n >> 1;
         for (; j & bit; bit >>= 1) j ^= bit;
@@ -132,6 +155,7 @@ static void fft_inplace(cplx *z, int n, int logn, const cplx *tw, int inverse)
 static void precond(vvo_run *r, const double *x, double *y)
 {
     const int nx = r->nx, nv = r->nv, nm = r->nmodes, nh = nx / 2;
+    const int nyq = (nx & 1) ? -1 : nh;       /* the real Nyquist mode of an even length */
     /* rfft of row pairs */
 #pragma omp parallel
     {
@@ -142,7 +166,7 @@ static void precond(vvo_run *r, const double *x, double *y)
             for (int i = 0; i < nx; ++i) z[i] = x[(long)j * nx + i] + I * (jb >= 0 ? x[(long)jb * nx + i] : 0.0);
             fft_inplace(z, nx, r->logn, r->tw, 0);
             for (int k = 0; k <= nh; ++k) {
-                const cplx zk = z[k], zm = conj(z[(nx - k) & (nx - 1)]);
+                const cplx zk = z[k], zm = conj(z[k == 0 ? 0 : nx - k]);
                 r->spec[(long)k * nv + j] = 0.5 * (zk + zm);
                 if (jb >= 0) r->spec[(long)k * nv + jb] = -0.5 * I * (zk - zm);
             }
@@ -168,12 +192,12 @@ static void precond(vvo_run *r, const double *x, double *y)
             for (int k = 0; k <= nh; ++k) {
                 cplx A = r->spec[(long)k * nv + j];
                 cplx B = jb >= 0 ? r->spec[(long)k * nv + jb] : 0.0;
-                if (k == 0 || k == nh) {
+                if (k == 0 || k == nyq) {
                     A = creal(A);
                     B = creal(B);
                 }
                 z[k] = A + I * B;
-                if (k != 0 && k != nh) z[nx - k] = conj(A) + I * conj(B);
+                if (k != 0 && k != nyq) z[nx - k] = conj(A) + I * conj(B);
             }
             fft_inplace(z, nx, r->logn, r->tw, 1);
             const double s = 1.0 / nx;
@@ -373,7 +397,7 @@ vvo_run *vvo_create(int nx, int nv, double hx, double hv, double ht, int bracket
                     const double *v, const double *eigen, const double *f0, const double *h0, double ksp_rtol,
                     double ksp_atol, int ksp_max_it, double snes_atol, int snes_max_iter, int restart, int nthreads)
 {
-    if (nx < 4 || (nx & (nx - 1)) || nv < 8 || restart < 1) return NULL;
+    if (nx < 4 || nv < 8 || restart < 1) return NULL;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else

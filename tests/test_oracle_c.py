@@ -15,9 +15,10 @@ def cb():
     return cbaseline
 
 
+@pytest.mark.parametrize("nx,nv", [(16, 40), (9, 20), (50, 21)])
 @pytest.mark.parametrize("kind", ['arakawaj1', 'arakawaj2', 'arakawaj4'])
-def test_c_operators_match_oracle(cb, kind):
-    g = O.Grid(16, 40, 4 * np.pi, 8.0, ht=0.05)
+def test_c_operators_match_oracle(cb, kind, nx, nv):
+    g = O.Grid(nx, nv, 4 * np.pi, 8.0, ht=0.05)
     rng = np.random.default_rng(3)
     f0 = O.normalise(O.initial_distribution(g, lambda x, v: (1 + 0.2 * np.cos(0.5 * x)) * O.maxwellian(v)), g)
     run = cb.CRun(g, f0.T, bracket=kind)
@@ -32,8 +33,9 @@ def test_c_operators_match_oracle(cb, kind):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
-def test_c_timesteps_match_oracle(cb):
-    g = O.Grid(32, 64, 4 * np.pi, 8.0, ht=0.05)
+@pytest.mark.parametrize("nx,nv", [(32, 64), (25, 48)])
+def test_c_timesteps_match_oracle(cb, nx, nv):
+    g = O.Grid(nx, nv, 4 * np.pi, 8.0, ht=0.05)
     prof = lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * O.maxwellian(v)
     orun = O.OracleRun(g, O.initial_distribution(g, prof))
     crun = cb.CRun(g, O.normalise(O.initial_distribution(g, prof), g).T)

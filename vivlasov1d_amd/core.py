@@ -9,6 +9,7 @@ subset that selects the solver), ``vlasov/run/run_base.py:118-137, 408-506,
 eigenvalues).
 """
 
+import importlib
 import math
 import os
 
@@ -216,6 +217,66 @@ def initial_distribution_nT(grid, n, T):
     f[:STENCIL, :] = 0.0
     f[grid.nv - STENCIL:, :] = 0.0
     return f
+
+
+def fourier_filter(f, nfourier):
+    """The initial-data Fourier filter along x (run_base.py:467-496):
+    scipy.fftpack.rfft over x, entries nfourier and up set to zero, irfft.
+    FFTPACK stores the coefficients as y0, Re y1, Im y1, Re y2, Im y2, ...
+    (Re y_{nx/2} last for even nx), so Re y_k survives when 2k - 1 < nfourier
+    and Im y_k when 2k < nfourier.  f in natural order, shape (nv, nx)."""
+    if nfourier <= 0:
+        return f
+    nx = f.shape[1]
+    Y = np.fft.rfft(f, axis=1)
+    k = np.arange(Y.shape[1])
+    keep_re = np.where(k == 0, True, 2 * k - 1 < nfourier)
+    keep_im = np.where(k == 0, True, 2 * k < nfourier)
+    Y = np.where(keep_re, Y.real, 0.0) + 1j * np.where(keep_im, Y.imag, 0.0)
+    return np.fft.irfft(Y, n=nx, axis=1)
+
+
+def initial_function(name, attr):
+    """The function `attr` of the initial-data module `name`: the reference
+    imports runs.<name> (run_base.py:418, 433, 445); a module of that name on
+    sys.path is taken as well."""
+    for modname in ("runs." + name, name):
+        try:
+            mod = importlib.import_module(modname)
+        except ImportError:
+            continue
+        if hasattr(mod, attr):
+            return getattr(mod, attr)
+    raise ImportError("initial-data module %r with a function %r not found (runs.%s or %s)"
+                      % (name, attr, name, name))
+
+
+def initial_data(cfg, grid, profile=None):
+    """f0 as run_base.py:408-496 builds it, before normalisation: a profile
+    (here a callable, or the config's distribution_python module's
+    distribution(x, v), evaluated per point as Toolbox.initialise_distribution_function
+    does), else a Maxwellian in n(x), T(x) from density_python /
+    temperature_python (density(x_i, L), temperature(x_i)) or the constant
+    density / temperature; then the nfourier filter.  Natural order (nv, nx)."""
+    idata = cfg['initial_data']
+    if profile is None and idata.get('distribution_python'):
+        dist = initial_function(idata['distribution_python'], 'distribution')
+        profile = np.vectorize(lambda x, v: float(dist(x, v)), otypes=[np.float64])
+    if profile is not None:
+        f = initial_distribution(grid, profile)
+    else:
+        if idata.get('density_python'):
+            dens = initial_function(idata['density_python'], 'density')
+            n = np.array([dens(x, grid.L) for x in grid.x], dtype=np.float64)
+        else:
+            n = np.full(grid.nx, float(idata['density']))
+        if idata.get('temperature_python'):
+            temp = initial_function(idata['temperature_python'], 'temperature')
+            T = np.array([temp(x) for x in grid.x], dtype=np.float64)
+        else:
+            T = np.full(grid.nx, float(idata['temperature']))
+        f = initial_distribution_nT(grid, n, T)
+    return fourier_filter(f, int(idata.get('nfourier') or 0))
 
 
 def normalise(f, grid):

@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .core import Config, Grid, initial_distribution, kinetic_hamiltonian, normalise
+from .core import Config, Grid, initial_data, kinetic_hamiltonian, normalise
 from .solvers import (Context, PETScArakawaRungeKutta, PETScArakawaSymplectic, PETScPoissonSolver,
                       PETScVlasovSolver, SNES, Toolbox)
 
@@ -56,11 +56,10 @@ class Runscript:
         self.pc_int, self.ph_int = ctx.new_x(), ctx.new_x()
         self.pc_ext = ctx.new_x()
         self.toolbox = Toolbox(None, None, grid, ctx)
-        # initial data (run_base.py:408-506)
+        # initial data (run_base.py:408-506): f0 as given, else the profile
+        # callable or the configuration's initial-data settings
         if f0 is None:
-            if profile is None:
-                raise ValueError("give an initial profile(x, v) or f0")
-            f0 = initial_distribution(grid, profile)
+            f0 = initial_data(cfg, grid, profile)
         f0 = np.asarray(f0, dtype=np.float64).reshape(grid.nv, grid.nx)
         if normalise_f0:
             f0 = normalise(f0, grid)
