@@ -434,6 +434,14 @@ def kinetic_bands(grid, eigen_k):
     return diagm, diag, diagp
 
 
+class IdentityPreconditioner:
+    """No preconditioner (preconditioner_type None, TensorProduct.pyx:66-86
+    returns None): function_snes_mult / mult apply F and J directly."""
+
+    def apply(self, x):
+        return x
+
+
 class Preconditioner:
     """P^-1 x = irfft_x( T_k^-1 rfft_x(x) )   (TensorProductKinetic.pyx:146-160).
 
@@ -872,9 +880,12 @@ class OracleRun:
     def __init__(self, grid, f0, bracket='arakawaj4', laplace='cfd4', charge=-1.0,
                  mass=1.0, ksp_rtol=1e-10, ksp_atol=1e-50, ksp_max_it=1000,
                  snes_atol=1e-10, snes_max_iter=20, restart=30, precond_variant='lapack',
-                 preconditioner='tensor_kinetic', dissipation=None, initial_guess=None, initial_iter=1):
+                 preconditioner='tensor_kinetic', dissipation=None, initial_guess=None, initial_iter=1,
+                 external=None):
         self.grid = grid
         self.kind = bracket
+        self.external = external
+        self.itime = 0
         self.diss = dissipation
         if initial_guess not in (None, 'None', '') + GUESSES:
             raise ValueError("initial guess %r" % (initial_guess,))
@@ -890,13 +901,17 @@ class OracleRun:
         self.fc = normalise(np.array(f0, dtype=np.float64), grid)
         self.h2c = np.zeros(grid.nx)
         self.h2h = np.zeros(grid.nx)
+        self.pc_ext = np.zeros(grid.nx)
         self.calculate_moments()
+        self.calculate_external(0.)             # run_base.py:514-525
         # TensorProductPreconditioner.create (TensorProduct.pyx:66-86), built
         # after the initial moments as in run_base.py (its phi is pc_int)
         if preconditioner in ('tensor_kinetic', 'tensor_kinetic_scipy'):
             self.pc = Preconditioner(grid, precond_variant)
         elif preconditioner in ('tensor_potential', 'tensor_potential_scipy'):
             self.pc = PotentialPreconditioner(grid, self.phi, precond_variant)
+        elif preconditioner in (None, 'None', 'none'):
+            self.pc = IdentityPreconditioner()        # preconditioner_type None: KSP on J itself
         else:
             raise ValueError(preconditioner)
         self.log = []
@@ -916,6 +931,21 @@ class OracleRun:
 
     def update_previous(self):
         self.H = h_average(self.h0, self.h1c, self.h1h, self.h2c, self.h2h)
+
+    def calculate_external(self, t):
+        """run_base.py:707-725: p_i = external(x_i, t) per point, the average
+        removed (VecSum in index order, VecShift), then h2 =
+        potential_to_hamiltonian(p) (copy_pext_to_h, :746-747).  No external
+        potential: p and h2 stay zero."""
+        if self.external is None:
+            return
+        p = np.array([float(self.external(x, t)) for x in self.grid.x])
+        psum = 0.0
+        for v in p:
+            psum += v
+        p = p + (-(psum / self.grid.nx))
+        self.pc_ext = p
+        self.h2c = potential_to_hamiltonian(p, self.grid.nx)
 
     def _mom(self):
         if not self.diss.collisions:
@@ -995,6 +1025,9 @@ class OracleRun:
         self.h2h = self.h2c.copy()
         if self.diss is not None and self.diss.collisions:
             self.nh, self.uh, self.ah = self.n.copy(), self.u.copy(), self.a.copy()
+        # calculate_external(current_time) (run_nonlinear_matrixfree_split_mp.py:113, 124-125)
+        self.itime += 1
+        self.calculate_external(self.grid.ht * self.itime)
         # update_history -> preconditioner.update_matrices (PETScVlasovSolver.pyx:135-139)
         if isinstance(self.pc, PotentialPreconditioner):
             self.pc.update(self.phi)

@@ -347,6 +347,11 @@ int arnoldi_step(vv_ctx *c, int it)
     if (c->op_mode != 0) {
         if (int r = linear_apply(c, v, &c->gs->scale[it], c->op_fbar, c->X, c->ht_inv, -0.5, 0.5, true)) return r;
         if (int r = apply_pc(c, c->X, w, done)) return r;
+    } else if (c->p.precond == VV_PRECOND_NONE) {
+        // no preconditioner (the reference's preconditioner_type None): J v
+        // goes straight into the next basis vector
+        VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], w), "jacobian");
+        if (int r = diss_jacobian(c, v, &c->gs->scale[it], w, done)) return r;
     } else {
         VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
         if (int r = diss_jacobian(c, v, &c->gs->scale[it], c->X, done)) return r;
@@ -393,8 +398,12 @@ int arnoldi_step32(vv_ctx *c, int it)
     float *w = c->basis32 + (size_t)(it + 1) * vs;
     float *X = reinterpret_cast<float *>(c->X);
     const double F = 4.0 * (double)c->N * c->B;
-    VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], X), "jacobian32");
-    if (int r = pc_apply32(c, X, w, done, it)) return r;
+    if (c->p.precond == VV_PRECOND_NONE) {
+        VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], w), "jacobian32");
+    } else {
+        VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], X), "jacobian32");
+        if (int r = pc_apply32(c, X, w, done, it)) return r;
+    }
     VV_PLAUNCH(P_MDOT32, F * (it + 2), it, launch_mdot32(c, it), "mdot32");
     VV_PLAUNCH(P_MAXPY32, F * (it + 3), it, launch_maxpy_norm32(c, it), "maxpy_norm32");
     return VV_OK;
@@ -514,8 +523,6 @@ int restart_cycles(vv_ctx *c, const double *b, double *x)
 
 int ksp_solve_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, vv_ksp_result *res)
 {
-    if (c->p.precond == VV_PRECOND_NONE)
-        return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve needs a tensor-product preconditioner");
     VV_CHECK(launch_zero(c, x, c->N * c->B), "zero x");
     VV_CHECK(launch_gmres_init(c, kp->rtol, kp->atol, kp->dtol, kp->max_it), "gmres init");
     if (int r = cycle_start_from(c, b)) return r;
@@ -584,8 +591,6 @@ int snes_graph(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp)
 
 int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, vv_ksp_result *res)
 {
-    if (c->p.precond == VV_PRECOND_NONE)
-        return fail(c, VV_ERR_UNSUPPORTED, "vv_snes_ksponly needs a tensor-product preconditioner");
     if (c->use_graph) {
         if (int r = no_diss(c, "graph replay")) return r;
         if (int r = snes_graph(c, f, fh, kp)) return r;
@@ -630,8 +635,6 @@ int ensure_mixed(vv_ctx *c)
 int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_params *kp, const vv_mixed_params *mp,
                          vv_ksp_result *res, int32_t *cycles)
 {
-    if (c->p.precond == VV_PRECOND_NONE)
-        return fail(c, VV_ERR_UNSUPPORTED, "vv_ksp_solve_mixed needs a tensor-product preconditioner");
     if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the mixed-precision solve takes one instance");
     if (int r = ensure_mixed(c)) return r;
     const int64_t N = c->N;

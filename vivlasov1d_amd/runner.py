@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .core import Config, Grid, initial_data, kinetic_hamiltonian, normalise
+from .core import Config, Grid, initial_data, initial_function, kinetic_hamiltonian, normalise
 from .solvers import (Context, PETScArakawaRungeKutta, PETScArakawaSymplectic, PETScPoissonSolver,
                       PETScVlasovSolver, SNES, Toolbox)
 
@@ -29,7 +29,7 @@ class ResidualTooLarge(RuntimeError):
 
 class Runscript:
     def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False, precision='double',
-                 inner_rtol=1e-5, normalise_f0=True, refine='newton', hdf5_out=None):
+                 inner_rtol=1e-5, normalise_f0=True, refine='newton', hdf5_out=None, external=None):
         if not isinstance(cfg, Config):
             cfg = Config(cfg)
         self.cfg = cfg
@@ -65,6 +65,11 @@ class Runscript:
             f0 = normalise(f0, grid)
         self.fc.copy_(torch.from_numpy(f0))
         self.calculate_moments()
+        # external potential external(x, t) (run_base.py:514-525): the callable
+        # given, else the configuration's external_python module
+        ext = cfg['initial_data'].get('external_python')
+        self.external = external if external is not None else (initial_function(ext, 'external') if ext else None)
+        self.calculate_external(0.)
         # solver objects (run_nonlinear_matrixfree_split_mp.py:26-80)
         self.vlasov_solver = PETScVlasovSolver(cfg, None, None, grid, self.h0, self.h1c, self.h1h, self.h2c,
                                                self.h2h, self.pc_int, self.charge, coll_freq=sol['coll_freq'],
@@ -132,6 +137,23 @@ class Runscript:
             self.ctx.velocity_moments(self.fc, self.U, self.E)
             self.ctx.collision_factor(self.N, self.U, self.E, self.A)
 
+    # run_base.py:707-725, 745-747 -------------------------------------------
+    def calculate_external(self, t):
+        """p_i = external(x_i, t), evaluated per point on the host as the
+        reference does, its average removed (index-order sum), then
+        h2 = potential_to_hamiltonian(p) on the device.  Without an external
+        potential p and h2 stay zero."""
+        if self.external is None:
+            return
+        g = self.grid
+        p = np.array([float(self.external(x, t)) for x in g.x])
+        psum = 0.0
+        for v in p:
+            psum += v
+        p = p + (-(psum / g.nx))
+        self.pc_ext.copy_(torch.from_numpy(p))
+        self.ctx.potential_to_hamiltonian(self.pc_ext, self.h2c)
+
     # run_base.py:749-763 ----------------------------------------------------
     def make_history(self):
         self.fh.copy_(self.fc)
@@ -192,7 +214,7 @@ class Runscript:
         """One timestep (run_nonlinear_matrixfree_split_mp.py:112-171)."""
         self.itime += 1
         self.make_history()
-        # calculate_external: no external potential -> h2c stays 0 (run_base.py:707-725)
+        self.calculate_external(self.itime * self.grid.ht)      # _mp.py:113, 124-125
         guess = self.initial_guess()
         self.vlasov_solver.update_previous(self.fc)
         pred = self.calculate_residual()

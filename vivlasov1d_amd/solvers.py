@@ -854,11 +854,9 @@ class SNES:
         return self.ksp
 
     def solve(self, b, x):
-        """SNESSolve_KSPONLY: F = P^-1 F(x); KSPSolve; x -= delta unless the
-        linear solve diverged."""
+        """SNESSolve_KSPONLY: F = P^-1 F(x) (F(x) without a preconditioner);
+        KSPSolve; x -= delta unless the linear solve diverged."""
         s = self.solver
-        if s.preconditioner is None:
-            raise _lib.VVError("the native SNES path needs the tensor_kinetic preconditioner")
         k = self.ksp
         if k.precision == 'mixed':
             self.last, k.cycles = s.ctx.snes_ksponly_mixed(x, s.Fh, *k.mixed_args())
