@@ -93,3 +93,16 @@ def test_distribution_module_and_missing_module(tmp_path, monkeypatch):
     cfg['initial_data']['distribution_python'] = 'vv_no_such_module'
     with pytest.raises(ImportError, match="vv_no_such_module"):
         initial_data(cfg, g)
+
+
+def test_unknown_preconditioner_scheme_runs_without_one():
+    """TensorProductPreconditioner.create returns None for names other than
+    the four tensor schemes (examples/twostream_bgk*.cfg use 'TensorFast')."""
+    from vivlasov1d_amd.solvers import effective_preconditioner
+    for s in ('tensor_kinetic', 'tensor_kinetic_scipy', 'tensor_potential', 'tensor_potential_scipy', ''):
+        assert effective_preconditioner(s) == s
+    cfg = Config({'solver': {'preconditioner_type': 'tensor', 'preconditioner_scheme': 'TensorFast'}})
+    with pytest.warns(UserWarning, match="without a preconditioner"):
+        assert effective_preconditioner(cfg.get_preconditioner()) == ''
+    with pytest.warns(UserWarning):
+        assert effective_preconditioner('tensor_diagonal') == ''

@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from .core import Config, Grid, initial_data, initial_function, kinetic_hamiltonian, normalise
 from .solvers import (Context, PETScArakawaRungeKutta, PETScArakawaSymplectic, PETScPoissonSolver,
-                      PETScVlasovSolver, SNES, Toolbox)
+                      PETScVlasovSolver, SNES, Toolbox, effective_preconditioner)
 
 
 class ResidualTooLarge(RuntimeError):
@@ -38,7 +38,8 @@ class Runscript:
         self.charge = cfg['initial_data']['charge']
         self.mass = cfg['initial_data']['mass']
         sol = cfg['solver']
-        self.ctx = Context(grid, bracket=cfg.get_poisson_bracket(), preconditioner=cfg.get_preconditioner(),
+        self.ctx = Context(grid, bracket=cfg.get_poisson_bracket(),
+                           preconditioner=effective_preconditioner(cfg.get_preconditioner()),
                            laplace=cfg.get_laplace_operator(), charge=self.charge, restart=restart)
         ctx = self.ctx
         # vectors (run_base.py:192-233); x-vectors are 1D, the Hamiltonians are their factors

@@ -22,6 +22,7 @@ builds (Toolbox.pyx:59, :187).
 """
 
 import ctypes
+import warnings
 
 import numpy as np
 import torch
@@ -556,9 +557,20 @@ class TensorProductPreconditioner:
             pc = TensorProductPreconditioner(ctx, phi)
             pc.update_matrices()
             return pc
-        if type == 'tensor_diagonal':
-            raise _lib.VVError("preconditioner %r is not provided (the reference has no implementation)" % type)
         return None
+
+
+def effective_preconditioner(scheme):
+    """The preconditioner a configured scheme gives: TensorProductPreconditioner.create
+    (TensorProduct.pyx:66-86) builds the four tensor schemes and returns None
+    for every other name ('TensorFast' in examples/twostream_bgk*.cfg,
+    'tensor_diagonal', ...), so such a run goes on without a preconditioner.
+    The same happens here, with a warning."""
+    if scheme in _lib.PRECONDITIONERS:
+        return scheme
+    warnings.warn("preconditioner scheme %r is not tensor_kinetic[_scipy] or tensor_potential[_scipy]: "
+                  "running without a preconditioner, as the reference does" % (scheme,))
+    return ''
 
     def tensorProduct(self, X, Y):
         """Y = P^-1 X (TensorProductKinetic.pyx:146-160, TensorProductPotential.pyx:157-173)."""
@@ -591,7 +603,7 @@ class PETScVlasovSolver:
         self.charge = charge
         self.H0, self.H1p, self.H1h, self.H2p, self.H2h = H0, H1p, H1h, H2p, H2h
         self.phi = phi
-        pc = config.get_preconditioner()
+        pc = effective_preconditioner(config.get_preconditioner())
         self.ctx = context if context is not None else Context(
             grid, bracket=config.get_poisson_bracket(), preconditioner=pc,
             laplace=config.get_laplace_operator(), charge=charge, restart=restart)
