@@ -120,3 +120,20 @@ def test_config_selects_dissipation():
     assert cfg.get_double_bracket() == 'fhh' and cfg.get_collision_operator() == ''
     cfg = Config({'solver': {'dissipation_type': 'collisions', 'collision_operator': 'collE', 'coll_freq': 0.0}})
     assert cfg.get_collision_operator() == '' and cfg.is_dissipation_none()
+
+
+def test_host_classes_keep_the_reference_method_names():
+    """The Python layer's classes carry the reference's callbacks
+    (PETScVlasovSolver.pyx:130-198, TensorProduct.pyx:66-160,
+    PETScPoissonSolver.pyx:58-100)."""
+    from vivlasov1d_amd import solvers
+    need = {
+        solvers.PETScVlasovSolver: ['set_moments', 'update_history', 'update_previous', 'snes_mult', 'mult',
+                                    'jacobian_mult', 'function_snes_mult', 'function_mult', 'function_norm',
+                                    'jacobian', 'function'],
+        solvers.TensorProductPreconditioner: ['create', 'tensorProduct', 'update_matrices'],
+        solvers.PETScPoissonSolver: ['formRHS', 'mult', 'function_mult'],
+    }
+    for cls, names in need.items():
+        for n in names:
+            assert callable(getattr(cls, n, None)), (cls.__name__, n)

@@ -559,6 +559,16 @@ class TensorProductPreconditioner:
             return pc
         return None
 
+    def tensorProduct(self, X, Y):
+        """Y = P^-1 X (TensorProductKinetic.pyx:146-160, TensorProductPotential.pyx:157-173)."""
+        self.ctx.precond(X, Y)
+
+    def update_matrices(self):
+        """The potential form's bands from phi (PotentialFast.pyx:205-264); the
+        kinetic form depends on the grid only (KineticFast.pyx:244-245)."""
+        if self.ctx.potential and self.phi is not None:
+            self.ctx.precond_update(self.phi)
+
 
 def effective_preconditioner(scheme):
     """The preconditioner a configured scheme gives: TensorProductPreconditioner.create
@@ -571,16 +581,6 @@ def effective_preconditioner(scheme):
     warnings.warn("preconditioner scheme %r is not tensor_kinetic[_scipy] or tensor_potential[_scipy]: "
                   "running without a preconditioner, as the reference does" % (scheme,))
     return ''
-
-    def tensorProduct(self, X, Y):
-        """Y = P^-1 X (TensorProductKinetic.pyx:146-160, TensorProductPotential.pyx:157-173)."""
-        self.ctx.precond(X, Y)
-
-    def update_matrices(self):
-        """The potential form's bands from phi (PotentialFast.pyx:205-264); the
-        kinetic form depends on the grid only (KineticFast.pyx:244-245)."""
-        if self.ctx.potential and self.phi is not None:
-            self.ctx.precond_update(self.phi)
 
 
 class PETScVlasovSolver:
