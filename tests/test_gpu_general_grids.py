@@ -233,3 +233,47 @@ def test_picard_steps_match_oracle(nx, nv, precond, cuda):
             assert abs(pg['gmres'] - po['gmres_its']) <= 1
         f = run.fc.cpu().numpy().T
         assert np.abs(f - orun.fc).max() <= 1e-9 * np.abs(orun.fc).max()
+
+
+@pytest.mark.parametrize("precond", ['tensor_kinetic', 'tensor_potential'])
+def test_batched_instances_match_single_contexts(precond, cuda):
+    """A batched context on a 100 x 200 grid (DFT along x, and along v for
+    the potential form): P^-1 and GMRES of every instance bit-identical to a
+    single-instance context with the same x-period."""
+    import torch
+    from vivlasov1d_amd.core import Grid as PGrid
+    from vivlasov1d_amd.solvers import Context
+    print("case", precond, flush=True)
+    nx, nv = 100, 200
+    Ls = [4 * np.pi, 2 * np.pi, 10 * np.pi]
+    B = len(Ls)
+    rng = np.random.default_rng(5)
+    grids = [PGrid(nx, nv, L, 8.0, ht=0.05) for L in Ls]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    cb = Context(grids[0], preconditioner=precond, hx=[gr.hx for gr in grids])
+    h0 = O.kinetic_hamiltonian(O.Grid(nx, nv, Ls[0], 8.0, ht=0.05))
+    h1 = 0.1 * rng.standard_normal((B, nx))
+    z = np.zeros((B, nx))
+    cb.set_hamiltonian(T(h0), T(h1), T(h1), T(z), T(z))
+    phis = 0.3 * np.cos(np.arange(nx) * 2 * np.pi / nx)[None, :] + 0.05 * rng.standard_normal((B, nx))
+    if precond == 'tensor_potential':
+        cb.precond_update(T(phis))
+    x = rng.standard_normal((B, nv, nx))
+    yb = cb.new_field()
+    cb.precond(T(x), yb)
+    bb = cb.new_field()
+    cb.jacobian(T(x), bb)
+    xb = cb.new_field()
+    rb = cb.ksp_solve(bb, xb, 1e-10, 1e-50, 1e5, 1000)
+    for b in range(B):
+        cs = Context(grids[b], preconditioner=precond)
+        cs.set_hamiltonian(T(h0), T(h1[b]), T(h1[b]), T(z[b]), T(z[b]))
+        if precond == 'tensor_potential':
+            cs.precond_update(T(phis[b]))
+        y1 = cs.new_field()
+        cs.precond(T(x[b]), y1)
+        assert torch.equal(yb[b], y1), b
+        x1 = cs.new_field()
+        r = cs.ksp_solve(bb[b].contiguous(), x1, 1e-10, 1e-50, 1e5, 1000)
+        assert (r.its, r.reason) == (rb[b].its, rb[b].reason), b
+        assert torch.equal(x1, xb[b]), b
