@@ -94,3 +94,5 @@ def test_unknown_initial_guess_is_an_error(cuda):
     prof = lambda x, v: np.exp(-0.5 * v ** 2)
     with pytest.raises(_lib.VVError, match="initial guess"):
         Runscript(_cfg('gear4', 1, 16, 32, 2 * np.pi, 6.0, 0.1), profile=prof)
+    with pytest.raises(_lib.VVError, match="Gear history"):
+        Runscript(_cfg('gear', 1, 16, 32, 2 * np.pi, 6.0, 0.1), profile=prof)

@@ -105,6 +105,12 @@ class Runscript:
                    'rk4_18': self.initial_guess_rk4, 'rk4_16': self.initial_guess_rk4_16,
                    'symplectic2': self.initial_guess_symplectic2, 'symplectic4': self.initial_guess_symplectic4}
         ig = sol.get('initial_guess')
+        if ig == 'gear':
+            # run_base.py:510-511 calls PETScArakawaGear.initialise_history,
+            # which is commented out (PETScArakawaGear.pyx:70-103): the
+            # reference stops at set-up with this option
+            raise _lib.VVError("initial guess 'gear' is not provided: the reference's Gear history set-up "
+                               "(PETScArakawaGear.initialise_history) does not exist")
         if ig not in guesses:
             raise _lib.VVError("initial guess %r is not provided (rk4, rk4_16, rk4_18, symplectic2, symplectic4)"
                                % (ig,))
