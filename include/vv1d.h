@@ -341,7 +341,8 @@ int vv_ksp_solve(vv_ctx *ctx, const double *b, double *x, const vv_ksp_params *k
 
 /* SNESSolve_KSPONLY as set up at run_nonlinear_matrixfree_split_mp.py:47-53:
  * b = P^-1 F(f); KSPSolve(P^-1 J, b) -> delta; f -= delta unless the linear
- * solve diverged.  f is updated in place. */
+ * solve diverged.  f is updated in place.  With VV_PRECOND_NONE P is the
+ * identity (the reference's preconditioner_type None). */
 int vv_snes_ksponly(vv_ctx *ctx, double *f, const double *fh,
                     const vv_ksp_params *kp, vv_ksp_result *res);
 
