@@ -277,3 +277,30 @@ def test_batched_instances_match_single_contexts(precond, cuda):
         r = cs.ksp_solve(bb[b].contiguous(), x1, 1e-10, 1e-50, 1e5, 1000)
         assert (r.its, r.reason) == (rb[b].its, rb[b].reason), b
         assert torch.equal(x1, xb[b]), b
+
+
+@pytest.mark.parametrize("nx,nv,precond", [(100, 200, 'tensor_kinetic'), (51, 101, 'tensor_kinetic'),
+                                           (100, 201, 'tensor_potential'), (64, 51, 'tensor_potential')])
+def test_transposed_and_fp32_preconditioners(nx, nv, precond, cuda):
+    """<P^-1 x, y> = <x, P^-T y> for the transposed preconditioner the
+    adjoint solve uses, and the FP32 form against the FP64 one, on lengths
+    that are not powers of two (odd ones included)."""
+    import torch
+    print("case", nx, nv, precond, flush=True)
+    g, ctx, H, rng, T = _setup(nx, nv, 'arakawaj4', 9 + nx, cuda, precond=precond)
+    if precond == 'tensor_potential':
+        ctx.precond_update(T(0.3 * np.cos(np.arange(nx) * 2 * np.pi / nx) + 0.05 * rng.standard_normal(nx)))
+    x, y = rng.standard_normal((2, nv, nx))
+    px, pty = ctx.new_field(), ctx.new_field()
+    ctx.precond(T(x), px)
+    ctx.precond_t(T(y), pty)
+    a = float((px.cpu().numpy() * y).sum())
+    b = float((x * pty.cpu().numpy()).sum())
+    assert abs(a - b) <= 1e-12 * np.linalg.norm(px.cpu().numpy()) * np.linalg.norm(y), (a, b)
+    x32 = x.astype(np.float32)
+    p32 = torch.empty((nv, nx), dtype=torch.float32, device=cuda)
+    ctx.precond32(T(x32), p32)
+    p64 = ctx.new_field()
+    ctx.precond(T(x32.astype(np.float64)), p64)
+    ref = p64.cpu().numpy()
+    assert np.abs(p32.double().cpu().numpy() - ref).max() <= 2e-6 * np.abs(ref).max()
