@@ -304,3 +304,35 @@ def test_transposed_and_fp32_preconditioners(nx, nv, precond, cuda):
     ctx.precond(T(x32.astype(np.float64)), p64)
     ref = p64.cpu().numpy()
     assert np.abs(p32.double().cpu().numpy() - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("nx,nv,precond,refine", [(51, 101, 'tensor_kinetic', 'newton'),
+                                                  (100, 200, 'tensor_kinetic', 'ksp'),
+                                                  (50, 101, None, 'newton')])
+def test_mixed_picard_steps_match_oracle(nx, nv, precond, refine, cuda):
+    """The mixed-precision solver (FP32 cycles, FP64 refinement) for three
+    Landau steps on grids of the reference's examples, odd N and no
+    preconditioner included, against the FP64 oracle run: f within 1e-9."""
+    from vivlasov1d_amd.core import Config
+    from vivlasov1d_amd.runner import Runscript
+    print("case", nx, nv, precond, refine, flush=True)
+    L, vmax, ht = 4 * np.pi, 8.0, 0.05
+    prof = lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * np.exp(-0.5 * v ** 2) / math.sqrt(2 * np.pi)
+    g = O.Grid(nx, nv, L, vmax, ht=ht)
+    orun = O.OracleRun(g, O.initial_distribution(g, prof), ksp_rtol=1e-10, snes_atol=1e-10,
+                       preconditioner=precond)
+    cfg = Config({'grid': {'nx': nx, 'nv': nv, 'L': L, 'vmax': vmax, 'ht': ht, 'nt': 3},
+                  'solver': {'poisson_bracket': 'ArakawaJ4', 'laplace_operator': 'CFD4',
+                             'preconditioner_type': 'tensor' if precond else None,
+                             'preconditioner_scheme': precond, 'petsc_ksp_rtol': 1e-10,
+                             'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
+                  'initial_data': {'charge': -1.0}})
+    run = Runscript(cfg, profile=prof, precision='mixed', refine=refine)
+    for s in range(3):
+        eo = orun.step()
+        eg = run.step()
+        print(s, len(eo['picard']), len(eg['picard']), [p['cycles'] for p in eg['picard']], flush=True)
+        for pg in eg['picard']:
+            assert pg['reason'] > 0 and pg['cycles'] >= 1
+        f = run.fc.cpu().numpy().T
+        assert np.abs(f - orun.fc).max() <= 1e-9 * np.abs(orun.fc).max(), s
