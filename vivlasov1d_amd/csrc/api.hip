@@ -398,12 +398,17 @@ int arnoldi_step32(vv_ctx *c, int it)
     float *w = c->basis32 + (size_t)(it + 1) * vs;
     float *X = reinterpret_cast<float *>(c->X);
     const double F = 4.0 * (double)c->N * c->B;
-    if (c->p.precond == VV_PRECOND_NONE) {
-        VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], w), "jacobian32");
-    } else {
-        VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], X), "jacobian32");
-        if (int r = pc_apply32(c, X, w, done, it)) return r;
+    float *ju = c->p.precond == VV_PRECOND_NONE ? w : X;
+    VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], ju), "jacobian32");
+    if (diss_on(c)) {
+        // the dissipation term in FP64 on the widened vector, folded into J u
+        VV_CHECK(launch_widen(c, v, c->w64), "widen");
+        VV_CHECK(launch_zero(c, c->d64, c->N * c->B), "zero");
+        if (int r = diss_jacobian(c, c->w64, &c->gs->scale[it], c->d64, done)) return r;
+        VV_CHECK(launch_narrow_add(c, ju, c->d64), "narrow add");
     }
+    if (c->p.precond != VV_PRECOND_NONE)
+        if (int r = pc_apply32(c, X, w, done, it)) return r;
     VV_PLAUNCH(P_MDOT32, F * (it + 2), it, launch_mdot32(c, it), "mdot32");
     VV_PLAUNCH(P_MAXPY32, F * (it + 3), it, launch_maxpy_norm32(c, it), "maxpy_norm32");
     return VV_OK;
@@ -620,6 +625,10 @@ int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, v
 // lazily allocated buffers of the mixed-precision solver (FP32 basis, FP64 residual)
 int ensure_mixed(vv_ctx *c)
 {
+    if (diss_on(c) && !c->w64) {
+        VV_CHECK(dalloc(&c->w64, (size_t)c->N * c->B), "alloc widened Krylov vector");
+        VV_CHECK(dalloc(&c->d64, (size_t)c->N * c->B), "alloc dissipation term");
+    }
     if (c->basis32) return VV_OK;
     VV_CHECK(dalloc(&c->basis32, (size_t)(c->p.restart + 1) * c->vs), "alloc FP32 basis");
     VV_CHECK(dalloc(&c->R, (size_t)c->N * c->B), "alloc residual");
@@ -658,6 +667,7 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
         rounds = 1;
         rnorm0 = st.rnorm0;
         VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
+        if (int r = diss_jacobian(c, x, nullptr, c->X, nullptr)) return r;
         if (int r = precond(c, c->X, c->R, nullptr)) return r;
         VV_CHECK(launch_residual_restart(c, b, c->R), "residual");
         VV_CHECK(launch_norm2(c, c->R, N, ds_norms(c)), "norm");
@@ -697,6 +707,7 @@ int ksp_solve_mixed_impl(vv_ctx *c, const double *b, double *x, const vv_ksp_par
         ++rounds;
         // true FP64 residual r = b - P^-1 J x
         VV_PLAUNCH(P_JAC, 2 * b_field(c), -1, launch_jacobian(c, x, nullptr, c->X), "jacobian");
+        if (int r = diss_jacobian(c, x, nullptr, c->X, nullptr)) return r;
         if (int r = precond(c, c->X, c->R, nullptr)) return r;
         VV_CHECK(launch_residual_restart(c, b, c->R), "residual");
         VV_CHECK(launch_norm2(c, c->R, N, ds_norms(c)), "norm");
@@ -939,7 +950,7 @@ int vv_ctx_destroy(vv_ctx *c)
                     c->gm,     c->g0,     c->gp,      c->twiddle, c->twiddle32, c->spec, c->pinv, c->pwork,
                     c->X,      c->rhs,    c->D,       c->basis,   c->basis32, c->R,   c->partial, c->dscalar,
                     c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid, c->tb,     c->fave,    c->db1,
-                    c->db2,    c->ex};
+                    c->db2,    c->ex,     c->w64,     c->d64};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);
@@ -1503,7 +1514,6 @@ int vv_ksp_solve_mixed(vv_ctx *c, const double *b, double *x, const vv_ksp_param
     if (kp->max_it < 1) return fail(c, VV_ERR_INVALID, "max_it must be >= 1");
     if (!(mp->inner_rtol > 0.0 && mp->inner_rtol < 1.0) || mp->max_cycles < 1)
         return fail(c, VV_ERR_INVALID, "inner_rtol must be in (0, 1) and max_cycles >= 1");
-    if (int r = no_diss(c, "the mixed-precision solve")) return r;
     return ksp_solve_mixed_impl(c, b, x, kp, mp, res, cycles);
 }
 
@@ -1516,7 +1526,6 @@ int vv_snes_ksponly_mixed(vv_ctx *c, double *f, const double *fh, const vv_ksp_p
     if (!(mp->inner_rtol > 0.0 && mp->inner_rtol < 1.0) || mp->max_cycles < 1)
         return fail(c, VV_ERR_INVALID, "inner_rtol must be in (0, 1) and max_cycles >= 1");
     if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the mixed-precision solve takes one instance");
-    if (int r = no_diss(c, "the mixed-precision solve")) return r;
     if (int r = vv_function(c, f, fh, c->rhs)) return r;
     vv_ksp_result kr{};
     if (int r = ksp_solve_mixed_impl(c, c->rhs, c->D, kp, mp, &kr, cycles)) return r;

@@ -76,7 +76,37 @@ __global__ __launch_bounds__(NT) void collision_factor_kernel(const double *n, c
     a[i] = n[i] * n[i] / (n[i] * e[i] - u[i] * u[i]);
 }
 
+// The FP32 GMRES cycle of the mixed solver takes the dissipation term in
+// FP64: its FP32 Krylov vector is widened, the FP64 kernels above add the
+// term into a zeroed FP64 field, and that is folded into the FP32 J u with
+// one rounding.
+__global__ __launch_bounds__(NT) void widen_kernel(const float *__restrict__ x, double *__restrict__ y, int64_t n)
+{
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k < n) y[k] = (double)x[k];
+}
+
+__global__ __launch_bounds__(NT) void narrow_add_kernel(float *y, const double *__restrict__ d, int64_t n)
+{
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k < n) y[k] = (float)((double)y[k] + d[k]);
+}
+
 }  // namespace
+
+hipError_t launch_widen(vv_ctx *c, const float *x, double *y)
+{
+    const int64_t n = c->N * c->B;
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, c->stream, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow_add(vv_ctx *c, float *y, const double *d)
+{
+    const int64_t n = c->N * c->B;
+    hipLaunchKernelGGL(narrow_add_kernel, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, c->stream, y, d, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_collision(vv_ctx *c, const double *f, const double *fscale, double *y, const double *n,
                             const double *U, const double *a, double factor, const int32_t *done)

@@ -101,6 +101,7 @@ struct vv_ctx {
     // mixed precision (allocated on first use)
     float *basis32;     // FP32 basis, same layout
     double *R;          // FP64 true residual of the refinement loop
+    double *w64, *d64;  // FP32 cycle with a dissipation term: the Krylov vector widened, the term in FP64
     // reductions
     double *partial;    // per instance: part_stride doubles
     int64_t part_stride;
@@ -154,6 +155,8 @@ hipError_t launch_residual(vv_ctx *c, const double *f, const double *fh, double 
 hipError_t launch_jacobian(vv_ctx *c, const double *u, const double *uscale, double *y);
 hipError_t launch_bracket_add(vv_ctx *c, const double *x, double *y, double factor);
 hipError_t launch_jacobian32(vv_ctx *c, const float *u, const double *uscale, float *y);
+hipError_t launch_widen(vv_ctx *c, const float *x, double *y);                  // y = (double) x
+hipError_t launch_narrow_add(vv_ctx *c, float *y, const double *d);           // y = (float)(y + d)
 // y = alpha u + factor {u~, H} on all rows (u~: u on rows 2..nv-3, else 0);
 // H = h2d (2D) or hxv (x only) when given, else H_ave
 hipError_t launch_bracket_t(vv_ctx *c, const double *u, const double *uscale, const double *h2d, const double *hxv,
