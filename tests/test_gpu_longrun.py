@@ -37,19 +37,19 @@ def _prof(x, v):
     return (1 + 0.5 * np.cos(0.5 * x)) * np.exp(-0.5 * v ** 2) / math.sqrt(2 * math.pi)
 
 
-def _threshold_tie(s, ec, eg, crun, run, atol=1e-10):
+def _threshold_tie(s, ec, eg, crun, run, atol=1e-10, band=0.01):
     """False when both sides took the same number of Picard iterations.
     Otherwise the step must be a threshold tie (the deciding residuals
-    straddle snes_atol within 1 %); the GPU run is then set to the oracle's
-    state and True returned."""
+    straddle snes_atol within `band`, 1 % by default); the GPU run is then
+    set to the oracle's state and True returned."""
     kg = len(eg['picard'])
     if ec['picard'] == kg:
         return False
     res = [p['residual'] for p in eg['picard']]
     if ec['picard'] < kg:       # the oracle stopped below atol, the GPU run went on
-        tie = 0.99 * atol <= ec['residual'] < atol <= res[ec['picard'] - 1] <= 1.01 * atol
+        tie = (1 - band) * atol <= ec['residual'] < atol <= res[ec['picard'] - 1] <= (1 + band) * atol
     else:                       # the GPU run stopped below atol, the oracle went on
-        tie = 0.99 * atol <= res[-1] < atol
+        tie = (1 - band) * atol <= res[-1] < atol
     assert tie, (s, ec, res)
     print("threshold tie at step", s, ec, res, flush=True)
     run.set_state(crun.f())
@@ -134,7 +134,10 @@ def test_jeans_strong_config_matches_c_oracle(cuda, tmp_path, monkeypatch):
     its nt = 100 steps against the C restatement: the same Picard count every
     step (threshold ties aside, as above) and f within 1e-8 relative (both
     sides converge to snes_atol; with ksp_rtol 1e-4 the inner solves stop
-    early, so the two converged f differ at the Picard tolerance level)."""
+    early, so the two converged f differ at the Picard tolerance level).
+    With ksp_rtol 1e-4 each Picard residual also carries the inexact
+    solves' error, about 1e-4 of the previous residual (1 % here), so a
+    threshold tie is accepted within 5 % of snes_atol."""
     from oracle import cbaseline
     from vivlasov1d_amd.core import Config
     from vivlasov1d_amd.runner import Runscript
@@ -158,7 +161,7 @@ def test_jeans_strong_config_matches_c_oracle(cuda, tmp_path, monkeypatch):
     ties, worst = 0, 0.0
     for s in range(100):
         ec, eg = crun.step(), run.step()
-        if _threshold_tie(s, ec, eg, crun, run):
+        if _threshold_tie(s, ec, eg, crun, run, band=0.05):
             ties += 1
             continue
         fc = crun.f()
