@@ -342,7 +342,7 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the timestep is not sharded across GPUs in this build "
+        print("bench.py measures one GPU: the timestep is not sharded across GPUs "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

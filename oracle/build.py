@@ -2,7 +2,7 @@
 
 Checker / CPU-baseline infrastructure only.  -ffp-contract=off keeps the
 bracket arithmetic unfused (the reference's order); -march=x86-64-v3 keeps the
-library runnable on any AVX2 host the GPU box may have.
+library runnable on any AVX2 host.
 
 The reference itself (Cython + petsc4py + PETSc + FFTW + MPI) cannot be built
 in this image, so there is no oracle/_ref build.
