@@ -74,3 +74,24 @@ def test_run_writes_its_states(tmp_path, nsave, cuda):
     ref = O.invariants(d["f"][k].T, O.kinetic_hamiltonian(g), d["h1"][k][0], np.zeros(nx), d["phi_int"][k], g, -1.0)
     for key in ("mass", "l2", "energy"):
         assert abs(ref[key] - inv_last[key]) <= 1e-12 * abs(ref[key]), key
+
+
+def test_run_with_external_potential_writes_it(tmp_path, cuda):
+    """phi_ext and h2 of every saved step are the external potential of that
+    step's time with its mean removed (run_base.py:707-747, 766-783)."""
+    from vivlasov1d_amd.runner import Runscript
+    nx, nv, nt = 32, 64, 3
+    path = str(tmp_path / "driven.hdf5")
+    drive = lambda x, t: 0.05 * math.cos(0.5 * x - 0.3 * t)
+    run = Runscript(_cfg(nx, nv, nt, 1), profile=_prof, hdf5_out=path, external=drive)
+    for _ in range(nt):
+        run.step()
+    run.close()
+    d = H.read(path)
+    g = O.Grid(nx, nv, 4 * math.pi, 8.0, ht=0.05)
+    for k in range(nt + 1):
+        p = 0.05 * np.cos(0.5 * g.x - 0.3 * 0.05 * k)
+        p = p - p.mean()
+        assert np.abs(d["phi_ext"][k] - p).max() <= 1e-15 * 8, k
+        assert np.allclose(d["h2"][k].T, np.broadcast_to(O.potential_to_hamiltonian(d["phi_ext"][k], nx)[:, None],
+                                                         (nx, nv)), rtol=0, atol=1e-16 * 8)
