@@ -361,7 +361,7 @@ def main():
                 "bytes_per_launch": d["bytes"] / d["launches"], "avg_launch_us": round(d["avg_us"], 3),
                 "launches": d["launches"]}
         # HBM bytes per launch measured by the PMC runs summarised in profiles/
-        from vivlasov1d_amd.profiles import read_summary
+        from bench_profiles import read_summary
         src = os.path.join("profiles", "%s_%s.txt" % (dom, args.workload))
         tk = read_summary(os.path.join(os.path.dirname(os.path.abspath(__file__)), src))
         if "traffic_bytes_per_launch" in tk:

@@ -17,7 +17,7 @@ Two inputs are summarised:
 
 Usage::
 
-    python -m vivlasov1d_amd.profiles KERNEL TRACE_DIR OUT.txt "COMMAND" \\
+    python bench_profiles.py KERNEL TRACE_DIR OUT.txt "COMMAND" \\
         [--bench BENCH_LOG] [--bench-plain UNPROFILED_BENCH_LOG]
         [--pmc FETCH_DIR WRITE_DIR CAL_FETCH_DIR CAL_WRITE_DIR CAL_MIB "PMC COMMAND"]
 

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 @pytest.mark.parametrize("name", ["maxpy_norm_twostream_1024x2048_J4.txt", "jacobian_twostream_1024x2048_J4.txt"])
 def test_profile_summary(name):
-    from vivlasov1d_amd.profiles import read_summary
+    from bench_profiles import read_summary
     k = read_summary(os.path.join(ROOT, "profiles", name))
     for key in ("kernel", "rocprof_avg_us_timed_region", "bench_avg_us_unprofiled", "traffic_bytes_per_launch",
                 "fetch_factor", "write_factor"):

@@ -25,14 +25,8 @@
 namespace vv {
 namespace {
 
-#ifndef VV_BRACKET_SLIDE
-#define VV_BRACKET_SLIDE 1
-#endif
-#ifndef VV_BRACKET_TY
-#define VV_BRACKET_TY 16
-#endif
 constexpr int TX = 64;
-constexpr int TY = VV_BRACKET_TY;
+constexpr int TY = 16;
 constexpr int HX = TX + 4;
 constexpr int HY = TY + 4;
 constexpr int NT = 256;
@@ -219,7 +213,6 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const int cx = lane + 2;
     const T ht_inv = (T)a.ht_inv, af1 = (T)a.inst[b].af1, af2 = (T)a.inst[b].af2, factor = (T)a.factor;
 
-#if VV_BRACKET_SLIDE
     // Each lane walks down its column: the stencil values of F and H live in
     // a register window (column 0: rows -2..2, columns +-1: rows -1..1,
     // columns +-2: row 0) that moves one row per step, so a row costs 5 + 5
@@ -342,79 +335,6 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     }
 #undef F
 #undef H
-#else
-#define F(di, dj) sF[cy + (dj)][cx + (di)]
-#define H(di, dj) sH[cy + (dj)][cx + (di)]
-    for (int q = 0; q < ROWS_PER_WAVE; ++q) {
-        const int rr = wave * ROWS_PER_WAVE + q;
-        const int j = j0 + rr;
-        if (j >= nv) break;
-        const int cy = rr + 2;
-        const size_t o = (size_t)j * nx + i;
-
-        T base;
-        if (MODE == MODE_RESIDUAL) {
-            const T fder = fin[o] - fhin[o];                 // Fder = (0 + F) - Fh
-            base = T(0) + ht_inv * fder;                     // Y = 0; Y += Fder/dt
-        } else if (MODE == MODE_JACOBIAN) {
-            base = T(0) + ht_inv * F(0, 0);                  // Y = 0; Y += u/dt
-        } else if (MODE == MODE_TRANSPOSED) {
-            base = (T)a.alpha * (fin[o] * us);
-        } else {
-            base = a.store ? T(0) : yout[o];
-        }
-        T out = base;
-        if (MODE == MODE_TRANSPOSED || (j >= 2 && j < nv - 2)) {
-            if (KIND == VV_BRACKET_J1 || KIND == VV_BRACKET_J4) {
-                // PoissonBracket.pyx:100-111 (J1 / second order)
-                const T jpp = (F(1, 0) - F(-1, 0)) * (H(0, 1) - H(0, -1))
-                            - (F(0, 1) - F(0, -1)) * (H(1, 0) - H(-1, 0));
-                const T jpc = F(1, 0) * (H(1, 1) - H(1, -1))
-                            - F(-1, 0) * (H(-1, 1) - H(-1, -1))
-                            - F(0, 1) * (H(1, 1) - H(-1, 1))
-                            + F(0, -1) * (H(1, -1) - H(-1, -1));
-                const T jcp = F(1, 1) * (H(0, 1) - H(1, 0))
-                            - F(-1, -1) * (H(-1, 0) - H(0, -1))
-                            - F(-1, 1) * (H(0, 1) - H(-1, 0))
-                            + F(1, -1) * (H(1, 0) - H(0, -1));
-                if (KIND == VV_BRACKET_J1) {
-                    out = base + factor * (jpp + jpc + jcp) * af1;
-                } else {
-                    // PoissonBracket.pyx:373-389 (J4 = 2 J1 - J2)
-                    const T jcc2 = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
-                                 - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
-                    const T jpc2 = F(2, 0) * (H(1, 1) - H(1, -1))
-                                 - F(-2, 0) * (H(-1, 1) - H(-1, -1))
-                                 - F(0, 2) * (H(1, 1) - H(-1, 1))
-                                 + F(0, -2) * (H(1, -1) - H(-1, -1));
-                    const T jcp2 = F(1, 1) * (H(0, 2) - H(2, 0))
-                                 - F(-1, -1) * (H(-2, 0) - H(0, -2))
-                                 - F(-1, 1) * (H(0, 2) - H(-2, 0))
-                                 + F(1, -1) * (H(2, 0) - H(0, -2));
-                    const T r1 = (jpp + jpc + jcp) * af1;
-                    const T r2 = (jcc2 + jpc2 + jcp2) * af2;
-                    out = base + factor * (T(2) * r1 - r2);
-                }
-            } else {
-                // PoissonBracket.pyx:221-234 (J2)
-                const T jcc = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
-                            - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
-                const T jpc = F(2, 0) * (H(1, 1) - H(1, -1))
-                            - F(-2, 0) * (H(-1, 1) - H(-1, -1))
-                            - F(0, 2) * (H(1, 1) - H(-1, 1))
-                            + F(0, -2) * (H(1, -1) - H(-1, -1));
-                const T jcp = F(1, 1) * (H(0, 2) - H(2, 0))
-                            - F(-1, -1) * (H(-2, 0) - H(0, -2))
-                            - F(-1, 1) * (H(0, 2) - H(-2, 0))
-                            + F(1, -1) * (H(2, 0) - H(0, -2));
-                out = base + factor * (jcc + jpc + jcp) * af2;
-            }
-        }
-        yout[o] = out;
-    }
-#undef F
-#undef H
-#endif
 }
 
 template <int MODE, typename T>

@@ -29,13 +29,6 @@
 // and rounded).
 #include "vv1d_internal.h"
 
-#ifndef VV_FFT_XCD
-#define VV_FFT_XCD 1
-#endif
-#ifndef VV_TRI_PREFETCH
-#define VV_TRI_PREFETCH 0
-#endif
-
 namespace vv {
 namespace {
 
@@ -62,20 +55,13 @@ template <typename C> __device__ __forceinline__ C cneg(C a) { return cmk<C>(-a.
 // the IEEE division sequence.  The dependent chain of the Thomas sweep is
 // shorter (21.5 -> 20.9 us per solve at 1024 x 2048); P^-1 moves by about
 // one ulp (2.8e-16 relative).  The pivots are O(1/dt), far from the
-// denormal and overflow ranges.  VV_FAST_RCP=0 restores the IEEE division.
-#ifndef VV_FAST_RCP
-#define VV_FAST_RCP 1
-#endif
+// denormal and overflow ranges.
 __device__ __forceinline__ double rcp_of(double d)
 {
-#if VV_FAST_RCP
     double r = __builtin_amdgcn_rcp(d);
     r = r * (2.0 - d * r);
     r = r * (2.0 - d * r);
     return r;
-#else
-    return 1.0 / d;
-#endif
 }
 __device__ __forceinline__ float rcp_of(float d) { return 1.0f / d; }
 
@@ -288,7 +274,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
     const int nx = a.nx, R = a.rows;
     // XCD-aware row-block order: the mode-major segments of neighbouring row
     // blocks share L2 lines, so they are kept on one XCD
-    const int j0 = (VV_FFT_XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * R;
+    const int j0 = xcd_tile(blockIdx.x, gridDim.x) * R;
     // Stage 0 of the FFT runs in registers: butterfly (p, j) of complex row p
     // (real rows 2p, 2p+1) takes z_p[j + m nx/4], m = 0..3, exactly the
     // elements the thread loads (coalesced along j); its twiddles are 1.
@@ -373,7 +359,7 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *buf = reinterpret_cast<C *>(lds_raw);
     const int nx = a.nx, R = a.rows, nh = nx >> 1;
-    const int j0 = (VV_FFT_XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * R;   // as in rfft_kernel
+    const int j0 = xcd_tile(blockIdx.x, gridDim.x) * R;   // as in rfft_kernel
     const int npair = R >> 1;
     // Z_k = A_k + i B_k (k <= nx/2) and its Hermitian extension for k > nx/2;
     // every global load of the thread is issued before the LDS stores.
@@ -573,14 +559,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     const C zero = cmk<C>(T(0), T(0));
     C *S = spec + (size_t)k * a.nv;
     const int c = threadIdx.x;
-    // the per-row factors of the thread's chunk, loaded with the spectrum so
-    // the sequential sweep below never waits on memory
-    double clo[M], cmid[M], chi[M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-        clo[q] = cmid[q] = chi[q] = 0.0;
-        if (VV_TRI_PREFETCH && c < P) tri_load<TR>(a, c * M + q, clo[q], cmid[q], chi[q]);
-    }
+    double clo[M], cmid[M], chi[M];   // per-row factors of the thread's chunk
     {
         // n = P*M <= NT*M: every load of the thread in flight before the LDS stores
         C v[M];
@@ -604,7 +583,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
         const int s = c * M, ls = c * (M + 1);
         // forward sweep: y_j + c'_j y_{j+1} = d'_j - a'_j y_{s-1}
         C aj, bj, cj;
-        if (!VV_TRI_PREFETCH) tri_load<TR>(a, s, clo[0], cmid[0], chi[0]);
+        tri_load<TR>(a, s, clo[0], cmid[0], chi[0]);
         tri_row<TR>(a, af, e, s, clo[0], cmid[0], chi[0], aj, bj, cj);
         C r = crecip(bj);
         Cm[0] = cmul(cj, r);
@@ -612,7 +591,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
         Am[0] = cmul(aj, r);
 #pragma unroll
         for (int q = 1; q < M; ++q) {
-            if (!VV_TRI_PREFETCH) tri_load<TR>(a, s + q, clo[q], cmid[q], chi[q]);
+            tri_load<TR>(a, s + q, clo[q], cmid[q], chi[q]);
             tri_row<TR>(a, af, e, s + q, clo[q], cmid[q], chi[q], aj, bj, cj);
             if (q == M - 1) {
                 ae = aj;
