@@ -8,9 +8,9 @@ GMRES(30) rtol 1e-10, Picard atol 1e-10 (at most 20 iterations).  A "step"
 is one full implicit timestep: Picard loop of SNES(ksponly) + GMRES on the
 preconditioned Jacobian, moments and Poisson after every solve.
 
-Timing: W untimed warm-up steps, then K steps between two
-torch.cuda.synchronize() calls.  value = K / elapsed  (timesteps/s) of the
-plain FP64 form.  The mixed-precision form (FP32 GMRES cycles + FP32
+Timing: W untimed warm-up steps, then 3 windows (--repeats) of K steps, each
+between two torch.cuda.synchronize() calls.  value = K / (median window
+time)  (timesteps/s) of the plain FP64 form.  The mixed-precision form (FP32 GMRES cycles + FP32
 preconditioner inside FP64 iterative refinement, FP64 residual / update /
 Poisson) is then timed the same way from the same initial state and reported
 under "mixed" with its L-infinity distance to the FP64 run after W + K steps.
@@ -88,11 +88,14 @@ def cpu_baseline(w, sample_steps, threads):
     return cbaseline.time_sample(w, sample_steps, threads)
 
 
-def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None):
-    """W warm-up steps, then K timed steps; returns the run and its figures."""
+def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeats=None):
+    """W warm-up steps, then `repeats` windows of K timed steps each; returns
+    the run, the median window time, the records of all timed steps and the
+    per-kernel event statistics (summed over the windows)."""
     import torch
     from vivlasov1d_amd.runner import Runscript
-    nt = args.warmup + args.steps
+    repeats = args.repeats if repeats is None else repeats
+    nt = args.warmup + repeats * args.steps
     run = Runscript(make_cfg(w, nt), profile=w["profile"], precision=precision, inner_rtol=args.inner_rtol,
                     refine=refine)
     if ksp_rtol is not None:
@@ -107,18 +110,40 @@ def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None):
     elif kernel not in (None, "none"):
         ctx.profile(tuple(kernel.split(",")))
     nrec0 = len(run.history)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run.step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    windows = []
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run.step()
+        torch.cuda.synchronize()
+        windows.append(time.perf_counter() - t0)
     ctx.profile(())
     kstats = {}
     for k in KERNELS:
         ms, n, by = ctx.profile_read(k)
         if n:
             kstats[k] = dict(ms=ms, launches=n, bytes=by, avg_us=1e3 * ms / n, gbs=by / (ms * 1e-3) / 1e9)
-    return run, t1 - t0, run.history[nrec0:], kstats
+    run.windows = windows
+    return run, sorted(windows)[len(windows) // 2], run.history[nrec0:], kstats
+
+
+def bracket_config2(args):
+    """The metric's bracket HBM GB/s at the nonlinear-Landau workload
+    (256x512, v in [-8, 8], amplitude 0.5, dt = 0.05) for J1, J2 and J4: the
+    residual (24 B/point) and Jacobian-apply (16 B/point) kernels, HIP-event
+    timed over a few timesteps."""
+    out = {}
+    base = workloads()["landau_256x512_J4"]
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.repeats = 5, 2, 1
+    for kind in ("arakawaj1", "arakawaj2", "arakawaj4"):
+        w = dict(base, bracket=kind)
+        brun, _, _, bk = timed_run(w, sub, "double", "jacobian,residual")
+        del brun
+        out[kind] = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
+                         "frac_hbm_peak": round(v["gbs"] / HBM_PEAK_GBS, 4)} for k, v in bk.items()}
+    return out
 
 
 def _its(recs):
@@ -162,7 +187,7 @@ def mixed_figures(w, args, run64, value64):
         del krun
         torch.cuda.empty_cache()
     # continue both runs (untimed) to the comparison step
-    done = args.warmup + args.steps
+    done = args.warmup + args.repeats * args.steps
     for _ in range(max(0, args.compare_steps - done)):
         run64.step()
         mrun.step()
@@ -297,6 +322,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="timed windows of --steps steps each; value is the median window's rate")
     ap.add_argument("--workload", default="twostream_1024x2048_J4")
     ap.add_argument("--kernel", default="maxpy_norm",
                     help="kernels timed with HIP events in the timed FP64 run ('all', 'none' or a comma list of "
@@ -307,6 +334,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="timesteps of the CPU baseline sample (about 10-20 s on 16 host threads)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all)")
+    ap.add_argument("--cpu1-steps", type=int, default=1,
+                    help="timesteps of the one-core CPU sample (0 skips it)")
+    ap.add_argument("--no-config2-bracket", action="store_true",
+                    help="skip the bracket GB/s at the 256x512 nonlinear-Landau size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
     ap.add_argument("--mixed-kernel", default="none", help="kernel timed in the mixed run (e.g. mdot32)")
@@ -370,7 +401,7 @@ def main():
             roof["traffic_over_algorithmic"] = round(tk["traffic_bytes_per_launch"] / roof["bytes_per_launch"], 3)
     bracket = None
     if args.bracket_kernels not in ("", "none"):
-        brun, _, _, bk = timed_run(w, args, "double", args.bracket_kernels)
+        brun, _, _, bk = timed_run(w, args, "double", args.bracket_kernels, repeats=1)
         del brun
         bracket = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
                        "frac_hbm_peak": round(v["gbs"] / HBM_PEAK_GBS, 4),
@@ -379,15 +410,19 @@ def main():
     picard = [len(r["picard"]) for r in recs]
     gm = [p["gmres"] for r in recs for p in r["picard"]]
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(w, args.cpu_steps, args.cpu_threads)
+            if args.cpu1_steps > 0:
+                cpu1 = cpu_baseline(w, args.cpu1_steps, 1)
         except Exception as e:      # reported, never silently replaced
             cpu = {"error": "%s: %s" % (type(e).__name__, e)}
+    bracket2 = None if args.no_config2_bracket else bracket_config2(args)
 
     inv = run.invariants()
     value = args.steps / elapsed
+    run_windows = run.windows
 
     mixed = None
     if not args.no_mixed:
@@ -413,21 +448,27 @@ def main():
                    "preconditioner": "tensor_kinetic", "parallelism": "single GPU"},
         "roofline": roof,
         "step_roofline": {"bytes_per_step": sb / max(1, len(recs)),
-                          "achieved": round(sb / elapsed / 1e9, 1), "unit": "GB/s",
-                          "frac": round(sb / elapsed / 1e9 / HBM_PEAK_GBS, 4)},
+                          "achieved": round(sb / max(1, len(recs)) * args.steps / elapsed / 1e9, 1), "unit": "GB/s",
+                          "frac": round(sb / max(1, len(recs)) * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 4)},
         "iterations": {"picard_per_step": float(np.mean(picard)), "gmres_per_solve": float(np.mean(gm)),
                        "gmres_per_step": float(sum(gm)) / max(1, len(recs))},
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
                         "ms": round(v["ms"], 3)} for k, v in kstats.items()},
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
+        "timing": {"repeats": args.repeats, "window_s": [round(t, 4) for t in run_windows],
+                   "value_from": "median window"},
         "invariants": {k: inv[k] for k in ("mass", "l2", "energy")},
         "bracket": bracket,
+        "bracket_landau_256x512": bracket2,
         "mixed": mixed,
         "batched": batched,
         "adjoint": adjoint,
     }
     if cpu and "value" in cpu and cpu["value"]:
         line["gpu_over_cpu"] = round(value / cpu["value"], 2)
+    if cpu1 and "value" in cpu1 and cpu1["value"]:
+        line["gpu_over_cpu_1core"] = round(value / cpu1["value"], 2)
     print(json.dumps(line), flush=True)
 
 

@@ -137,7 +137,10 @@ int vv_ctx_destroy(vv_ctx *ctx);
 const char *vv_last_error(const vv_ctx *ctx);
 int vv_set_stream(vv_ctx *ctx, void *stream);
 int vv_synchronize(vv_ctx *ctx);
-/* bytes of device scratch the context owns (Krylov basis, spectra, ...) */
+/* bytes of device memory the context owns: what vv_ctx_create allocated
+ * (Krylov basis, spectra, work vectors, ...) plus the buffers allocated on
+ * first use (FP32 basis of the mixed solve, adjoint scratch, explicit-guess
+ * stages, double-bracket fields) */
 int64_t vv_scratch_bytes(const vv_ctx *ctx);
 
 /* PETScVlasovSolverBase.update_previous (PETScVlasovSolver.pyx:142-157):

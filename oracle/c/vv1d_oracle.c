@@ -495,7 +495,8 @@ void vvo_destroy(vvo_run *r)
     free(r);
 }
 
-/* One timestep; stats: [0] Picard iterations, [1] GMRES iterations, [2] final residual. */
+/* One timestep; stats (3 + snes_max_iter entries): [0] Picard iterations, [1] GMRES
+ * iterations, [2] final residual, [3 + q] the residual after Picard iteration q. */
 int vvo_step(vvo_run *r, double *stats)
 {
     const long N = r->N;
@@ -523,6 +524,7 @@ int vvo_step(vvo_run *r, double *stats)
         update_previous(r);
         const double prev = pred;
         pred = calc_residual(r);
+        if (stats) stats[2 + i] = pred;
         if ((pred > prev && i > 1) || pred < r->snes_atol || i >= r->snes_max_iter) {
             if (pred > prev) {
                 memcpy(r->fc, r->fl, sizeof(double) * N);
@@ -540,6 +542,13 @@ int vvo_step(vvo_run *r, double *stats)
 }
 
 void vvo_get_f(const vvo_run *r, double *out) { memcpy(out, r->fc, sizeof(double) * r->N); }
+
+/* replace the state (natural order) and recompute its moments, as after set-up */
+void vvo_set_f(vvo_run *r, const double *f)
+{
+    memcpy(r->fc, f, sizeof(double) * r->N);
+    moments(r);
+}
 
 /* component entry points for cross-checks (H_ave from the given factors) */
 void vvo_set_h(vvo_run *r, const double *h1p, const double *h1h, const double *h2p, const double *h2h)
@@ -561,6 +570,15 @@ void vvo_moments(vvo_run *r, double *n, double *phi, double *h1, double *pnorm)
     memcpy(h1, r->h1c, sizeof(double) * r->nx);
     *pnorm = r->pnorm;
 }
+void vvo_set_threads(int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
 int vvo_threads(void)
 {
 #ifdef _OPENMP

@@ -29,12 +29,14 @@ def load():
         lib.vvo_step.restype = I
         lib.vvo_step.argtypes = [P, PD]
         lib.vvo_get_f.argtypes = [P, PD]
+        lib.vvo_set_f.argtypes = [P, PD]
         lib.vvo_set_h.argtypes = [P, PD, PD, PD, PD]
         lib.vvo_residual.argtypes = [P, PD, PD, PD]
         lib.vvo_jacobian.argtypes = [P, PD, PD]
         lib.vvo_precond.argtypes = [P, PD, PD]
         lib.vvo_moments.argtypes = [P, PD, PD, PD, PD]
         lib.vvo_threads.restype = I
+        lib.vvo_set_threads.argtypes = [I]
         _lib = lib
     return _lib
 
@@ -51,6 +53,7 @@ class CRun:
                  restart=30, threads=0):
         lib = load()
         self.grid = grid
+        self.snes_max_iter = int(snes_max_iter)
         self._v = np.ascontiguousarray(grid.v)
         e = O.kinetic_eigenvalues(grid.nx)
         self._eig = np.ascontiguousarray(np.stack([e.real, e.imag], axis=1).reshape(-1))
@@ -65,14 +68,23 @@ class CRun:
         self.lib = lib
 
     def step(self):
-        st = np.zeros(3)
+        """One timestep: Picard count, GMRES total, final residual and the
+        residual after every Picard iteration (`residuals`)."""
+        st = np.zeros(3 + self.snes_max_iter)
         rc = self.lib.vvo_step(self.h, _p(st))
-        return dict(picard=int(st[0]), gmres=int(st[1]), residual=float(st[2]), rc=rc)
+        k = int(st[0])
+        return dict(picard=k, gmres=int(st[1]), residual=float(st[2]), residuals=[float(x) for x in st[3:3 + k]],
+                    rc=rc)
 
     def f(self):
         out = np.empty((self.grid.nv, self.grid.nx))
         self.lib.vvo_get_f(self.h, _p(out))
         return out
+
+    def set_f(self, f_nat):
+        """Replace the state (natural order (nv, nx)) and recompute its moments."""
+        self._fs = np.ascontiguousarray(f_nat, dtype=np.float64)
+        self.lib.vvo_set_f(self.h, _p(self._fs))
 
     def set_h(self, h1p, h1h, h2p, h2h):
         self._hs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h1p, h1h, h2p, h2h)]
@@ -107,17 +119,24 @@ class CRun:
 
 def time_sample(w, steps=1, threads=0):
     """Time `steps` timesteps of workload w (dict from bench.workloads) after
-    one untimed step; returns the cpu_baseline record of bench.py."""
+    one untimed step, on `threads` OpenMP threads (0: all); returns the
+    cpu_baseline record of bench.py.  The thread count in force before the
+    call is restored."""
+    lib = load()
+    before = lib.vvo_threads()
     g = O.Grid(w["nx"], w["nv"], w["L"], w["vmax"], ht=w["ht"])
     f0 = O.normalise(O.initial_distribution(g, w["profile"]), g)
-    run = CRun(g, f0.T, bracket=w["bracket"], threads=threads)
-    cores = load().vvo_threads()
-    run.step()                                     # warm-up step (page faults, caches)
-    t0 = time.perf_counter()
-    stats = [run.step() for _ in range(steps)]
-    dt = time.perf_counter() - t0
-    run.close()
+    try:
+        run = CRun(g, f0.T, bracket=w["bracket"], threads=threads)
+        cores = lib.vvo_threads()
+        run.step()                                     # warm-up step (page faults, caches)
+        t0 = time.perf_counter()
+        stats = [run.step() for _ in range(steps)]
+        dt = time.perf_counter() - t0
+        run.close()
+    finally:
+        lib.vvo_set_threads(before)
     return {"value": round(steps / dt, 4), "unit": "timesteps/s", "cores": cores, "kind": "port",
             "sample": "%d timestep(s) of the same workload after 1 warm-up step (%.1f s); C/OpenMP "
-                      "restatement of the timestep (oracle/c), %d threads; Picard/GMRES per step %s"
+                      "restatement of the timestep (oracle/c), %d thread(s); Picard/GMRES per step %s"
                       % (steps, dt, cores, [(s["picard"], s["gmres"]) for s in stats])}

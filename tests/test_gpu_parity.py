@@ -45,7 +45,7 @@ def _setup(nx, nv, kind, seed, cuda, precond='tensor_kinetic', restart=30):
     return g, ctx, H, rng, T
 
 
-@pytest.mark.parametrize("nx,nv", GRIDS)
+@pytest.mark.parametrize("nx,nv", GRIDS + [(1024, 2048)])
 @pytest.mark.parametrize("kind", KINDS)
 def test_residual_and_jacobian_bitwise(nx, nv, kind, cuda):
     import torch

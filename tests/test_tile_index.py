@@ -1,0 +1,39 @@
+"""Host-side pin of the bracket kernel's index arithmetic.
+
+The kernel takes its tile origin, halo rows / columns and periodic wraps from
+vivlasov1d_amd/csrc/tile_index.h.  tests/native/tile_index_check.cpp compiles
+that header for the host and enumerates, for every grid nx in [8, 130] x
+nv in [8, 70] and the benchmark / reference-example sizes, every tile, every
+staged halo cell, every x- and v-factor load and every stencil read: each must
+be inside its buffer and be the periodic-in-x neighbour the stencil needs.
+An over-read in a halo cell that feeds no stored output passes every parity
+test; this one fails on it (a single periodic wrap, the fault of an earlier
+version on nx < 66, reports thousands of violations)."""
+
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "native", "tile_index_check.cpp")
+
+
+def _hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    pytest.skip("hipcc not available")
+
+
+def test_bracket_tile_indices_in_bounds(tmp_path):
+    exe = str(tmp_path / "tile_index_check")
+    subprocess.run([_hipcc(), "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", SRC, "-o", exe],
+                   check=True, cwd=str(tmp_path))
+    sizes = ["1024", "2048", "256", "512", "64", "128", "100", "200", "50", "201", "51", "101", "9", "9",
+             "4096", "16", "16", "4096", "512", "1024"]
+    out = subprocess.run([exe, "8", "130", "8", "70"] + sizes, capture_output=True, text=True)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout
+    assert "0 failures" in out.stdout

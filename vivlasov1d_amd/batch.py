@@ -115,6 +115,7 @@ class Bucket:
             picard = np.zeros(self.B, dtype=np.int32)
             gmres = np.zeros(self.B, dtype=np.int64)
             reverted = np.zeros(self.B, dtype=bool)
+            history = []             # residual of every Picard iteration, per instance (NaN: not active)
             i = 0
             while active.any():
                 i += 1
@@ -124,6 +125,7 @@ class Bucket:
                 ctx.moments(self.fc, self.n, self.phi, self.h1c, want_norm=False)
                 self._update_previous()
                 new = self._residual()
+                history.append(np.where(active, new, np.nan))
                 revert = []
                 for b in np.nonzero(active)[0]:
                     picard[b] += 1
@@ -140,7 +142,8 @@ class Bucket:
                     ctx.moments(self.fc, self.n, self.phi, self.h1c, want_norm=False)
                     self._update_previous()
             ctx.set_active(None)
-        rec = dict(step=self.itime, initial=initial, final=pred, picard=picard, gmres=gmres, reverted=reverted)
+        rec = dict(step=self.itime, initial=initial, final=pred, picard=picard, gmres=gmres, reverted=reverted,
+                   residuals=np.array(history).reshape(-1, self.B))
         self.history.append(rec)
         bad = np.nonzero(pred > 10.)[0]
         if bad.size:

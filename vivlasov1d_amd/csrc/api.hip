@@ -113,10 +113,16 @@ int hipfail(vv_ctx *c, hipError_t e, const char *where)
         if (_e != hipSuccess) return hipfail(c, _e, where);    \
     } while (0)
 
+// Device allocation owned by the context; its size is added to the context's
+// ledger, which vv_scratch_bytes reports (buffers allocated on first use
+// included).
 template <typename T>
-hipError_t dalloc(T **p, size_t count)
+hipError_t dalloc(vv_ctx *c, T **p, size_t count)
 {
-    return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+    const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
+    if (e == hipSuccess) c->dev_bytes += (int64_t)bytes;
+    return e;
 }
 
 // count doubles from device to host (through the pinned staging buffer)
@@ -195,8 +201,8 @@ int precond_t(vv_ctx *c, const double *x, double *y, const int32_t *done)
 int ensure_linear(vv_ctx *c)
 {
     if (c->T2) return VV_OK;
-    VV_CHECK(dalloc(&c->T2, (size_t)c->N * c->B), "alloc linearisation scratch");
-    VV_CHECK(dalloc(&c->xs, (size_t)4 * c->p.nx * c->B), "alloc x scratch");
+    VV_CHECK(dalloc(c, &c->T2, (size_t)c->N * c->B), "alloc linearisation scratch");
+    VV_CHECK(dalloc(c, &c->xs, (size_t)4 * c->p.nx * c->B), "alloc x scratch");
     return VV_OK;
 }
 
@@ -626,12 +632,12 @@ int snes_impl(vv_ctx *c, double *f, const double *fh, const vv_ksp_params *kp, v
 int ensure_mixed(vv_ctx *c)
 {
     if (diss_on(c) && !c->w64) {
-        VV_CHECK(dalloc(&c->w64, (size_t)c->N * c->B), "alloc widened Krylov vector");
-        VV_CHECK(dalloc(&c->d64, (size_t)c->N * c->B), "alloc dissipation term");
+        VV_CHECK(dalloc(c, &c->w64, (size_t)c->N * c->B), "alloc widened Krylov vector");
+        VV_CHECK(dalloc(c, &c->d64, (size_t)c->N * c->B), "alloc dissipation term");
     }
     if (c->basis32) return VV_OK;
-    VV_CHECK(dalloc(&c->basis32, (size_t)(c->p.restart + 1) * c->vs), "alloc FP32 basis");
-    VV_CHECK(dalloc(&c->R, (size_t)c->N * c->B), "alloc residual");
+    VV_CHECK(dalloc(c, &c->basis32, (size_t)(c->p.restart + 1) * c->vs), "alloc FP32 basis");
+    VV_CHECK(dalloc(c, &c->R, (size_t)c->N * c->B), "alloc residual");
     return VV_OK;
 }
 
@@ -811,33 +817,33 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     if (ncb > nrows) nrows = ncb;
     c->part_stride = (int64_t)nrows * (kMaxRestart + 2);
     const size_t Bz = (size_t)B;
-    VV_CCHECK(dalloc(&c->inst, Bz), "alloc instance constants");
-    VV_CCHECK(dalloc(&c->active, Bz), "alloc active flags");
-    VV_CCHECK(dalloc(&c->h0, nv), "alloc h0");
-    VV_CCHECK(dalloc(&c->h1p, nx * Bz), "alloc h1p");
-    VV_CCHECK(dalloc(&c->h1h, nx * Bz), "alloc h1h");
-    VV_CCHECK(dalloc(&c->h2p, nx * Bz), "alloc h2p");
-    VV_CCHECK(dalloc(&c->h2h, nx * Bz), "alloc h2h");
-    VV_CCHECK(dalloc(&c->eigen, c->nmodes), "alloc eigen");
+    VV_CCHECK(dalloc(c, &c->inst, Bz), "alloc instance constants");
+    VV_CCHECK(dalloc(c, &c->active, Bz), "alloc active flags");
+    VV_CCHECK(dalloc(c, &c->h0, nv), "alloc h0");
+    VV_CCHECK(dalloc(c, &c->h1p, nx * Bz), "alloc h1p");
+    VV_CCHECK(dalloc(c, &c->h1h, nx * Bz), "alloc h1h");
+    VV_CCHECK(dalloc(c, &c->h2p, nx * Bz), "alloc h2p");
+    VV_CCHECK(dalloc(c, &c->h2h, nx * Bz), "alloc h2h");
+    VV_CCHECK(dalloc(c, &c->eigen, c->nmodes), "alloc eigen");
     const size_t ncoef = pot ? (size_t)nx * Bz : (size_t)nv;
-    VV_CCHECK(dalloc(&c->gm, ncoef), "alloc gm");
-    VV_CCHECK(dalloc(&c->g0, ncoef), "alloc g0");
-    VV_CCHECK(dalloc(&c->gp, ncoef), "alloc gp");
-    if (pot) VV_CCHECK(dalloc(&c->tb, c->N * Bz), "alloc transposed field");
-    VV_CCHECK(dalloc(&c->vgrid, nv), "alloc v grid");
-    VV_CCHECK(dalloc(&c->twiddle, c->fft_n), "alloc twiddle");
-    VV_CCHECK(dalloc(&c->twiddle32, c->fft_n), "alloc twiddle32");
-    VV_CCHECK(dalloc(&c->spec, (size_t)c->nmodes * c->fft_rows * Bz), "alloc spectrum");
-    VV_CCHECK(dalloc(&c->pinv, nx * Bz), "alloc pinv");
-    VV_CCHECK(dalloc(&c->pwork, (3 * (size_t)nx + 1) * Bz), "alloc Poisson scratch");
-    VV_CCHECK(dalloc(&c->X, c->N * Bz), "alloc X");
-    VV_CCHECK(dalloc(&c->rhs, c->N * Bz), "alloc rhs");
-    VV_CCHECK(dalloc(&c->D, c->N * Bz), "alloc D");
-    VV_CCHECK(dalloc(&c->basis, (size_t)(p->restart + 1) * c->vs), "alloc Krylov basis");
-    VV_CCHECK(dalloc(&c->partial, (size_t)c->part_stride * Bz), "alloc partial");
-    VV_CCHECK(dalloc(&c->dscalar, ds_count(B)), "alloc scalars");
-    VV_CCHECK(dalloc(&c->npart, (size_t)64 * nx * Bz), "alloc density partials");
-    VV_CCHECK(dalloc(&c->gs, Bz), "alloc gmres state");
+    VV_CCHECK(dalloc(c, &c->gm, ncoef), "alloc gm");
+    VV_CCHECK(dalloc(c, &c->g0, ncoef), "alloc g0");
+    VV_CCHECK(dalloc(c, &c->gp, ncoef), "alloc gp");
+    if (pot) VV_CCHECK(dalloc(c, &c->tb, c->N * Bz), "alloc transposed field");
+    VV_CCHECK(dalloc(c, &c->vgrid, nv), "alloc v grid");
+    VV_CCHECK(dalloc(c, &c->twiddle, c->fft_n), "alloc twiddle");
+    VV_CCHECK(dalloc(c, &c->twiddle32, c->fft_n), "alloc twiddle32");
+    VV_CCHECK(dalloc(c, &c->spec, (size_t)c->nmodes * c->fft_rows * Bz), "alloc spectrum");
+    VV_CCHECK(dalloc(c, &c->pinv, nx * Bz), "alloc pinv");
+    VV_CCHECK(dalloc(c, &c->pwork, (3 * (size_t)nx + 1) * Bz), "alloc Poisson scratch");
+    VV_CCHECK(dalloc(c, &c->X, c->N * Bz), "alloc X");
+    VV_CCHECK(dalloc(c, &c->rhs, c->N * Bz), "alloc rhs");
+    VV_CCHECK(dalloc(c, &c->D, c->N * Bz), "alloc D");
+    VV_CCHECK(dalloc(c, &c->basis, (size_t)(p->restart + 1) * c->vs), "alloc Krylov basis");
+    VV_CCHECK(dalloc(c, &c->partial, (size_t)c->part_stride * Bz), "alloc partial");
+    VV_CCHECK(dalloc(c, &c->dscalar, ds_count(B)), "alloc scalars");
+    VV_CCHECK(dalloc(c, &c->npart, (size_t)64 * nx * Bz), "alloc density partials");
+    VV_CCHECK(dalloc(c, &c->gs, Bz), "alloc gmres state");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_status), 2 * Bz * sizeof(GmresStatus)),
               "pinned status");
     VV_CCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->host_status_dev), c->host_status, 0),
@@ -991,16 +997,7 @@ int vv_synchronize(vv_ctx *c)
 
 int64_t vv_scratch_bytes(const vv_ctx *c)
 {
-    if (!c) return 0;
-    const int64_t nx = c->p.nx, nv = c->p.nv, B = c->B;
-    int64_t bytes = (int64_t)sizeof(double) * B * ((c->p.restart + 4) * c->N + 64 * nx + 8 * nx + 1 + c->part_stride) +
-                    (int64_t)sizeof(double) * (4 * nv + (int64_t)ds_count(c->B)) +
-                    (int64_t)sizeof(double2) * (B * (int64_t)c->nmodes * c->fft_rows + c->fft_n + c->nmodes) +
-                    (int64_t)sizeof(float2) * c->fft_n + (int64_t)(sizeof(GmresState) + sizeof(InstConst) + 4) * B;
-    if (c->tb) bytes += (int64_t)sizeof(double) * B * (c->N + 2 * nx);
-    if (c->basis32) bytes += (int64_t)sizeof(float) * (c->p.restart + 1) * c->vs + 8 * B * c->N;
-    bytes += (int64_t)sizeof(double) * (c->p.restart + 1) * (c->vs - B * c->N);   // basis stride padding
-    return bytes;
+    return c ? c->dev_bytes : 0;
 }
 
 int vv_set_active(vv_ctx *c, const int32_t *active)
@@ -1264,7 +1261,7 @@ int explicit_setup(vv_ctx *c, double *x, int32_t niter)
     if (!x) return fail(c, VV_ERR_INVALID, "null field");
     if (niter < 1) return fail(c, VV_ERR_INVALID, "niter must be >= 1");
     if (c->B != 1) return fail(c, VV_ERR_UNSUPPORTED, "the explicit steppers take a single-instance context");
-    if (!c->ex) VV_CHECK(dalloc(&c->ex, 6 * (size_t)c->N), "alloc explicit-stepper scratch");
+    if (!c->ex) VV_CHECK(dalloc(c, &c->ex, 6 * (size_t)c->N), "alloc explicit-stepper scratch");
     return VV_OK;
 }
 
@@ -1343,9 +1340,9 @@ int vv_set_dissipation(vv_ctx *c, const vv_dissipation *d)
     if (kind != VV_DISSIPATION_NONE && c->use_graph)
         return fail(c, VV_ERR_UNSUPPORTED, "switch graph replay off before setting a dissipation term");
     if ((kind == VV_DISSIPATION_FFH || kind == VV_DISSIPATION_FHH) && !c->fave) {
-        VV_CHECK(dalloc(&c->fave, (size_t)c->N), "alloc double-bracket average");
-        VV_CHECK(dalloc(&c->db1, (size_t)c->N), "alloc double-bracket scratch");
-        VV_CHECK(dalloc(&c->db2, (size_t)c->N), "alloc double-bracket scratch");
+        VV_CHECK(dalloc(c, &c->fave, (size_t)c->N), "alloc double-bracket average");
+        VV_CHECK(dalloc(c, &c->db1, (size_t)c->N), "alloc double-bracket scratch");
+        VV_CHECK(dalloc(c, &c->db2, (size_t)c->N), "alloc double-bracket scratch");
         VV_CHECK(launch_zero(c, c->fave, c->N), "zero");    // the solver's Fave starts as a zero Vec
     }
     c->diss_kind = kind;

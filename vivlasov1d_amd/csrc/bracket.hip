@@ -25,10 +25,10 @@
 namespace vv {
 namespace {
 
-constexpr int TX = 64;
-constexpr int TY = 16;
-constexpr int HX = TX + 4;
-constexpr int HY = TY + 4;
+constexpr int TX = kBrTX;
+constexpr int TY = kBrTY;
+constexpr int HX = kBrHX;
+constexpr int HY = kBrHY;
 constexpr int NT = 256;
 constexpr int ROWS_PER_WAVE = TY / (NT / 64);
 
@@ -65,18 +65,6 @@ struct BrArgs {
                            // Vec.set(0.) followed by PoissonBracket.function)
 };
 
-// Periodic x index of a tile column: the tile with its halo spans columns
-// i0-2 .. i0+TX+1, which on grids narrower than TX+2 (nx < 66) reaches past
-// nx + nx - 1; the extra columns feed no stored output but must still load
-// from inside the row.
-__device__ __forceinline__ int wrap_x(int gi, int nx)
-{
-    if (gi < 0) gi += nx;
-    if (gi >= nx) gi -= nx;
-    if (gi >= nx) gi %= nx;
-    return gi;
-}
-
 template <int KIND, int MODE, typename T>
 __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 {
@@ -87,11 +75,8 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     __shared__ double sV[HY];         // h0 of the tile's rows
 
     const int nx = a.nx, nv = a.nv;
-    const int ntx = (nx + TX - 1) / TX;
-    const int nty = (nv + TY - 1) / TY;
-    const int t = xcd_tile(blockIdx.x, ntx * nty);
-    const int i0 = (t % ntx) * TX;
-    const int j0 = (t / ntx) * TY;
+    const BrTile tile = br_tile(blockIdx.x, nx, nv);
+    const int i0 = tile.i0, j0 = tile.j0;
     const size_t fo = (size_t)b * a.N, xo = (size_t)b * nx;
     const T *__restrict__ fin = a.f + fo;
     const T *__restrict__ fhin = (MODE == MODE_RESIDUAL) ? a.fh + fo : nullptr;
@@ -110,7 +95,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
                            : 1.0;
     double xf0 = 0.0, xf1 = 0.0, xf2 = 0.0, xf3 = 0.0, vf = 0.0;
     if (threadIdx.x < HX) {
-        const int gi = wrap_x(i0 - 2 + (int)threadIdx.x, nx);
+        const int gi = br_halo_column(tile, (int)threadIdx.x, nx);
         if (a.hxv != nullptr) {
             xf0 = a.hxv[xo + gi];
         } else if (factors) {
@@ -120,8 +105,8 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             xf3 = a.h2h[xo + gi];
         }
     } else if (threadIdx.x < HX + HY) {
-        const int gj = j0 - 2 + ((int)threadIdx.x - HX);
-        if (factors && gj >= 0 && gj < nv) vf = a.h0[gj];
+        const int gj = br_halo_row(tile, (int)threadIdx.x - HX, nv);
+        if (factors && gj >= 0) vf = a.h0[gj];
     }
     constexpr int NQ = (HY * HX + NT - 1) / NT;
     T fa[NQ], fb[NQ];
@@ -135,10 +120,8 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
         if (idx < HY * HX) {
             const int r = idx / HX;
             const int c = idx - r * HX;
-            const int gj = j0 - 2 + r;
-            const int gi = wrap_x(i0 - 2 + c, nx);
-            if (gj >= 0 && gj < nv) {
-                const size_t o = (size_t)gj * nx + gi;
+            const int64_t o = br_halo_offset(tile, r, c, nx, nv);
+            if (o >= 0) {
                 fa[q] = fin[o];
                 if (MODE == MODE_RESIDUAL) fb[q] = fhin[o];
                 if (a.h2d != nullptr) hq[q] = a.h2d[fo + o];
@@ -174,8 +157,8 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
         if (idx < HY * HX) {
             const int r = idx / HX;
             const int c = idx - r * HX;
-            const int gj = j0 - 2 + r;
-            const bool in = (gj >= 0 && gj < nv);
+            const int gj = br_halo_row(tile, r, nv);
+            const bool in = gj >= 0;
             T fv;
             if (MODE == MODE_RESIDUAL) {
                 fv = half * fb[q] + half * fa[q];            // Fave (PETScNLVlasovMP.pyx:44-46)
@@ -340,9 +323,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 template <int MODE, typename T>
 hipError_t launch_mode(vv_ctx *c, BrArgs<T> a)
 {
-    const int ntx = (c->p.nx + TX - 1) / TX;
-    const int nty = (c->p.nv + TY - 1) / TY;
-    dim3 grid(ntx * nty, c->B), block(NT);
+    dim3 grid(br_tiles(c->p.nx, c->p.nv), c->B), block(NT);
     switch (c->p.bracket) {
     case VV_BRACKET_J1:
         launch_timed(c, bracket_kernel<VV_BRACKET_J1, MODE, T>, grid, block, 0, a);

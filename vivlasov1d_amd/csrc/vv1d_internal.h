@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/vv1d.h"
+#include "tile_index.h"
 
 namespace vv {
 
@@ -144,6 +145,7 @@ struct vv_ctx {
     double *db2;        // scratch bracket of the Jacobian
     bool db1_valid;
     double *ex;         // explicit steppers: H | t1 | t2 | t3 | t4 | stage (6 N)
+    int64_t dev_bytes;  // device memory the context has allocated so far
     std::string err;
 };
 
@@ -242,12 +244,5 @@ inline void launch_timed(vv_ctx *c, K kernel, dim3 grid, dim3 block, uint32_t sh
 
 // element stride between the spectra of consecutive instances
 inline int64_t spec_stride(const vv_ctx *c) { return (int64_t)c->nmodes * c->fft_rows; }
-
-// XCD-aware block order: spread consecutive tiles so that each of the 8 XCDs
-// works on one contiguous range of tiles (halo lines stay in that XCD's L2).
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-    if ((nb & 7) != 0) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
-}
 
 }  // namespace vv

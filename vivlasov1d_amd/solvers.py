@@ -105,6 +105,10 @@ class Context:
     def handle(self):
         return self._ptr
 
+    def scratch_bytes(self):
+        """Device bytes the context owns (vv_scratch_bytes)."""
+        return int(self.lib.vv_scratch_bytes(self._ptr))
+
     def call(self, name, *args):
         return _lib.check(getattr(self.lib, name)(self._ptr, *args), self._ptr)
 
