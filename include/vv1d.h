@@ -315,6 +315,21 @@ int vv_potential_to_hamiltonian(vv_ctx *ctx, const double *phi, double *h1);
 int vv_moments(vv_ctx *ctx, const double *f, double *n, double *phi, double *h1,
                double *pnorm);
 
+/* vv_moments without any allocation -- no counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same results as
+ * vv_moments bit for bit (same kernels), but the density partial sums and the
+ * Poisson work vectors live in a file-scope device array of the library
+ * (2.125 MiB on every GPU the library runs on, held until the process ends)
+ * instead of the context's buffers, and the residual norm
+ * ||A phi + (n - mean n) q||_2 of instance b goes to pnorm[b], a DEVICE array
+ * of batch doubles (may be NULL), so nothing is copied to the host.
+ * Limit: nx * batch <= 4096; larger contexts get VV_ERR_INVALID.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (several
+ * streams or host threads) race and must be serialised by the caller. */
+int vv_moments_noalloc(vv_ctx *ctx, const double *f, double *n, double *phi, double *h1,
+                       double *pnorm);
+
 /* VecNorm(NORM_2), VecDot, VecAXPY over len doubles. */
 int vv_norm2(vv_ctx *ctx, const double *x, int64_t len, double *out);
 int vv_dot(vv_ctx *ctx, const double *x, const double *y, int64_t len, double *out);

@@ -106,6 +106,7 @@ SIGNATURES = {
     "vv_poisson_function_mult": (ctypes.c_int, [_P, _P, _P, _P]),
     "vv_potential_to_hamiltonian": (ctypes.c_int, [_P, _P, _P]),
     "vv_moments": (ctypes.c_int, [_P, _P, _P, _P, _P, _PD]),
+    "vv_moments_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "vv_norm2": (ctypes.c_int, [_P, _P, _I64, _PD]),
     "vv_dot": (ctypes.c_int, [_P, _P, _P, _I64, _PD]),
     "vv_axpy": (ctypes.c_int, [_P, _D, _P, _P, _I64]),

@@ -216,7 +216,7 @@ inline double *xs_h1(vv_ctx *c) { return c->xs + 2 * (size_t)c->p.nx * c->B; }
 int h1_of(vv_ctx *c, const double *g)
 {
     const size_t bx = (size_t)c->p.nx * c->B;
-    VV_CHECK(launch_density_partial(c, g), "density");
+    VV_CHECK(launch_density_partial(c, g, nullptr), "density");
     VV_CHECK(launch_poisson(c, nullptr, c->xs, c->xs + bx, c->xs + 2 * bx, nullptr, 1), "poisson");
     return VV_OK;
 }
@@ -749,6 +749,24 @@ int check_params(const vv_params *p)
     return VV_OK;
 }
 
+// Scratch of vv_moments_noalloc: a file-scope device array, so that the entry
+// point needs no allocation at all.  Every GPU holds its own copy until the
+// process ends; its device address is looked up once per context, with the
+// context's device current (vv_ctx_create), and only that address is used
+// from the host.  Layout for X = nx * batch x-points: density partials
+// (kDensityChunks * X doubles) | Poisson work (3 nx + 1 per instance, at most
+// 4 X doubles).
+constexpr int64_t kNoallocMaxX = 4096;          // nx * batch
+constexpr int64_t kNoallocDoubles = (kDensityChunks + 4) * kNoallocMaxX;     // 2.125 MiB
+
+}  // namespace
+
+namespace vv {
+__device__ __align__(256) double scratch[kNoallocDoubles];
+}  // namespace vv
+
+namespace {
+
 int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_ctx **out)
 {
     vv_ctx *c = nullptr;
@@ -919,6 +937,9 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(hipMemcpy(c->twiddle, tw.data(), nf * sizeof(double2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->twiddle32, tw32.data(), nf * sizeof(float2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->pinv, pinv.data(), nx * Bz * sizeof(double), hipMemcpyHostToDevice), "upload");
+    // this device's copy of the allocation-free entry point's scratch array
+    VV_CCHECK(hipGetSymbolAddress(reinterpret_cast<void **>(&c->static_scratch), HIP_SYMBOL(vv::scratch)),
+              "static scratch address");
 #undef VV_CCHECK
     *out = c;
     return VV_OK;
@@ -1454,12 +1475,29 @@ int vv_moments(vv_ctx *c, const double *f, double *n, double *phi, double *h1, d
 {
     if (int r = check_ctx(c)) return r;
     if (!f || !n || !phi || !h1) return fail(c, VV_ERR_INVALID, "null argument");
-    VV_CHECK(launch_density_partial(c, f), "density");
+    VV_CHECK(launch_density_partial(c, f, nullptr), "density");
     VV_CHECK(launch_poisson(c, nullptr, n, phi, h1, ds_pres(c), 1), "poisson");
     if (!pnorm) return VV_OK;
     std::vector<double> tmp(16 * (size_t)c->B);
     if (int r = read_scalars(c, ds_pres(c), 16 * (size_t)(c->B - 1) + 1, tmp.data())) return r;
     for (int b = 0; b < c->B; ++b) pnorm[b] = tmp[16 * (size_t)b];
+    return VV_OK;
+}
+
+int vv_moments_noalloc(vv_ctx *c, const double *f, double *n, double *phi, double *h1, double *pnorm)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!f || !n || !phi || !h1) return fail(c, VV_ERR_INVALID, "null argument");
+    const int64_t X = (int64_t)c->p.nx * c->B;
+    if (X > kNoallocMaxX)
+        return fail(c, VV_ERR_INVALID, "vv_moments_noalloc: nx * batch = " + std::to_string(X) +
+                                           " exceeds its static scratch (nx * batch <= 4096)");
+    if (!c->static_scratch) return fail(c, VV_ERR_HIP, "vv_moments_noalloc: static scratch address unknown");
+    double *npart = c->static_scratch;
+    double *work = c->static_scratch + kDensityChunks * X;
+    // every partial and work entry is written before it is read: nothing to clear
+    VV_CHECK(launch_density_partial(c, f, npart), "density");
+    VV_CHECK(launch_poisson(c, nullptr, n, phi, h1, pnorm, 1, npart, work, 1), "poisson");
     return VV_OK;
 }
 

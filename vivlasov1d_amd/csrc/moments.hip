@@ -28,7 +28,7 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int PT = 1024;     // threads of the single-workgroup Poisson kernel
-constexpr int DCH = 64;      // row chunks of the density reduction
+constexpr int DCH = kDensityChunks;   // row chunks of the density reduction
 
 __global__ __launch_bounds__(NT) void density_kernel(const double *__restrict__ f, int nx, int nv,
                                                      double *__restrict__ npart)
@@ -89,6 +89,7 @@ struct PoiArgs {
     const double *pinv;
     double *work;          // 3*nx + 1 scratch: b | phi | n | mean n
     double *n_out, *phi, *h1, *pres;
+    int pres_stride;       // doubles between the instances' residual norms in pres
 };
 
 // A x for the periodic CFD4 / CFD2 stencils, in the reference's order
@@ -114,7 +115,7 @@ __device__ __forceinline__ void poi_instance(PoiArgs &a)
     if (a.n_out) a.n_out += b * nx;
     if (a.phi) a.phi += b * nx;
     if (a.h1) a.h1 += b * nx;
-    if (a.pres) a.pres += b * 16;
+    if (a.pres) a.pres += b * a.pres_stride;
 }
 
 // one workgroup: n (from the density partials), mean n, b = -(n - mean n) q
@@ -328,16 +329,17 @@ __global__ __launch_bounds__(NT) void invariants_finish_kernel(InvArgs a, int nb
 
 }  // namespace
 
-hipError_t launch_density_partial(vv_ctx *c, const double *f)
+hipError_t launch_density_partial(vv_ctx *c, const double *f, double *npart)
 {
     dim3 grid((c->p.nx + NT - 1) / NT, DCH, c->B);
-    hipLaunchKernelGGL(density_kernel, grid, dim3(NT), 0, c->stream, f, c->p.nx, c->p.nv, c->npart);
+    hipLaunchKernelGGL(density_kernel, grid, dim3(NT), 0, c->stream, f, c->p.nx, c->p.nv,
+                       npart != nullptr ? npart : c->npart);
     return hipGetLastError();
 }
 
 hipError_t launch_density(vv_ctx *c, const double *f, double *n)
 {
-    hipError_t e = launch_density_partial(c, f);
+    hipError_t e = launch_density_partial(c, f, nullptr);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(density_finish_kernel, dim3((c->p.nx + NT - 1) / NT, c->B), dim3(NT), 0, c->stream, c->npart,
                        DCH,
@@ -353,10 +355,11 @@ hipError_t launch_velocity_moments(vv_ctx *c, const double *f, double *u, double
 }
 
 hipError_t launch_poisson(vv_ctx *c, const double *n_in, double *n_out, double *phi, double *h1, double *pres,
-                          int from_part)
+                          int from_part, const double *npart, double *work, int pres_stride)
 {
     PoiArgs a{c->p.nx, c->p.laplace, from_part, c->p.hv, c->p.charge, c->inst,
-              from_part ? c->npart : nullptr, n_in, c->pinv, c->pwork, n_out, phi, h1, pres};
+              from_part ? (npart != nullptr ? npart : c->npart) : nullptr, n_in, c->pinv,
+              work != nullptr ? work : c->pwork, n_out, phi, h1, pres, pres_stride};
     const int nx = c->p.nx;
     hipLaunchKernelGGL(poisson_prep_kernel, dim3(1, c->B), dim3(PT), 0, c->stream, a);
     hipLaunchKernelGGL(poisson_conv_kernel, dim3((nx + NT - 1) / NT, c->B), dim3(NT), 2 * nx * sizeof(double),

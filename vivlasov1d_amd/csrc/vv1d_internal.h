@@ -17,6 +17,7 @@ constexpr int kMaxRestart = 64;     // GMRES restart limit (basis of restart+1 v
 constexpr int kMaxNx = 4096;        // largest x size
 constexpr int kMaxNv = 8192;
 constexpr int kReduceBlocks = 512;  // fixed block count of N-sized reductions
+constexpr int kDensityChunks = 64;  // row chunks of the density partial sums (DCH x nx per instance)
 
 // Host-visible part of the GMRES state; copied to pinned host memory after
 // each chunk of iterations.
@@ -146,6 +147,8 @@ struct vv_ctx {
     bool db1_valid;
     double *ex;         // explicit steppers: H | t1 | t2 | t3 | t4 | stage (6 N)
     int64_t dev_bytes;  // device memory the context has allocated so far
+    double *static_scratch;   // device address of this device's copy of the allocation-free
+                              // entry point's file-scope scratch array
     std::string err;
 };
 
@@ -218,11 +221,17 @@ hipError_t launch_copy(vv_ctx *c, const double *x, double *y, int64_t n);
 hipError_t launch_zero(vv_ctx *c, double *y, int64_t n);
 hipError_t launch_xadd(vv_ctx *c, double *y, const double *w, double alpha);  // y[b][j][i] += alpha w[b][i]
 // moments.hip
-hipError_t launch_density_partial(vv_ctx *c, const double *f);
+// density partials (DCH x nx per instance) into npart, or the context's own
+// buffer when npart is null
+hipError_t launch_density_partial(vv_ctx *c, const double *f, double *npart);
 hipError_t launch_density(vv_ctx *c, const double *f, double *n);
 hipError_t launch_velocity_moments(vv_ctx *c, const double *f, double *u, double *e);
+// Poisson pipeline; npart / work (3 nx + 1 per instance) default to the
+// context's buffers when null; the residual norm of instance b goes to
+// pres[b * pres_stride]
 hipError_t launch_poisson(vv_ctx *c, const double *n_in, double *n_out, double *phi, double *h1, double *pres,
-                          int from_part);
+                          int from_part, const double *npart = nullptr, double *work = nullptr,
+                          int pres_stride = 16);
 hipError_t launch_poisson_mult(vv_ctx *c, const double *x, const double *n, double *y);
 hipError_t launch_pot_to_ham(vv_ctx *c, const double *phi, double *h1);
 hipError_t launch_invariants(vv_ctx *c, const double *f, const double *h0, const double *h1,

@@ -285,6 +285,12 @@ class Context:
         self.call("vv_moments", self.F(f), self.X(n), self.X(phi), self.X(h1), out if want_norm else None)
         return self._per_instance(out) if want_norm else None
 
+    def moments_noalloc(self, f, n, phi, h1, pnorm=None):
+        """vv_moments_noalloc: the moments pipeline without allocation; the
+        Poisson residual norms go to the device tensor pnorm (batch doubles)."""
+        self.call("vv_moments_noalloc", self.F(f), self.X(n), self.X(phi), self.X(h1),
+                  None if pnorm is None else _dev_ptr(pnorm, self.B, "pnorm"))
+
     def norm2(self, x):
         out = ctypes.c_double()
         self.call("vv_norm2", _dev_ptr(x), ctypes.c_int64(x.numel()), ctypes.byref(out))
