@@ -6,19 +6,28 @@ iterate is then reverted).  Two runs that differ only by rounding (the GPU and
 the C restatement sum in different orders) can part in two places, and only
 there:
 
-* threshold tie: at the iteration where they part, the residuals of both
-  sides lie within `band` of snes_atol, the side that stopped below it and the
-  side that went on at or above it;
-* floor tie: both sides have stopped contracting (each side's residual at
-  the parting iteration is at least `stall` times its previous one: the
-  nonlinear iteration sits at its rounding floor), and the two residuals agree
-  within `agree`; the rise that triggers the reference's revert is then
-  rounding noise.
+* threshold tie: at the iteration where they part, the side that stopped is
+  below snes_atol, the side that went on at or above it, and the two
+  residuals agree within `band`;
+* floor tie: both sides have reached the rounding floor of the residual
+  ||F(f)|| + ||A phi + (n - mean n) q||, where the stop decision (below
+  snes_atol or not, rose or not) is rounding noise: at the parting iteration
+  both residuals are within `near` times snes_atol and within a factor
+  `ratio` of each other, or both have stopped contracting (each at least
+  `stall` times its previous value) and agree within `agree`.  On the
+  largest grids (512 x 1024) that floor lies at about snes_atol = 1e-10 and
+  the two implementations' values there differ by up to tens of percent.
 
-Any other parting is a real difference and fails the test."""
+A parting anywhere else (a residual still on its contraction line, or the
+two sides far apart) is a real difference.  A test that accepts a tie also
+checks that the two states still agree to `tie_f_rtol`, the bar of the steps
+where the counts agree (the extra Picard update at the floor moves f by
+about 1e-15 relative), before it resynchronises them."""
+
+tie_f_rtol = 1e-12
 
 
-def parting_kind(rc, rg, atol, band=0.01, stall=0.9, agree=0.05):
+def parting_kind(rc, rg, atol, band=0.05, near=3.0, ratio=2.0, stall=0.9, agree=0.1):
     """rc, rg: residual after every Picard iteration of each side.  Returns
     None when both took the same count, else 'threshold' or 'floor'; raises
     AssertionError for any other parting."""
@@ -26,11 +35,12 @@ def parting_kind(rc, rg, atol, band=0.01, stall=0.9, agree=0.05):
     if kc == kg:
         return None
     q = min(kc, kg) - 1
-    stopped, went_on = (rc[q], rg[q]) if kc < kg else (rg[q], rc[q])
-    if (1 - band) * atol <= stopped < atol <= went_on <= (1 + band) * atol:
+    a, b = rc[q], rg[q]
+    stopped, went_on = (a, b) if kc < kg else (b, a)
+    if stopped < atol <= went_on and went_on - stopped <= band * went_on:
         return 'threshold'
-    if q >= 1:
-        stalled = rc[q] >= stall * rc[q - 1] and rg[q] >= stall * rg[q - 1]
-        if stalled and abs(rc[q] - rg[q]) <= agree * max(rc[q], rg[q]):
-            return 'floor'
+    if max(a, b) <= near * atol and max(a, b) <= ratio * min(a, b):
+        return 'floor'
+    if q >= 1 and a >= stall * rc[q - 1] and b >= stall * rg[q - 1] and abs(a - b) <= agree * max(a, b):
+        return 'floor'
     raise AssertionError("Picard counts differ (%d vs %d) away from a tie: %s / %s" % (kc, kg, rc, rg))
