@@ -372,6 +372,10 @@ int arnoldi_step(vv_ctx *c, int it)
 // FP32 P^-1 (the mixed solver's cycle), as pc_apply
 int pc_apply32(vv_ctx *c, const float *x, float *y, const int32_t *done, int it)
 {
+    if (c->lu64 && !c->lu32) {      // FP32 pivots of the kinetic solve, on first FP32 use
+        VV_CHECK(dalloc(c, &c->lu32, (size_t)c->nmodes * c->fft_rows * c->B), "alloc FP32 pivots");
+        VV_CHECK(launch_tridiag_factor(c, nullptr, c->lu32), "FP32 pivots");
+    }
     float2 *spec = reinterpret_cast<float2 *>(c->spec);
     const double F = 4.0 * (double)c->N * c->B, S = 0.5 * b_spec(c);
     const bool pot = c->tb != nullptr;
@@ -852,6 +856,8 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(dalloc(c, &c->twiddle, c->fft_n), "alloc twiddle");
     VV_CCHECK(dalloc(c, &c->twiddle32, c->fft_n), "alloc twiddle32");
     VV_CCHECK(dalloc(c, &c->spec, (size_t)c->nmodes * c->fft_rows * Bz), "alloc spectrum");
+    if (p->precond == VV_PRECOND_TENSOR_KINETIC)
+        VV_CCHECK(dalloc(c, &c->lu64, (size_t)c->nmodes * c->fft_rows * Bz), "alloc pivots");
     VV_CCHECK(dalloc(c, &c->pinv, nx * Bz), "alloc pinv");
     VV_CCHECK(dalloc(c, &c->pwork, (3 * (size_t)nx + 1) * Bz), "alloc Poisson scratch");
     VV_CCHECK(dalloc(c, &c->X, c->N * Bz), "alloc X");
@@ -937,6 +943,8 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(hipMemcpy(c->twiddle, tw.data(), nf * sizeof(double2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->twiddle32, tw32.data(), nf * sizeof(float2), hipMemcpyHostToDevice), "upload");
     VV_CCHECK(hipMemcpy(c->pinv, pinv.data(), nx * Bz * sizeof(double), hipMemcpyHostToDevice), "upload");
+    // reciprocal LU pivots of the kinetic T_k (fixed for the run)
+    if (c->lu64) VV_CCHECK(launch_tridiag_factor(c, c->lu64, nullptr), "pivots");
     // this device's copy of the allocation-free entry point's scratch array
     VV_CCHECK(hipGetSymbolAddress(reinterpret_cast<void **>(&c->static_scratch), HIP_SYMBOL(vv::scratch)),
               "static scratch address");
@@ -977,7 +985,7 @@ int vv_ctx_destroy(vv_ctx *c)
                     c->gm,     c->g0,     c->gp,      c->twiddle, c->twiddle32, c->spec, c->pinv, c->pwork,
                     c->X,      c->rhs,    c->D,       c->basis,   c->basis32, c->R,   c->partial, c->dscalar,
                     c->npart,  c->gs,     c->T2,      c->xs,   c->vgrid, c->tb,     c->fave,    c->db1,
-                    c->db2,    c->ex,     c->w64,     c->d64};
+                    c->db2,    c->ex,     c->w64,     c->d64,   c->lu64,   c->lu32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->host_status) (void)hipHostFree(c->host_status);

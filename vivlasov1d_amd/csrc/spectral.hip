@@ -679,6 +679,175 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
     for (int j = threadIdx.x; j < a.nv; j += NT) S[j] = sd[j + j / M];
 }
 
+// ---- kinetic solve with precomputed pivots ---------------------------------
+// The kinetic T_k do not change during a run, so their LU factors are formed
+// once per context (KineticFast.pyx:88-90 factors once with zgbtrf as well):
+// without pivoting (the bands are diagonally dominant: |Im b_j| is about
+// twice |a_j| + |c_j|, so zgbtrf's partial pivoting swaps no rows),
+//   u_0 = b_0,  u_j = b_j - (a_j / u_{j-1}) c_{j-1},   rp[k][j] = 1 / u_j,
+// in FP64, one thread per mode (and instance), stored as C (FP64, or rounded
+// for the FP32 solve).  The solve then has no divisions:
+//   forward   y_j = d_j - (a_j rp_{j-1}) y_{j-1},
+//   backward  x_j = rp_j (y_j - c_j x_{j+1}),
+// a_j and c_j being rebuilt from e_k and the per-row factors as in tri_row.
+template <typename T>
+__global__ __launch_bounds__(NT) void tridiag_factor_kernel(TriArgs a, int nmodes, cx_t<T> *__restrict__ rp)
+{
+    const int k = blockIdx.x * NT + threadIdx.x;
+    if (k >= nmodes) return;
+    const double af = a.inst[blockIdx.y].afpc;
+    const double2 e = a.eigen[k];
+    cx_t<T> *R = rp + blockIdx.y * a.S + (size_t)k * a.nv;
+    double lo, mid, hi;
+    double2 aj, bj, cj;
+    tri_load<false>(a, 0, lo, mid, hi);
+    tri_row<false>(a, af, e, 0, lo, mid, hi, aj, bj, cj);
+    double2 r = crecip(bj);
+    R[0] = cmk<cx_t<T>>((T)r.x, (T)r.y);
+    for (int j = 1; j < a.nv; ++j) {
+        const double2 cprev = cj;
+        tri_load<false>(a, j, lo, mid, hi);
+        tri_row<false>(a, af, e, j, lo, mid, hi, aj, bj, cj);
+        r = crecip(csub(bj, cmul(cmul(aj, r), cprev)));
+        R[j] = cmk<cx_t<T>>((T)r.x, (T)r.y);
+    }
+}
+
+template <typename C> __device__ __forceinline__ C shfl_up_cx(C v, int off)
+{
+    return cmk<C>(__shfl_up(v.x, off), __shfl_up(v.y, off));
+}
+template <typename C> __device__ __forceinline__ C shfl_down_cx(C v, int off)
+{
+    return cmk<C>(__shfl_down(v.x, off), __shfl_down(v.y, off));
+}
+
+// One workgroup per mode (blockIdx.x) and instance (blockIdx.y); thread t
+// owns the chunk of M rows [t M, t M + M).  The spectrum row and its pivots
+// are staged in LDS (coalesced loads).  Across chunks the recurrences are
+// affine maps y_out = A + B y_in, composed by a scan over the wavefront's
+// lanes (shuffles) and over the 4 wavefronts (one LDS exchange), first
+// forward, then backward; no thread waits on a division.
+template <int M, typename T>
+__global__ __launch_bounds__(NT) void tridiag_lu_kernel(TriArgs a, const cx_t<T> *__restrict__ rp,
+                                                        cx_t<T> *__restrict__ spec)
+{
+    using C = cx_t<T>;
+    const int32_t dflag = cycle_done(a.done, a.gstride);
+    const int k = blockIdx.x;
+    const double af = a.inst[blockIdx.y].afpc;
+    const double2 e = a.eigen[k];
+    C *S = spec + blockIdx.y * a.S + (size_t)k * a.nv;
+    const C *Rg = rp + blockIdx.y * a.S + (size_t)k * a.nv;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *sd = reinterpret_cast<C *>(lds_raw);      // NT (M+1): right-hand side, then y, then x
+    C *sr = sd + NT * (M + 1);                   // NT (M+1): pivots
+    __shared__ C wa[NT / 64], wb[NT / 64];
+    const C zero = cmk<C>(T(0), T(0)), one = cmk<C>(T(1), T(0));
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    {
+        C v[M], w[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int j = t + q * NT;
+            v[q] = load_or_zero(S, j, j < a.nv);
+            w[q] = load_or_zero(Rg, j, j < a.nv);
+        }
+        if (dflag) return;             // uniform over the block, before any barrier
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int j = t + q * NT;
+            sd[j + j / M] = v[q];     // one pad slot per chunk
+            sr[j + j / M] = w[q];
+        }
+    }
+    __syncthreads();
+    const int s = t * M, ls = t * (M + 1);
+    // sub- and super-diagonals of the chunk's rows (rows past nv: identity rows)
+    C av[M], cv[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        double lo, mid, hi;
+        C bj;
+        tri_load<false>(a, s + q, lo, mid, hi);
+        tri_row<false>(a, af, e, s + q, lo, mid, hi, av[q], bj, cv[q]);
+    }
+    const C rprev = t > 0 ? sr[ls - 2] : zero;   // pivot of row s - 1 (last row of chunk t - 1)
+    // forward map of the chunk: y_{s+M-1} = A + B y_{s-1}
+    C A = sd[ls], B = cneg(cmul(av[0], rprev));
+#pragma unroll
+    for (int q = 1; q < M; ++q) {
+        const C m = cmul(av[q], sr[ls + q - 1]);
+        A = csub(sd[ls + q], cmul(m, A));
+        B = cneg(cmul(m, B));
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {     // inclusive scan, earlier lanes first
+        const C pa = shfl_up_cx(A, off), pb = shfl_up_cx(B, off);
+        if (lane >= off) {
+            A = cadd(A, cmul(B, pa));
+            B = cmul(B, pb);
+        }
+    }
+    if (lane == 63) {
+        wa[wave] = A;
+        wb[wave] = B;
+    }
+    C ea = shfl_up_cx(A, 1), eb = shfl_up_cx(B, 1);   // lanes 0 .. lane-1
+    if (lane == 0) {
+        ea = zero;
+        eb = one;
+    }
+    __syncthreads();
+    C y = zero;                                   // y at the end of the previous wavefronts' rows
+    for (int w = 0; w < wave; ++w) y = cadd(wa[w], cmul(wb[w], y));
+    y = cadd(ea, cmul(eb, y));                    // y_{s-1}
+    y = csub(sd[ls], cmul(cmul(av[0], rprev), y));
+    sd[ls] = y;
+#pragma unroll
+    for (int q = 1; q < M; ++q) {
+        y = csub(sd[ls + q], cmul(cmul(av[q], sr[ls + q - 1]), y));
+        sd[ls + q] = y;
+    }
+    // backward map of the chunk: x_s = G + D x_{s+M}
+    C G = cmul(sr[ls + M - 1], sd[ls + M - 1]), D = cneg(cmul(sr[ls + M - 1], cv[M - 1]));
+#pragma unroll
+    for (int q = M - 2; q >= 0; --q) {
+        const C rc = cmul(sr[ls + q], cv[q]);
+        G = csub(cmul(sr[ls + q], sd[ls + q]), cmul(rc, G));
+        D = cneg(cmul(rc, D));
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {     // inclusive scan, later lanes first
+        const C na = shfl_down_cx(G, off), nb = shfl_down_cx(D, off);
+        if (lane + off < 64) {
+            G = cadd(G, cmul(D, na));
+            D = cmul(D, nb);
+        }
+    }
+    __syncthreads();                              // every thread has read wa / wb
+    if (lane == 0) {
+        wa[wave] = G;
+        wb[wave] = D;
+    }
+    C xa = shfl_down_cx(G, 1), xb = shfl_down_cx(D, 1);   // lanes lane+1 .. 63
+    if (lane == 63) {
+        xa = zero;
+        xb = one;
+    }
+    __syncthreads();
+    C x = zero;                                   // x at the first row of the next wavefronts
+    for (int w = NT / 64 - 1; w > wave; --w) x = cadd(wa[w], cmul(wb[w], x));
+    x = cadd(xa, cmul(xb, x));                    // x_{s+M}
+#pragma unroll
+    for (int q = M - 1; q >= 0; --q) {
+        x = cmul(sr[ls + q], csub(sd[ls + q], cmul(cv[q], x)));
+        sd[ls + q] = x;
+    }
+    __syncthreads();
+    for (int j = t; j < a.nv; j += NT) S[j] = sd[j + j / M];
+}
+
 // out = in^T for every instance (blockIdx.z): in holds R rows of C, out C
 // rows of R.  32x32 tiles through LDS (one pad column), loads and stores
 // coalesced along the contiguous axis of each side; clamped addresses for the
@@ -832,9 +1001,41 @@ void tridiag_shape(int nv, int *P, int *M)
     }
 }
 
+// rows per thread of the pivoted solve (NT threads per mode)
+int lu_rows(int nv)
+{
+    const int m = (nv + NT - 1) / NT;
+    return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : 16;
+}
+
+template <typename T>
+const cx_t<T> *lu_factors(const vv_ctx *c);
+template <> const double2 *lu_factors<double>(const vv_ctx *c) { return c->lu64; }
+template <> const float2 *lu_factors<float>(const vv_ctx *c) { return c->lu32; }
+
+template <typename T>
+hipError_t tridiag_lu_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
+{
+    TriArgs a{c->fft_rows, NT, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp,
+              c->coef_stride, c->cf_lo, c->cf_mid, c->cf_hi, c->inst, c->ht_inv, done};
+    const int M = lu_rows(c->fft_rows);
+    const size_t lds = (size_t)2 * NT * (M + 1) * sizeof(cx_t<T>);
+    const cx_t<T> *rp = lu_factors<T>(c);
+    dim3 grid(c->nmodes, c->B), block(NT);
+    switch (M) {
+    case 1: launch_timed(c, tridiag_lu_kernel<1, T>, grid, block, (uint32_t)lds, a, rp, spec); break;
+    case 2: launch_timed(c, tridiag_lu_kernel<2, T>, grid, block, (uint32_t)lds, a, rp, spec); break;
+    case 4: launch_timed(c, tridiag_lu_kernel<4, T>, grid, block, (uint32_t)lds, a, rp, spec); break;
+    case 8: launch_timed(c, tridiag_lu_kernel<8, T>, grid, block, (uint32_t)lds, a, rp, spec); break;
+    default: launch_timed(c, tridiag_lu_kernel<16, T>, grid, block, (uint32_t)lds, a, rp, spec); break;
+    }
+    return hipGetLastError();
+}
+
 template <typename T, bool TR = false>
 hipError_t tridiag_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
 {
+    if (!TR && lu_factors<T>(c) != nullptr) return tridiag_lu_t<T>(c, spec, done);
     int P, M;
     tridiag_shape(c->fft_rows, &P, &M);
     TriArgs a{c->fft_rows, P, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp,
@@ -891,6 +1092,16 @@ hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t
 hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done)
 {
     return tridiag_t<float>(c, spec, done);
+}
+
+hipError_t launch_tridiag_factor(vv_ctx *c, double2 *rp64, float2 *rp32)
+{
+    TriArgs a{c->fft_rows, NT, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp,
+              c->coef_stride, c->cf_lo, c->cf_mid, c->cf_hi, c->inst, c->ht_inv, nullptr};
+    dim3 grid((c->nmodes + NT - 1) / NT, c->B);
+    if (rp64) hipLaunchKernelGGL(tridiag_factor_kernel<double>, grid, dim3(NT), 0, c->stream, a, c->nmodes, rp64);
+    if (rp32) hipLaunchKernelGGL(tridiag_factor_kernel<float>, grid, dim3(NT), 0, c->stream, a, c->nmodes, rp32);
+    return hipGetLastError();
 }
 
 hipError_t launch_transpose(vv_ctx *c, const double *in, double *out, bool back, const int32_t *done)

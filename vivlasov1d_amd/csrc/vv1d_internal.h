@@ -92,6 +92,11 @@ struct vv_ctx {
     double2 *twiddle;   // exp(-2 pi i m / fft_n), m = 0..fft_n-1
     float2 *twiddle32;  // the same, rounded to float (FP32 preconditioner)
     double2 *spec;      // nmodes * fft_rows complex spectrum per instance, mode-major
+    // kinetic preconditioner: reciprocal LU pivots 1/u_j of every T_k, same
+    // layout as the spectrum (FP64 at creation; FP32 on first FP32 use);
+    // null for the potential preconditioner (its bands change every step)
+    double2 *lu64;
+    float2 *lu32;
     // Poisson
     double *pinv;       // first row of the mean-free circulant inverse (nx per instance)
     double *pwork;      // Poisson scratch (3*nx + 1 per instance)
@@ -197,6 +202,8 @@ hipError_t launch_rfft32(vv_ctx *c, const float *x, float2 *spec, const int32_t 
 hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done);
 hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t *done);
 // potential preconditioner: field (nv rows of nx) <-> transposed (nx rows of nv)
+// reciprocal LU pivots of the kinetic T_k into rp64 (FP64) and / or rp32 (rounded)
+hipError_t launch_tridiag_factor(vv_ctx *c, double2 *rp64, float2 *rp32);
 hipError_t launch_transpose(vv_ctx *c, const double *in, double *out, bool back, const int32_t *done);
 hipError_t launch_transpose32(vv_ctx *c, const float *in, float *out, bool back, const int32_t *done);
 hipError_t launch_potential_bands(vv_ctx *c, const double *phi);
