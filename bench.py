@@ -317,6 +317,14 @@ def batched_figures(args):
     return out
 
 
+def progress(msg):
+    """One line per finished leg on stderr (stdout carries only the JSON line)."""
+    print("bench: %s (%.0f s)" % (msg, time.perf_counter() - T_START), file=sys.stderr, flush=True)
+
+
+T_START = time.perf_counter()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +391,7 @@ def main():
     w = workloads()[args.workload]
     N = w["nx"] * w["nv"]
     run, elapsed, recs, kstats = timed_run(w, args, "double", args.kernel)
+    progress("FP64 timed")
     dom = max(kstats, key=lambda k: kstats[k]["ms"]) if kstats else None
     roof = None
     if dom:
@@ -402,6 +411,7 @@ def main():
     bracket = None
     if args.bracket_kernels not in ("", "none"):
         brun, _, _, bk = timed_run(w, args, "double", args.bracket_kernels, repeats=1)
+        progress("bracket kernels timed")
         del brun
         bracket = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
                        "frac_hbm_peak": round(v["gbs"] / HBM_PEAK_GBS, 4),
@@ -414,11 +424,14 @@ def main():
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(w, args.cpu_steps, args.cpu_threads)
+            progress("CPU baseline (all cores)")
             if args.cpu1_steps > 0:
                 cpu1 = cpu_baseline(w, args.cpu1_steps, 1)
+                progress("CPU baseline (1 core)")
         except Exception as e:      # reported, never silently replaced
             cpu = {"error": "%s: %s" % (type(e).__name__, e)}
     bracket2 = None if args.no_config2_bracket else bracket_config2(args)
+    progress("bracket at 256x512")
 
     inv = run.invariants()
     value = args.steps / elapsed
@@ -427,14 +440,17 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_figures(w, args, run, value)
+        progress("mixed precision")
         del run
         torch.cuda.empty_cache()
     batched = None
     if not args.no_batched:
         batched = batched_figures(args)
+        progress("batched ensemble")
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_figures(args)
+        progress("adjoint")
 
     line = {
         "metric": "timesteps/s at 1024x2048 J4 (plain FP64)" if args.workload.startswith("twostream_1024")
