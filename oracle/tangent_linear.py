@@ -78,17 +78,42 @@ def tangent_linear(kind, states, d0, h0, grid, charge, laplace='cfd4'):
 
 
 def field_energy(f, grid, charge, laplace='cfd4'):
-    """1/2 sum_i (phi_{i+1} - phi_i)^2 / hx (potential.pyx:57-59 without the charge)."""
+    """E = 1/2 q sum_i (phi_{i+1} - phi_i)^2 / hx of the potential of f
+    (potential.pyx:48-59, calculate_energy, with its charge factor)."""
     phi = O.poisson_solve(O.poisson_rhs(O.density(f, grid), charge, grid.nx), grid.hx, laplace)
     d = np.roll(phi, -1) - phi
-    return 0.5 * (d * d).sum() / grid.hx
+    return 0.5 * charge * (d * d).sum() / grid.hx
 
 
 def field_energy_gradient(f, grid, charge, laplace='cfd4'):
     """dE/df, by the chain rule through density -> rhs -> phi (all linear)."""
     phi = O.poisson_solve(O.poisson_rhs(O.density(f, grid), charge, grid.nx), grid.hx, laplace)
-    g = ((phi - np.roll(phi, 1)) - (np.roll(phi, -1) - phi)) / grid.hx      # dE/dphi
+    g = charge * ((phi - np.roll(phi, 1)) - (np.roll(phi, -1) - phi)) / grid.hx      # dE/dphi
     # phi = S b with S the (symmetric) mean-free solve, b = -q (n - mean n), n = hv sum_j f
     w = O.poisson_solve(g, grid.hx, laplace)                                 # S^T g
     dn = -charge * (w - w.mean())
     return np.broadcast_to((grid.hv * dn)[:, None], f.shape).copy()
+
+
+def tangent_linear_mf(kind, states, d0, h0, grid, charge, laplace='cfd4', rtol=1e-13):
+    """The tangent-linear propagation of tangent_linear() without assembling
+    A: each step solves A d1 = -A0 d0 by SciPy's GMRES on the matrix-free
+    linear_apply, preconditioned by the oracle's kinetic P^-1 (SciPy band
+    solves), to rtol.  For grids where the dense matrices do not fit (256 x 512
+    and up); pinned against the dense model on small grids."""
+    from scipy.sparse.linalg import LinearOperator, gmres
+    N = grid.nx * grid.nv
+    shape = (grid.nx, grid.nv)
+    pc = O.Preconditioner(grid, 'scipy')
+    d = np.asarray(d0, dtype=np.float64).reshape(shape)
+    for n in range(len(states) - 1):
+        f0, f1 = states[n], states[n + 1]
+        A = LinearOperator((N, N), matvec=lambda u: linear_apply(kind, u.reshape(shape), f0, f1, h0, grid, charge,
+                                                                 'A', laplace).ravel(), dtype=np.float64)
+        M = LinearOperator((N, N), matvec=lambda u: pc.apply(u.reshape(shape)).ravel(), dtype=np.float64)
+        b = -linear_apply(kind, d, f0, f1, h0, grid, charge, 'A0', laplace).ravel()
+        x, info = gmres(A, b, M=M, rtol=rtol, atol=0.0, restart=60, maxiter=50)
+        if info != 0:
+            raise RuntimeError("tangent-linear GMRES did not converge (info %d)" % info)
+        d = x.reshape(shape)
+    return d

@@ -171,3 +171,89 @@ def test_gradient_matches_finite_differences(cuda):
         fd = (energy(a0 + h * d).item() - energy(a0 - h * d).item()) / (2 * h)
         got = float((grad * d.cpu().numpy()).sum())
         assert abs(fd - got) <= 1e-5 * abs(fd)
+
+
+def test_config5_gradient_every_amplitude(cuda):
+    """The adjoint workload at its full size: bump-on-tail base state
+    (k = 0.3, 256 x 512, v in [-10, 10]) plus sum_m a_m cos(m k x) M(v),
+    m = 1..16, a_m ~ N(0, 1e-3) from default_rng(0); J4, dt = 0.05, 200
+    steps, checkpoint every 20; loss = the field energy of the final state
+    (potential.pyx:48-59, charge -1); Picard / GMRES tolerances snes_atol
+    1e-14, ksp_rtol 1e-13 so the forward map is smooth far below the
+    difference step.  Every one of the 16 components of dE/da from the
+    adjoint against a central difference (h = 1e-6): |fd - grad_m| <= 1e-6
+    max |grad| (and the relative error printed)."""
+    import torch
+    from vivlasov1d_amd.adjoint import FieldEnergy, ImplicitTimesteps, Stepper, perturbation_basis
+    from vivlasov1d_amd.core import Config, bump_on_tail, initial_distribution, normalise
+    nx, nv, k, K = 256, 512, 0.3, 200
+    cfg = Config({'grid': {'nx': nx, 'nv': nv, 'L': 2 * math.pi / k, 'vmax': 10.0, 'ht': 0.05, 'nt': K},
+                  'solver': {'poisson_bracket': 'arakawaj4', 'laplace_operator': 'CFD4',
+                             'preconditioner_type': 'tensor', 'preconditioner_scheme': 'tensor_kinetic',
+                             'petsc_ksp_rtol': 1e-13, 'petsc_snes_atol': 1e-14, 'petsc_snes_max_iter': 30},
+                  'initial_data': {'charge': -1.0}})
+    st = Stepper(cfg, ksp_rtol_adj=1e-13, checkpoint=20)
+    base = torch.from_numpy(normalise(initial_distribution(st.grid, bump_on_tail(k, 0.0)), st.grid)).to(cuda)
+    basis = torch.from_numpy(perturbation_basis(st.grid, k, 16)).to(cuda)
+    a0 = torch.from_numpy(np.random.default_rng(0).normal(0.0, 1e-3, 16)).to(cuda)
+
+    def energy(a):
+        return FieldEnergy.apply(ImplicitTimesteps.apply(base + (a[:, None, None] * basis).sum(0), st, K), st)
+
+    a = a0.clone().requires_grad_(True)
+    E = energy(a)
+    E.backward()
+    grad = a.grad.cpu().numpy()
+    gmax = np.abs(grad).max()
+    print("case config5 loss", E.item(), "grad", grad, flush=True)
+    h = 1e-6
+    worst = 0.0
+    with torch.no_grad():
+        for m in range(16):
+            e = torch.zeros(16, dtype=torch.float64, device=cuda)
+            e[m] = h
+            fd = (energy(a0 + e).item() - energy(a0 - e).item()) / (2 * h)
+            worst = max(worst, abs(fd - grad[m]) / gmax)
+            print("  a_%d: fd %.12e adjoint %.12e rel %.2e" % (m + 1, fd, grad[m], abs(fd - grad[m]) / abs(fd)),
+                  flush=True)
+            assert abs(fd - grad[m]) <= 1e-6 * gmax, (m, fd, grad[m])
+    print("worst |fd - grad| / max |grad|", worst, flush=True)
+
+
+def test_config5_size_gradient_matches_matrix_free_tangent_linear(cuda):
+    """At the adjoint workload's grid (256 x 512, bump-on-tail k = 0.3 with
+    a 1e-3 cos(k x) perturbation), 3 timesteps: the adjoint gradient dE/df0
+    along two random directions d in the span of the 16 perturbation modes
+    against <dE/df_K, d_K> from the CPU FP64
+    matrix-free tangent-linear model (oracle.tangent_linear.tangent_linear_mf,
+    pinned against the dense model), <= 1e-8 relative."""
+    import torch
+    from vivlasov1d_amd.adjoint import FieldEnergy, ImplicitTimesteps, Stepper, perturbation_basis
+    from vivlasov1d_amd.core import Config, bump_on_tail, initial_distribution, normalise
+    nx, nv, k, K = 256, 512, 0.3, 3
+    cfg = Config({'grid': {'nx': nx, 'nv': nv, 'L': 2 * math.pi / k, 'vmax': 10.0, 'ht': 0.05, 'nt': K},
+                  'solver': {'poisson_bracket': 'arakawaj4', 'laplace_operator': 'CFD4',
+                             'preconditioner_type': 'tensor', 'preconditioner_scheme': 'tensor_kinetic',
+                             'petsc_ksp_rtol': 1e-13, 'petsc_snes_atol': 1e-14, 'petsc_snes_max_iter': 30},
+                  'initial_data': {'charge': -1.0}})
+    st = Stepper(cfg, ksp_rtol_adj=1e-13, checkpoint=2)
+    f0 = torch.from_numpy(normalise(initial_distribution(st.grid, bump_on_tail(k, 1e-3)), st.grid)).to(cuda)
+    f0.requires_grad_(True)
+    E = FieldEnergy.apply(ImplicitTimesteps.apply(f0, st, K), st)
+    E.backward()
+    grad = f0.grad.cpu().numpy().T
+    states = [s.cpu().numpy().T.copy() for s in st.advance(f0.detach(), K)]
+    g = O.Grid(nx, nv, 2 * math.pi / k, 10.0, ht=0.05)
+    assert np.array_equal(g.v, st.grid.v) and g.hx == st.grid.hx
+    h0 = O.kinetic_hamiltonian(g)
+    gE = TL.field_energy_gradient(states[-1], g, -1.0)
+    assert abs(TL.field_energy(states[-1], g, -1.0) - E.item()) <= 1e-11 * abs(E.item())
+    rng = np.random.default_rng(12)
+    basis = perturbation_basis(st.grid, k, 16)          # (16, nv, nx): the amplitudes' directions
+    for q in range(2):
+        d = np.tensordot(rng.standard_normal(16), basis, axes=1).T
+        dK = TL.tangent_linear_mf('arakawaj4', states, d, h0, g, -1.0)
+        ref = (gE * dK).sum()
+        got = (grad * d).sum()
+        print("case tlm 256x512 direction", q, ref, got, abs(got - ref) / abs(ref), flush=True)
+        assert abs(got - ref) <= 1e-8 * abs(ref)

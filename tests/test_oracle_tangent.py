@@ -96,3 +96,37 @@ def test_tangent_linear_matches_picard_differences():
     fd = (fp - fm) / (2 * eps)
     err = np.abs(fd - dK).max() / np.abs(dK).max()
     assert err <= 1e-6, err
+
+
+def test_field_energy_has_the_reference_charge_factor():
+    """potential.pyx:48-59: E = 0.5 * charge * sum (phi_{i+1} - phi_i)^2 / hx;
+    with q = +1 the sum itself, with q = -1 its negative (phi itself flips
+    sign with q, so the squares do not)."""
+    g, f0, f1, rng = _pair(16, 24, 5)
+    phi = O.poisson_solve(O.poisson_rhs(O.density(f1, g), 1.0, g.nx), g.hx, 'cfd4')
+    s = 0.0
+    for i in range(g.nx):
+        s += (phi[(i + 1) % g.nx] - phi[i]) ** 2
+    assert abs(TL.field_energy(f1, g, 1.0) - 0.5 * s / g.hx) <= 1e-13 * abs(s / g.hx)
+    assert abs(TL.field_energy(f1, g, -1.0) + 0.5 * s / g.hx) <= 1e-13 * abs(s / g.hx)
+
+
+def test_matrix_free_tangent_linear_matches_dense():
+    """The matrix-free model (SciPy GMRES on linear_apply, kinetic P^-1)
+    propagates a perturbation through 3 steps like the dense one."""
+    nx, nv = 16, 32
+    g = O.Grid(nx, nv, 2 * math.pi / 0.3, 8.0, ht=0.05)
+    prof = lambda x, v: (1 + 0.3 * np.cos(0.3 * x)) * np.exp(-0.5 * v ** 2) / math.sqrt(2 * math.pi)
+    r = O.OracleRun(g, O.normalise(O.initial_distribution(g, prof), g), bracket='arakawaj4', ksp_rtol=1e-14,
+                    snes_atol=1e-14, snes_max_iter=30)
+    states = [r.fc.copy()]
+    for _ in range(3):
+        r.step()
+        states.append(r.fc.copy())
+    h0 = O.kinetic_hamiltonian(g)
+    rng = np.random.default_rng(8)
+    d0 = rng.standard_normal((nx, nv))
+    dense = TL.tangent_linear('arakawaj4', states, d0, h0, g, -1.0)
+    mf = TL.tangent_linear_mf('arakawaj4', states, d0, h0, g, -1.0)
+    err = np.abs(mf - dense).max() / np.abs(dense).max()
+    assert err <= 1e-10, err

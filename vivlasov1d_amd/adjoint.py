@@ -109,8 +109,9 @@ class ImplicitTimesteps(torch.autograd.Function):
 
 
 class FieldEnergy(torch.autograd.Function):
-    """E = 1/2 sum_i (phi_{i+1} - phi_i)^2 / hx of the potential of f (the
-    potential.pyx:57-59 form without the charge factor)."""
+    """E = 1/2 q sum_i (phi_{i+1} - phi_i)^2 / hx of the potential of f: the
+    field energy of potential.pyx:48-59 (calculate_energy), charge factor
+    included (negative for q = -1)."""
 
     @staticmethod
     def forward(ctx, f, stepper):
@@ -119,14 +120,14 @@ class FieldEnergy(torch.autograd.Function):
         c.moments(f.detach().contiguous(), n, phi, h1, want_norm=False)
         d = torch.roll(phi, -1) - phi
         ctx.stepper, ctx.phi = stepper, phi
-        return 0.5 * (d * d).sum() / stepper.grid.hx
+        return 0.5 * stepper.run.charge * (d * d).sum() / stepper.grid.hx
 
     @staticmethod
     def backward(ctx, grad):
         st, phi = ctx.stepper, ctx.phi
         g = st.grid
-        # dE/dphi = D^T D phi / hx (D: periodic forward difference)
-        dphi = ((phi - torch.roll(phi, 1)) - (torch.roll(phi, -1) - phi)) / g.hx
+        # dE/dphi = q D^T D phi / hx (D: periodic forward difference)
+        dphi = st.run.charge * ((phi - torch.roll(phi, 1)) - (torch.roll(phi, -1) - phi)) / g.hx
         # dE/dn = (-q) (I - mean) C dE/dphi: the Poisson solve of the moments
         # pipeline applied to dE/dphi (mean-free already); dn/df = hv sum_j
         w = st.ctx.new_x()
