@@ -19,7 +19,17 @@ Usage::
 
     python bench_profiles.py KERNEL TRACE_DIR OUT.txt "COMMAND" \\
         [--bench BENCH_LOG] [--bench-plain UNPROFILED_BENCH_LOG]
-        [--pmc FETCH_DIR WRITE_DIR CAL_FETCH_DIR CAL_WRITE_DIR CAL_MIB "PMC COMMAND"]
+        [--pmc FETCH_DIR WRITE_DIR CAL_FETCH_DIR CAL_WRITE_DIR CAL_MIB "PMC COMMAND" [--pmc-bench PMC_BENCH_LOG]]
+    python bench_profiles.py --counters OUT.txt "COMMAND" PASS_DIR [PASS_DIR ...]
+    python bench_profiles.py --busy OUT.txt "COMMAND" TRACE_DIR
+
+With --pmc-bench (the bench.py log of the PMC runs, run with --kernel KERNEL)
+the counters are averaged over that run's timed-region dispatches only and
+compared with their own algorithmic bytes (traffic_over_algorithmic).
+--counters writes, per kernel of the step, the occupancy / VALU / LDS
+counters of the given rocprofv3 --pmc runs (one counter set per run) over
+the working dispatches.  --busy writes the GPU busy fraction and kernel
+concurrency per 50 ms of a kernel trace (the batched ensemble).
 
 KERNEL is the library's kernel name (``mdot``, ``maxpy_norm``, ``jacobian``,
 ...).  The summary ends with machine-readable ``key: value`` lines, which
@@ -36,12 +46,12 @@ import sys
 SYMBOLS = {
     "mdot": "mdot_kernel<", "maxpy_norm": "maxpy_norm_kernel<", "jacobian": "bracket_kernel<4, 1, double>",
     "residual": "bracket_kernel<4, 0, double>", "rfft": "rfft_kernel<double", "irfft": "irfft_kernel<double",
-    "tridiag": "tridiag_kernel<8, double, false>", "mdot32": "mdot_kernel<", "maxpy_norm32": "maxpy_norm_kernel<",
+    "tridiag": "tridiag_lu_kernel<8, double>", "mdot32": "mdot_kernel<", "maxpy_norm32": "maxpy_norm_kernel<",
     "jacobian32": "bracket_kernel<4, 1, float>",
 }
 # kernels that return at once when enqueued past a cycle's convergence
 EARLY = ("mdot_kernel", "maxpy_norm_kernel", "bracket_kernel<4, 1", "rfft_kernel", "tridiag_kernel",
-         "irfft_kernel", "gmres_update_kernel", "mdot_finish_kernel")
+         "tridiag_lu_kernel", "irfft_kernel", "gmres_update_kernel", "mdot_finish_kernel")
 
 
 def _short(name):
@@ -83,10 +93,11 @@ def kernel_launches(rows, kernel, idle_us=8.0):
     return [d for d in (_dur_us(r) for r in sel) if d > idle_us]
 
 
-def pmc_kib(counter_dir, counter, sym, kernel=None, idle_us=8.0, big_only=False):
+def pmc_kib(counter_dir, counter, sym, kernel=None, idle_us=8.0, big_only=False, last=None):
     """Mean counter value (KiB) over the working dispatches whose symbol contains
     sym (big_only: only dispatches reading at least half the largest, i.e. the
-    calibration copies themselves)."""
+    calibration copies themselves; last: only the last `last` working
+    dispatches, in dispatch order -- the timed region of the bench run)."""
     path = os.path.join(counter_dir, "run_counter_collection.csv")
     vals = []
     for r in csv.DictReader(open(path)):
@@ -96,12 +107,25 @@ def pmc_kib(counter_dir, counter, sym, kernel=None, idle_us=8.0, big_only=False)
             continue
         if kernel is not None and _dur_us(r) <= idle_us:
             continue
-        vals.append(float(r["Counter_Value"]))
+        vals.append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
     if not vals:
         raise ValueError("no %s dispatches of %s in %s" % (counter, sym, path))
+    vals = [v for _, v in sorted(vals)]
     if big_only:
         vals = [v for v in vals if v >= 0.5 * max(vals)]
+    if last:
+        vals = vals[-last:]
     return statistics.mean(vals), len(vals)
+
+
+def bench_roofline(log):
+    """(algorithmic bytes per launch, launches) of the timed kernel in a bench.py log."""
+    for ln in open(log):
+        if ln.startswith("{"):
+            r = json.loads(ln).get("roofline") or {}
+            if "bytes_per_launch" in r:
+                return r["bytes_per_launch"], int(r["launches"])
+    raise ValueError("no roofline object in %s" % log)
 
 
 def summarise(kernel, trace_dir, out, command, bench_log=None, pmc=None, plain_log=None):
@@ -142,10 +166,15 @@ def summarise(kernel, trace_dir, out, command, bench_log=None, pmc=None, plain_l
                                  "(the per-dispatch tracing lengthens event-timed dispatches) %.2f us, value %s %s"
                                  % (kernel, ks["avg_us"], j["value"], j["unit"]))
     if pmc:
-        fdir, wdir, cfdir, cwdir, cal_mib, pcmd = pmc
+        fdir, wdir, cfdir, cwdir, cal_mib, pcmd = pmc[:6]
         sym = SYMBOLS[kernel]
-        f_kib, nf = pmc_kib(fdir, "FETCH_SIZE", sym, kernel)
-        w_kib, nw = pmc_kib(wdir, "WRITE_SIZE", sym, kernel)
+        last = algo = None
+        if len(pmc) > 6:
+            # the PMC runs' own bench line: the algorithmic bytes of exactly the
+            # dispatches whose counters are averaged (its timed region)
+            algo, last = bench_roofline(pmc[6])
+        f_kib, nf = pmc_kib(fdir, "FETCH_SIZE", sym, kernel, last=last)
+        w_kib, nw = pmc_kib(wdir, "WRITE_SIZE", sym, kernel, last=last)
         cf_kib, _ = pmc_kib(cfdir, "FETCH_SIZE", "copyBuffer", big_only=True)
         cw_kib, _ = pmc_kib(cwdir, "WRITE_SIZE", "copyBuffer", big_only=True)
         cal_kib = float(cal_mib) * 1024.0
@@ -161,10 +190,93 @@ def summarise(kernel, trace_dir, out, command, bench_log=None, pmc=None, plain_l
         keys.update(fetch_bytes_per_launch=round(fetch), write_bytes_per_launch=round(write),
                     traffic_bytes_per_launch=round(fetch + write), fetch_factor=round(ff, 4),
                     write_factor=round(fw, 4))
+        if algo:
+            lines.append("  the same %d dispatches (the PMC runs' timed region) carry %.0f algorithmic bytes each: "
+                         "traffic / algorithmic = %.3f" % (last, algo, (fetch + write) / algo))
+            keys.update(pmc_algorithmic_bytes_per_launch=round(algo),
+                        traffic_over_algorithmic=round((fetch + write) / algo, 4))
     lines += [""] + ["%s: %s" % (k, v) for k, v in keys.items()]
     with open(out, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     return keys
+
+
+def counters_table(pass_dirs, idle_us=8.0):
+    """Per kernel: the mean of every counter of the given --pmc runs over the
+    working dispatches (longer than idle_us), plus the derived ratios."""
+    per = {}
+    for d in pass_dirs:
+        for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+            if _dur_us(r) <= idle_us:
+                continue
+            per.setdefault(_short(r["Kernel_Name"]), {}).setdefault(r["Counter_Name"], []).append(
+                float(r["Counter_Value"]))
+    lines = ["%-40s %6s %6s %6s %6s %8s %8s %8s" % ("kernel (working dispatches)", "occ%", "VALU%", "LDS%",
+                                                    "bank%", "wait", "VALU/wv", "LDS/wv")]
+    keys = {}
+    for name in sorted(per, key=lambda n: -sum(per[n].get("SQ_WAVE_CYCLES", [0]))):
+        m = {c: statistics.mean(v) for c, v in per[name].items()}
+        if "SQ_WAVES" not in m or m["SQ_WAVES"] < 64:
+            continue
+        wait = m["SQ_WAIT_INST_ANY"] / m["SQ_WAVE_CYCLES"] if m.get("SQ_WAVE_CYCLES") else float("nan")
+        lines.append("%-40s %6.1f %6.1f %6.1f %6.2f %8.2f %8.0f %8.0f" % (
+            name[:40], m.get("OccupancyPercent", float("nan")), m.get("VALUBusy", float("nan")),
+            m.get("LdsUtil", float("nan")), m.get("LdsBankConflict", float("nan")), wait,
+            m.get("SQ_INSTS_VALU", float("nan")) / m["SQ_WAVES"], m.get("SQ_INSTS_LDS", float("nan")) / m["SQ_WAVES"]))
+        tag = name.split("<")[0]
+        if tag not in keys:
+            keys[tag] = m
+    return lines, keys
+
+
+def write_counters(out, command, pass_dirs):
+    lines, _ = counters_table(pass_dirs)
+    head = ["command: " + command, "",
+            "occ%: OccupancyPercent (waves resident / wave slots, averaged over the kernel); VALU%: VALUBusy;",
+            "LDS%: LdsUtil; bank%: LdsBankConflict; wait: SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES (share of a",
+            "wave's life spent waiting on a dependency); VALU/wv, LDS/wv: SQ_INSTS_VALU, SQ_INSTS_LDS per wave.",
+            "Averages over the working dispatches (Arnoldi-step kernels enqueued past convergence return",
+            "at once and are left out: duration <= 8 us).", ""]
+    with open(out, "w") as fh:
+        fh.write("\n".join(head + lines) + "\n")
+
+
+def busy_table(trace_csv, window_ms=50.0):
+    """GPU busy fraction (union of kernel intervals) and kernel concurrency
+    (summed kernel time / window) per window of a kernel trace, plus the
+    share of the trace spent with one kernel at a time."""
+    ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Stream_Id"])
+                for r in csv.DictReader(open(trace_csv)))
+    t0, t1 = ev[0][0], max(e for _, e, _ in ev)
+    W = int(window_ms * 1e6)
+    lines = ["%8s %8s %8s %8s %8s" % ("t_ms", "busy", "concur", "kernels", "streams")]
+    for w in range(int((t1 - t0) // W) + 1):
+        a, b = t0 + w * W, t0 + (w + 1) * W
+        sub = sorted((max(s, a), min(e, b), st) for s, e, st in ev if e > a and s < b)
+        if not sub:
+            continue
+        busy, cs, ce = 0, None, None
+        for s, e, _ in sub:
+            if ce is None or s > ce:
+                if ce is not None:
+                    busy += ce - cs
+                cs, ce = s, e
+            else:
+                ce = max(ce, e)
+        busy += ce - cs
+        tot = sum(e - s for s, e, _ in sub)
+        lines.append("%8.0f %8.2f %8.2f %8d %8d" % ((a - t0) / 1e6, busy / W, tot / W, len(sub),
+                                                  len({st for _, _, st in sub})))
+    return lines
+
+
+def write_busy(out, command, trace_dir):
+    head = ["command: " + command, "",
+            "per 50 ms window of the kernel trace: busy = share of the window with at least one kernel",
+            "running; concur = summed kernel time / window (kernels running at once); streams = HIP",
+            "streams with a kernel in the window.", ""]
+    with open(out, "w") as fh:
+        fh.write("\n".join(head + busy_table(os.path.join(trace_dir, "run_kernel_trace.csv"))) + "\n")
 
 
 def read_summary(path):
@@ -186,6 +298,12 @@ def read_summary(path):
 
 
 def main(argv):
+    if argv and argv[0] == "--counters":
+        write_counters(argv[1], argv[2], argv[3:])
+        return
+    if argv and argv[0] == "--busy":
+        write_busy(argv[1], argv[2], argv[3])
+        return
     kernel, trace_dir, out, command = argv[:4]
     rest = argv[4:]
     bench_log = rest[rest.index("--bench") + 1] if "--bench" in rest else None
@@ -194,6 +312,8 @@ def main(argv):
     if "--pmc" in rest:
         i = rest.index("--pmc")
         pmc = rest[i + 1:i + 7]
+        if "--pmc-bench" in rest:
+            pmc.append(rest[rest.index("--pmc-bench") + 1])
     keys = summarise(kernel, trace_dir, out, command, bench_log, pmc, plain_log)
     print(json.dumps(keys))
 
