@@ -38,7 +38,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
-           "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32")
+           "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32", "irfft_mdot", "irfft_mdot32")
 
 
 def workloads():
@@ -102,6 +102,8 @@ def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeat
         run.snes.getKSP().setTolerances(rtol=ksp_rtol)
     run.inv0 = run.invariants()
     ctx = run.ctx
+    if args.no_fusion:
+        ctx.set_fusion(False)
     for _ in range(args.warmup):
         run.step()
     torch.cuda.synchronize()
@@ -344,6 +346,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all)")
     ap.add_argument("--cpu1-steps", type=int, default=1,
                     help="timesteps of the one-core CPU sample (0 skips it)")
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="run the Arnoldi step's irFFT and Gram-Schmidt dots as separate launches")
     ap.add_argument("--no-config2-bracket", action="store_true",
                     help="skip the bracket GB/s at the 256x512 nonlinear-Landau size")
     ap.add_argument("--no-cpu", action="store_true")

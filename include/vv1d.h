@@ -132,6 +132,13 @@ int vv_set_graph(vv_ctx *ctx, int enable);
 int vv_set_tuning(vv_ctx *ctx, int32_t fft_rows, int32_t cgs_ept);
 int vv_get_tuning(vv_ctx *ctx, int32_t *fft_rows, int32_t *cgs_ept);
 int vv_autotune(vv_ctx *ctx, int32_t *fft_rows, int32_t *cgs_ept);
+/* Kernel fusion (no counterpart in the reference; on by default): where the
+ * FFT workgroup's rows are exactly one Gram-Schmidt slice (power-of-two nx
+ * <= 1024, kinetic preconditioner, rows per FFT workgroup x nx = 256 x
+ * cgs_ept x 16 bytes / element size), the Arnoldi step runs the irFFT of
+ * P^-1 J v and the CGS dots of the new vector in one launch.  Results are
+ * bit-identical either way; 0 runs the separate kernels (for timing). */
+int vv_set_fusion(vv_ctx *ctx, int enable);
 
 int vv_ctx_destroy(vv_ctx *ctx);
 const char *vv_last_error(const vv_ctx *ctx);

@@ -75,6 +75,7 @@ SIGNATURES = {
     "vv_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "vv_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "vv_autotune": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "vv_set_fusion": (ctypes.c_int, [_P, ctypes.c_int]),
     "vv_ctx_destroy": (ctypes.c_int, [_P]),
     "vv_last_error": (ctypes.c_char_p, [_P]),
     "vv_set_stream": (ctypes.c_int, [_P, _P]),

@@ -20,11 +20,11 @@ thread_local std::string g_create_error;
 
 enum ProfKind {
     P_JAC = 0, P_RES, P_RFFT, P_TRI, P_IRFFT, P_MDOT, P_MAXPY,
-    P_JAC32, P_RFFT32, P_TRI32, P_IRFFT32, P_MDOT32, P_MAXPY32, P_NKINDS
+    P_JAC32, P_RFFT32, P_TRI32, P_IRFFT32, P_MDOT32, P_MAXPY32, P_IRFFT_MDOT, P_IRFFT_MDOT32, P_NKINDS
 };
 const char *kProfNames[P_NKINDS] = {"jacobian",   "residual", "rfft",      "tridiag", "irfft",
                                     "mdot",       "maxpy_norm", "jacobian32", "rfft32",  "tridiag32",
-                                    "irfft32",    "mdot32",   "maxpy_norm32"};
+                                    "irfft32",    "mdot32",   "maxpy_norm32", "irfft_mdot", "irfft_mdot32"};
 
 // device scalar regions (doubles): per-instance norms | Poisson residuals
 // (16 apart) | invariants (6 per instance)
@@ -358,6 +358,15 @@ int arnoldi_step(vv_ctx *c, int it)
         // goes straight into the next basis vector
         VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], w), "jacobian");
         if (int r = diss_jacobian(c, v, &c->gs->scale[it], w, done)) return r;
+    } else if (const int rows = irfft_mdot_rows(c, false)) {
+        // P^-1 J v with the irFFT and the CGS dots of w in one launch
+        VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
+        if (int r = diss_jacobian(c, v, &c->gs->scale[it], c->X, done)) return r;
+        VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, c->X, c->spec, done), "rfft");
+        VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
+        VV_PLAUNCH(P_IRFFT_MDOT, S + F * (it + 2), it, launch_irfft_mdot(c, c->spec, it, done), "irfft_mdot");
+        VV_PLAUNCH(P_MAXPY, F * (it + 3), it, launch_maxpy_norm(c, it, rows), "maxpy_norm");
+        return VV_OK;
     } else {
         VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
         if (int r = diss_jacobian(c, v, &c->gs->scale[it], c->X, done)) return r;
@@ -370,12 +379,19 @@ int arnoldi_step(vv_ctx *c, int it)
 
 // y = P^-1 x in FP32 (complex64 spectrum in c->spec)
 // FP32 P^-1 (the mixed solver's cycle), as pc_apply
-int pc_apply32(vv_ctx *c, const float *x, float *y, const int32_t *done, int it)
+// FP32 pivots of the kinetic solve, formed on first FP32 use
+int pc_prepare32(vv_ctx *c)
 {
-    if (c->lu64 && !c->lu32) {      // FP32 pivots of the kinetic solve, on first FP32 use
+    if (c->lu64 && !c->lu32) {
         VV_CHECK(dalloc(c, &c->lu32, (size_t)c->nmodes * c->fft_rows * c->B), "alloc FP32 pivots");
         VV_CHECK(launch_tridiag_factor(c, nullptr, c->lu32), "FP32 pivots");
     }
+    return VV_OK;
+}
+
+int pc_apply32(vv_ctx *c, const float *x, float *y, const int32_t *done, int it)
+{
+    if (int r = pc_prepare32(c)) return r;
     float2 *spec = reinterpret_cast<float2 *>(c->spec);
     const double F = 4.0 * (double)c->N * c->B, S = 0.5 * b_spec(c);
     const bool pot = c->tb != nullptr;
@@ -416,6 +432,18 @@ int arnoldi_step32(vv_ctx *c, int it)
         VV_CHECK(launch_zero(c, c->d64, c->N * c->B), "zero");
         if (int r = diss_jacobian(c, c->w64, &c->gs->scale[it], c->d64, done)) return r;
         VV_CHECK(launch_narrow_add(c, ju, c->d64), "narrow add");
+    }
+    const int rows = c->p.precond != VV_PRECOND_NONE ? irfft_mdot_rows(c, true) : 0;
+    if (rows) {
+        // P32^-1 with the irFFT and the CGS dots of w in one launch
+        if (int r = pc_prepare32(c)) return r;
+        float2 *spec = reinterpret_cast<float2 *>(c->spec);
+        const double S = 0.5 * b_spec(c);
+        VV_PLAUNCH(P_RFFT32, F + S, it, launch_rfft32(c, X, spec, done), "rfft32");
+        VV_PLAUNCH(P_TRI32, 2 * S, it, launch_tridiag32(c, spec, done), "tridiag32");
+        VV_PLAUNCH(P_IRFFT_MDOT32, S + F * (it + 2), it, launch_irfft_mdot32(c, spec, it, done), "irfft_mdot32");
+        VV_PLAUNCH(P_MAXPY32, F * (it + 3), it, launch_maxpy_norm32(c, it, rows), "maxpy_norm32");
+        return VV_OK;
     }
     if (c->p.precond != VV_PRECOND_NONE)
         if (int r = pc_apply32(c, X, w, done, it)) return r;
@@ -790,6 +818,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     c->p.eigen = nullptr;
     c->p.hx = hxs[0];
     c->B = B;
+    c->fuse = true;
     c->stream = reinterpret_cast<hipStream_t>(stream);
     const int nx = p->nx, nv = p->nv;
     c->N = (int64_t)nx * nv;
@@ -1069,6 +1098,17 @@ int vv_set_tuning(vv_ctx *c, int32_t fft_rows, int32_t cgs_ept)
     }
     c->tune_fft_rows = fft_rows;
     c->tune_cgs_ept = cgs_ept;
+    return VV_OK;
+}
+
+int vv_set_fusion(vv_ctx *c, int enable)
+{
+    if (int r = check_ctx(c)) return r;
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->fuse = enable != 0;
     return VV_OK;
 }
 

@@ -21,13 +21,13 @@
 // the Hessenberg / Givens bookkeeping).  The kernel boundary orders the
 // partials before the finish, so no device-scope fence (an L2 write-back on
 // this GPU) is needed inside the streaming kernels.
-#include "vv1d_internal.h"
+#include "cgs_vec.h"
 
 namespace vv {
 namespace {
 
-constexpr int NT = 256;
-constexpr int NW = NT / 64;
+constexpr int NT = kCgsNT;
+constexpr int NW = kCgsNW;
 
 // Sum over the block, returned in every thread (fixed tree).
 __device__ __forceinline__ double block_sum_all(double v, double *red)
@@ -45,12 +45,6 @@ __device__ __forceinline__ double block_sum_all(double v, double *red)
     const double s = red[NW];
     __syncthreads();
     return s;
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    return v;   // lane 0
 }
 
 // Sum of part[0..nb) by the whole block (thread t adds t, t+NT, ...).
@@ -193,41 +187,6 @@ __global__ __launch_bounds__(NT) void reduce_finish_kernel(const double *partial
     }
 }
 
-// 16-byte vector of the basis storage type: double2 (FP64 cycle) or float4
-// (FP32 cycle of the mixed-precision solver).  Arithmetic is FP64 either way.
-template <typename S> struct V16;
-template <> struct V16<double> {
-    using type = double2;
-    static constexpr int L = 2;
-};
-template <> struct V16<float> {
-    using type = float4;
-    static constexpr int L = 4;
-};
-__device__ __forceinline__ double vget(const double2 &v, int l) { return l == 0 ? v.x : v.y; }
-__device__ __forceinline__ double vget(const float4 &v, int l)
-{
-    return (double)(l == 0 ? v.x : (l == 1 ? v.y : (l == 2 ? v.z : v.w)));
-}
-__device__ __forceinline__ void vset(double2 &v, int l, double x)
-{
-    if (l == 0) v.x = x;
-    else v.y = x;
-}
-__device__ __forceinline__ void vset(float4 &v, int l, double x)
-{
-    const float f = (float)x;
-    if (l == 0) v.x = f;
-    else if (l == 1) v.y = f;
-    else if (l == 2) v.z = f;
-    else v.w = f;
-}
-
-// The basis is vector-major across instances: V_j of instance b at
-// basis + j * vstride + b * N (vstride = B N rounded up to a multiple of 4
-// elements, so every V_j starts on a 32-byte boundary), so V_j of all
-// instances is one contiguous field and the operator kernels see instance
-// stride N.
 // TAIL: N is not a multiple of the vector width L (odd grids such as 9 x 9;
 // one instance only, a batched context needs N % 4 == 0).  Thread t < N - L
 // nvv of workgroup 0 then also owns the scalar element L nvv + t.
@@ -237,34 +196,6 @@ __device__ __forceinline__ int64_t tail_element(int64_t N, int64_t nvv)
     constexpr int L = V16<S>::L;
     return (TAIL && blockIdx.x == 0 && (int64_t)threadIdx.x < N - nvv * L) ? nvv * L + threadIdx.x : -1;
 }
-template <typename S>
-struct CgsArgs {
-    const S *basis;
-    int64_t N;
-    int64_t vstride;
-    double *part;
-    int64_t pstride;
-    GmresState *g;
-    int64_t noff;      // maxpy: offset of its norm partials in part (the dot partials stay at 0)
-    int nbd;           // maxpy: rows of dot partials (the mdot grid)
-};
-
-template <int EPT, int G, typename S>
-__device__ __forceinline__ void load_group(const S *basis, int64_t vstride, int v0, int64_t e0, int64_t nvv,
-                                           typename V16<S>::type (&x)[G][EPT])
-{
-    using V = typename V16<S>::type;
-#pragma unroll
-    for (int gg = 0; gg < G; ++gg) {
-        const V *Vp = reinterpret_cast<const V *>(basis + (size_t)(v0 + gg) * vstride);
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int64_t e = e0 + (int64_t)q * NT;
-            x[gg][q] = (e < nvv) ? Vp[e] : V{};
-        }
-    }
-}
-
 // VecMDot of w = V_{col+1} against v_0..v_col, written as HH(:, col) and the
 // VecMAXPY coefficients -HH(:, col).
 template <int EPT, int PV, typename S, bool TAIL>
@@ -300,13 +231,8 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
 #pragma unroll
         for (int gg = 0; gg < 2; ++gg) {
             const double sv = s_scale[v + gg];
-            double acc = 0.0;
-#pragma unroll
-            for (int q = 0; q < EPT; ++q) {
-#pragma unroll
-                for (int l = 0; l < L; ++l) acc += vget(w[q], l) * (vget(x[gg][q], l) * sv);
-            }
-            if (TAIL && et >= 0) acc += wt * ((double)a.basis[(size_t)(v + gg) * a.vstride + et] * sv);
+            double acc = dot_slice<EPT, S>(w, x[gg], sv);
+            if (TAIL && et >= 0) acc = __builtin_fma(wt, (double)a.basis[(size_t)(v + gg) * a.vstride + et] * sv, acc);
             acc = wave_sum(acc);
             if (lane == 0) red[wave][v + gg] = acc;
         }
@@ -315,13 +241,8 @@ __global__ __launch_bounds__(NT) void mdot_kernel(CgsArgs<S> a, int col)
         V x[1][EPT];
         load_group<EPT, 1, S>(a.basis, a.vstride, v, e0, nvv, x);
         const double sv = s_scale[v];
-        double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-#pragma unroll
-            for (int l = 0; l < L; ++l) acc += vget(w[q], l) * (vget(x[0][q], l) * sv);
-        }
-        if (TAIL && et >= 0) acc += wt * ((double)a.basis[(size_t)v * a.vstride + et] * sv);
+        double acc = dot_slice<EPT, S>(w, x[0], sv);
+        if (TAIL && et >= 0) acc = __builtin_fma(wt, (double)a.basis[(size_t)v * a.vstride + et] * sv, acc);
         acc = wave_sum(acc);
         if (lane == 0) red[wave][v] = acc;
     }
@@ -718,6 +639,8 @@ int64_t norm_partial_offset(const vv_ctx *c) { return c->part_stride / (kMaxRest
 
 }  // namespace
 
+int cgs_ept(const vv_ctx *c, int64_t nvv) { return cgs_ept_v(c, nvv); }
+
 // partial-sum rows the CGS kernels can need (one vector per thread)
 int cgs_blocks_max(int64_t N) { return (int)(((N >> 1) + NT - 1) / NT); }
 
@@ -741,10 +664,11 @@ hipError_t launch_norm_start(vv_ctx *c)
 namespace {
 
 template <int E, typename S, bool TAIL>
-void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x)
+void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x, int dot_rows)
 {
     const int nb = cgs_blocks_v(c, c->N / V16<S>::L);
-    CgsArgs<S> a{basis, c->N, c->vs, c->partial, c->part_stride, c->gs, norm_partial_offset(c), nb};
+    CgsArgs<S> a{basis, c->N, c->vs, c->partial, c->part_stride, c->gs, norm_partial_offset(c),
+                 dot_rows > 0 ? dot_rows : nb};
     const dim3 grid(nb, c->B), block(NT);
     if (which == 0) {
         if (c->p.restart < 32) launch_timed(c, mdot_kernel<E, 32, S, TAIL>, grid, block, 0, a, arg);
@@ -758,21 +682,21 @@ void cgs_launch(vv_ctx *c, const S *basis, int which, int arg, double *x)
 }
 
 template <typename S, bool TAIL>
-void cgs_dispatch_e(vv_ctx *c, const S *basis, int which, int arg, double *x)
+void cgs_dispatch_e(vv_ctx *c, const S *basis, int which, int arg, double *x, int dot_rows)
 {
     switch (cgs_ept_v(c, c->N / V16<S>::L)) {
-    case 8: cgs_launch<8, S, TAIL>(c, basis, which, arg, x); break;
-    case 4: cgs_launch<4, S, TAIL>(c, basis, which, arg, x); break;
-    case 2: cgs_launch<2, S, TAIL>(c, basis, which, arg, x); break;
-    default: cgs_launch<1, S, TAIL>(c, basis, which, arg, x); break;
+    case 8: cgs_launch<8, S, TAIL>(c, basis, which, arg, x, dot_rows); break;
+    case 4: cgs_launch<4, S, TAIL>(c, basis, which, arg, x, dot_rows); break;
+    case 2: cgs_launch<2, S, TAIL>(c, basis, which, arg, x, dot_rows); break;
+    default: cgs_launch<1, S, TAIL>(c, basis, which, arg, x, dot_rows); break;
     }
 }
 
 template <typename S>
-hipError_t cgs_dispatch(vv_ctx *c, const S *basis, int which, int arg, double *x)
+hipError_t cgs_dispatch(vv_ctx *c, const S *basis, int which, int arg, double *x, int dot_rows = 0)
 {
-    if (c->N % V16<S>::L) cgs_dispatch_e<S, true>(c, basis, which, arg, x);
-    else cgs_dispatch_e<S, false>(c, basis, which, arg, x);
+    if (c->N % V16<S>::L) cgs_dispatch_e<S, true>(c, basis, which, arg, x, dot_rows);
+    else cgs_dispatch_e<S, false>(c, basis, which, arg, x, dot_rows);
     return hipGetLastError();
 }
 
@@ -785,9 +709,9 @@ hipError_t mdot_t(vv_ctx *c, const S *basis, int col)
 }
 
 template <typename S>
-hipError_t maxpy_norm_t(vv_ctx *c, const S *basis, int it)
+hipError_t maxpy_norm_t(vv_ctx *c, const S *basis, int it, int dot_rows)
 {
-    hipError_t e = cgs_dispatch(c, basis, 1, it, nullptr);
+    hipError_t e = cgs_dispatch(c, basis, 1, it, nullptr, dot_rows);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(gmres_update_kernel, dim3(1, c->B), dim3(NT), 0, c->stream, c->partial + norm_partial_offset(c),
                        cgs_blocks_v(c, c->N / V16<S>::L), c->part_stride, it, c->gs, c->status_out);
@@ -804,10 +728,16 @@ hipError_t build_solution_t(vv_ctx *c, const S *basis, double *x)
 }  // namespace
 
 hipError_t launch_mdot(vv_ctx *c, int col) { return mdot_t<double>(c, c->basis, col); }
-hipError_t launch_maxpy_norm(vv_ctx *c, int it) { return maxpy_norm_t<double>(c, c->basis, it); }
+hipError_t launch_maxpy_norm(vv_ctx *c, int it, int dot_rows)
+{
+    return maxpy_norm_t<double>(c, c->basis, it, dot_rows);
+}
 hipError_t launch_build_solution(vv_ctx *c, double *x) { return build_solution_t<double>(c, c->basis, x); }
 hipError_t launch_mdot32(vv_ctx *c, int col) { return mdot_t<float>(c, c->basis32, col); }
-hipError_t launch_maxpy_norm32(vv_ctx *c, int it) { return maxpy_norm_t<float>(c, c->basis32, it); }
+hipError_t launch_maxpy_norm32(vv_ctx *c, int it, int dot_rows)
+{
+    return maxpy_norm_t<float>(c, c->basis32, it, dot_rows);
+}
 hipError_t launch_build_solution32(vv_ctx *c, double *x) { return build_solution_t<float>(c, c->basis32, x); }
 
 hipError_t launch_start32(vv_ctx *c, const double *r)

@@ -27,7 +27,7 @@
 // float for the FP32 preconditioner inside the mixed-precision GMRES cycle
 // (complex64 spectrum, float twiddles; T_k's coefficients are formed in FP64
 // and rounded).
-#include "vv1d_internal.h"
+#include "cgs_vec.h"
 
 namespace vv {
 namespace {
@@ -396,6 +396,143 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
     __syncthreads();
     // the last stage's outputs (natural order) go straight to the output rows
     lds_fft<true, PRE>(buf, nx, a.logn, npair, a.tw, pt, 0, IrfftSink<T>{y, j0, nx, a.logn, a.nv});
+}
+
+// Real rows of the irFFT left in the workgroup's LDS buffer itself, natural
+// order (row r of the block at wr[r nx + i]), backward-normalised as in
+// IrfftSink.  The last FFT stage reads every input before its barrier, so
+// the buffer can take its outputs.
+template <typename T>
+struct RealRowsSink {
+    T *wr;
+    int nx, logn;
+    __device__ __forceinline__ void operator()(int pos, cx_t<T> z) const
+    {
+        const int b = pos >> logn, i = pos & (nx - 1);
+        const T inv_n = T(1) / (T)nx;
+        wr[(2 * b) * nx + i] = z.x * inv_n;
+        wr[(2 * b + 1) * nx + i] = z.y * inv_n;
+    }
+};
+
+// The Arnoldi step's irFFT fused with the classical Gram-Schmidt dots that
+// follow it: w = V_{col+1} = irfft(spec) is written to the basis and, from
+// the workgroup's LDS, dotted with v_0..v_col.  The workgroup's R rows are
+// one CGS slice of NT EPT 16-byte vectors (R nx = NT EPT L, nv % R == 0);
+// thread t takes the slice's vectors t + q NT, and the per-slice partials
+// part[slice PV + j] are summed by maxpy_norm_kernel as mdot_kernel's are.
+// Where EPT equals the CGS launches' own, w and the partials are
+// bit-identical to irfft_kernel followed by mdot_kernel.  What the fusion
+// saves: the second read of w, one launch, and the wait for the first basis
+// vectors, whose loads are in flight while the FFT stages run.
+template <typename T, int EPT, int PV>
+__global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec,
+                                                        CgsArgs<T> g, int col)
+{
+    using C = cx_t<T>;
+    using V = typename V16<T>::type;
+    const int32_t dflag = cycle_done(a.done, a.gstride);
+    spec += blockIdx.y * a.S;
+    const T *basis = g.basis + blockIdx.y * g.N;
+    const GmresState *gs = g.g + blockIdx.y;
+    double *part = g.part + blockIdx.y * g.pstride;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *buf = reinterpret_cast<C *>(lds_raw);
+    __shared__ double red[kCgsNW][PV];
+    __shared__ double s_scale[PV];
+    const int nx = a.nx, R = a.rows, nh = nx >> 1;
+    const int slice = xcd_tile(blockIdx.x, gridDim.x);
+    const int j0 = slice * R;
+    const int npair = R >> 1;
+    const int nvec = col + 1;
+    const int total = (nh + 1) * npair;
+    constexpr int PER = (MAXE / 2 + 64 + NT - 1) / NT;
+    C va[PER], vb[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        const int k = idx / npair;
+        const int ja = j0 + 2 * (idx - k * npair);
+        va[q] = load_or_zero(spec, (int64_t)k * a.nv + ja, idx < total);
+        vb[q] = load_or_zero(spec, (int64_t)k * a.nv + ja + 1, idx < total);
+    }
+    const double sc = (int)threadIdx.x < nvec ? gs->scale[threadIdx.x] : 0.0;
+    FftTw<true, C> pt;
+    pt.load(a.tw, nx, a.logn);
+    if (dflag) return;                 // uniform over the block, before any barrier
+    if ((int)threadIdx.x < PV) s_scale[threadIdx.x] = sc;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < total) {
+            const int k = idx / npair;
+            const int p = idx - k * npair;
+            C A = va[q], B = vb[q];
+            if (k == 0 || k == nh) {
+                A.y = T(0);
+                B.y = T(0);
+            }
+            C *z = buf + p * nx;
+            z[k] = cmk<C>(A.x - B.y, A.y + B.x);
+            if (k != 0 && k != nh) z[nx - k] = cmk<C>(A.x + B.y, B.x - A.y);
+        }
+    }
+    // the first pair of basis vectors is loaded while the FFT stages run
+    const int64_t nvv = g.N / V16<T>::L;
+    const int64_t e0 = (int64_t)slice * NT * EPT + threadIdx.x;
+    V x[2][EPT];
+#pragma unroll
+    for (int gg = 0; gg < 2; ++gg) {
+        const V *Vp = reinterpret_cast<const V *>(basis + (size_t)gg * g.vstride);
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int64_t e = e0 + (int64_t)q * NT;
+            x[gg][q] = (gg < nvec && e < nvv) ? Vp[e] : V{};
+        }
+    }
+    __syncthreads();
+    T *wr = reinterpret_cast<T *>(buf);
+    lds_fft<true, true>(buf, nx, a.logn, npair, a.tw, pt, 0, RealRowsSink<T>{wr, nx, a.logn});
+
+    // w of the slice: to the basis, and into registers for the dots
+    const V *wl = reinterpret_cast<const V *>(wr);
+    V *W = reinterpret_cast<V *>(const_cast<T *>(basis) + (size_t)nvec * g.vstride);
+    V w[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        w[q] = wl[threadIdx.x + q * NT];
+        W[e0 + (int64_t)q * NT] = w[q];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int v = 0;
+    for (; v + 2 <= nvec; v += 2) {
+        if (v > 0) load_group<EPT, 2, T>(basis, g.vstride, v, e0, nvv, x);
+#pragma unroll
+        for (int gg = 0; gg < 2; ++gg) {
+            double acc = dot_slice<EPT, T>(w, x[gg], s_scale[v + gg]);
+            acc = wave_sum(acc);
+            if (lane == 0) red[wave][v + gg] = acc;
+        }
+    }
+    if (v < nvec) {
+        V y[1][EPT];
+        if (v > 0) {
+            load_group<EPT, 1, T>(basis, g.vstride, v, e0, nvv, y);
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) y[0][q] = x[0][q];
+        }
+        double acc = dot_slice<EPT, T>(w, y[0], s_scale[v]);
+        acc = wave_sum(acc);
+        if (lane == 0) red[wave][v] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < PV) {
+        double s = 0.0;
+        if ((int)threadIdx.x < nvec)
+            for (int wv = 0; wv < kCgsNW; ++wv) s += red[wv][threadIdx.x];
+        part[(size_t)slice * PV + threadIdx.x] = s;
+    }
 }
 
 // Transforms of a length n that is not a power of two (n = 100, 201, ...):
@@ -987,6 +1124,52 @@ hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, cons
     return hipGetLastError();
 }
 
+// 16-byte vectors per thread of the slice one irFFT workgroup covers when the
+// fused irFFT + dots kernel applies (power-of-two nx <= 4 NT, the kinetic
+// form, full workgroups, R nx = NT EPT L with EPT in 1, 2, 4, 8), else 0
+template <typename T>
+int irfft_mdot_ept(const vv_ctx *c)
+{
+    const int nx = c->fft_n;
+    constexpr int L = V16<T>::L;
+    if (!c->fuse || (nx & (nx - 1)) || nx > 4 * NT || c->tb != nullptr) return 0;
+    const int R = fft_rows(c);
+    if (c->fft_rows % R || c->N % L) return 0;
+    const int64_t pts = (int64_t)R * nx;
+    if (pts % (NT * L)) return 0;
+    const int64_t ept = pts / (NT * L);
+    return (ept == 1 || ept == 2 || ept == 4 || ept == 8) ? (int)ept : 0;
+}
+
+template <typename T, int EPT>
+void irfft_mdot_launch(vv_ctx *c, const FftArgs<T> &a, dim3 grid, size_t lds, const cx_t<T> *spec,
+                       const CgsArgs<T> &g, int col)
+{
+    if (c->p.restart < 32) launch_timed(c, irfft_mdot_kernel<T, EPT, 32>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+    else launch_timed(c, irfft_mdot_kernel<T, EPT, 64>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+}
+
+template <typename T>
+hipError_t irfft_mdot_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, const T *basis, int col,
+                        const int32_t *done)
+{
+    const int ept = irfft_mdot_ept<T>(c);
+    FftArgs<T> a{c->fft_n, c->logn, c->fft_rows, fft_rows(c), c->N, spec_stride(c),
+                 (int64_t)sizeof(GmresState), tw, done};
+    const int nb = a.nv / a.rows;
+    const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
+    const CgsArgs<T> g{basis, c->N, c->vs, c->partial, c->part_stride, c->gs, 0, nb};
+    const dim3 grid(nb, c->B);
+    switch (ept) {
+    case 8: irfft_mdot_launch<T, 8>(c, a, grid, lds, spec, g, col); break;
+    case 4: irfft_mdot_launch<T, 4>(c, a, grid, lds, spec, g, col); break;
+    case 2: irfft_mdot_launch<T, 2>(c, a, grid, lds, spec, g, col); break;
+    case 1: irfft_mdot_launch<T, 1>(c, a, grid, lds, spec, g, col); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 void tridiag_shape(int nv, int *P, int *M)
 {
     if (nv <= 512) {
@@ -1067,6 +1250,21 @@ hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t 
 hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done)
 {
     return irfft_t<double>(c, c->twiddle, spec, y, done);
+}
+
+int irfft_mdot_rows(const vv_ctx *c, bool fp32)
+{
+    return (fp32 ? irfft_mdot_ept<float>(c) : irfft_mdot_ept<double>(c)) > 0 ? c->fft_rows / fft_rows(c) : 0;
+}
+
+hipError_t launch_irfft_mdot(vv_ctx *c, const double2 *spec, int col, const int32_t *done)
+{
+    return irfft_mdot_t<double>(c, c->twiddle, spec, c->basis, col, done);
+}
+
+hipError_t launch_irfft_mdot32(vv_ctx *c, const float2 *spec, int col, const int32_t *done)
+{
+    return irfft_mdot_t<float>(c, c->twiddle32, spec, c->basis32, col, done);
 }
 
 hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done)

@@ -136,6 +136,9 @@ struct vv_ctx {
     // launch-shape tuning (0: built-in heuristic): v-rows per FFT workgroup,
     // 16-byte vectors per thread of the CGS kernels
     int tune_fft_rows, tune_cgs_ept;
+    // kernel fusion (default on): the Arnoldi step's irFFT and CGS dots in
+    // one launch where the geometry allows; results are bit-identical
+    bool fuse;
     // captured first GMRES cycle of SNESSolve_KSPONLY (graph replay mode)
     bool use_graph, capturing;
     hipGraphExec_t graph_exec;
@@ -201,6 +204,12 @@ hipError_t launch_tridiag_t(vv_ctx *c, double2 *spec, const int32_t *done);   //
 hipError_t launch_rfft32(vv_ctx *c, const float *x, float2 *spec, const int32_t *done);
 hipError_t launch_tridiag32(vv_ctx *c, float2 *spec, const int32_t *done);
 hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t *done);
+// irFFT into basis[col+1] fused with its CGS dots against basis[0..col]
+// (per-slice partials, as launch_mdot); irfft_mdot_rows: the partial rows it
+// writes (its grid), 0 where the fused kernel does not apply
+int irfft_mdot_rows(const vv_ctx *c, bool fp32);
+hipError_t launch_irfft_mdot(vv_ctx *c, const double2 *spec, int col, const int32_t *done);
+hipError_t launch_irfft_mdot32(vv_ctx *c, const float2 *spec, int col, const int32_t *done);
 // potential preconditioner: field (nv rows of nx) <-> transposed (nx rows of nv)
 // reciprocal LU pivots of the kinetic T_k into rp64 (FP64) and / or rp32 (rounded)
 hipError_t launch_tridiag_factor(vv_ctx *c, double2 *rp64, float2 *rp32);
@@ -211,12 +220,15 @@ hipError_t launch_potential_bands(vv_ctx *c, const double *phi);
 hipError_t launch_gmres_init(vv_ctx *c, double rtol, double atol, double dtol, int max_it);
 hipError_t launch_norm_start(vv_ctx *c);                  // cycle start on basis[0]
 int cgs_blocks_max(int64_t N);                            // partial rows the CGS kernels may use
+int cgs_ept(const vv_ctx *c, int64_t nvv);              // 16-byte vectors per thread of a CGS launch
 hipError_t launch_mdot(vv_ctx *c, int col);               // CGS dots of basis[col+1] vs 0..col
-hipError_t launch_maxpy_norm(vv_ctx *c, int it);          // CGS update of basis[it+1] + Givens
+// CGS update of basis[it+1] + Givens; dot_rows: partial rows of the dots
+// launch before it (0: launch_mdot's grid)
+hipError_t launch_maxpy_norm(vv_ctx *c, int it, int dot_rows = 0);
 hipError_t launch_build_solution(vv_ctx *c, double *x);  // x += V y
 hipError_t launch_start32(vv_ctx *c, const double *r);    // FP32 cycle start: V_0 = (float) r
 hipError_t launch_mdot32(vv_ctx *c, int col);
-hipError_t launch_maxpy_norm32(vv_ctx *c, int it);
+hipError_t launch_maxpy_norm32(vv_ctx *c, int it, int dot_rows = 0);
 hipError_t launch_build_solution32(vv_ctx *c, double *x);  // x (FP64) += V y (FP32 basis)
 hipError_t launch_residual_restart(vv_ctx *c, const double *b, double *y);  // y = b - y (B fields)
 hipError_t launch_norms(vv_ctx *c, const double *x, double *out_dev);       // ||x_b|| of the B fields

@@ -168,6 +168,10 @@ class Context:
         self.call("vv_get_tuning", ctypes.byref(r), ctypes.byref(e))
         return r.value, e.value
 
+    def set_fusion(self, enable=True):
+        """Fused irFFT + Gram-Schmidt dots in the Arnoldi step (default on; bit-identical)."""
+        self.call("vv_set_fusion", ctypes.c_int(1 if enable else 0))
+
     def autotune(self):
         """Time the launch-shape candidates; returns (fft_rows, cgs_ept) kept."""
         r, e = ctypes.c_int32(), ctypes.c_int32()
@@ -386,7 +390,8 @@ class Context:
         return res if self.batched else res[0]
 
     PROFILE_KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
-                       "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32")
+                       "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
+                       "irfft_mdot", "irfft_mdot32")
 
     def profile(self, kernels=None):
         """Time the named kernels (all when None; () switches timing off)."""
