@@ -11,9 +11,10 @@ preconditioned Jacobian, moments and Poisson after every solve.
 Timing: W untimed warm-up steps, then 3 windows (--repeats) of K steps, each
 between two torch.cuda.synchronize() calls.  value = K / (median window
 time)  (timesteps/s) of the plain FP64 form.  The mixed-precision form (FP32 GMRES cycles + FP32
-preconditioner inside FP64 iterative refinement, FP64 residual / update /
-Poisson) is then timed the same way from the same initial state and reported
-under "mixed" with its L-infinity distance to the FP64 run after W + K steps.
+preconditioner for the Picard solves that start from a residual below
+snes_atol * 2^20, inside the FP64 Picard residual / update / Poisson loop) is then timed the same way from
+the same initial state and reported under "mixed", next to FP64 at the same
+linear tolerance and with its distance to the FP64 run after --compare-steps.
 The batched form (256 independent instances with uneven grids, bucketed by
 shape into batched contexts, one captured graph replay per Newton step,
 autotuned launch shapes) is reported under "batched": aggregate
@@ -88,7 +89,7 @@ def cpu_baseline(w, sample_steps, threads):
     return cbaseline.time_sample(w, sample_steps, threads)
 
 
-def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeats=None):
+def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeats=None, fp64_above="auto"):
     """W warm-up steps, then `repeats` windows of K timed steps each; returns
     the run, the median window time, the records of all timed steps and the
     per-kernel event statistics (summed over the windows)."""
@@ -97,7 +98,7 @@ def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeat
     repeats = args.repeats if repeats is None else repeats
     nt = args.warmup + repeats * args.steps
     run = Runscript(make_cfg(w, nt), profile=w["profile"], precision=precision, inner_rtol=args.inner_rtol,
-                    refine=refine)
+                    refine=refine, fp64_above=fp64_above)
     if ksp_rtol is not None:
         run.snes.getKSP().setTolerances(rtol=ksp_rtol)
     run.inv0 = run.invariants()
@@ -167,21 +168,31 @@ def mixed_figures(w, args, run64, value64):
     import torch
     mrun, melapsed, mrecs, mk = timed_run(w, args, "mixed", args.mixed_kernel)
     mval = args.steps / melapsed
-    mcy = [p["cycles"] for r in mrecs for p in r["picard"]]
+    solves = [p for r in mrecs for p in r["picard"]]
+    fp32_solves = [p for p in solves if not p.get("fp64")]
     out = {
         "value": round(mval, 3), "unit": "timesteps/s", "ms_per_step": round(1e3 * melapsed / args.steps, 3),
         "speedup_vs_fp64": round(mval / value64, 3),
-        "dtype": "f32 GMRES/P^-1/J.u (f64 dots, Hessenberg) inside the f64 Picard residual, update and phi",
-        "refine": "newton", "inner_rtol": args.inner_rtol, "snes_atol": 1e-10,
-        "iterations": dict(_its(mrecs), cycles_per_solve=float(np.mean(mcy))),
+        "dtype": "f32 GMRES/P^-1/J.u (f64 dots, Hessenberg) for the Picard solves that start below "
+                 "snes_atol * 2^20 (the larger corrections in f64 at the same rtol), inside the f64 Picard "
+                 "residual, update and phi",
+        "refine": "newton", "fp64_above": 1e-10 * 2.0 ** 20, "inner_rtol": args.inner_rtol, "snes_atol": 1e-10,
+        "iterations": dict(_its(mrecs), fp32_solves=len(fp32_solves), fp64_solves=len(solves) - len(fp32_solves),
+                           cycles_per_fp32_solve=float(np.mean([p["cycles"] for p in fp32_solves]))
+                           if fp32_solves else 0.0),
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
                         "GB/s": round(v["gbs"], 1), "ms": round(v["ms"], 3)} for k, v in mk.items()},
     }
     if not args.no_mixed_refs:
         srun, sel, srecs, _ = timed_run(w, args, "double", None, ksp_rtol=args.inner_rtol)
-        out["fp64_same_linear_rtol"] = {"value": round(args.steps / sel, 3), "ksp_rtol": args.inner_rtol,
+        sval = args.steps / sel
+        out["fp64_same_linear_rtol"] = {"value": round(sval, 3), "ksp_rtol": args.inner_rtol,
                                         "iterations": _its(srecs)}
+        out["ratio_vs_fp64_same_linear_rtol"] = round(mval / sval, 3)
         del srun
+        arun, ael, arecs, _ = timed_run(w, args, "mixed", None, fp64_above=None)
+        out["all_solves_fp32"] = {"value": round(args.steps / ael, 3), "iterations": _its(arecs)}
+        del arun
         krun, kel, krecs, _ = timed_run(w, args, "mixed", None, refine="ksp")
         kcy = [p["cycles"] for r in krecs for p in r["picard"]]
         out["ksp_refined"] = {"value": round(args.steps / kel, 3), "speedup_vs_fp64": round(args.steps / kel / value64, 3),

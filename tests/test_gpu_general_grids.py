@@ -333,6 +333,7 @@ def test_mixed_picard_steps_match_oracle(nx, nv, precond, refine, cuda):
         eg = run.step()
         print(s, len(eo['picard']), len(eg['picard']), [p['cycles'] for p in eg['picard']], flush=True)
         for pg in eg['picard']:
-            assert pg['reason'] > 0 and pg['cycles'] >= 1
+            # solves that start above snes_atol * 2^20 run in FP64 (refine='newton')
+            assert pg['reason'] > 0 and (pg['cycles'] >= 1 or pg['fp64'])
         f = run.fc.cpu().numpy().T
         assert np.abs(f - orun.fc).max() <= 1e-9 * np.abs(orun.fc).max(), s

@@ -144,11 +144,13 @@ def test_mixed_single_cycle(nx, nv, kind, restart, rtol, cuda):
         assert np.linalg.norm(r) <= rtol * np.linalg.norm(b) * (1 + 1e-6)
 
 
-@pytest.mark.parametrize("refine", ['newton', 'ksp'])
+@pytest.mark.parametrize("refine,fp64_above", [('newton', 'auto'), ('newton', None), ('ksp', 'auto')])
 @pytest.mark.parametrize("kind", KINDS)
-def test_mixed_picard_steps_match_oracle(kind, refine, cuda):
+def test_mixed_picard_steps_match_oracle(kind, refine, fp64_above, cuda):
     """Three implicit timesteps of the Landau problem on 32x64 with the mixed
-    solver against the FP64 oracle run."""
+    solver against the FP64 oracle run (fp64_above='auto': Picard solves that
+    start above snes_atol * 2^20 in FP64, the rest FP32; None: every solve
+    FP32)."""
     from vivlasov1d_amd.core import Config
     from vivlasov1d_amd.runner import Runscript
     nx, nv, L, vmax, ht = 32, 64, 4 * np.pi, 8.0, 0.05
@@ -160,15 +162,19 @@ def test_mixed_picard_steps_match_oracle(kind, refine, cuda):
                              'preconditioner_scheme': 'tensor_kinetic', 'petsc_ksp_rtol': 1e-10,
                              'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
                   'initial_data': {'charge': -1.0}})
-    run = Runscript(cfg, profile=prof, precision='mixed', refine=refine)
+    run = Runscript(cfg, profile=prof, precision='mixed', refine=refine, fp64_above=fp64_above)
+    thr = 1e-10 * 2.0 ** 20
     for s in range(3):
         eo = orun.step()
         eg = run.step()
         print(s, eo['initial'], eg['initial'], len(eo['picard']), len(eg['picard']),
               [p['cycles'] for p in eg['picard']], flush=True)
         assert abs(eg['initial'] - eo['initial']) <= 1e-8 * eo['initial']
-        for pg in eg['picard']:
-            assert pg['reason'] > 0 and pg['cycles'] >= 1
+        start = [eg['initial']] + [p['residual'] for p in eg['picard'][:-1]]
+        for pg, r0 in zip(eg['picard'], start):
+            assert pg['reason'] > 0
+            assert pg['fp64'] == (refine == 'newton' and fp64_above == 'auto' and r0 > thr)
+            assert pg['cycles'] == 0 if pg['fp64'] else pg['cycles'] >= 1
         f = run.fc.cpu().numpy().T
         assert np.abs(f - orun.fc).max() <= 1e-9 * np.abs(orun.fc).max()
     inv = run.invariants()
@@ -205,3 +211,46 @@ def test_mixed_twostream_step_matches_fp64(refine, cuda):
         f64 = r64.fc.cpu().numpy()
         fmx = rmx.fc.cpu().numpy()
         assert np.abs(fmx - f64).max() <= 1e-9 * np.abs(f64).max()
+
+
+def test_mixed_fp64_above(cuda):
+    """Two-stream at 256x512 J4, three timesteps: with the Picard solves that
+    start above snes_atol * 2^20 in FP64, the FP32 solves take no more GMRES
+    iterations than with every solve in FP32 (whose large corrections leave
+    broadband FP32 rounding noise in f), the Picard counts equal the FP64
+    run's at the same linear tolerance, and f stays within 1e-9 of it."""
+    from vivlasov1d_amd.core import Config
+    from vivlasov1d_amd.runner import Runscript
+    nx, nv, L, vmax, ht = 256, 512, 2 * np.pi / 0.2, 10.0, 0.05
+
+    def prof(x, v):
+        m = lambda w: np.exp(-0.5 * w ** 2) / math.sqrt(2 * np.pi)
+        return (1 + 0.001 * np.cos(0.2 * x)) * 0.5 * (m(v - 2.4) + m(v + 2.4))
+
+    cfg = lambda rtol: Config({'grid': {'nx': nx, 'nv': nv, 'L': L, 'vmax': vmax, 'ht': ht, 'nt': 3},
+                               'solver': {'poisson_bracket': 'arakawaj4', 'laplace_operator': 'CFD4',
+                                          'preconditioner_type': 'tensor',
+                                          'preconditioner_scheme': 'tensor_kinetic', 'petsc_ksp_rtol': rtol,
+                                          'petsc_snes_atol': 1e-10, 'petsc_snes_max_iter': 20},
+                               'initial_data': {'charge': -1.0}})
+    thr = 1e-10 * 2.0 ** 20
+    r64 = Runscript(cfg(1e-5), profile=prof)
+    r1 = Runscript(cfg(1e-10), profile=prof, precision='mixed')
+    r2 = Runscript(cfg(1e-10), profile=prof, precision='mixed', fp64_above=None)
+    t1 = t2 = 0
+    for s in range(3):
+        e64, e1, e2 = r64.step(), r1.step(), r2.step()
+        g64 = [p['gmres'] for p in e64['picard']]
+        g1 = [p['gmres'] for p in e1['picard']]
+        g2 = [p['gmres'] for p in e2['picard']]
+        print(s, g64, g1, g2, [p['fp64'] for p in e1['picard']], flush=True)
+        start = [e1['initial']] + [p['residual'] for p in e1['picard'][:-1]]
+        assert [p['fp64'] for p in e1['picard']] == [r0 > thr for r0 in start]
+        assert any(p['fp64'] for p in e1['picard']) and not all(p['fp64'] for p in e1['picard'])
+        assert not any(p['fp64'] for p in e2['picard'])
+        assert len(g1) == len(g64)
+        t1 += sum(g1)
+        t2 += sum(g2)
+        f64 = r64.fc.cpu().numpy()
+        assert np.abs(r1.fc.cpu().numpy() - f64).max() <= 1e-9 * np.abs(f64).max()
+    assert t1 <= t2, (t1, t2)

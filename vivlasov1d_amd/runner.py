@@ -29,7 +29,8 @@ class ResidualTooLarge(RuntimeError):
 
 class Runscript:
     def __init__(self, cfg, profile=None, f0=None, log=None, restart=30, verbose=False, precision='double',
-                 inner_rtol=1e-5, normalise_f0=True, refine='newton', hdf5_out=None, external=None):
+                 inner_rtol=1e-5, normalise_f0=True, refine='newton', hdf5_out=None, external=None,
+                 fp64_above='auto'):
         if not isinstance(cfg, Config):
             cfg = Config(cfg)
         self.cfg = cfg
@@ -88,10 +89,17 @@ class Runscript:
         # Poisson solve stay FP64.  refine='newton': the Picard loop below is
         # the refinement (one FP32 cycle per solve, to inner_rtol, until the
         # FP64 residual meets snes_atol); 'ksp': each solve refined in FP64 to
-        # petsc_ksp_rtol
-        self.snes.getKSP().setPrecision(precision, inner_rtol=inner_rtol, refine=refine)
-        self.precision = precision
+        # petsc_ksp_rtol.  fp64_above (with 'newton'): a Picard solve that
+        # starts from a residual above it runs in FP64 at the same relative
+        # tolerance; 'auto' = snes_atol * 2^20: the FP32 rounding of a
+        # correction (2^-24 of it, the correction being about the size of the
+        # residual it removes) then stays 16 times below the nonlinear
+        # tolerance; None: every solve FP32
         self.snes_atol = sol['petsc_snes_atol']
+        if fp64_above == 'auto':
+            fp64_above = self.snes_atol * 2.0 ** 20
+        self.snes.getKSP().setPrecision(precision, inner_rtol=inner_rtol, refine=refine, fp64_above=fp64_above)
+        self.precision = precision
         self.snes_max_iter = sol['petsc_snes_max_iter']
         # initial guess (run_base.py:148, 171-183, 368-372): the steppers see the
         # history potentials h1h, h2h; 'rk4' is the 3/8 rule the reference's
@@ -232,7 +240,9 @@ class Runscript:
         while True:
             i += 1
             self.fl.copy_(self.fc)
+            self.snes.picard_residual = pred
             self.snes.solve(None, self.fc)
+            self.snes.picard_residual = None
             self.calculate_moments()
             self.vlasov_solver.update_previous(self.fc)
             prev, pred = pred, self.calculate_residual()
@@ -240,6 +250,7 @@ class Runscript:
             it_rec = dict(gmres=ksp.getIterationNumber(), reason=ksp.getConvergedReason(), residual=pred)
             if ksp.precision == 'mixed':
                 it_rec['cycles'] = ksp.cycles
+                it_rec['fp64'] = ksp.last_precision == 'double'
             rec['picard'].append(it_rec)
             if self.verbose:
                 print("  Nonlinear Solver: %5i GMRES  iterations, residual = %24.16E"

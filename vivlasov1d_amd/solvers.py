@@ -786,13 +786,24 @@ class KSP:
         # the Poisson solve of the outer loop drive the error to snes_atol) or
         # inside the solve (refine='ksp': FP64 iterative refinement on the true
         # preconditioned residual to the KSP's own rtol).
+        # fp64_above (refine='newton'): a Picard solve whose starting residual
+        # exceeds it runs in FP64 at the same relative tolerance.  The FP32
+        # cycle leaves broadband rounding noise of about 2^-24 times the
+        # correction in f, which GMRES removes slowly in every later solve of
+        # the step; where that noise would come within a factor 16 of the
+        # nonlinear tolerance (residual > snes_atol * 2^20, the run script's
+        # choice) the solve is FP64.  None: every solve FP32.
         self.precision = 'double'
         self.refine = 'newton'
+        self.fp64_above = None
         self.inner_rtol = 1e-5
         self.max_cycles = 50
         self.cycles = 0
+        self.last_precision = 'double'
 
-    def setPrecision(self, precision, inner_rtol=None, max_cycles=None, refine=None):
+    def setPrecision(self, precision, inner_rtol=None, max_cycles=None, refine=None, fp64_above=False):
+        """fp64_above: residual above which a mixed solve runs in FP64 (None:
+        never; the default False leaves it unchanged)."""
         if precision not in ('double', 'mixed'):
             raise _lib.VVError("precision must be 'double' or 'mixed'")
         if refine is not None and refine not in ('newton', 'ksp'):
@@ -800,10 +811,18 @@ class KSP:
         self.precision = precision
         if refine is not None:
             self.refine = refine
+        if fp64_above is not False:
+            self.fp64_above = None if fp64_above is None else float(fp64_above)
         if inner_rtol is not None:
             self.inner_rtol = float(inner_rtol)
         if max_cycles is not None:
             self.max_cycles = int(max_cycles)
+
+    def fp64_solve(self, residual):
+        """Whether a mixed-precision solve starting from this Picard residual
+        runs in FP64 (refine='newton' only; None: residual unknown)."""
+        return (self.refine == 'newton' and self.fp64_above is not None and residual is not None
+                and residual > self.fp64_above)
 
     def mixed_args(self):
         """(rtol, atol, dtol, max_it, inner_rtol, max_cycles, fp64_check) of a mixed solve."""
@@ -859,6 +878,9 @@ class SNES:
         self.type = 'ksponly'
         self.linear_its = 0
         self.last = None
+        # Picard residual the next solve starts from (set by the run script;
+        # None: unknown), for the mixed-precision KSP's choice of precision
+        self.picard_residual = None
 
     def setType(self, t):
         if t != 'ksponly':
@@ -885,10 +907,16 @@ class SNES:
         KSPSolve; x -= delta unless the linear solve diverged."""
         s = self.solver
         k = self.ksp
-        if k.precision == 'mixed':
+        if k.precision == 'mixed' and k.fp64_solve(self.picard_residual):
+            self.last = s.ctx.snes_ksponly(x, s.Fh, max(k.rtol, k.inner_rtol), k.atol, k.dtol, k.max_it)
+            k.cycles = 0
+            k.last_precision = 'double'
+        elif k.precision == 'mixed':
             self.last, k.cycles = s.ctx.snes_ksponly_mixed(x, s.Fh, *k.mixed_args())
+            k.last_precision = 'mixed'
         else:
             self.last = s.ctx.snes_ksponly(x, s.Fh, k.rtol, k.atol, k.dtol, k.max_it)
+            k.last_precision = 'double'
         k._res = self.last
         self.linear_its = self.last.its
 
