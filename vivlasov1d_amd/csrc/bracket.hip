@@ -90,6 +90,9 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const int32_t dflag =
         (a.done != nullptr) ? *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(a.done) + b * a.gstride)
                             : 0;
+    // batched: a finished instance skips its loads (one instance keeps them
+    // in flight with the flag's)
+    if (gridDim.y > 1 && dflag) return;
     const double usd = (a.uscale != nullptr)
                            ? *reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.uscale) + b * a.gstride)
                            : 1.0;

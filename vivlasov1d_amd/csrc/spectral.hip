@@ -267,6 +267,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
 {
     using C = cx_t<T>;
     const int32_t dflag = cycle_done(a.done, a.gstride);
+    if (gridDim.y > 1 && dflag) return;   // batched: a finished instance skips its loads
     x += blockIdx.y * a.N;
     spec += blockIdx.y * a.S;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -354,6 +355,7 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
 {
     using C = cx_t<T>;
     const int32_t dflag = cycle_done(a.done, a.gstride);
+    if (gridDim.y > 1 && dflag) return;   // batched: a finished instance skips its loads
     spec += blockIdx.y * a.S;
     y += blockIdx.y * a.N;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -432,6 +434,7 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
     using C = cx_t<T>;
     using V = typename V16<T>::type;
     const int32_t dflag = cycle_done(a.done, a.gstride);
+    if (gridDim.y > 1 && dflag) return;   // batched: a finished instance skips its loads
     spec += blockIdx.y * a.S;
     const T *basis = g.basis + blockIdx.y * g.N;
     const GmresState *gs = g.g + blockIdx.y;
@@ -456,6 +459,19 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
         va[q] = load_or_zero(spec, (int64_t)k * a.nv + ja, idx < total);
         vb[q] = load_or_zero(spec, (int64_t)k * a.nv + ja + 1, idx < total);
     }
+    // the first pair of basis vectors is in flight while the FFT stages run
+    const int64_t nvv = g.N / V16<T>::L;
+    const int64_t e0 = (int64_t)slice * NT * EPT + threadIdx.x;
+    V x[2][EPT];
+#pragma unroll
+    for (int gg = 0; gg < 2; ++gg) {
+        const V *Vp = reinterpret_cast<const V *>(basis + (size_t)gg * g.vstride);
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int64_t e = e0 + (int64_t)q * NT;
+            x[gg][q] = (gg < nvec && e < nvv) ? Vp[e] : V{};
+        }
+    }
     const double sc = (int)threadIdx.x < nvec ? gs->scale[threadIdx.x] : 0.0;
     FftTw<true, C> pt;
     pt.load(a.tw, nx, a.logn);
@@ -475,19 +491,6 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
             C *z = buf + p * nx;
             z[k] = cmk<C>(A.x - B.y, A.y + B.x);
             if (k != 0 && k != nh) z[nx - k] = cmk<C>(A.x + B.y, B.x - A.y);
-        }
-    }
-    // the first pair of basis vectors is loaded while the FFT stages run
-    const int64_t nvv = g.N / V16<T>::L;
-    const int64_t e0 = (int64_t)slice * NT * EPT + threadIdx.x;
-    V x[2][EPT];
-#pragma unroll
-    for (int gg = 0; gg < 2; ++gg) {
-        const V *Vp = reinterpret_cast<const V *>(basis + (size_t)gg * g.vstride);
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int64_t e = e0 + (int64_t)q * NT;
-            x[gg][q] = (gg < nvec && e < nvv) ? Vp[e] : V{};
         }
     }
     __syncthreads();
@@ -682,6 +685,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
 {
     using C = cx_t<T>;
     const int32_t dflag = cycle_done(a.done, a.gstride);
+    if (gridDim.y > 1 && dflag) return;   // batched: a finished instance skips its loads
     spec += blockIdx.y * a.S;
     a.gm += blockIdx.y * a.cstride;
     a.g0 += blockIdx.y * a.cstride;
@@ -871,6 +875,7 @@ __global__ __launch_bounds__(NT) void tridiag_lu_kernel(TriArgs a, const cx_t<T>
 {
     using C = cx_t<T>;
     const int32_t dflag = cycle_done(a.done, a.gstride);
+    if (gridDim.y > 1 && dflag) return;   // batched: a finished instance skips its loads
     const int k = blockIdx.x;
     const double af = a.inst[blockIdx.y].afpc;
     const double2 e = a.eigen[k];
