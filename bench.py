@@ -346,12 +346,13 @@ def main():
     ap.add_argument("--repeats", type=int, default=3,
                     help="timed windows of --steps steps each; value is the median window's rate")
     ap.add_argument("--workload", default="twostream_1024x2048_J4")
-    ap.add_argument("--kernel", default="maxpy_norm",
+    ap.add_argument("--kernel", default="irfft_mdot",
                     help="kernels timed with HIP events in the timed FP64 run ('all', 'none' or a comma list of "
                          "%s); the roofline is the one with the largest total time.  Each timed dispatch costs "
                          "about 2 percent of throughput, so the default times the dominant kernel only" % (KERNELS,))
-    ap.add_argument("--bracket-kernels", default="jacobian,residual",
-                    help="kernels timed in a second, untimed FP64 run for the bracket HBM GB/s ('none' skips)")
+    ap.add_argument("--bracket-kernels", default="jacobian,residual,rfft,tridiag,maxpy_norm",
+                    help="kernels timed in a second, untimed FP64 run: the bracket HBM GB/s (jacobian, "
+                         "residual) and the other kernels of the step ('none' skips)")
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="timesteps of the CPU baseline sample (about 10-20 s on 16 host threads)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all)")
@@ -426,14 +427,15 @@ def main():
             # PMC runs' timed region), as the summary records it
             if "traffic_over_algorithmic" in tk:
                 roof["traffic_over_algorithmic"] = tk["traffic_over_algorithmic"]
-    bracket = None
+    bracket = step_kernels = None
     if args.bracket_kernels not in ("", "none"):
         brun, _, _, bk = timed_run(w, args, "double", args.bracket_kernels, repeats=1)
         progress("bracket kernels timed")
         del brun
-        bracket = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
-                       "frac_hbm_peak": round(v["gbs"] / HBM_PEAK_GBS, 4),
-                       "bytes_per_launch": v["bytes"] / v["launches"]} for k, v in bk.items()}
+        step_kernels = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "GB/s": round(v["gbs"], 1),
+                            "frac_hbm_peak": round(v["gbs"] / HBM_PEAK_GBS, 4),
+                            "bytes_per_launch": v["bytes"] / v["launches"]} for k, v in bk.items()}
+        bracket = {k: v for k, v in step_kernels.items() if k in ("jacobian", "residual")}
     sb = sum(step_bytes(N, r) for r in recs)
     picard = [len(r["picard"]) for r in recs]
     gm = [p["gmres"] for r in recs for p in r["picard"]]
@@ -494,6 +496,7 @@ def main():
                    "value_from": "median window"},
         "invariants": {k: inv[k] for k in ("mass", "l2", "energy")},
         "bracket": bracket,
+        "step_kernels": step_kernels,
         "bracket_landau_256x512": bracket2,
         "mixed": mixed,
         "batched": batched,

@@ -28,8 +28,8 @@ def test_profile_summary(name):
 def test_counter_summary_covers_the_step_kernels():
     text = open(os.path.join(ROOT, "profiles", "counters_twostream_1024x2048_J4.txt")).read()
     assert "command: cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc" in text
-    for k in ("maxpy_norm_kernel", "mdot_kernel", "bracket_kernel<4, 1", "rfft_kernel", "tridiag_lu_kernel",
-              "irfft_kernel"):
+    for k in ("maxpy_norm_kernel", "irfft_mdot_kernel", "bracket_kernel<4, 1", "rfft_kernel",
+              "tridiag_lu_kernel"):
         assert k in text, k
 
 

@@ -459,6 +459,10 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
         va[q] = load_or_zero(spec, (int64_t)k * a.nv + ja, idx < total);
         vb[q] = load_or_zero(spec, (int64_t)k * a.nv + ja + 1, idx < total);
     }
+    const double sc = (int)threadIdx.x < nvec ? gs->scale[threadIdx.x] : 0.0;
+    FftTw<true, C> pt;
+    pt.load(a.tw, nx, a.logn);
+    if (dflag) return;                 // uniform over the block, before any barrier
     // the first pair of basis vectors is in flight while the FFT stages run
     const int64_t nvv = g.N / V16<T>::L;
     const int64_t e0 = (int64_t)slice * NT * EPT + threadIdx.x;
@@ -472,10 +476,6 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
             x[gg][q] = (gg < nvec && e < nvv) ? Vp[e] : V{};
         }
     }
-    const double sc = (int)threadIdx.x < nvec ? gs->scale[threadIdx.x] : 0.0;
-    FftTw<true, C> pt;
-    pt.load(a.tw, nx, a.logn);
-    if (dflag) return;                 // uniform over the block, before any barrier
     if ((int)threadIdx.x < PV) s_scale[threadIdx.x] = sc;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
