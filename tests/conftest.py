@@ -19,3 +19,13 @@ def cuda():
     if not torch.cuda.is_available():
         pytest.fail("GPU test selected but torch sees no GPU")
     return torch.device("cuda")
+
+
+@pytest.fixture
+def progress(capsys):
+    """Print a progress line of a long test as it happens, past pytest's
+    output capture (so a run that watches its output sees it moving)."""
+    def say(*args):
+        with capsys.disabled():
+            print(*args, flush=True)
+    return say

@@ -173,7 +173,7 @@ def test_gradient_matches_finite_differences(cuda):
         assert abs(fd - got) <= 1e-5 * abs(fd)
 
 
-def test_config5_gradient_every_amplitude(cuda):
+def test_config5_gradient_every_amplitude(cuda, progress):
     """The adjoint workload at its full size: bump-on-tail base state
     (k = 0.3, 256 x 512, v in [-10, 10]) plus sum_m a_m cos(m k x) M(v),
     m = 1..16, a_m ~ N(0, 1e-3) from default_rng(0); J4, dt = 0.05, 200
@@ -214,13 +214,12 @@ def test_config5_gradient_every_amplitude(cuda):
             e[m] = h
             fd = (energy(a0 + e).item() - energy(a0 - e).item()) / (2 * h)
             worst = max(worst, abs(fd - grad[m]) / gmax)
-            print("  a_%d: fd %.12e adjoint %.12e rel %.2e" % (m + 1, fd, grad[m], abs(fd - grad[m]) / abs(fd)),
-                  flush=True)
+            progress("  a_%d: fd %.12e adjoint %.12e rel %.2e" % (m + 1, fd, grad[m], abs(fd - grad[m]) / abs(fd)))
             assert abs(fd - grad[m]) <= 1e-6 * gmax, (m, fd, grad[m])
     print("worst |fd - grad| / max |grad|", worst, flush=True)
 
 
-def test_config5_size_gradient_matches_matrix_free_tangent_linear(cuda):
+def test_config5_size_gradient_matches_matrix_free_tangent_linear(cuda, progress):
     """At the adjoint workload's grid (256 x 512, bump-on-tail k = 0.3 with
     a 1e-3 cos(k x) perturbation), 3 timesteps: the adjoint gradient dE/df0
     along two random directions d in the span of the 16 perturbation modes
@@ -251,6 +250,7 @@ def test_config5_size_gradient_matches_matrix_free_tangent_linear(cuda):
     rng = np.random.default_rng(12)
     basis = perturbation_basis(st.grid, k, 16)          # (16, nv, nx): the amplitudes' directions
     for q in range(2):
+        progress("  tangent-linear model (CPU), direction %d" % q)
         d = np.tensordot(rng.standard_normal(16), basis, axes=1).T
         dK = TL.tangent_linear_mf('arakawaj4', states, d, h0, g, -1.0)
         ref = (gE * dK).sum()

@@ -27,7 +27,7 @@ STEPS = 2
 ATOL = 1e-10
 
 
-def test_ensemble_of_256_matches_c_oracle(cuda):
+def test_ensemble_of_256_matches_c_oracle(cuda, progress):
     from oracle import cbaseline
     from picard_ties import parting_kind, tie_f_rtol
     from vivlasov1d_amd.batch import BatchRun, make_instances
@@ -46,7 +46,9 @@ def test_ensemble_of_256_matches_c_oracle(cuda):
         states.append({s['index']: br.state(s['index'])[0] for s in specs})
         print("batched step", t + 1, "done", flush=True)
     ties, worst = 0, 0.0
-    for s in specs:
+    for n, s in enumerate(specs):
+        if n % 32 == 0:
+            progress("  ensemble: C restatement of instance %d of %d" % (n, len(specs)))
         bi, q = where[s['index']]
         g = O.Grid(s['nx'], s['nv'], s['L'], s['vmax'], ht=s['ht'])
         crun = cbaseline.CRun(g, states[0][s['index']], bracket='arakawaj4')

@@ -59,7 +59,7 @@ def _threshold_tie(s, ec, eg, crun, run, atol=1e-10, band=0.05, f_rtol=None):
 
 
 @pytest.mark.parametrize("kind,steps", [("arakawaj4", 500), ("arakawaj1", 500), ("arakawaj2", 500)])
-def test_many_steps_match_c_oracle(kind, steps, cuda):
+def test_many_steps_match_c_oracle(kind, steps, cuda, progress):
     from oracle import cbaseline
     from vivlasov1d_amd.runner import Runscript
     print("case", kind, steps, flush=True)
@@ -81,7 +81,7 @@ def test_many_steps_match_c_oracle(kind, steps, cuda):
         worst = max(worst, err)
         assert err <= 1e-12, (s, err)
         if (s + 1) % 25 == 0:
-            print("  step", s + 1, "worst f error so far", worst, flush=True)
+            progress("  %s step %d worst f error so far %.3e" % (kind, s + 1, worst))
     print("worst f error", worst, "threshold ties", ties, flush=True)
     assert ties <= 3
     inv = run.invariants()
@@ -130,7 +130,7 @@ def test_linear_landau_100_steps_match_c_oracle(cuda):
 JEANS_STRONG_DENSITY = "import math\ndef density(x, L):\n    return 1. + 0.01 * math.cos(0.1 * (x - 0.5 * L))\n"
 
 
-def test_jeans_strong_config_matches_c_oracle(cuda, tmp_path, monkeypatch):
+def test_jeans_strong_config_matches_c_oracle(cuda, tmp_path, monkeypatch, progress):
     """examples/jeans_strong.cfg as written -- 100 x 200, v in [-60, 60],
     dt = 0.1, J4, CFD4, tensor_kinetic, ksp_rtol 1e-4, snes_atol 1e-10,
     snes_max_iter 100, density_python (the example's n(x) = 1 + 0.01
@@ -172,12 +172,14 @@ def test_jeans_strong_config_matches_c_oracle(cuda, tmp_path, monkeypatch):
         err = np.abs(run.fc.cpu().numpy() - fc).max() / np.abs(fc).max()
         worst = max(worst, err)
         assert err <= 1e-8, (s, err)
+        if (s + 1) % 25 == 0:
+            progress("  jeans_strong step %d worst f error so far %.3e" % (s + 1, worst))
     print("worst f error", worst, "threshold ties", ties, flush=True)
     assert ties <= 3
     crun.close()
 
 
-def test_twostream_1024x2048_steps_match_c_oracle(cuda):
+def test_twostream_1024x2048_steps_match_c_oracle(cuda, progress):
     """The benchmark workload itself -- two-stream instability, nx x nv =
     1024 x 2048, x in [0, 2 pi/0.2), v in [-10, 10], f0 = (1 + 0.001 cos 0.2x)
     1/2 [M(v - 2.4) + M(v + 2.4)], J4, dt = 0.05, ksp_rtol 1e-10, snes_atol
@@ -200,7 +202,7 @@ def test_twostream_1024x2048_steps_match_c_oracle(cuda):
     ties = 0
     for s in range(3):
         ec, eg = crun.step(), run.step()
-        print("step", s + 1, "picard", ec['picard'], len(eg['picard']), flush=True)
+        progress("  twostream 1024x2048 step %d picard %d %d" % (s + 1, ec['picard'], len(eg['picard'])))
         if _threshold_tie(s, ec, eg, crun, run):
             ties += 1
             continue
