@@ -383,7 +383,8 @@ int arnoldi_step(vv_ctx *c, int it)
 int pc_prepare32(vv_ctx *c)
 {
     if (c->lu64 && !c->lu32) {
-        VV_CHECK(dalloc(c, &c->lu32, (size_t)c->nmodes * c->fft_rows * c->B), "alloc FP32 pivots");
+        VV_CHECK(dalloc(c, &c->lu32, (size_t)c->nmodes * lu_pivots_per_mode(c->fft_rows) * c->B),
+                 "alloc FP32 pivots");
         VV_CHECK(launch_tridiag_factor(c, nullptr, c->lu32), "FP32 pivots");
     }
     return VV_OK;
@@ -886,7 +887,7 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(dalloc(c, &c->twiddle32, c->fft_n), "alloc twiddle32");
     VV_CCHECK(dalloc(c, &c->spec, (size_t)c->nmodes * c->fft_rows * Bz), "alloc spectrum");
     if (p->precond == VV_PRECOND_TENSOR_KINETIC)
-        VV_CCHECK(dalloc(c, &c->lu64, (size_t)c->nmodes * c->fft_rows * Bz), "alloc pivots");
+        VV_CCHECK(dalloc(c, &c->lu64, (size_t)c->nmodes * lu_pivots_per_mode(c->fft_rows) * Bz), "alloc pivots");
     VV_CCHECK(dalloc(c, &c->pinv, nx * Bz), "alloc pinv");
     VV_CCHECK(dalloc(c, &c->pwork, (3 * (size_t)nx + 1) * Bz), "alloc Poisson scratch");
     VV_CCHECK(dalloc(c, &c->X, c->N * Bz), "alloc X");

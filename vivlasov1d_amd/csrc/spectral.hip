@@ -832,13 +832,17 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(TriArgs a, cx_t<T> *__restr
 //   backward  x_j = rp_j (y_j - c_j x_{j+1}),
 // a_j and c_j being rebuilt from e_k and the per-row factors as in tri_row.
 template <typename T>
-__global__ __launch_bounds__(NT) void tridiag_factor_kernel(TriArgs a, int nmodes, cx_t<T> *__restrict__ rp)
+__global__ __launch_bounds__(NT) void tridiag_factor_kernel(TriArgs a, int nmodes, int M, cx_t<T> *__restrict__ rp)
 {
     const int k = blockIdx.x * NT + threadIdx.x;
     if (k >= nmodes) return;
     const double af = a.inst[blockIdx.y].afpc;
     const double2 e = a.eigen[k];
-    cx_t<T> *R = rp + blockIdx.y * a.S + (size_t)k * a.nv;
+    // chunk-interleaved: the pivot of row j = t M + q at q NT + t, so that the
+    // solve's thread t, which owns rows t M .. t M + M - 1, loads them
+    // coalesced; rows nv .. NT M - 1 hold zero
+    const int L = NT * M;
+    cx_t<T> *R = rp + ((size_t)blockIdx.y * nmodes + k) * L;
     double lo, mid, hi;
     double2 aj, bj, cj;
     tri_load<false>(a, 0, lo, mid, hi);
@@ -850,8 +854,9 @@ __global__ __launch_bounds__(NT) void tridiag_factor_kernel(TriArgs a, int nmode
         tri_load<false>(a, j, lo, mid, hi);
         tri_row<false>(a, af, e, j, lo, mid, hi, aj, bj, cj);
         r = crecip(csub(bj, cmul(cmul(aj, r), cprev)));
-        R[j] = cmk<cx_t<T>>((T)r.x, (T)r.y);
+        R[(j % M) * NT + j / M] = cmk<cx_t<T>>((T)r.x, (T)r.y);
     }
+    for (int j = a.nv; j < L; ++j) R[(j % M) * NT + j / M] = cmk<cx_t<T>>(T(0), T(0));
 }
 
 template <typename C> __device__ __forceinline__ C shfl_up_cx(C v, int off)
@@ -880,27 +885,30 @@ __global__ __launch_bounds__(NT) void tridiag_lu_kernel(TriArgs a, const cx_t<T>
     const double af = a.inst[blockIdx.y].afpc;
     const double2 e = a.eigen[k];
     C *S = spec + blockIdx.y * a.S + (size_t)k * a.nv;
-    const C *Rg = rp + blockIdx.y * a.S + (size_t)k * a.nv;
+    // pivots in the chunk-interleaved order of tridiag_factor_kernel: thread
+    // t's rows t M + q at q NT + t, loaded coalesced straight into registers
+    const C *Rg = rp + ((size_t)blockIdx.y * gridDim.x + k) * (NT * M);
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *sd = reinterpret_cast<C *>(lds_raw);      // NT (M+1): right-hand side, then y, then x
-    C *sr = sd + NT * (M + 1);                   // NT (M+1): pivots
     __shared__ C wa[NT / 64], wb[NT / 64];
     const C zero = cmk<C>(T(0), T(0)), one = cmk<C>(T(1), T(0));
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    C rpv[M];                                     // pivots of rows t M .. t M + M - 1
+    C rprev;                                      // pivot of row t M - 1 (last row of chunk t - 1)
     {
-        C v[M], w[M];
+        C v[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) {
             const int j = t + q * NT;
             v[q] = load_or_zero(S, j, j < a.nv);
-            w[q] = load_or_zero(Rg, j, j < a.nv);
+            rpv[q] = Rg[q * NT + t];
         }
+        rprev = load_or_zero(Rg, (M - 1) * NT + t - 1, t > 0);
         if (dflag) return;             // uniform over the block, before any barrier
 #pragma unroll
         for (int q = 0; q < M; ++q) {
             const int j = t + q * NT;
             sd[j + j / M] = v[q];     // one pad slot per chunk
-            sr[j + j / M] = w[q];
         }
     }
     __syncthreads();
@@ -914,12 +922,11 @@ __global__ __launch_bounds__(NT) void tridiag_lu_kernel(TriArgs a, const cx_t<T>
         tri_load<false>(a, s + q, lo, mid, hi);
         tri_row<false>(a, af, e, s + q, lo, mid, hi, av[q], bj, cv[q]);
     }
-    const C rprev = t > 0 ? sr[ls - 2] : zero;   // pivot of row s - 1 (last row of chunk t - 1)
     // forward map of the chunk: y_{s+M-1} = A + B y_{s-1}
     C A = sd[ls], B = cneg(cmul(av[0], rprev));
 #pragma unroll
     for (int q = 1; q < M; ++q) {
-        const C m = cmul(av[q], sr[ls + q - 1]);
+        const C m = cmul(av[q], rpv[q - 1]);
         A = csub(sd[ls + q], cmul(m, A));
         B = cneg(cmul(m, B));
     }
@@ -948,15 +955,15 @@ __global__ __launch_bounds__(NT) void tridiag_lu_kernel(TriArgs a, const cx_t<T>
     sd[ls] = y;
 #pragma unroll
     for (int q = 1; q < M; ++q) {
-        y = csub(sd[ls + q], cmul(cmul(av[q], sr[ls + q - 1]), y));
+        y = csub(sd[ls + q], cmul(cmul(av[q], rpv[q - 1]), y));
         sd[ls + q] = y;
     }
     // backward map of the chunk: x_s = G + D x_{s+M}
-    C G = cmul(sr[ls + M - 1], sd[ls + M - 1]), D = cneg(cmul(sr[ls + M - 1], cv[M - 1]));
+    C G = cmul(rpv[M - 1], sd[ls + M - 1]), D = cneg(cmul(rpv[M - 1], cv[M - 1]));
 #pragma unroll
     for (int q = M - 2; q >= 0; --q) {
-        const C rc = cmul(sr[ls + q], cv[q]);
-        G = csub(cmul(sr[ls + q], sd[ls + q]), cmul(rc, G));
+        const C rc = cmul(rpv[q], cv[q]);
+        G = csub(cmul(rpv[q], sd[ls + q]), cmul(rc, G));
         D = cneg(cmul(rc, D));
     }
 #pragma unroll
@@ -983,7 +990,7 @@ __global__ __launch_bounds__(NT) void tridiag_lu_kernel(TriArgs a, const cx_t<T>
     x = cadd(xa, cmul(xb, x));                    // x_{s+M}
 #pragma unroll
     for (int q = M - 1; q >= 0; --q) {
-        x = cmul(sr[ls + q], csub(sd[ls + q], cmul(cv[q], x)));
+        x = cmul(rpv[q], csub(sd[ls + q], cmul(cv[q], x)));
         sd[ls + q] = x;
     }
     __syncthreads();
@@ -1189,12 +1196,6 @@ void tridiag_shape(int nv, int *P, int *M)
     }
 }
 
-// rows per thread of the pivoted solve (NT threads per mode)
-int lu_rows(int nv)
-{
-    const int m = (nv + NT - 1) / NT;
-    return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : 16;
-}
 
 template <typename T>
 const cx_t<T> *lu_factors(const vv_ctx *c);
@@ -1207,7 +1208,7 @@ hipError_t tridiag_lu_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
     TriArgs a{c->fft_rows, NT, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp,
               c->coef_stride, c->cf_lo, c->cf_mid, c->cf_hi, c->inst, c->ht_inv, done};
     const int M = lu_rows(c->fft_rows);
-    const size_t lds = (size_t)2 * NT * (M + 1) * sizeof(cx_t<T>);
+    const size_t lds = (size_t)NT * (M + 1) * sizeof(cx_t<T>);
     const cx_t<T> *rp = lu_factors<T>(c);
     dim3 grid(c->nmodes, c->B), block(NT);
     switch (M) {
@@ -1240,6 +1241,16 @@ hipError_t tridiag_t(vv_ctx *c, cx_t<T> *spec, const int32_t *done)
 }
 
 }  // namespace
+
+// rows per thread of the pivoted solve (NT threads per mode), and the pivots
+// stored per mode (NT x rows per thread, chunk-interleaved)
+int lu_rows(int nv)
+{
+    const int m = (nv + NT - 1) / NT;
+    return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : 16;
+}
+
+int lu_pivots_per_mode(int nv) { return NT * lu_rows(nv); }
 
 int fft_blocks(vv_ctx *c)
 {
@@ -1302,8 +1313,9 @@ hipError_t launch_tridiag_factor(vv_ctx *c, double2 *rp64, float2 *rp32)
     TriArgs a{c->fft_rows, NT, spec_stride(c), (int64_t)sizeof(GmresState), c->eigen, c->gm, c->g0, c->gp,
               c->coef_stride, c->cf_lo, c->cf_mid, c->cf_hi, c->inst, c->ht_inv, nullptr};
     dim3 grid((c->nmodes + NT - 1) / NT, c->B);
-    if (rp64) hipLaunchKernelGGL(tridiag_factor_kernel<double>, grid, dim3(NT), 0, c->stream, a, c->nmodes, rp64);
-    if (rp32) hipLaunchKernelGGL(tridiag_factor_kernel<float>, grid, dim3(NT), 0, c->stream, a, c->nmodes, rp32);
+    const int M = lu_rows(c->fft_rows);
+    if (rp64) hipLaunchKernelGGL(tridiag_factor_kernel<double>, grid, dim3(NT), 0, c->stream, a, c->nmodes, M, rp64);
+    if (rp32) hipLaunchKernelGGL(tridiag_factor_kernel<float>, grid, dim3(NT), 0, c->stream, a, c->nmodes, M, rp32);
     return hipGetLastError();
 }
 

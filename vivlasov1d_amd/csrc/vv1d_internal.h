@@ -208,6 +208,10 @@ hipError_t launch_irfft32(vv_ctx *c, const float2 *spec, float *y, const int32_t
 // (per-slice partials, as launch_mdot); irfft_mdot_rows: the partial rows it
 // writes (its grid), 0 where the fused kernel does not apply
 int irfft_mdot_rows(const vv_ctx *c, bool fp32);
+// rows per thread of the LU tridiagonal solve, and the pivots it stores per
+// mode (chunk-interleaved, >= nv)
+int lu_rows(int nv);
+int lu_pivots_per_mode(int nv);
 hipError_t launch_irfft_mdot(vv_ctx *c, const double2 *spec, int col, const int32_t *done);
 hipError_t launch_irfft_mdot32(vv_ctx *c, const float2 *spec, int col, const int32_t *done);
 // potential preconditioner: field (nv rows of nx) <-> transposed (nx rows of nv)
