@@ -89,7 +89,8 @@ def cpu_baseline(w, sample_steps, threads):
     return cbaseline.time_sample(w, sample_steps, threads)
 
 
-def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeats=None, fp64_above="auto"):
+def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeats=None, fp64_above="auto",
+              every=1):
     """W warm-up steps, then `repeats` windows of K timed steps each; returns
     the run, the median window time, the records of all timed steps and the
     per-kernel event statistics (summed over the windows)."""
@@ -109,9 +110,9 @@ def timed_run(w, args, precision, kernel, refine="newton", ksp_rtol=None, repeat
         run.step()
     torch.cuda.synchronize()
     if kernel == "all":
-        ctx.profile(None)
+        ctx.profile(None, every=every)
     elif kernel not in (None, "none"):
-        ctx.profile(tuple(kernel.split(",")))
+        ctx.profile(tuple(kernel.split(",")), every=every)
     nrec0 = len(run.history)
     windows = []
     for _ in range(repeats):
@@ -348,8 +349,13 @@ def main():
     ap.add_argument("--workload", default="twostream_1024x2048_J4")
     ap.add_argument("--kernel", default="irfft_mdot",
                     help="kernels timed with HIP events in the timed FP64 run ('all', 'none' or a comma list of "
-                         "%s); the roofline is the one with the largest total time.  Each timed dispatch costs "
-                         "about 2 percent of throughput, so the default times the dominant kernel only" % (KERNELS,))
+                         "%s); the roofline is the one with the largest total time.  Timed dispatches cost "
+                         "throughput (about 4 percent when every launch of the dominant kernel is timed), so the "
+                         "default times the dominant kernel only, one launch in --kernel-every" % (KERNELS,))
+    ap.add_argument("--kernel-every", type=int, default=8,
+                    help="time one launch in every N of --kernel in the timed run (a timed dispatch idles the "
+                         "GPU for several microseconds around it; a sample keeps the value run close to an "
+                         "untimed one)")
     ap.add_argument("--bracket-kernels", default="jacobian,residual,rfft,tridiag,maxpy_norm",
                     help="kernels timed in a second, untimed FP64 run: the bracket HBM GB/s (jacobian, "
                          "residual) and the other kernels of the step ('none' skips)")
@@ -406,7 +412,7 @@ def main():
 
     w = workloads()[args.workload]
     N = w["nx"] * w["nv"]
-    run, elapsed, recs, kstats = timed_run(w, args, "double", args.kernel)
+    run, elapsed, recs, kstats = timed_run(w, args, "double", args.kernel, every=args.kernel_every)
     progress("FP64 timed")
     dom = max(kstats, key=lambda k: kstats[k]["ms"]) if kstats else None
     roof = None
@@ -415,7 +421,7 @@ def main():
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": d["bytes"] / d["launches"], "avg_launch_us": round(d["avg_us"], 3),
-                "launches": d["launches"]}
+                "launches": d["launches"], "sampled_every": args.kernel_every}
         # HBM bytes per launch measured by the PMC runs summarised in profiles/
         from bench_profiles import read_summary
         src = os.path.join("profiles", "%s_%s.txt" % (dom, args.workload))

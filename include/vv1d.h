@@ -447,15 +447,21 @@ int vv_invariants(vv_ctx *ctx, const double *f, const double *h0,
 
 /* Per-kernel timing with HIP events attached to each timed kernel's own dispatch (for the
  * roofline report).  kernel: "jacobian", "residual", "rfft", "tridiag",
- * "irfft", "mdot", "maxpy_norm" and the FP32 kernels of the mixed solver
- * "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32".
+ * "irfft", "mdot", "maxpy_norm", the FP32 kernels of the mixed solver
+ * "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
+ * and the fused irFFT + Gram-Schmidt dots "irfft_mdot", "irfft_mdot32".
  * Enable, run, then read the summed time in ms, the launch count and the
  * summed algorithmic HBM bytes of those launches (Krylov launches that
  * returned at once after convergence are not counted).  Reading clears that
  * kernel's records.  enable is a bit mask over the kernels in the order
  * listed (jacobian = 1, residual = 2, rfft = 4, ..., maxpy_norm = 64,
- * jacobian32 = 128, ..., maxpy_norm32 = 4096; -1 all, 0 off). */
+ * jacobian32 = 128, ..., maxpy_norm32 = 4096, irfft_mdot = 8192,
+ * irfft_mdot32 = 16384; -1 all, 0 off). */
 int vv_profile_enable(vv_ctx *ctx, int enable);
+/* Time one launch in every `every` (>= 1, default 1) of each enabled kernel
+ * kind: a HIP-event-timed dispatch has idle time of its own around it, so a
+ * sample keeps the measured run close to an untimed one. */
+int vv_profile_sample(vv_ctx *ctx, int every);
 int vv_profile_read(vv_ctx *ctx, const char *kernel, double *ms, int64_t *launches,
                     double *bytes);
 

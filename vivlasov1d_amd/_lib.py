@@ -125,6 +125,7 @@ SIGNATURES = {
     "vv_adjoint_step": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(KspResult)]),
     "vv_invariants": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _PD]),
     "vv_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
+    "vv_profile_sample": (ctypes.c_int, [_P, ctypes.c_int]),
     "vv_profile_read": (ctypes.c_int, [_P, ctypes.c_char_p, _PD, ctypes.POINTER(ctypes.c_int64), _PD]),
 }
 

@@ -43,6 +43,11 @@ struct ProfRec {
 struct Prof {
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
+    // time one launch in every `every` of each enabled kind: the events put
+    // idle time of their own around a timed dispatch (several microseconds),
+    // so timing every launch would slow the run it measures
+    int every = 1;
+    int64_t seen[P_NKINDS] = {};
 };
 
 // one profiler per context, kept outside the C struct
@@ -71,7 +76,7 @@ struct ProfScope {
     // Nothing is timed while a graph is being captured.
     ProfScope(vv_ctx *cc, int k, double by, int tg = -1) : c(cc), kind(k), tag(tg), bytes(by)
     {
-        if (((c->prof >> kind) & 1) && !c->capturing) {
+        if (((c->prof >> kind) & 1) && !c->capturing && prof_of(c)->seen[kind]++ % prof_of(c)->every == 0) {
             Prof *p = prof_of(c);
             a = prof_event(p);
             b = prof_event(p);
@@ -1708,6 +1713,16 @@ int vv_profile_enable(vv_ctx *c, int enable)
 {
     if (int r = check_ctx(c)) return r;
     c->prof = enable;
+    return VV_OK;
+}
+
+int vv_profile_sample(vv_ctx *c, int every)
+{
+    if (int r = check_ctx(c)) return r;
+    if (every < 1) return fail(c, VV_ERR_INVALID, "every must be >= 1");
+    Prof *p = prof_of(c);
+    p->every = every;
+    for (auto &n : p->seen) n = 0;
     return VV_OK;
 }
 

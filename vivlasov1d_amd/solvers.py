@@ -393,8 +393,10 @@ class Context:
                        "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
                        "irfft_mdot", "irfft_mdot32")
 
-    def profile(self, kernels=None):
-        """Time the named kernels (all when None; () switches timing off)."""
+    def profile(self, kernels=None, every=1):
+        """Time the named kernels (all when None; () switches timing off),
+        one launch in every `every` of each."""
+        self.call("vv_profile_sample", ctypes.c_int(int(every)))
         if kernels is None:
             mask = -1
         else:
