@@ -130,11 +130,36 @@ hipError_t dalloc(vv_ctx *c, T **p, size_t count)
     return e;
 }
 
-// count doubles from device to host (through the pinned staging buffer)
+// count doubles from device memory into the mapped pinned staging buffer
+__global__ void scalars_to_host_kernel(const double *__restrict__ src, double *dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// the five 1D factors of H in one launch (n = nx B for the x-factors, nv for h0)
+struct FactorCopy {
+    const double *src[5];
+    double *dst[5];
+    int n[5];
+};
+__global__ void factors_copy_kernel(FactorCopy a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        if (i < a.n[q]) a.dst[q][i] = a.src[q][i];
+}
+
+// count doubles from device to host: a small kernel writes them into the
+// mapped pinned buffer (no copy-engine dispatch and its host-side set-up),
+// then the stream is drained
 int read_scalars(vv_ctx *c, const double *dev, size_t count, double *out)
 {
-    VV_CHECK(hipMemcpyAsync(c->host_scalars, dev, count * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-             "copy scalars");
+    const int n = (int)count;
+    hipLaunchKernelGGL(scalars_to_host_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, dev,
+                       c->host_scalars_dev, n);
+    VV_CHECK(hipGetLastError(), "copy scalars");
     VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
     for (size_t q = 0; q < count; ++q) out[q] = c->host_scalars[q];
     return VV_OK;
@@ -909,6 +934,8 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
               "pinned status device address");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_scalars), ds_count(B) * sizeof(double)),
               "pinned scalars");
+    VV_CCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->host_scalars_dev), c->host_scalars, 0),
+              "pinned scalars device address");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_active), Bz * sizeof(int32_t)), "pinned mask");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming), "event");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming), "event");
@@ -1211,12 +1238,11 @@ int vv_set_hamiltonian(vv_ctx *c, const double *h0, const double *h1p, const dou
 {
     if (int r = check_ctx(c)) return r;
     if (!h0 || !h1p || !h1h || !h2p || !h2h) return fail(c, VV_ERR_INVALID, "null Hamiltonian factor");
-    const size_t bx = (size_t)c->p.nx * c->B * sizeof(double), bv = c->p.nv * sizeof(double);
-    VV_CHECK(hipMemcpyAsync(c->h0, h0, bv, hipMemcpyDeviceToDevice, c->stream), "copy h0");
-    VV_CHECK(hipMemcpyAsync(c->h1p, h1p, bx, hipMemcpyDeviceToDevice, c->stream), "copy h1p");
-    VV_CHECK(hipMemcpyAsync(c->h1h, h1h, bx, hipMemcpyDeviceToDevice, c->stream), "copy h1h");
-    VV_CHECK(hipMemcpyAsync(c->h2p, h2p, bx, hipMemcpyDeviceToDevice, c->stream), "copy h2p");
-    VV_CHECK(hipMemcpyAsync(c->h2h, h2h, bx, hipMemcpyDeviceToDevice, c->stream), "copy h2h");
+    const int nxb = c->p.nx * c->B, nv = c->p.nv;
+    const FactorCopy a{{h0, h1p, h1h, h2p, h2h}, {c->h0, c->h1p, c->h1h, c->h2p, c->h2h}, {nv, nxb, nxb, nxb, nxb}};
+    const int n = nxb > nv ? nxb : nv;
+    hipLaunchKernelGGL(factors_copy_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, a);
+    VV_CHECK(hipGetLastError(), "copy Hamiltonian factors");
     c->db1_valid = false;
     return VV_OK;
 }

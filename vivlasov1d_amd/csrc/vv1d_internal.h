@@ -119,6 +119,7 @@ struct vv_ctx {
     vv::GmresStatus *host_status_dev;   // the same memory as the device addresses it
     vv::GmresStatus *status_out;    // set for one launch: the Hessenberg update also stores the statuses there
     double *host_scalars;           // pinned staging of device scalars
+    double *host_scalars_dev;       // the same memory as the device addresses it
     int32_t *host_active;           // pinned staging of the instance mask
     hipEvent_t ev[2];
     // profiling (per-kernel HIP event pairs)
