@@ -356,8 +356,26 @@ static void moments(vvo_run *r)
     for (int i = 0; i < nx; ++i) b[i] = -(r->n[i] - nmean) * r->charge;
     double ps = 0.0;
     for (int i = 0; i < nx; ++i) {
-        double s = 0.0;
-        for (int m = 0; m < nx; ++m) s += r->pinv[m] * b[(i - m + nx) % nx];
+        /* phi_i = sum_m pinv_m b_{(i-m) mod nx} in the device kernel's order:
+           four quarters of the m range, each with four interleaved partial
+           sums, the quarters added in order (a summation order of this
+           restatement's own; the reference solves with a PETSc KSP) */
+        const int q = (nx + 3) / 4;
+        double p[4];
+        for (int w = 0; w < 4; ++w) {
+            const int m0 = w * q, m1 = m0 + q < nx ? m0 + q : nx;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int m = m0;
+            for (; m + 4 <= m1; m += 4) {
+                s0 += r->pinv[m] * b[(i - m + nx) % nx];
+                s1 += r->pinv[m + 1] * b[(i - m - 1 + nx) % nx];
+                s2 += r->pinv[m + 2] * b[(i - m - 2 + nx) % nx];
+                s3 += r->pinv[m + 3] * b[(i - m - 3 + nx) % nx];
+            }
+            for (; m < m1; ++m) s0 += r->pinv[m] * b[(i - m + nx) % nx];
+            p[w] = (s0 + s1) + (s2 + s3);
+        }
+        const double s = ((p[0] + p[1]) + p[2]) + p[3];
         r->phi[i] = s;
         ps += s;
     }

@@ -149,7 +149,10 @@ def test_moments_and_poisson(nx, nv, laplace, cuda):
     h1_ref = O.potential_to_hamiltonian(phi_ref, nx)
     assert np.abs(h1.cpu().numpy() - h1_ref).max() <= 1e-11 * scale
     pr = np.linalg.norm(O.poisson_function_mult(phi2.cpu().numpy(), n_ref, -1.0, g.hx2_inv, laplace))
-    assert abs(pn - pr) <= 1e-6 * max(pr, 1e-300) + 1e-14
+    # the residual sits at the rounding floor; the device and NumPy form the
+    # mean density in different orders, which moves every entry by up to a
+    # few ulps of |n| (|q| = 1): sqrt(nx) 8 eps max|n| bounds the difference
+    assert abs(pn - pr) <= 1e-6 * max(pr, 1e-300) + 8 * np.sqrt(nx) * np.finfo(float).eps * np.abs(n_ref).max()
     h1b = ctx.new_x()
     ctx.potential_to_hamiltonian(phi2, h1b)
     assert np.abs(h1b.cpu().numpy() - h1.cpu().numpy()).max() <= 1e-15 * scale

@@ -379,20 +379,28 @@ __global__ __launch_bounds__(NT) void gmres_update_kernel(const double *part, in
     g += blockIdx.y;
     part += blockIdx.y * pstride;
     if (hout) hout += blockIdx.y;
-    if (g->st.done) {
-        if (hout && threadIdx.x == 0) put_status(hout, g->st);
-        return;
-    }
+    // every load of the block is issued before the early-exit test waits on
+    // the flag's (reading the state of a finished cycle is harmless)
+    const int32_t done = g->st.done;
     __shared__ double red[NW + 1];
     __shared__ double s_h[kMaxRestart + 2], s_c[kMaxRestart], s_s[kMaxRestart];
     double *hh = g->hh + (size_t)it * (kMaxRestart + 1);
-    if (threadIdx.x <= it) s_h[threadIdx.x] = hh[threadIdx.x];
-    if (threadIdx.x < it) {
-        s_c[threadIdx.x] = g->cc[threadIdx.x];
-        s_s[threadIdx.x] = g->ss[threadIdx.x];
-    }
+    const double h_t = threadIdx.x <= it ? hh[threadIdx.x] : 0.0;
+    const double c_t = threadIdx.x < it ? g->cc[threadIdx.x] : 0.0;
+    const double s_t = threadIdx.x < it ? g->ss[threadIdx.x] : 0.0;
     const double grs_it = g->grs[it];
-    const double tot = final_sum_1(part, nb, red);     // barriers: the LDS stores above are visible
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += NT) acc += part[b];
+    if (done) {                        // uniform over the block, before any barrier
+        if (hout && threadIdx.x == 0) put_status(hout, g->st);
+        return;
+    }
+    if (threadIdx.x <= it) s_h[threadIdx.x] = h_t;
+    if (threadIdx.x < it) {
+        s_c[threadIdx.x] = c_t;
+        s_s[threadIdx.x] = s_t;
+    }
+    const double tot = block_sum_all(acc, red);        // barriers: the LDS stores above are visible
     if (threadIdx.x != 0) return;
     const double tt = sqrt(tot);
     s_h[it + 1] = tt;
@@ -445,6 +453,7 @@ __global__ __launch_bounds__(NT) void backsolve_kernel(GmresState *g)
     g += blockIdx.y;
     __shared__ double sh[kMaxRestart][kMaxRestart];
     __shared__ double sg[kMaxRestart];
+    __shared__ double y[kMaxRestart];     // in LDS: a runtime-indexed register array would live in scratch
     const int n = g->st.it;
     if (n <= 0) return;
     for (int idx = threadIdx.x; idx < n * n; idx += NT) {
@@ -455,7 +464,6 @@ __global__ __launch_bounds__(NT) void backsolve_kernel(GmresState *g)
     __syncthreads();
     if (threadIdx.x != 0) return;
     const int k = n - 1;
-    double y[kMaxRestart];
     if (sh[k][k] != 0.0) {
         y[k] = sg[k] / sh[k][k];
     } else {

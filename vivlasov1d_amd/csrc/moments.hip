@@ -148,10 +148,15 @@ __global__ __launch_bounds__(PT) void poisson_prep_kernel(PoiArgs a)
     if (threadIdx.x == 0) a.work[3 * nx] = nmean;
 }
 
-// phi_i = sum_m c_m b_{i-m}: C the mean-free circulant inverse of A
+// phi_i = sum_m c_m b_{i-m}: C the mean-free circulant inverse of A.  A
+// workgroup forms 64 outputs (one per lane); its 4 wavefronts take a quarter
+// of the m range each, with 4 interleaved accumulators per lane, and the
+// quarters are added in order: fixed summation order, chains of nx/16 terms.
+constexpr int CONV_OUT = 64;
 __global__ __launch_bounds__(NT) void poisson_conv_kernel(PoiArgs a)
 {
     extern __shared__ double sm[];
+    __shared__ double part[NT / 64][CONV_OUT];
     poi_instance(a);
     const int nx = a.nx;
     double *sb = sm, *sc = sm + nx;
@@ -160,11 +165,37 @@ __global__ __launch_bounds__(NT) void poisson_conv_kernel(PoiArgs a)
         sc[i] = a.pinv[i];
     }
     __syncthreads();
-    const int i = blockIdx.x * NT + threadIdx.x;
-    if (i >= nx) return;
-    double s = 0.0;
-    for (int m = 0, d = i; m < nx; ++m, --d) s += sc[m] * sb[d < 0 ? d + nx : d];   // sb[(i - m) mod nx]
-    a.work[nx + i] = s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * CONV_OUT + lane;
+    const int q = (nx + NT / 64 - 1) / (NT / 64);
+    const int m0 = wave * q, m1 = m0 + q < nx ? m0 + q : nx;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (i < nx) {
+        int m = m0;
+        int d = i - m0;                          // (i - m) mod nx, kept in [0, nx)
+        d = d < 0 ? d + nx : d;
+        for (; m + 4 <= m1; m += 4) {
+            const int d1 = d - 1 < 0 ? d - 1 + nx : d - 1;
+            const int d2 = d1 - 1 < 0 ? d1 - 1 + nx : d1 - 1;
+            const int d3 = d2 - 1 < 0 ? d2 - 1 + nx : d2 - 1;
+            s0 += sc[m] * sb[d];
+            s1 += sc[m + 1] * sb[d1];
+            s2 += sc[m + 2] * sb[d2];
+            s3 += sc[m + 3] * sb[d3];
+            d = d3 - 1 < 0 ? d3 - 1 + nx : d3 - 1;
+        }
+        for (; m < m1; ++m) {
+            s0 += sc[m] * sb[d];
+            d = d - 1 < 0 ? d - 1 + nx : d - 1;
+        }
+    }
+    part[wave][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (wave == 0 && i < nx) {
+        double s = part[0][lane];
+        for (int w = 1; w < NT / 64; ++w) s += part[w][lane];
+        a.work[nx + i] = s;
+    }
 }
 
 // one workgroup: h1 = phi - mean phi and ||A phi + (n - mean n) q||
@@ -362,8 +393,8 @@ hipError_t launch_poisson(vv_ctx *c, const double *n_in, double *n_out, double *
               work != nullptr ? work : c->pwork, n_out, phi, h1, pres, pres_stride};
     const int nx = c->p.nx;
     hipLaunchKernelGGL(poisson_prep_kernel, dim3(1, c->B), dim3(PT), 0, c->stream, a);
-    hipLaunchKernelGGL(poisson_conv_kernel, dim3((nx + NT - 1) / NT, c->B), dim3(NT), 2 * nx * sizeof(double),
-                       c->stream, a);
+    hipLaunchKernelGGL(poisson_conv_kernel, dim3((nx + CONV_OUT - 1) / CONV_OUT, c->B), dim3(NT),
+                       2 * nx * sizeof(double), c->stream, a);
     hipLaunchKernelGGL(poisson_post_kernel, dim3(1, c->B), dim3(PT), nx * sizeof(double), c->stream, a);
     return hipGetLastError();
 }
