@@ -137,6 +137,29 @@ __global__ void scalars_to_host_kernel(const double *__restrict__ src, double *d
     if (i < n) dst[i] = src[i];
 }
 
+// the statuses of the B instances into a mapped pinned slot, one lane per
+// instance (per-field stores)
+__global__ void status_to_host_kernel(const GmresState *__restrict__ g, GmresStatus *dst, int B)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const GmresStatus st = g[b].st;
+    dst[b].it = st.it;
+    dst[b].its = st.its;
+    dst[b].reason = st.reason;
+    dst[b].done = st.done;
+    dst[b].active = st.active;
+    dst[b].res = st.res;
+    dst[b].rnorm0 = st.rnorm0;
+}
+
+// the instance mask from its mapped pinned staging into device memory
+__global__ void mask_from_host_kernel(const int32_t *src, int32_t *__restrict__ dst, int B)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) dst[b] = src[b];
+}
+
 // the five 1D factors of H in one launch (n = nx B for the x-factors, nv for h0)
 struct FactorCopy {
     const double *src[5];
@@ -499,11 +522,13 @@ int max_it_of(const GmresStatus *s, int B)
     return m;
 }
 
+// the device statuses into host slot `slot` by a kernel writing the mapped
+// pinned memory (no copy-engine dispatch)
 int copy_status(vv_ctx *c, int slot)
 {
-    VV_CHECK(hipMemcpy2DAsync(status_slot(c, slot), sizeof(GmresStatus), &c->gs->st, sizeof(GmresState),
-                              sizeof(GmresStatus), c->B, hipMemcpyDeviceToHost, c->stream),
-             "status copy");
+    hipLaunchKernelGGL(status_to_host_kernel, dim3((c->B + 255) / 256), dim3(256), 0, c->stream, c->gs,
+                       c->host_status_dev + (size_t)slot * c->B, c->B);
+    VV_CHECK(hipGetLastError(), "status copy");
     return VV_OK;
 }
 
@@ -937,6 +962,8 @@ int create_impl(const vv_params *p, int B, const double *hxs, void *stream, vv_c
     VV_CCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->host_scalars_dev), c->host_scalars, 0),
               "pinned scalars device address");
     VV_CCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->host_active), Bz * sizeof(int32_t)), "pinned mask");
+    VV_CCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->host_active_dev), c->host_active, 0),
+              "pinned mask device address");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming), "event");
     VV_CCHECK(hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming), "event");
     VV_CCHECK(hipMemset(c->gs, 0, Bz * sizeof(GmresState)), "zero gmres state");
@@ -1094,13 +1121,15 @@ int64_t vv_scratch_bytes(const vv_ctx *c)
 int vv_set_active(vv_ctx *c, const int32_t *active)
 {
     if (int r = check_ctx(c)) return r;
-    // stream-ordered copy from pinned memory: no legacy-stream operation, so
-    // other streams may be capturing graphs meanwhile
+    // stream-ordered: a kernel on the context's stream reads the mapped
+    // pinned staging (no legacy-stream operation, so other streams may be
+    // capturing graphs meanwhile); the drain first keeps an earlier upload
+    // from seeing the new values
     VV_CHECK(hipStreamSynchronize(c->stream), "synchronize");
     for (int b = 0; b < c->B; ++b) c->host_active[b] = active ? (active[b] ? 1 : 0) : 1;
-    VV_CHECK(hipMemcpyAsync(c->active, c->host_active, (size_t)c->B * sizeof(int32_t), hipMemcpyHostToDevice,
-                            c->stream),
-             "upload mask");
+    hipLaunchKernelGGL(mask_from_host_kernel, dim3((c->B + 255) / 256), dim3(256), 0, c->stream,
+                       c->host_active_dev, c->active, c->B);
+    VV_CHECK(hipGetLastError(), "upload mask");
     return VV_OK;
 }
 

@@ -121,6 +121,7 @@ struct vv_ctx {
     double *host_scalars;           // pinned staging of device scalars
     double *host_scalars_dev;       // the same memory as the device addresses it
     int32_t *host_active;           // pinned staging of the instance mask
+    int32_t *host_active_dev;       // the same memory as the device addresses it
     hipEvent_t ev[2];
     // profiling (per-kernel HIP event pairs)
     int prof;           // bit k set: time kernel kind k
