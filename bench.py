@@ -39,7 +39,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
-           "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32", "irfft_mdot", "irfft_mdot32")
+           "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32", "irfft_mdot", "irfft_mdot32",
+           "jac_rfft")
 
 
 def workloads():

@@ -136,8 +136,10 @@ int vv_autotune(vv_ctx *ctx, int32_t *fft_rows, int32_t *cgs_ept);
  * FFT workgroup's rows are exactly one Gram-Schmidt slice (power-of-two nx
  * <= 1024, kinetic preconditioner, rows per FFT workgroup x nx = 256 x
  * cgs_ept x 16 bytes / element size), the Arnoldi step runs the irFFT of
- * P^-1 J v and the CGS dots of the new vector in one launch.  Results are
- * bit-identical either way; 0 runs the separate kernels (for timing). */
+ * P^-1 J v and the CGS dots of the new vector in one launch, and where
+ * besides nx >= 256 and no dissipation term is set, J v and its rFFT in one
+ * launch (J v is not stored).  Results are identical either way (up to the
+ * sign of exact zeros); 0 runs the separate kernels (for timing). */
 int vv_set_fusion(vv_ctx *ctx, int enable);
 
 int vv_ctx_destroy(vv_ctx *ctx);
@@ -449,14 +451,15 @@ int vv_invariants(vv_ctx *ctx, const double *f, const double *h0,
  * roofline report).  kernel: "jacobian", "residual", "rfft", "tridiag",
  * "irfft", "mdot", "maxpy_norm", the FP32 kernels of the mixed solver
  * "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
- * and the fused irFFT + Gram-Schmidt dots "irfft_mdot", "irfft_mdot32".
+ * the fused irFFT + Gram-Schmidt dots "irfft_mdot", "irfft_mdot32", and
+ * the Jacobian-vector product fused with the rFFT "jac_rfft".
  * Enable, run, then read the summed time in ms, the launch count and the
  * summed algorithmic HBM bytes of those launches (Krylov launches that
  * returned at once after convergence are not counted).  Reading clears that
  * kernel's records.  enable is a bit mask over the kernels in the order
  * listed (jacobian = 1, residual = 2, rfft = 4, ..., maxpy_norm = 64,
  * jacobian32 = 128, ..., maxpy_norm32 = 4096, irfft_mdot = 8192,
- * irfft_mdot32 = 16384; -1 all, 0 off). */
+ * irfft_mdot32 = 16384, jac_rfft = 32768; -1 all, 0 off). */
 int vv_profile_enable(vv_ctx *ctx, int enable);
 /* Time one launch in every `every` (>= 1, default 1) of each enabled kernel
  * kind: a HIP-event-timed dispatch has idle time of its own around it, so a

@@ -2,7 +2,9 @@
 
 Each translation unit is compiled separately so that the bracket kernels can
 use ``-ffp-contract=off`` (bit-faithful to the reference's unfused double
-arithmetic) while the spectral and Krylov kernels keep FMA contraction.
+arithmetic) while the spectral kernels keep FMA contraction; the spectral
+unit honours ``#pragma clang fp contract(off)``, which the stencil code it
+shares with the bracket kernels (arakawa.h) carries.
 Usage: ``python -m vivlasov1d_amd.build`` or ``build()``.
 """
 
@@ -24,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 
 UNITS = {
     "bracket.hip": ["-ffp-contract=off"],
-    "spectral.hip": ["-ffp-contract=fast"],
+    "spectral.hip": ["-ffp-contract=fast-honor-pragmas"],
     "krylov.hip": ["-ffp-contract=off"],
     "moments.hip": ["-ffp-contract=off"],
     "dissipation.hip": ["-ffp-contract=off"],

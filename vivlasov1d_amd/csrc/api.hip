@@ -20,11 +20,12 @@ thread_local std::string g_create_error;
 
 enum ProfKind {
     P_JAC = 0, P_RES, P_RFFT, P_TRI, P_IRFFT, P_MDOT, P_MAXPY,
-    P_JAC32, P_RFFT32, P_TRI32, P_IRFFT32, P_MDOT32, P_MAXPY32, P_IRFFT_MDOT, P_IRFFT_MDOT32, P_NKINDS
+    P_JAC32, P_RFFT32, P_TRI32, P_IRFFT32, P_MDOT32, P_MAXPY32, P_IRFFT_MDOT, P_IRFFT_MDOT32, P_JAC_RFFT, P_NKINDS
 };
 const char *kProfNames[P_NKINDS] = {"jacobian",   "residual", "rfft",      "tridiag", "irfft",
                                     "mdot",       "maxpy_norm", "jacobian32", "rfft32",  "tridiag32",
-                                    "irfft32",    "mdot32",   "maxpy_norm32", "irfft_mdot", "irfft_mdot32"};
+                                    "irfft32",    "mdot32",   "maxpy_norm32", "irfft_mdot", "irfft_mdot32",
+                                    "jac_rfft"};
 
 // device scalar regions (doubles): per-instance norms | Poisson residuals
 // (16 apart) | invariants (6 per instance)
@@ -412,10 +413,15 @@ int arnoldi_step(vv_ctx *c, int it)
         VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], w), "jacobian");
         if (int r = diss_jacobian(c, v, &c->gs->scale[it], w, done)) return r;
     } else if (const int rows = irfft_mdot_rows(c, false)) {
-        // P^-1 J v with the irFFT and the CGS dots of w in one launch
-        VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
-        if (int r = diss_jacobian(c, v, &c->gs->scale[it], c->X, done)) return r;
-        VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, c->X, c->spec, done), "rfft");
+        // P^-1 J v with the irFFT and the CGS dots of w in one launch; J v
+        // and the rFFT in one launch too where no dissipation term is added
+        if (jac_rfft_rows(c) && !diss_on(c)) {
+            VV_PLAUNCH(P_JAC_RFFT, F + S, it, launch_jac_rfft(c, v, &c->gs->scale[it], c->spec, done), "jac_rfft");
+        } else {
+            VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
+            if (int r = diss_jacobian(c, v, &c->gs->scale[it], c->X, done)) return r;
+            VV_PLAUNCH(P_RFFT, F + S, it, launch_rfft(c, c->X, c->spec, done), "rfft");
+        }
         VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
         VV_PLAUNCH(P_IRFFT_MDOT, S + F * (it + 2), it, launch_irfft_mdot(c, c->spec, it, done), "irfft_mdot");
         VV_PLAUNCH(P_MAXPY, F * (it + 3), it, launch_maxpy_norm(c, it, rows), "maxpy_norm");

@@ -20,6 +20,7 @@
 // and walks down its rows with the stencil values in a sliding register
 // window.
 
+#include "arakawa.h"
 #include "vv1d_internal.h"
 
 namespace vv {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     double href = 0.0;
     if (sizeof(T) == 4 && factors) {
         const int rc = HY / 2, cc = HX / 2;
-        href = (((sV[rc] + 0.5 * sX[0][cc]) + 0.5 * sX[1][cc]) + 0.5 * sX[2][cc]) + 0.5 * sX[3][cc];
+        href = compose_have(sV[rc], sX[0][cc], sX[1][cc], sX[2][cc], sX[3][cc]);
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -181,12 +182,9 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             } else if (a.hxv != nullptr) {
                 sH[r][c] = in ? (T)sX[0][c] : T(0);
             } else if (sizeof(T) == 8) {
-                sH[r][c] = in ? (T)((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
-                              : T(0);
+                sH[r][c] = in ? (T)compose_have(sV[r], sX[0][c], sX[1][c], sX[2][c], sX[3][c]) : T(0);
             } else {
-                sH[r][c] = in ? (T)(((((sV[r] + 0.5 * sX[0][c]) + 0.5 * sX[1][c]) + 0.5 * sX[2][c]) + 0.5 * sX[3][c])
-                                    - href)
-                              : T(0);
+                sH[r][c] = in ? (T)(compose_have(sV[r], sX[0][c], sX[1][c], sX[2][c], sX[3][c]) - href) : T(0);
             }
         }
     }
@@ -199,33 +197,11 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     const int cx = lane + 2;
     const T ht_inv = (T)a.ht_inv, af1 = (T)a.inst[b].af1, af2 = (T)a.inst[b].af2, factor = (T)a.factor;
 
-    // Each lane walks down its column: the stencil values of F and H live in
-    // a register window (column 0: rows -2..2, columns +-1: rows -1..1,
-    // columns +-2: row 0) that moves one row per step, so a row costs 5 + 5
-    // LDS reads instead of 25.  Same values, same expressions.
+    // Each lane walks down its column with the stencil values of F and H in
+    // a sliding register window (arakawa.h).  Same values, same expressions.
     const int rr0 = wave * ROWS_PER_WAVE;
-    T f0[5], fm[3], fp[3], fm2, fp2, g0[5], gm[3], gp[3], gm2, gp2;
-    {
-        const int cy = rr0 + 2;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            f0[k] = sF[cy - 2 + k][cx];
-            g0[k] = sH[cy - 2 + k][cx];
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            fm[k] = sF[cy - 1 + k][cx - 1];
-            fp[k] = sF[cy - 1 + k][cx + 1];
-            gm[k] = sH[cy - 1 + k][cx - 1];
-            gp[k] = sH[cy - 1 + k][cx + 1];
-        }
-        fm2 = sF[cy][cx - 2];
-        fp2 = sF[cy][cx + 2];
-        gm2 = sH[cy][cx - 2];
-        gp2 = sH[cy][cx + 2];
-    }
-#define F(di, dj) ((di) == 0 ? f0[(dj) + 2] : (di) == -1 ? fm[(dj) + 1] : (di) == 1 ? fp[(dj) + 1] : (di) == -2 ? fm2 : fp2)
-#define H(di, dj) ((di) == 0 ? g0[(dj) + 2] : (di) == -1 ? gm[(dj) + 1] : (di) == 1 ? gp[(dj) + 1] : (di) == -2 ? gm2 : gp2)
+    ArakawaWin<T> win;
+    win.load(&sF[0][0], &sH[0][0], HX, rr0 + 2, cx);
 #pragma unroll
     for (int q = 0; q < ROWS_PER_WAVE; ++q) {
         const int rr = rr0 + q;
@@ -237,90 +213,20 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
                 const T fder = fin[o] - fhin[o];                 // Fder = (0 + F) - Fh
                 base = T(0) + ht_inv * fder;                     // Y = 0; Y += Fder/dt
             } else if (MODE == MODE_JACOBIAN) {
-                base = T(0) + ht_inv * F(0, 0);                  // Y = 0; Y += u/dt
+                base = jacobian_base(ht_inv, win.f0[2]);         // Y = 0; Y += u/dt
             } else if (MODE == MODE_TRANSPOSED) {
                 base = (T)a.alpha * (fin[o] * us);
             } else {
                 base = a.store ? T(0) : yout[o];
             }
             T out = base;
-            if (MODE == MODE_TRANSPOSED || (j >= 2 && j < nv - 2)) {
-                if (KIND == VV_BRACKET_J1 || KIND == VV_BRACKET_J4) {
-                    // PoissonBracket.pyx:100-111 (J1 / second order)
-                    const T jpp = (F(1, 0) - F(-1, 0)) * (H(0, 1) - H(0, -1))
-                                - (F(0, 1) - F(0, -1)) * (H(1, 0) - H(-1, 0));
-                    const T jpc = F(1, 0) * (H(1, 1) - H(1, -1))
-                                - F(-1, 0) * (H(-1, 1) - H(-1, -1))
-                                - F(0, 1) * (H(1, 1) - H(-1, 1))
-                                + F(0, -1) * (H(1, -1) - H(-1, -1));
-                    const T jcp = F(1, 1) * (H(0, 1) - H(1, 0))
-                                - F(-1, -1) * (H(-1, 0) - H(0, -1))
-                                - F(-1, 1) * (H(0, 1) - H(-1, 0))
-                                + F(1, -1) * (H(1, 0) - H(0, -1));
-                    if (KIND == VV_BRACKET_J1) {
-                        out = base + factor * (jpp + jpc + jcp) * af1;
-                    } else {
-                        // PoissonBracket.pyx:373-389 (J4 = 2 J1 - J2)
-                        const T jcc2 = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
-                                     - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
-                        const T jpc2 = F(2, 0) * (H(1, 1) - H(1, -1))
-                                     - F(-2, 0) * (H(-1, 1) - H(-1, -1))
-                                     - F(0, 2) * (H(1, 1) - H(-1, 1))
-                                     + F(0, -2) * (H(1, -1) - H(-1, -1));
-                        const T jcp2 = F(1, 1) * (H(0, 2) - H(2, 0))
-                                     - F(-1, -1) * (H(-2, 0) - H(0, -2))
-                                     - F(-1, 1) * (H(0, 2) - H(-2, 0))
-                                     + F(1, -1) * (H(2, 0) - H(0, -2));
-                        const T r1 = (jpp + jpc + jcp) * af1;
-                        const T r2 = (jcc2 + jpc2 + jcp2) * af2;
-                        out = base + factor * (T(2) * r1 - r2);
-                    }
-                } else {
-                    // PoissonBracket.pyx:221-234 (J2)
-                    const T jcc = (F(1, 1) - F(-1, -1)) * (H(-1, 1) - H(1, -1))
-                                - (F(-1, 1) - F(1, -1)) * (H(1, 1) - H(-1, -1));
-                    const T jpc = F(2, 0) * (H(1, 1) - H(1, -1))
-                                - F(-2, 0) * (H(-1, 1) - H(-1, -1))
-                                - F(0, 2) * (H(1, 1) - H(-1, 1))
-                                + F(0, -2) * (H(1, -1) - H(-1, -1));
-                    const T jcp = F(1, 1) * (H(0, 2) - H(2, 0))
-                                - F(-1, -1) * (H(-2, 0) - H(0, -2))
-                                - F(-1, 1) * (H(0, 2) - H(-2, 0))
-                                + F(1, -1) * (H(2, 0) - H(0, -2));
-                    out = base + factor * (jcc + jpc + jcp) * af2;
-                }
-            }
+            if (MODE == MODE_TRANSPOSED || (j >= 2 && j < nv - 2))
+                out = arakawa_add<KIND>(win, base, factor, af1, af2);
             yout[o] = out;
         }
-        if (q + 1 < ROWS_PER_WAVE) {
-            // move the window one row down (rows stay inside the staged tile)
-            const int cn = rr + 3;    // LDS row of the next output row
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                f0[k] = f0[k + 1];
-                g0[k] = g0[k + 1];
-            }
-            f0[4] = sF[cn + 2][cx];
-            g0[4] = sH[cn + 2][cx];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                fm[k] = fm[k + 1];
-                fp[k] = fp[k + 1];
-                gm[k] = gm[k + 1];
-                gp[k] = gp[k + 1];
-            }
-            fm[2] = sF[cn + 1][cx - 1];
-            fp[2] = sF[cn + 1][cx + 1];
-            gm[2] = sH[cn + 1][cx - 1];
-            gp[2] = sH[cn + 1][cx + 1];
-            fm2 = sF[cn][cx - 2];
-            fp2 = sF[cn][cx + 2];
-            gm2 = sH[cn][cx - 2];
-            gp2 = sH[cn][cx + 2];
-        }
+        // move the window one row down (rows stay inside the staged tile)
+        if (q + 1 < ROWS_PER_WAVE) win.advance(&sF[0][0], &sH[0][0], HX, rr + 3, cx);
     }
-#undef F
-#undef H
 }
 
 template <int MODE, typename T>

@@ -27,6 +27,7 @@
 // float for the FP32 preconditioner inside the mixed-precision GMRES cycle
 // (complex64 spectrum, float twiddles; T_k's coefficients are formed in FP64
 // and rounded).
+#include "arakawa.h"
 #include "cgs_vec.h"
 
 namespace vv {
@@ -261,6 +262,28 @@ __device__ __forceinline__ int32_t cycle_done(const int32_t *done, int64_t gstri
                            : 0;
 }
 
+// The R real rows' spectra from their packed transforms in buf (complex row
+// p = real rows 2p + i 2p+1), stored mode-major: S[k][j0 + r], k <= nx/2.
+template <typename C>
+__device__ __forceinline__ void store_real_spectrum(const C *buf, int nx, int R, int j0, int nv, C *__restrict__ spec)
+{
+    using T = typename C::value_type;
+    const int nk = (nx >> 1) + 1;
+    const T half = T(0.5);
+    for (int idx = threadIdx.x; idx < nk * R; idx += NT) {
+        const int k = idx / R;
+        const int r = idx - k * R;
+        if (j0 + r >= nv) continue;
+        const C *z = buf + (r >> 1) * nx;
+        const C zk = z[k];
+        const C zm = z[(nx - k) & (nx - 1)];
+        C out;
+        if ((r & 1) == 0) out = cmk<C>((zk.x + zm.x) * half, (zk.y - zm.y) * half);
+        else out = cmk<C>((zk.y + zm.y) * half, (zm.x - zk.x) * half);
+        spec[(size_t)k * nv + j0 + r] = out;
+    }
+}
+
 // PRE (nx <= 4 NT): the thread's twiddles are loaded with its tile.
 template <typename T, bool PRE>
 __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
@@ -318,20 +341,179 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
     }
     __syncthreads();
     lds_fft<false, PRE>(buf, nx, a.logn, R >> 1, a.tw, pt, 1, LdsSink<C>{buf});
-    const int nk = (nx >> 1) + 1;
-    const T half = T(0.5);
-    for (int idx = threadIdx.x; idx < nk * R; idx += NT) {
-        const int k = idx / R;
-        const int r = idx - k * R;
-        if (j0 + r >= a.nv) continue;
-        const C *z = buf + (r >> 1) * nx;
-        const C zk = z[k];
-        const C zm = z[(nx - k) & (nx - 1)];
-        C out;
-        if ((r & 1) == 0) out = cmk<C>((zk.x + zm.x) * half, (zk.y - zm.y) * half);
-        else out = cmk<C>((zk.y + zm.y) * half, (zm.x - zk.x) * half);
-        spec[(size_t)k * a.nv + j0 + r] = out;
+    store_real_spectrum(buf, nx, R, j0, a.nv, spec);
+}
+
+// Jacobian-vector product fused with the forward rFFT of its rows:
+// X = J u (the bracket kernel's MODE_JACOBIAN, PETScNLVlasovMP.pyx:33-37:
+// Y = 0; Y += u/dt; Y += 0.5 {u, H_ave} on rows 2..nv-3) is formed in LDS row
+// block by row block and transformed there, so X never goes to HBM.
+// The workgroup's R rows are computed in strips of NT columns: the strip of
+// u (times the Krylov vector's scale) and of H_ave with a 2-cell halo,
+// (R+4) x (NT+4) cells each, is staged in LDS; thread t walks column t of
+// the strip down the R rows with the bracket kernel's stencil window
+// (arakawa.h), so every value is the bracket kernel's, bit for bit.  The next
+// strip's loads are in flight during the current strip's walk, and the
+// results stay in registers until the last walk is done, so the FFT buffer
+// reuses the strips' LDS.  nx = 4 NT: thread t's columns over the 4 strips,
+// t + m NT, are exactly the inputs of radix-4 butterfly t of the first FFT
+// stage, which then runs in registers as in rfft_kernel (the same
+// arithmetic, bit for bit).  nx = NT, 2 NT: the rows go to the FFT buffer and
+// the transform starts from LDS (its unit twiddles can change only the sign
+// of an exact zero).  R in 2, 4, 8.
+struct JacRows {
+    const double *uscale;                       // optional device scale of u (GMRES state)
+    const double *h0, *h1p, *h1h, *h2p, *h2h;   // 1D factors of H_ave
+    const InstConst *inst;
+    double ht_inv, factor;
+};
+
+template <int R, typename T>
+struct JacStrip {
+    T u[R + 4];          // own column, rows j0-2 .. j0+R+1 (0 outside the grid)
+    double x[4];         // own column's x-factors
+    T eu;                // halo cell (threads t < 4 (R+4))
+    double ex[4], eh0;
+};
+
+template <int R, typename T>
+__device__ __forceinline__ void jac_strip_load(const T *__restrict__ u, const JacRows &jr, size_t xo, int nx, int nv,
+                                               int j0, int cb, JacStrip<R, T> &g)
+{
+    constexpr int RH = R + 4;
+    const int t = threadIdx.x;
+    const int i = cb + t;
+#pragma unroll
+    for (int r = 0; r < RH; ++r) {
+        const int gj = j0 - 2 + r;
+        g.u[r] = (gj >= 0 && gj < nv) ? u[(size_t)gj * nx + i] : T(0);
     }
+    g.x[0] = jr.h1p[xo + i];
+    g.x[1] = jr.h1h[xo + i];
+    g.x[2] = jr.h2p[xo + i];
+    g.x[3] = jr.h2h[xo + i];
+    if (t < 4 * RH) {
+        const int hc = t / RH, hr = t - hc * RH;
+        const int tc = hc < 2 ? hc : NT + hc;          // tile columns 0, 1, NT+2, NT+3
+        const int gi = wrap_x(cb - 2 + tc, nx);
+        const int gj = j0 - 2 + hr;
+        const bool in = gj >= 0 && gj < nv;
+        g.eu = in ? u[(size_t)gj * nx + gi] : T(0);
+        g.eh0 = in ? jr.h0[gj] : 0.0;
+        g.ex[0] = jr.h1p[xo + gi];
+        g.ex[1] = jr.h1h[xo + gi];
+        g.ex[2] = jr.h2p[xo + gi];
+        g.ex[3] = jr.h2h[xo + gi];
+    }
+}
+
+template <int KIND, int R, int NS, typename T>
+__device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T *__restrict__ u,
+                                              cx_t<T> *__restrict__ spec)
+{
+    using C = cx_t<T>;
+    constexpr int RH = R + 4, SP = NT + 4, nx = NS * NT;
+    const int32_t dflag = cycle_done(a.done, a.gstride);
+    if (gridDim.y > 1 && dflag) return;   // batched: a finished instance skips its loads
+    const int b = blockIdx.y;
+    u += b * a.N;
+    spec += b * a.S;
+    const size_t xo = (size_t)b * nx;
+    // the FFT buffer reuses the strips' LDS: the strips' results wait in
+    // registers until the last walk is done
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *buf = reinterpret_cast<C *>(lds_raw);
+    T *sF = reinterpret_cast<T *>(lds_raw);
+    T *sH = sF + RH * SP;
+    const int nv = a.nv, t = threadIdx.x;
+    const int j0 = xcd_tile(blockIdx.x, gridDim.x) * R;
+    const double usd =
+        jr.uscale != nullptr ? *reinterpret_cast<const double *>(reinterpret_cast<const char *>(jr.uscale) + b * a.gstride)
+                             : 1.0;
+    double h0r[RH];
+#pragma unroll
+    for (int r = 0; r < RH; ++r) {
+        const int gj = j0 - 2 + r;
+        h0r[r] = (gj >= 0 && gj < nv) ? jr.h0[gj] : 0.0;
+    }
+    JacStrip<R, T> g;
+    jac_strip_load<R>(u, jr, xo, nx, nv, j0, 0, g);
+    if (dflag) return;                 // uniform over the block, before any barrier
+    const T us = (T)usd, ht_inv = (T)jr.ht_inv, factor = (T)jr.factor;
+    const T af1 = (T)jr.inst[b].af1, af2 = (T)jr.inst[b].af2;
+    T res[NS][R];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        // stage the strip: u times its scale (== the VecScale'd Krylov
+        // vector) and H_ave (0 outside the grid), as the bracket kernel does
+#pragma unroll
+        for (int r = 0; r < RH; ++r) {
+            const int gj = j0 - 2 + r;
+            sF[r * SP + t + 2] = g.u[r] * us;
+            sH[r * SP + t + 2] = (gj >= 0 && gj < nv) ? (T)compose_have(h0r[r], g.x[0], g.x[1], g.x[2], g.x[3]) : T(0);
+        }
+        if (t < 4 * RH) {
+            const int hc = t / RH, hr = t - hc * RH;
+            const int tc = hc < 2 ? hc : NT + hc;
+            const int gj = j0 - 2 + hr;
+            sF[hr * SP + tc] = g.eu * us;
+            sH[hr * SP + tc] = (gj >= 0 && gj < nv) ? (T)compose_have(g.eh0, g.ex[0], g.ex[1], g.ex[2], g.ex[3]) : T(0);
+        }
+        __syncthreads();
+        if (s + 1 < NS) jac_strip_load<R>(u, jr, xo, nx, nv, j0, (s + 1) * NT, g);
+        ArakawaWin<T> win;
+        win.load(sF, sH, SP, 2, t + 2);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int j = j0 + q;
+            T out = T(0);                                 // rows past nv: the rFFT's zero rows
+            if (j < nv) {
+                const T base = jacobian_base(ht_inv, win.f0[2]);
+                out = (j >= 2 && j < nv - 2) ? arakawa_add<KIND>(win, base, factor, af1, af2) : base;
+            }
+            res[s][q] = out;
+            if (q + 1 < R) win.advance(sF, sH, SP, q + 3, t + 2);
+        }
+        __syncthreads();
+    }
+    // twiddles in flight during the first stage (not held through the walks)
+    FftTw<false, C> pt;
+    pt.load(a.tw, nx, a.logn);
+    if (NS == 4) {
+        // stage 0 of the transform in registers, as rfft_kernel: butterfly
+        // (p, t) of complex row p (real rows 2p, 2p+1) takes columns t + m NT
+#pragma unroll
+        for (int p = 0; p < (R >> 1); ++p) {
+            const C v0 = cmk<C>(res[0][2 * p], res[0][2 * p + 1]), v1 = cmk<C>(res[1 % NS][2 * p], res[1 % NS][2 * p + 1]);
+            const C v2 = cmk<C>(res[2 % NS][2 * p], res[2 % NS][2 * p + 1]);
+            const C v3 = cmk<C>(res[3 % NS][2 * p], res[3 % NS][2 * p + 1]);
+            const C a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), d = csub(v1, v3);
+            const C a3 = cmk<C>(d.y, -d.x);             // (v1 - v3) * (-i)
+            C *z = buf + (p << a.logn) + (t << 2);      // stage 0 outputs: 4t + r
+            z[0] = cadd(a0, a2);
+            z[1] = cadd(a1, a3);
+            z[2] = csub(a0, a2);
+            z[3] = csub(a1, a3);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int p = 0; p < (R >> 1); ++p) buf[p * nx + s * NT + t] = cmk<C>(res[s][2 * p], res[s][2 * p + 1]);
+    }
+    __syncthreads();
+    lds_fft<false, true>(buf, nx, a.logn, R >> 1, a.tw, pt, NS == 4 ? 1 : 0, LdsSink<C>{buf});
+    store_real_spectrum(buf, nx, R, j0, nv, spec);
+}
+
+// LDS max(R/2 nx 16, 2 (R+4)(NT+4) 8) bytes (33 KB at R = 4), so three
+// workgroups per CU fit when the registers do (R <= 4; four cost register
+// spills and were slower)
+template <int KIND, int R, int NS, typename T>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 4 ? 3 : 2))) void jac_rfft_kernel(
+    FftArgs<T> a, JacRows jr, const T *__restrict__ u, cx_t<T> *__restrict__ spec)
+{
+    jac_rfft_body<KIND, R, NS>(a, jr, u, spec);
 }
 
 // Real rows 2b, 2b+1 of an irFFT workgroup from its complex transform b
@@ -1123,6 +1305,34 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
     return hipGetLastError();
 }
 
+template <int KIND, int R, int NS>
+void jac_rfft_launch(vv_ctx *c, const FftArgs<double> &a, const JacRows &jr, const double *u, double2 *spec)
+{
+    const size_t fft = (size_t)(R >> 1) * a.nx * sizeof(double2);
+    const size_t strip = 2 * (size_t)(R + 4) * (NT + 4) * sizeof(double);
+    const dim3 grid((a.nv + R - 1) / R, c->B);
+    launch_timed(c, jac_rfft_kernel<KIND, R, NS, double>, grid, dim3(NT), (uint32_t)(fft > strip ? fft : strip), a, jr,
+                 u, spec);
+}
+
+template <int KIND, int R>
+void jac_rfft_width(vv_ctx *c, const FftArgs<double> &a, const JacRows &jr, const double *u, double2 *spec)
+{
+    if (a.nx == 4 * NT) jac_rfft_launch<KIND, R, 4>(c, a, jr, u, spec);
+    else if (a.nx == 2 * NT) jac_rfft_launch<KIND, R, 2>(c, a, jr, u, spec);
+    else jac_rfft_launch<KIND, R, 1>(c, a, jr, u, spec);
+}
+
+template <int R>
+void jac_rfft_kind(vv_ctx *c, const FftArgs<double> &a, const JacRows &jr, const double *u, double2 *spec)
+{
+    switch (c->p.bracket) {
+    case VV_BRACKET_J1: jac_rfft_width<VV_BRACKET_J1, R>(c, a, jr, u, spec); break;
+    case VV_BRACKET_J2: jac_rfft_width<VV_BRACKET_J2, R>(c, a, jr, u, spec); break;
+    default: jac_rfft_width<VV_BRACKET_J4, R>(c, a, jr, u, spec); break;
+    }
+}
+
 template <typename T>
 hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, const int32_t *done)
 {
@@ -1261,6 +1471,29 @@ int fft_blocks(vv_ctx *c)
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done)
 {
     return rfft_t<double>(c, c->twiddle, x, spec, done);
+}
+
+int jac_rfft_rows(const vv_ctx *c)
+{
+    const int nx = c->fft_n;
+    if (!c->fuse || c->tb != nullptr || nx != c->p.nx || (nx & (nx - 1)) || nx < NT || nx > 4 * NT) return 0;
+    const int R = fft_rows(c);
+    return (R == 2 || R == 4 || R == 8) ? R : 0;
+}
+
+hipError_t launch_jac_rfft(vv_ctx *c, const double *u, const double *uscale, double2 *spec, const int32_t *done)
+{
+    const int R = jac_rfft_rows(c);
+    const FftArgs<double> a{c->fft_n, c->logn, c->fft_rows, R, c->N, spec_stride(c),
+                            (int64_t)sizeof(GmresState), c->twiddle, done};
+    const JacRows jr{uscale, c->h0, c->h1p, c->h1h, c->h2p, c->h2h, c->inst, c->ht_inv, 0.5};
+    switch (R) {
+    case 2: jac_rfft_kind<2>(c, a, jr, u, spec); break;
+    case 4: jac_rfft_kind<4>(c, a, jr, u, spec); break;
+    case 8: jac_rfft_kind<8>(c, a, jr, u, spec); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done)

@@ -200,6 +200,12 @@ hipError_t launch_rk4_end(vv_ctx *c, bool three_eighths, double *x, const double
 // spectral.hip
 int fft_blocks(vv_ctx *c);
 hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t *done);
+// J u (u times its optional device scale) and its rFFT in one launch, the
+// Jacobian never stored; jac_rfft_rows: the rows per workgroup where the
+// fused kernel applies to the context (the kinetic form with a power-of-two
+// nx in [256, 1024]), else 0.  No dissipation terms (the caller checks).
+int jac_rfft_rows(const vv_ctx *c);
+hipError_t launch_jac_rfft(vv_ctx *c, const double *u, const double *uscale, double2 *spec, const int32_t *done);
 hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done);
 hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done);
 hipError_t launch_tridiag_t(vv_ctx *c, double2 *spec, const int32_t *done);   // T_k^-H (transposed P)

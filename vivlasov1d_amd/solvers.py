@@ -169,7 +169,8 @@ class Context:
         return r.value, e.value
 
     def set_fusion(self, enable=True):
-        """Fused irFFT + Gram-Schmidt dots in the Arnoldi step (default on; bit-identical)."""
+        """Fused kernels in the Arnoldi step (default on): irFFT + Gram-Schmidt
+        dots, and J v + rFFT; results identical up to the sign of exact zeros."""
         self.call("vv_set_fusion", ctypes.c_int(1 if enable else 0))
 
     def autotune(self):
@@ -391,7 +392,7 @@ class Context:
 
     PROFILE_KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
                        "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
-                       "irfft_mdot", "irfft_mdot32")
+                       "irfft_mdot", "irfft_mdot32", "jac_rfft")
 
     def profile(self, kernels=None, every=1):
         """Time the named kernels (all when None; () switches timing off),
