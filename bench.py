@@ -40,7 +40,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
            "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32", "irfft_mdot", "irfft_mdot32",
-           "jac_rfft")
+           "jac_rfft", "jac_rfft32")
 
 
 def workloads():

@@ -48,7 +48,8 @@ SYMBOLS = {
     "residual": "bracket_kernel<4, 0, double>", "rfft": "::rfft_kernel<double", "irfft": "::irfft_kernel<double",
     "tridiag": "tridiag_lu_kernel<8, double>", "mdot32": "::mdot_kernel<", "maxpy_norm32": "::maxpy_norm_kernel<",
     "jacobian32": "bracket_kernel<4, 1, float>", "irfft_mdot": "::irfft_mdot_kernel<double",
-    "irfft_mdot32": "::irfft_mdot_kernel<float", "jac_rfft": "::jac_rfft_kernel<",
+    "irfft_mdot32": "::irfft_mdot_kernel<float", "jac_rfft": "::jac_rfft_kernel<4, 4, 4, double>",
+    "jac_rfft32": "::jac_rfft_kernel<4, 4, 4, float>",
 }
 # kernels that return at once when enqueued past a cycle's convergence
 EARLY = ("mdot_kernel", "maxpy_norm_kernel", "bracket_kernel<4, 1", "rfft_kernel", "tridiag_kernel",

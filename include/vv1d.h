@@ -452,14 +452,14 @@ int vv_invariants(vv_ctx *ctx, const double *f, const double *h0,
  * "irfft", "mdot", "maxpy_norm", the FP32 kernels of the mixed solver
  * "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
  * the fused irFFT + Gram-Schmidt dots "irfft_mdot", "irfft_mdot32", and
- * the Jacobian-vector product fused with the rFFT "jac_rfft".
+ * the Jacobian-vector product fused with the rFFT "jac_rfft", "jac_rfft32".
  * Enable, run, then read the summed time in ms, the launch count and the
  * summed algorithmic HBM bytes of those launches (Krylov launches that
  * returned at once after convergence are not counted).  Reading clears that
  * kernel's records.  enable is a bit mask over the kernels in the order
  * listed (jacobian = 1, residual = 2, rfft = 4, ..., maxpy_norm = 64,
  * jacobian32 = 128, ..., maxpy_norm32 = 4096, irfft_mdot = 8192,
- * irfft_mdot32 = 16384, jac_rfft = 32768; -1 all, 0 off). */
+ * irfft_mdot32 = 16384, jac_rfft = 32768, jac_rfft32 = 65536; -1 all, 0 off). */
 int vv_profile_enable(vv_ctx *ctx, int enable);
 /* Time one launch in every `every` (>= 1, default 1) of each enabled kernel
  * kind: a HIP-event-timed dispatch has idle time of its own around it, so a

@@ -3,8 +3,9 @@
 
 Where the FFT workgroup's rows are one CGS slice with the CGS kernels' own
 vectors per thread (256x512 and 1024x2048 in FP64; 1024x2048 in FP32 with
-cgs_ept = 4), the fused launches must reproduce jacobian + rfft and irfft +
-mdot exactly: same iterations, same converged reason, equal solutions (the
+cgs_ept = 4, where the FP32 J u's tile offsets of H follow the bracket
+kernel's 64 x 16 tiles), the fused launches must reproduce jacobian + rfft
+and irfft + mdot exactly: same iterations, same converged reason, equal solutions (the
 fused rFFT's first radix-4 stage multiplies by the unit twiddle, which can
 change only the sign of an exact zero), for single and batched contexts, for
 J1, J2 and J4 and 2, 4 and 8 rows per FFT workgroup.  At 512x1024 the fused
@@ -47,7 +48,7 @@ def _ctx(nx, nv, cuda, hx=None, restart=30, bracket='arakawaj4'):
 def _solve(ctx, b, fused, mixed=False, counts=None):
     import torch
     ctx.set_fusion(fused)
-    ctx.profile(("irfft_mdot", "irfft_mdot32", "mdot", "mdot32", "jac_rfft", "rfft"))
+    ctx.profile(("irfft_mdot", "irfft_mdot32", "mdot", "mdot32", "jac_rfft", "rfft", "jac_rfft32", "rfft32"))
     x = torch.zeros_like(b)
     # a random right-hand side: 120 iterations (four restart cycles) whether
     # or not they reach rtol, the comparison is of the two paths
@@ -59,8 +60,9 @@ def _solve(ctx, b, fused, mixed=False, counts=None):
     n_fused = ctx.profile_read("irfft_mdot32" if mixed else "irfft_mdot")[1]
     n_mdot = ctx.profile_read("mdot32" if mixed else "mdot")[1]
     if counts is not None:
-        counts["jac_rfft"] = ctx.profile_read("jac_rfft")[1]
-        counts["rfft"] = ctx.profile_read("rfft")[1]
+        sfx = "32" if mixed else ""
+        counts["jac_rfft"] = ctx.profile_read("jac_rfft" + sfx)[1]
+        counts["rfft"] = ctx.profile_read("rfft" + sfx)[1]
     ctx.profile(())
     return res, x.cpu().numpy(), n_fused, n_mdot
 
@@ -118,9 +120,11 @@ def test_fused_fp32_bitwise(cuda):
     print("case fused fp32 1024x2048", flush=True)
     ctx, b = _ctx(1024, 2048, cuda)
     ctx.set_tuning(0, 4)         # CGS slices of 4 float4 per thread: the fused slice
-    r1, x1, nf1, nm1 = _solve(ctx, b, True, mixed=True)
-    r0, x0, nf0, nm0 = _solve(ctx, b, False, mixed=True)
+    c1, c0 = {}, {}
+    r1, x1, nf1, nm1 = _solve(ctx, b, True, mixed=True, counts=c1)
+    r0, x0, nf0, nm0 = _solve(ctx, b, False, mixed=True, counts=c0)
     assert nf1 > 0 and nm1 == 0 and nf0 == 0 and nm0 > 0
+    assert c1["jac_rfft"] == nf1 and c0["jac_rfft"] == 0 and c0["rfft"] > c1["rfft"], (c1, c0)
     assert r1.reason in (2, -3) and _its(r1) == _its(r0)
     assert np.array_equal(x1, x0)
 

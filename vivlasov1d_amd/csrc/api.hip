@@ -20,12 +20,13 @@ thread_local std::string g_create_error;
 
 enum ProfKind {
     P_JAC = 0, P_RES, P_RFFT, P_TRI, P_IRFFT, P_MDOT, P_MAXPY,
-    P_JAC32, P_RFFT32, P_TRI32, P_IRFFT32, P_MDOT32, P_MAXPY32, P_IRFFT_MDOT, P_IRFFT_MDOT32, P_JAC_RFFT, P_NKINDS
+    P_JAC32, P_RFFT32, P_TRI32, P_IRFFT32, P_MDOT32, P_MAXPY32, P_IRFFT_MDOT, P_IRFFT_MDOT32, P_JAC_RFFT, P_JAC_RFFT32,
+    P_NKINDS
 };
 const char *kProfNames[P_NKINDS] = {"jacobian",   "residual", "rfft",      "tridiag", "irfft",
                                     "mdot",       "maxpy_norm", "jacobian32", "rfft32",  "tridiag32",
                                     "irfft32",    "mdot32",   "maxpy_norm32", "irfft_mdot", "irfft_mdot32",
-                                    "jac_rfft"};
+                                    "jac_rfft",   "jac_rfft32"};
 
 // device scalar regions (doubles): per-instance norms | Poisson residuals
 // (16 apart) | invariants (6 per instance)
@@ -485,7 +486,13 @@ int arnoldi_step32(vv_ctx *c, int it)
     float *X = reinterpret_cast<float *>(c->X);
     const double F = 4.0 * (double)c->N * c->B;
     float *ju = c->p.precond == VV_PRECOND_NONE ? w : X;
-    VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], ju), "jacobian32");
+    const int rows = c->p.precond != VV_PRECOND_NONE ? irfft_mdot_rows(c, true) : 0;
+    // J u and its rFFT in one launch where the fused kernels apply and no
+    // dissipation term is added
+    const bool fuse_ju = rows && jac_rfft_rows(c) && !diss_on(c);
+    if (!fuse_ju) {
+        VV_PLAUNCH(P_JAC32, 2 * F, it, launch_jacobian32(c, v, &c->gs->scale[it], ju), "jacobian32");
+    }
     if (diss_on(c)) {
         // the dissipation term in FP64 on the widened vector, folded into J u
         VV_CHECK(launch_widen(c, v, c->w64), "widen");
@@ -493,13 +500,16 @@ int arnoldi_step32(vv_ctx *c, int it)
         if (int r = diss_jacobian(c, c->w64, &c->gs->scale[it], c->d64, done)) return r;
         VV_CHECK(launch_narrow_add(c, ju, c->d64), "narrow add");
     }
-    const int rows = c->p.precond != VV_PRECOND_NONE ? irfft_mdot_rows(c, true) : 0;
     if (rows) {
         // P32^-1 with the irFFT and the CGS dots of w in one launch
         if (int r = pc_prepare32(c)) return r;
         float2 *spec = reinterpret_cast<float2 *>(c->spec);
         const double S = 0.5 * b_spec(c);
-        VV_PLAUNCH(P_RFFT32, F + S, it, launch_rfft32(c, X, spec, done), "rfft32");
+        if (fuse_ju) {
+            VV_PLAUNCH(P_JAC_RFFT32, F + S, it, launch_jac_rfft32(c, v, &c->gs->scale[it], spec, done), "jac_rfft32");
+        } else {
+            VV_PLAUNCH(P_RFFT32, F + S, it, launch_rfft32(c, X, spec, done), "rfft32");
+        }
         VV_PLAUNCH(P_TRI32, 2 * S, it, launch_tridiag32(c, spec, done), "tridiag32");
         VV_PLAUNCH(P_IRFFT_MDOT32, S + F * (it + 2), it, launch_irfft_mdot32(c, spec, it, done), "irfft_mdot32");
         VV_PLAUNCH(P_MAXPY32, F * (it + 3), it, launch_maxpy_norm32(c, it, rows), "maxpy_norm32");

@@ -9,11 +9,11 @@
 // pragmas), so both kernels produce the reference's unfused double results
 // bit for bit.
 //
-// The window of a lane at column cx, output row cy of an LDS tile (pitch P
-// elements per row) holds the first argument F and the second argument H at
-// column 0: rows cy-2..cy+2, columns +-1: rows cy-1..cy+1, columns +-2: row
-// cy; it moves one row down per output row, so a row costs 5 + 5 LDS reads
-// instead of 25.
+// The window of a lane at column cx, output row cy of a tile (read through
+// a functor of (row, column, column offset)) holds the first argument F and the second
+// argument H at column 0: rows cy-2..cy+2, columns +-1: rows cy-1..cy+1,
+// columns +-2: row cy; it moves one row down per output row, so a row costs
+// 5 + 5 LDS reads instead of 25.
 #pragma once
 
 #include "vv1d_internal.h"
@@ -21,41 +21,51 @@
 namespace vv {
 namespace {
 
+// Cell (r, cx + d) of an LDS tile of pitch P (the column offset d is a
+// compile-time constant at every call, so it folds into the LDS address).
+template <typename T, int P>
+struct TileRd {
+    const T *p;
+    __device__ __forceinline__ T operator()(int r, int cx, int d) const { return p[r * P + cx + d]; }
+};
+
 template <typename T>
 struct ArakawaWin {
     T f0[5], fm[3], fp[3], fm2, fp2;
     T g0[5], gm[3], gp[3], gm2, gp2;
 
-    __device__ __forceinline__ void load(const T *sF, const T *sH, int P, int cy, int cx)
+    template <typename RF, typename RH>
+    __device__ __forceinline__ void load(const RF &F, const RH &H, int cy, int cx)
     {
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            f0[k] = sF[(cy - 2 + k) * P + cx];
-            g0[k] = sH[(cy - 2 + k) * P + cx];
+            f0[k] = F(cy - 2 + k, cx, 0);
+            g0[k] = H(cy - 2 + k, cx, 0);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            fm[k] = sF[(cy - 1 + k) * P + cx - 1];
-            fp[k] = sF[(cy - 1 + k) * P + cx + 1];
-            gm[k] = sH[(cy - 1 + k) * P + cx - 1];
-            gp[k] = sH[(cy - 1 + k) * P + cx + 1];
+            fm[k] = F(cy - 1 + k, cx, -1);
+            fp[k] = F(cy - 1 + k, cx, 1);
+            gm[k] = H(cy - 1 + k, cx, -1);
+            gp[k] = H(cy - 1 + k, cx, 1);
         }
-        fm2 = sF[cy * P + cx - 2];
-        fp2 = sF[cy * P + cx + 2];
-        gm2 = sH[cy * P + cx - 2];
-        gp2 = sH[cy * P + cx + 2];
+        fm2 = F(cy, cx, -2);
+        fp2 = F(cy, cx, 2);
+        gm2 = H(cy, cx, -2);
+        gp2 = H(cy, cx, 2);
     }
 
     // move to output row cn (= the current row + 1)
-    __device__ __forceinline__ void advance(const T *sF, const T *sH, int P, int cn, int cx)
+    template <typename RF, typename RH>
+    __device__ __forceinline__ void advance(const RF &F, const RH &H, int cn, int cx)
     {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             f0[k] = f0[k + 1];
             g0[k] = g0[k + 1];
         }
-        f0[4] = sF[(cn + 2) * P + cx];
-        g0[4] = sH[(cn + 2) * P + cx];
+        f0[4] = F(cn + 2, cx, 0);
+        g0[4] = H(cn + 2, cx, 0);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             fm[k] = fm[k + 1];
@@ -63,14 +73,14 @@ struct ArakawaWin {
             gm[k] = gm[k + 1];
             gp[k] = gp[k + 1];
         }
-        fm[2] = sF[(cn + 1) * P + cx - 1];
-        fp[2] = sF[(cn + 1) * P + cx + 1];
-        gm[2] = sH[(cn + 1) * P + cx - 1];
-        gp[2] = sH[(cn + 1) * P + cx + 1];
-        fm2 = sF[cn * P + cx - 2];
-        fp2 = sF[cn * P + cx + 2];
-        gm2 = sH[cn * P + cx - 2];
-        gp2 = sH[cn * P + cx + 2];
+        fm[2] = F(cn + 1, cx, -1);
+        fp[2] = F(cn + 1, cx, 1);
+        gm[2] = H(cn + 1, cx, -1);
+        gp[2] = H(cn + 1, cx, 1);
+        fm2 = F(cn, cx, -2);
+        fp2 = F(cn, cx, 2);
+        gm2 = H(cn, cx, -2);
+        gp2 = H(cn, cx, 2);
     }
 };
 

@@ -147,14 +147,14 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
 
     const T us = (T)usd;
     // FP32: H_ave enters every Arakawa term only through differences, so the
-    // float tile holds H_ave - H_ave(tile centre), formed in FP64 and rounded
-    // once; the rounding error is then relative to H's variation over the
-    // tile, not to |H| (which would cost ~|H|/(hv dH/dv) ulps on fine v grids)
-    double href = 0.0;
-    if (sizeof(T) == 4 && factors) {
-        const int rc = HY / 2, cc = HX / 2;
-        href = compose_have(sV[rc], sX[0][cc], sX[1][cc], sX[2][cc], sX[3][cc]);
-    }
+    // float tile holds H_ave - h0(j0 + TY/2) (h0 at the centre row of the
+    // tile's 16-row band, 0 past the grid), formed in FP64 and rounded once;
+    // the rounding error is then relative to H's variation over the band
+    // (h0's over its rows plus the x-dependent part), not to |H| (which would
+    // cost ~|H|/(hv dH/dv) ulps on fine v grids).  The offset depends on the
+    // band only, so a kernel that stages other tiles of the same rows (the
+    // fused J u + rFFT) forms the same values.
+    const double href = (sizeof(T) == 4 && factors) ? sV[HY / 2] : 0.0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int idx = threadIdx.x + q * NT;
@@ -200,8 +200,9 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
     // Each lane walks down its column with the stencil values of F and H in
     // a sliding register window (arakawa.h).  Same values, same expressions.
     const int rr0 = wave * ROWS_PER_WAVE;
+    const TileRd<T, HX> rF{&sF[0][0]}, rH{&sH[0][0]};
     ArakawaWin<T> win;
-    win.load(&sF[0][0], &sH[0][0], HX, rr0 + 2, cx);
+    win.load(rF, rH, rr0 + 2, cx);
 #pragma unroll
     for (int q = 0; q < ROWS_PER_WAVE; ++q) {
         const int rr = rr0 + q;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(NT) void bracket_kernel(BrArgs<T> a)
             yout[o] = out;
         }
         // move the window one row down (rows stay inside the staged tile)
-        if (q + 1 < ROWS_PER_WAVE) win.advance(&sF[0][0], &sH[0][0], HX, rr + 3, cx);
+        if (q + 1 < ROWS_PER_WAVE) win.advance(rF, rH, rr + 3, cx);
     }
 }
 

@@ -407,6 +407,37 @@ __device__ __forceinline__ void jac_strip_load(const T *__restrict__ u, const Ja
     }
 }
 
+// H_ave from its factors in the tile's type: FP64 as is, FP32 relative to
+// href (formed in FP64, rounded once), as the bracket kernel stages it
+template <typename T>
+__device__ __forceinline__ T have_rel(double h0, const double (&x)[4], double href)
+{
+    if constexpr (sizeof(T) == 8) return compose_have(h0, x[0], x[1], x[2], x[3]);
+    else return (T)(compose_have(h0, x[0], x[1], x[2], x[3]) - href);
+}
+
+// Thread t's column of the staged strip down the R output rows (tile rows
+// 2 .. R+1): J u, or 0 on rows past nv (the rFFT's zero rows).
+template <int KIND, int R, typename T, typename RF, typename RH>
+__device__ __forceinline__ void jac_walk(const RF &rF, const RH &rH, int j0, int nv, T ht_inv, T factor, T af1,
+                                         T af2, T (&res)[R])
+{
+    const int cx = threadIdx.x + 2;
+    ArakawaWin<T> win;
+    win.load(rF, rH, 2, cx);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int j = j0 + q;
+        T out = T(0);
+        if (j < nv) {
+            const T base = jacobian_base(ht_inv, win.f0[2]);
+            out = (j >= 2 && j < nv - 2) ? arakawa_add<KIND>(win, base, factor, af1, af2) : base;
+        }
+        res[q] = out;
+        if (q + 1 < R) win.advance(rF, rH, q + 3, cx);
+    }
+}
+
 template <int KIND, int R, int NS, typename T>
 __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T *__restrict__ u,
                                               cx_t<T> *__restrict__ spec)
@@ -427,6 +458,11 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
     T *sH = sF + RH * SP;
     const int nv = a.nv, t = threadIdx.x;
     const int j0 = xcd_tile(blockIdx.x, gridDim.x) * R;
+    // FP32: H_ave - href, href = h0 at the centre row of the bracket kernel's
+    // 16-row band that holds the workgroup's rows (R divides 16), as the FP32
+    // bracket kernel stages it
+    const int jc = ((j0 >> 4) << 4) + 8;
+    const double href = (sizeof(T) == 4 && jc < nv) ? jr.h0[jc] : 0.0;
     const double usd =
         jr.uscale != nullptr ? *reinterpret_cast<const double *>(reinterpret_cast<const char *>(jr.uscale) + b * a.gstride)
                              : 1.0;
@@ -450,30 +486,18 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
         for (int r = 0; r < RH; ++r) {
             const int gj = j0 - 2 + r;
             sF[r * SP + t + 2] = g.u[r] * us;
-            sH[r * SP + t + 2] = (gj >= 0 && gj < nv) ? (T)compose_have(h0r[r], g.x[0], g.x[1], g.x[2], g.x[3]) : T(0);
+            sH[r * SP + t + 2] = (gj >= 0 && gj < nv) ? have_rel<T>(h0r[r], g.x, href) : T(0);
         }
         if (t < 4 * RH) {
             const int hc = t / RH, hr = t - hc * RH;
             const int tc = hc < 2 ? hc : NT + hc;
             const int gj = j0 - 2 + hr;
             sF[hr * SP + tc] = g.eu * us;
-            sH[hr * SP + tc] = (gj >= 0 && gj < nv) ? (T)compose_have(g.eh0, g.ex[0], g.ex[1], g.ex[2], g.ex[3]) : T(0);
+            sH[hr * SP + tc] = (gj >= 0 && gj < nv) ? have_rel<T>(g.eh0, g.ex, href) : T(0);
         }
         __syncthreads();
         if (s + 1 < NS) jac_strip_load<R>(u, jr, xo, nx, nv, j0, (s + 1) * NT, g);
-        ArakawaWin<T> win;
-        win.load(sF, sH, SP, 2, t + 2);
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int j = j0 + q;
-            T out = T(0);                                 // rows past nv: the rFFT's zero rows
-            if (j < nv) {
-                const T base = jacobian_base(ht_inv, win.f0[2]);
-                out = (j >= 2 && j < nv - 2) ? arakawa_add<KIND>(win, base, factor, af1, af2) : base;
-            }
-            res[s][q] = out;
-            if (q + 1 < R) win.advance(sF, sH, SP, q + 3, t + 2);
-        }
+        jac_walk<KIND, R>(TileRd<T, SP>{sF}, TileRd<T, SP>{sH}, j0, nv, ht_inv, factor, af1, af2, res[s]);
         __syncthreads();
     }
     // twiddles in flight during the first stage (not held through the walks)
@@ -1305,32 +1329,48 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
     return hipGetLastError();
 }
 
-template <int KIND, int R, int NS>
-void jac_rfft_launch(vv_ctx *c, const FftArgs<double> &a, const JacRows &jr, const double *u, double2 *spec)
+template <int KIND, int R, int NS, typename T>
+void jac_rfft_launch(vv_ctx *c, const FftArgs<T> &a, const JacRows &jr, const T *u, cx_t<T> *spec)
 {
-    const size_t fft = (size_t)(R >> 1) * a.nx * sizeof(double2);
-    const size_t strip = 2 * (size_t)(R + 4) * (NT + 4) * sizeof(double);
+    const size_t fft = (size_t)(R >> 1) * a.nx * sizeof(cx_t<T>);
+    const size_t strip = 2 * (size_t)(R + 4) * (NT + 4) * sizeof(T);
     const dim3 grid((a.nv + R - 1) / R, c->B);
-    launch_timed(c, jac_rfft_kernel<KIND, R, NS, double>, grid, dim3(NT), (uint32_t)(fft > strip ? fft : strip), a, jr,
-                 u, spec);
+    launch_timed(c, jac_rfft_kernel<KIND, R, NS, T>, grid, dim3(NT), (uint32_t)(fft > strip ? fft : strip), a, jr, u,
+                 spec);
 }
 
-template <int KIND, int R>
-void jac_rfft_width(vv_ctx *c, const FftArgs<double> &a, const JacRows &jr, const double *u, double2 *spec)
+template <int KIND, int R, typename T>
+void jac_rfft_width(vv_ctx *c, const FftArgs<T> &a, const JacRows &jr, const T *u, cx_t<T> *spec)
 {
     if (a.nx == 4 * NT) jac_rfft_launch<KIND, R, 4>(c, a, jr, u, spec);
     else if (a.nx == 2 * NT) jac_rfft_launch<KIND, R, 2>(c, a, jr, u, spec);
     else jac_rfft_launch<KIND, R, 1>(c, a, jr, u, spec);
 }
 
-template <int R>
-void jac_rfft_kind(vv_ctx *c, const FftArgs<double> &a, const JacRows &jr, const double *u, double2 *spec)
+template <int R, typename T>
+void jac_rfft_kind(vv_ctx *c, const FftArgs<T> &a, const JacRows &jr, const T *u, cx_t<T> *spec)
 {
     switch (c->p.bracket) {
     case VV_BRACKET_J1: jac_rfft_width<VV_BRACKET_J1, R>(c, a, jr, u, spec); break;
     case VV_BRACKET_J2: jac_rfft_width<VV_BRACKET_J2, R>(c, a, jr, u, spec); break;
     default: jac_rfft_width<VV_BRACKET_J4, R>(c, a, jr, u, spec); break;
     }
+}
+
+template <typename T>
+hipError_t jac_rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *u, const double *uscale, cx_t<T> *spec,
+                      const int32_t *done)
+{
+    const int R = jac_rfft_rows(c);
+    const FftArgs<T> a{c->fft_n, c->logn, c->fft_rows, R, c->N, spec_stride(c), (int64_t)sizeof(GmresState), tw, done};
+    const JacRows jr{uscale, c->h0, c->h1p, c->h1h, c->h2p, c->h2h, c->inst, c->ht_inv, 0.5};
+    switch (R) {
+    case 2: jac_rfft_kind<2>(c, a, jr, u, spec); break;
+    case 4: jac_rfft_kind<4>(c, a, jr, u, spec); break;
+    case 8: jac_rfft_kind<8>(c, a, jr, u, spec); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -1483,17 +1523,12 @@ int jac_rfft_rows(const vv_ctx *c)
 
 hipError_t launch_jac_rfft(vv_ctx *c, const double *u, const double *uscale, double2 *spec, const int32_t *done)
 {
-    const int R = jac_rfft_rows(c);
-    const FftArgs<double> a{c->fft_n, c->logn, c->fft_rows, R, c->N, spec_stride(c),
-                            (int64_t)sizeof(GmresState), c->twiddle, done};
-    const JacRows jr{uscale, c->h0, c->h1p, c->h1h, c->h2p, c->h2h, c->inst, c->ht_inv, 0.5};
-    switch (R) {
-    case 2: jac_rfft_kind<2>(c, a, jr, u, spec); break;
-    case 4: jac_rfft_kind<4>(c, a, jr, u, spec); break;
-    case 8: jac_rfft_kind<8>(c, a, jr, u, spec); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return jac_rfft_t<double>(c, c->twiddle, u, uscale, spec, done);
+}
+
+hipError_t launch_jac_rfft32(vv_ctx *c, const float *u, const double *uscale, float2 *spec, const int32_t *done)
+{
+    return jac_rfft_t<float>(c, c->twiddle32, u, uscale, spec, done);
 }
 
 hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done)

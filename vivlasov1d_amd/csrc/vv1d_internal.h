@@ -206,6 +206,7 @@ hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t 
 // nx in [256, 1024]), else 0.  No dissipation terms (the caller checks).
 int jac_rfft_rows(const vv_ctx *c);
 hipError_t launch_jac_rfft(vv_ctx *c, const double *u, const double *uscale, double2 *spec, const int32_t *done);
+hipError_t launch_jac_rfft32(vv_ctx *c, const float *u, const double *uscale, float2 *spec, const int32_t *done);
 hipError_t launch_tridiag_k(vv_ctx *c, double2 *spec, const int32_t *done);
 hipError_t launch_irfft(vv_ctx *c, const double2 *spec, double *y, const int32_t *done);
 hipError_t launch_tridiag_t(vv_ctx *c, double2 *spec, const int32_t *done);   // T_k^-H (transposed P)

@@ -392,7 +392,7 @@ class Context:
 
     PROFILE_KERNELS = ("jacobian", "residual", "rfft", "tridiag", "irfft", "mdot", "maxpy_norm",
                        "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
-                       "irfft_mdot", "irfft_mdot32", "jac_rfft")
+                       "irfft_mdot", "irfft_mdot32", "jac_rfft", "jac_rfft32")
 
     def profile(self, kernels=None, every=1):
         """Time the named kernels (all when None; () switches timing off),
