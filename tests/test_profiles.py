@@ -10,7 +10,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.mark.parametrize("name", ["maxpy_norm_twostream_1024x2048_J4.txt", "jacobian_twostream_1024x2048_J4.txt"])
+@pytest.mark.parametrize("name", ["irfft_mdot_twostream_1024x2048_J4.txt", "maxpy_norm_twostream_1024x2048_J4.txt",
+                                  "jac_rfft_twostream_1024x2048_J4.txt"])
 def test_profile_summary(name):
     from bench_profiles import read_summary
     k = read_summary(os.path.join(ROOT, "profiles", name))
@@ -28,7 +29,7 @@ def test_profile_summary(name):
 def test_counter_summary_covers_the_step_kernels():
     text = open(os.path.join(ROOT, "profiles", "counters_twostream_1024x2048_J4.txt")).read()
     assert "command: cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc" in text
-    for k in ("maxpy_norm_kernel", "irfft_mdot_kernel", "bracket_kernel<4, 1", "rfft_kernel",
+    for k in ("maxpy_norm_kernel", "irfft_mdot_kernel", "jac_rfft_kernel", "bracket_kernel<4, 0",
               "tridiag_lu_kernel"):
         assert k in text, k
 
