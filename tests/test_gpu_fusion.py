@@ -106,6 +106,22 @@ def test_fused_jac_rfft_kinds_and_rows(bracket, rows, cuda):
     assert np.array_equal(x1, x0)
 
 
+def test_fused_jac_rfft_ragged_rows(cuda):
+    # odd nv: the last FFT workgroup's second row is past the grid, and the
+    # fused irFFT + dots does not apply (rows per workgroup must divide nv),
+    # so J v + rFFT is fused alone, before the separate irFFT and dots
+    print("case fused jac_rfft 256x509", flush=True)
+    ctx, b = _ctx(256, 509, cuda)
+    c1, c0 = {}, {}
+    r1, x1, nf1, nm1 = _solve(ctx, b, True, counts=c1)
+    r0, x0, nf0, nm0 = _solve(ctx, b, False, counts=c0)
+    assert nf1 == 0 and nf0 == 0 and nm1 > 0 and nm1 == nm0, (nf1, nf0, nm1, nm0)
+    assert c1["jac_rfft"] == nm1 and c0["jac_rfft"] == 0, (c1, c0)
+    assert r1.reason in (2, -3) and _its(r1) == _its(r0)
+    assert r1.rnorm == r0.rnorm
+    assert np.array_equal(x1, x0)
+
+
 def test_fused_fp64_other_slice(cuda):
     print("case fused fp64 512x1024", flush=True)
     ctx, b = _ctx(512, 1024, cuda)

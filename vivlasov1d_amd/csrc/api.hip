@@ -427,6 +427,11 @@ int arnoldi_step(vv_ctx *c, int it)
         VV_PLAUNCH(P_IRFFT_MDOT, S + F * (it + 2), it, launch_irfft_mdot(c, c->spec, it, done), "irfft_mdot");
         VV_PLAUNCH(P_MAXPY, F * (it + 3), it, launch_maxpy_norm(c, it, rows), "maxpy_norm");
         return VV_OK;
+    } else if (jac_rfft_rows(c) && !diss_on(c)) {
+        // the kinetic P^-1 J v with J v fused into the rFFT; irFFT and dots apart
+        VV_PLAUNCH(P_JAC_RFFT, F + S, it, launch_jac_rfft(c, v, &c->gs->scale[it], c->spec, done), "jac_rfft");
+        VV_PLAUNCH(P_TRI, 2 * S, it, launch_tridiag_k(c, c->spec, done), "tridiag");
+        VV_PLAUNCH(P_IRFFT, S + F, it, launch_irfft(c, c->spec, w, done), "irfft");
     } else {
         VV_PLAUNCH(P_JAC, 2 * F, it, launch_jacobian(c, v, &c->gs->scale[it], c->X), "jacobian");
         if (int r = diss_jacobian(c, v, &c->gs->scale[it], c->X, done)) return r;

@@ -1513,12 +1513,20 @@ hipError_t launch_rfft(vv_ctx *c, const double *x, double2 *spec, const int32_t 
     return rfft_t<double>(c, c->twiddle, x, spec, done);
 }
 
+// Rows per workgroup of the fused J u + rFFT (its own choice: the spectrum's
+// layout does not depend on it): the largest of 8, 4, 2 that still leaves
+// >= 6 workgroups per CU (fewer halo rows per output row; batched contexts),
+// else the FFT kernels' rows when those are 2, 4 or 8, else 2.  Measured at
+// 512x1024 with 22 instances: 108 / 82 / 80 us at 2 / 4 / 8 rows; a single
+// 1024x2048 instance: 23.4 / 23.2 / 28.9 us.
 int jac_rfft_rows(const vv_ctx *c)
 {
     const int nx = c->fft_n;
     if (!c->fuse || c->tb != nullptr || nx != c->p.nx || (nx & (nx - 1)) || nx < NT || nx > 4 * NT) return 0;
+    for (int R = 8; R >= 2; R >>= 1)
+        if ((int64_t)((c->fft_rows + R - 1) / R) * c->B >= 6 * 256) return R;
     const int R = fft_rows(c);
-    return (R == 2 || R == 4 || R == 8) ? R : 0;
+    return (R == 2 || R == 4 || R == 8) ? R : 2;
 }
 
 hipError_t launch_jac_rfft(vv_ctx *c, const double *u, const double *uscale, double2 *spec, const int32_t *done)
