@@ -37,3 +37,17 @@ def test_bracket_tile_indices_in_bounds(tmp_path):
     print(out.stdout)
     assert out.returncode == 0, out.stdout
     assert "0 failures" in out.stdout
+
+
+def test_fused_jac_rfft_indices_in_bounds(tmp_path):
+    # the fused J u + rFFT kernel's row blocks, strips, halo cells, stencil
+    # window and FFT-buffer stores for nx = 256, 512, 1024, 2 / 4 / 8 rows per
+    # workgroup, nv in [8, 80] and the benchmark / reference sizes
+    exe = str(tmp_path / "tile_index_check")
+    subprocess.run([_hipcc(), "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", SRC, "-o", exe],
+                   check=True, cwd=str(tmp_path))
+    out = subprocess.run([exe, "jr", "8", "80", "128", "200", "201", "256", "509", "512", "1024", "2048"],
+                         capture_output=True, text=True)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout
+    assert "0 failures" in out.stdout

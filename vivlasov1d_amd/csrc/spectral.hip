@@ -382,22 +382,22 @@ __device__ __forceinline__ void jac_strip_load(const T *__restrict__ u, const Ja
 {
     constexpr int RH = R + 4;
     const int t = threadIdx.x;
-    const int i = cb + t;
+    const int i = cb + t;                              // staged column t + 2 (inside the row: nx >= cb + NT)
 #pragma unroll
     for (int r = 0; r < RH; ++r) {
-        const int gj = j0 - 2 + r;
-        g.u[r] = (gj >= 0 && gj < nv) ? u[(size_t)gj * nx + i] : T(0);
+        const int gj = jr_row(j0, r, nv);
+        g.u[r] = gj >= 0 ? u[(size_t)gj * nx + i] : T(0);
     }
     g.x[0] = jr.h1p[xo + i];
     g.x[1] = jr.h1h[xo + i];
     g.x[2] = jr.h2p[xo + i];
     g.x[3] = jr.h2h[xo + i];
     if (t < 4 * RH) {
-        const int hc = t / RH, hr = t - hc * RH;
-        const int tc = hc < 2 ? hc : NT + hc;          // tile columns 0, 1, NT+2, NT+3
-        const int gi = wrap_x(cb - 2 + tc, nx);
-        const int gj = j0 - 2 + hr;
-        const bool in = gj >= 0 && gj < nv;
+        int hr, tc;
+        jr_halo_cell(t, RH, &hr, &tc);
+        const int gi = jr_col(cb, tc, nx);
+        const int gj = jr_row(j0, hr, nv);
+        const bool in = gj >= 0;
         g.eu = in ? u[(size_t)gj * nx + gi] : T(0);
         g.eh0 = in ? jr.h0[gj] : 0.0;
         g.ex[0] = jr.h1p[xo + gi];
@@ -443,6 +443,7 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
                                               cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
+    static_assert(NT == kJrSW, "a strip is one column per thread");
     constexpr int RH = R + 4, SP = NT + 4, nx = NS * NT;
     const int32_t dflag = cycle_done(a.done, a.gstride);
     if (gridDim.y > 1 && dflag) return;   // batched: a finished instance skips its loads
@@ -469,8 +470,8 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
     double h0r[RH];
 #pragma unroll
     for (int r = 0; r < RH; ++r) {
-        const int gj = j0 - 2 + r;
-        h0r[r] = (gj >= 0 && gj < nv) ? jr.h0[gj] : 0.0;
+        const int gj = jr_row(j0, r, nv);
+        h0r[r] = gj >= 0 ? jr.h0[gj] : 0.0;
     }
     JacStrip<R, T> g;
     jac_strip_load<R>(u, jr, xo, nx, nv, j0, 0, g);
@@ -484,16 +485,14 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
         // vector) and H_ave (0 outside the grid), as the bracket kernel does
 #pragma unroll
         for (int r = 0; r < RH; ++r) {
-            const int gj = j0 - 2 + r;
             sF[r * SP + t + 2] = g.u[r] * us;
-            sH[r * SP + t + 2] = (gj >= 0 && gj < nv) ? have_rel<T>(h0r[r], g.x, href) : T(0);
+            sH[r * SP + t + 2] = jr_row(j0, r, nv) >= 0 ? have_rel<T>(h0r[r], g.x, href) : T(0);
         }
         if (t < 4 * RH) {
-            const int hc = t / RH, hr = t - hc * RH;
-            const int tc = hc < 2 ? hc : NT + hc;
-            const int gj = j0 - 2 + hr;
+            int hr, tc;
+            jr_halo_cell(t, RH, &hr, &tc);
             sF[hr * SP + tc] = g.eu * us;
-            sH[hr * SP + tc] = (gj >= 0 && gj < nv) ? have_rel<T>(g.eh0, g.ex, href) : T(0);
+            sH[hr * SP + tc] = jr_row(j0, hr, nv) >= 0 ? have_rel<T>(g.eh0, g.ex, href) : T(0);
         }
         __syncthreads();
         if (s + 1 < NS) jac_strip_load<R>(u, jr, xo, nx, nv, j0, (s + 1) * NT, g);

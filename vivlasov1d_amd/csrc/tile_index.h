@@ -1,7 +1,8 @@
 // Index arithmetic of the tiled kernels, shared by the device code and a
 // host-side checker (tests/native/tile_index_check.cpp) that enumerates every
-// tile, halo load and output of the bracket kernel for whole ranges of grid
-// sizes and asserts that each index stays inside its buffer.  A halo load that
+// tile, halo load and output of the bracket kernel and of the fused J u +
+// rFFT kernel for whole ranges of grid sizes and asserts that each index
+// stays inside its buffer.  A halo load that
 // reads past a buffer feeds no stored output, so no parity test can see it;
 // this checker can.
 #pragma once
@@ -83,5 +84,31 @@ __host__ __device__ __forceinline__ int br_halo_column(const BrTile &t, int c, i
 {
     return wrap_x(t.i0 - 2 + c, nx);
 }
+
+// Geometry of the fused J u + rFFT kernel: a workgroup's R output rows
+// j0 .. j0+R-1 (j0 = xcd_tile(block) R) are computed in strips of kJrSW
+// columns starting at cb; the strip with a 2-cell halo, (R+4) x (kJrSW+4)
+// cells, is staged; thread t owns staged column t + 2 (grid column cb + t),
+// threads t < 4 (R+4) also load one halo cell.
+constexpr int kJrSW = 256;
+
+// Grid row of staged row r (0 <= r < R+4), or -1 outside the grid (zero).
+__host__ __device__ __forceinline__ int jr_row(int j0, int r, int nv)
+{
+    const int gj = j0 - 2 + r;
+    return (gj >= 0 && gj < nv) ? gj : -1;
+}
+
+// Staged cell (row, column) that halo thread t (< 4 RH, RH = R+4) loads:
+// columns 0, 1, kJrSW+2, kJrSW+3, one row each.
+__host__ __device__ __forceinline__ void jr_halo_cell(int t, int RH, int *row, int *col)
+{
+    const int hc = t / RH;
+    *row = t - hc * RH;
+    *col = hc < 2 ? hc : kJrSW + hc;
+}
+
+// Grid column of staged column c of the strip starting at cb, in [0, nx).
+__host__ __device__ __forceinline__ int jr_col(int cb, int c, int nx) { return wrap_x(cb - 2 + c, nx); }
 
 }  // namespace vv
