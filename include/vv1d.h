@@ -373,6 +373,39 @@ int vv_ksp_solve(vv_ctx *ctx, const double *b, double *x, const vv_ksp_params *k
 int vv_snes_ksponly(vv_ctx *ctx, double *f, const double *fh,
                     const vv_ksp_params *kp, vv_ksp_result *res);
 
+/* One timestep of every instance of a batched context, the whole Picard
+ * loop in native code (no counterpart in the reference, which advances one
+ * instance per process; the rules are its run loop's,
+ * run_nonlinear_matrixfree_split_mp.py:111-174 and run_base.py:631-763,
+ * applied per instance): make_history (fh = f, h1h = h1, h2h = h2,
+ * phih = phi), update_previous, the initial residual ||F(f)|| + ||Poisson
+ * residual||; then, while any instance is active, with the others masked
+ * (vv_set_active): fl = f, vv_snes_ksponly, moments (density, phi, h1),
+ * update_previous, the residual; an instance stops when its residual rose
+ * (after the first iteration), fell below snes_atol or took snes_max_iter
+ * iterations, and a rise restores its f from fl (moments and H again).
+ * The caller's Python or C driver thus holds no lock between the launches.
+ * history (optional, history_rows x batch doubles): the residual of every
+ * iteration per instance, NaN where the instance was not active. */
+typedef struct {
+    double *f, *fh, *fl, *fy;    /* fields (batch, nv, nx): current, history, last iterate, work */
+    const double *h0;            /* nv */
+    double *h1, *h1h, *h2, *h2h; /* (batch, nx): factors of H, current and history */
+    double *n, *phi, *phih;      /* (batch, nx): density, potential, its history */
+} vv_batch_fields;
+
+typedef struct {
+    int32_t picard;    /* Picard iterations taken          */
+    int32_t reverted;  /* the last iterate was undone      */
+    int64_t gmres;     /* GMRES iterations over them       */
+    double  initial;   /* residual before the loop         */
+    double  final;     /* residual at its end              */
+} vv_instance_step;
+
+int vv_batched_timestep(vv_ctx *ctx, const vv_batch_fields *s, const vv_ksp_params *kp,
+                        double snes_atol, int32_t snes_max_iter, vv_instance_step *rec,
+                        double *history, int32_t history_rows, int32_t *iterations);
+
 /* Mixed precision (no counterpart in the reference, whose solve is FP64
  * throughout; it replaces the same KSPSolve / SNESSolve_KSPONLY call sites):
  * iterative refinement on (P^-1 J) x = b.  Each correction equation is solved

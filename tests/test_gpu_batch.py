@@ -9,7 +9,8 @@ instance's grid (which the parity tests pin to the oracle).  Bars:
   * graph replay of the SNES step: bit-identical to the eager path on every
     replay, with eager work and synchronisations between the replays;
   * the bucketed multi-instance driver: per-instance states after two
-    timesteps bit-identical to running each instance alone;
+    timesteps bit-identical to running each instance alone; its native
+    timestep (the Picard loop in C++) identical to the same loop in Python;
   * autotuned launch shapes: a valid choice, results within 1e-12 (the
     Gram-Schmidt summation tree changes with the shape).
 """
@@ -185,6 +186,30 @@ def test_bucketed_driver_matches_single_runs(cuda):
         fs, phis = states[s['index']]
         print(s['index'], s['nx'], s['nv'], np.abs(f - fs).max(), flush=True)
         assert np.array_equal(f, fs) and np.array_equal(phi, phis)
+
+
+def test_native_step_matches_python_loop(cuda):
+    # vv_batched_timestep runs the bucket's Picard loop in native code; the
+    # same loop in Python (native=False) must give the same states and
+    # records, reverts and per-iteration residual history included
+    from vivlasov1d_amd.batch import Bucket, make_instances
+    specs = make_instances(64, seed=0)
+    pick = [s for s in specs if (s['nx'], s['nv']) == (64, 128)][:6]
+    assert len(pick) >= 3
+    a = Bucket(pick, graph=True, autotune=False, native=True)
+    b = Bucket(pick, graph=True, autotune=False, native=False)
+    for _ in range(3):
+        ra, rb = a.step(), b.step()
+        print("picard", ra["picard"], rb["picard"], "reverted", ra["reverted"], flush=True)
+        for k in ("picard", "gmres", "reverted", "initial", "final"):
+            assert np.array_equal(ra[k], rb[k]), k
+        assert ra["residuals"].shape == rb["residuals"].shape
+        assert np.array_equal(np.isnan(ra["residuals"]), np.isnan(rb["residuals"]))
+        assert np.array_equal(np.nan_to_num(ra["residuals"]), np.nan_to_num(rb["residuals"]))
+    a.stream.synchronize()
+    b.stream.synchronize()
+    for x, y in ((a.fc, b.fc), (a.phi, b.phi), (a.h1c, b.h1c), (a.fh, b.fh)):
+        assert np.array_equal(x.cpu().numpy(), y.cpu().numpy())
 
 
 def test_autotune(cuda):

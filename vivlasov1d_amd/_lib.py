@@ -50,6 +50,16 @@ class KspResult(ctypes.Structure):
                 ("rnorm0", ctypes.c_double)]
 
 
+class BatchFields(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("f", "fh", "fl", "fy", "h0", "h1", "h1h", "h2", "h2h", "n", "phi",
+                                               "phih")]
+
+
+class InstanceStep(ctypes.Structure):
+    _fields_ = [("picard", ctypes.c_int32), ("reverted", ctypes.c_int32), ("gmres", ctypes.c_int64),
+                ("initial", ctypes.c_double), ("final", ctypes.c_double)]
+
+
 class MixedParams(ctypes.Structure):
     _fields_ = [("inner_rtol", ctypes.c_double), ("max_cycles", ctypes.c_int32), ("fp64_check", ctypes.c_int32)]
 
@@ -113,6 +123,9 @@ SIGNATURES = {
     "vv_axpy": (ctypes.c_int, [_P, _D, _P, _P, _I64]),
     "vv_ksp_solve": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(KspResult)]),
     "vv_snes_ksponly": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(KspResult)]),
+    "vv_batched_timestep": (ctypes.c_int, [_P, ctypes.POINTER(BatchFields), ctypes.POINTER(KspParams), ctypes.c_double,
+                                           ctypes.c_int32, ctypes.POINTER(InstanceStep), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "vv_ksp_solve_mixed": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(MixedParams),
                                           ctypes.POINTER(KspResult), ctypes.POINTER(ctypes.c_int32)]),
     "vv_snes_ksponly_mixed": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(KspParams), ctypes.POINTER(MixedParams),

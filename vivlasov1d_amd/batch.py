@@ -61,7 +61,7 @@ class Bucket:
 
     def __init__(self, specs, ksp_rtol=1e-10, ksp_atol=1e-50, ksp_max_it=1000, snes_atol=1e-10,
                  snes_max_iter=20, restart=30, charge=-1.0, mass=1.0, laplace='cfd4', graph=True,
-                 autotune=True):
+                 autotune=True, native=True):
         s0 = specs[0]
         self.specs = specs
         self.B = len(specs)
@@ -87,6 +87,7 @@ class Bucket:
             ctx.moments(self.fc, self.n, self.phi, self.h1c, want_norm=False)
         if graph:
             ctx.set_graph(True)
+        self.native = native
         self.itime = 0
         self.history = []
 
@@ -100,7 +101,12 @@ class Bucket:
 
     def step(self):
         """One timestep of every instance (the reference's run() body, per
-        instance, with masks)."""
+        instance, with masks): in native code (vv_batched_timestep, no
+        Python between the launches, so the buckets' host threads do not
+        contend for the interpreter), or with ``native=False`` the same loop
+        here."""
+        if self.native:
+            return self._step_native()
         ctx = self.ctx
         with torch.cuda.stream(self.stream):
             self.itime += 1
@@ -146,6 +152,21 @@ class Bucket:
                    residuals=np.array(history).reshape(-1, self.B))
         self.history.append(rec)
         bad = np.nonzero(pred > 10.)[0]
+        if bad.size:
+            raise ResidualTooLarge("ERROR: Residual of nonlinear solver too large (instances %s)."
+                                   % [self.specs[b]['index'] for b in bad])
+        return rec
+
+    def _step_native(self):
+        fields = dict(f=self.fc, fh=self.fh, fl=self.fl, fy=self.fy, h0=self.h0, h1=self.h1c, h1h=self.h1h,
+                      h2=self.h2c, h2h=self.h2h, n=self.n, phi=self.phi, phih=self.phih)
+        with torch.cuda.stream(self.stream):
+            self.itime += 1
+            out, history, _ = self.ctx.batched_timestep(fields, *self.ksp, self.snes_atol, self.snes_max_iter)
+        rec = dict(step=self.itime, initial=out["initial"], final=out["final"], picard=out["picard"].astype(np.int32),
+                   gmres=out["gmres"].astype(np.int64), reverted=out["reverted"].astype(bool), residuals=history)
+        self.history.append(rec)
+        bad = np.nonzero(rec["final"] > 10.)[0]
         if bad.size:
             raise ResidualTooLarge("ERROR: Residual of nonlinear solver too large (instances %s)."
                                    % [self.specs[b]['index'] for b in bad])

@@ -1369,6 +1369,86 @@ int vv_function_norm(vv_ctx *c, const double *f, const double *fh, double *y, do
     return read_scalars(c, ds_norms(c), (size_t)c->B, out);
 }
 
+int vv_batched_timestep(vv_ctx *c, const vv_batch_fields *s, const vv_ksp_params *kp, double snes_atol,
+                        int32_t snes_max_iter, vv_instance_step *rec, double *history, int32_t history_rows,
+                        int32_t *iterations)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!s || !kp || !rec || !s->f || !s->fh || !s->fl || !s->fy || !s->h0 || !s->h1 || !s->h1h || !s->h2 ||
+        !s->h2h || !s->n || !s->phi || !s->phih)
+        return fail(c, VV_ERR_INVALID, "null argument");
+    if (snes_max_iter < 1 || history_rows < 0 || (history_rows > 0 && !history))
+        return fail(c, VV_ERR_INVALID, "batched_timestep: bad iteration limit or history");
+    const int B = c->B;
+    const int64_t N = c->N, NB = c->N * c->B, XB = (int64_t)c->p.nx * c->B;
+    std::vector<double> fo(B), po(B), pred(B);
+    std::vector<int32_t> active(B, 1);
+    std::vector<vv_ksp_result> res(B);
+    auto residual = [&](std::vector<double> &out) -> int {
+        if (int r = vv_residual_norms(c, s->f, s->fh, s->fy, fo.data(), po.data())) return r;
+        for (int b = 0; b < B; ++b) out[b] = fo[b] + po[b];
+        return VV_OK;
+    };
+    // make_history, update_previous and the residual before the loop
+    VV_CHECK(launch_copy(c, s->f, s->fh, NB), "history");
+    VV_CHECK(launch_copy(c, s->h1, s->h1h, XB), "history");
+    VV_CHECK(launch_copy(c, s->h2, s->h2h, XB), "history");
+    VV_CHECK(launch_copy(c, s->phi, s->phih, XB), "history");
+    if (int r = vv_set_hamiltonian(c, s->h0, s->h1, s->h1h, s->h2, s->h2h)) return r;
+    if (int r = residual(pred)) return r;
+    for (int b = 0; b < B; ++b) rec[b] = vv_instance_step{0, 0, 0, pred[b], pred[b]};
+    std::vector<double> now(B);
+    int i = 0, rc = VV_OK;
+    for (;;) {
+        bool any = false;
+        for (int b = 0; b < B; ++b) any = any || active[b];
+        if (!any) break;
+        ++i;
+        if ((rc = vv_set_active(c, active.data()))) break;
+        if (const hipError_t e = launch_copy(c, s->f, s->fl, NB)) {
+            rc = hipfail(c, e, "backup");
+            break;
+        }
+        if ((rc = vv_snes_ksponly(c, s->f, s->fh, kp, res.data()))) break;
+        if ((rc = vv_moments(c, s->f, s->n, s->phi, s->h1, nullptr))) break;
+        if ((rc = vv_set_hamiltonian(c, s->h0, s->h1, s->h1h, s->h2, s->h2h))) break;
+        if ((rc = residual(now))) break;
+        if (i <= history_rows)
+            for (int b = 0; b < B; ++b) history[(size_t)(i - 1) * B + b] = active[b] ? now[b] : std::nan("");
+        bool revert = false;
+        for (int b = 0; b < B; ++b) {
+            if (!active[b]) continue;
+            rec[b].picard += 1;
+            rec[b].gmres += res[b].its;
+            const double prev = pred[b];
+            pred[b] = now[b];
+            if ((pred[b] > prev && i > 1) || pred[b] < snes_atol || i >= snes_max_iter) {
+                active[b] = 0;
+                if (pred[b] > prev) {
+                    // the iterate of this instance is undone
+                    if (const hipError_t e = launch_copy(c, s->fl + (size_t)b * N, s->f + (size_t)b * N, N)) {
+                        rc = hipfail(c, e, "revert");
+                        break;
+                    }
+                    rec[b].reverted = 1;
+                    revert = true;
+                }
+            }
+        }
+        if (rc) break;
+        if (revert) {
+            if ((rc = vv_moments(c, s->f, s->n, s->phi, s->h1, nullptr))) break;
+            if ((rc = vv_set_hamiltonian(c, s->h0, s->h1, s->h1h, s->h2, s->h2h))) break;
+        }
+    }
+    const int r2 = vv_set_active(c, nullptr);
+    if (rc) return rc;
+    if (r2) return r2;
+    for (int b = 0; b < B; ++b) rec[b].final = pred[b];
+    if (iterations) *iterations = i;
+    return VV_OK;
+}
+
 int vv_residual_norms(vv_ctx *c, const double *f, const double *fh, double *y, double *fnorm, double *pnorm)
 {
     if (int r = check_ctx(c)) return r;

@@ -326,6 +326,30 @@ class Context:
         self.call("vv_snes_ksponly", self.F(f), self.F(fh), ctypes.byref(kp), res)
         return list(res) if self.batched else res[0]
 
+    def batched_timestep(self, fields, rtol, atol, dtol, max_it, snes_atol, snes_max_iter):
+        """vv_batched_timestep: one timestep of every instance, the Picard loop
+        in native code.  fields: dict of the device tensors f, fh, fl, fy
+        (fields), h0 (v-vector), h1, h1h, h2, h2h, n, phi, phih (x-vectors).
+        Returns (per-instance records as a dict of numpy arrays, residual
+        history rows x B with NaN where inactive, iterations)."""
+        bf = _lib.BatchFields()
+        for k in ("f", "fh", "fl", "fy"):
+            setattr(bf, k, self.F(fields[k], k).value)
+        bf.h0 = self.V(fields["h0"], "h0").value
+        for k in ("h1", "h1h", "h2", "h2h", "n", "phi", "phih"):
+            setattr(bf, k, self.X(fields[k], k).value)
+        kp = _lib.KspParams(rtol, atol, dtol, int(max_it))
+        rec = (_lib.InstanceStep * self.B)()
+        rows = int(snes_max_iter)
+        hist = (ctypes.c_double * (rows * self.B))()
+        its = ctypes.c_int32()
+        self.call("vv_batched_timestep", ctypes.byref(bf), ctypes.byref(kp), ctypes.c_double(snes_atol),
+                  ctypes.c_int32(rows), rec, hist, ctypes.c_int32(rows), ctypes.byref(its))
+        n = its.value
+        out = {k: np.array([getattr(r, k) for r in rec]) for k in ("picard", "reverted", "gmres", "initial", "final")}
+        history = np.array(hist[:n * self.B], dtype=np.float64).reshape(n, self.B)
+        return out, history, n
+
     # mixed precision ----------------------------------------------------------
     def F32(self, t, name="field"):
         return _dev_ptr(t, self.N * self.B, name, torch.float32)
