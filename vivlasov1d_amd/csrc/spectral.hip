@@ -107,12 +107,32 @@ constexpr int MAXS = 6;     // radix-4 stages of the longest transform (MAXE = 4
 // in the radix-2 stage of odd logn, where n <= 2 NT), so 3 twiddles per stage
 // cover them; their loads are in flight with the tile's own loads instead of
 // being waited on after every barrier of the transform.
+// n = 1024 runs the plan 4 x 8 x 8 x 4 (see lds_fft1024): its 17 twiddles
+// take the same registers, flat index i (always a compile-time index) at
+// w[i / 3][i % 3] for i < 15, then w2 and w8 (0..6 and 7..13 for the radix-8
+// stages with sub-transforms of 4 and 32, 14..16 for the last radix-4 stage;
+// the last row of w, used only by n = 4096, stays out of it).
+constexpr int N8 = 1024;
 template <bool INV, typename C>
 struct FftTw {
     C w[MAXS][3];
-    C w2;
+    C w2, w8;
+    // P8: the n = 1024 plan's twiddles (n must be 1024), else the radix-4 ones
+    template <bool P8 = false>
     __device__ __forceinline__ void load(const C *__restrict__ tw, int n, int logn)
     {
+        if (P8) {
+            const int t = threadIdx.x, ka = t & 3, kb = t & 31, kc = t & 255;
+#pragma unroll
+            for (int r = 1; r < 8; ++r) {
+                w[(r - 1) / 3][(r - 1) % 3] = twiddle<INV>(tw, r * ka * 32);
+                w[(6 + r) / 3][(6 + r) % 3] = twiddle<INV>(tw, r * kb * 4);
+            }
+            w[4][2] = twiddle<INV>(tw, kc);
+            w2 = twiddle<INV>(tw, 2 * kc);
+            w8 = twiddle<INV>(tw, 3 * kc);
+            return;
+        }
         const int ns = logn >> 1;
         const int j = threadIdx.x & ((n >> 2) - 1);
 #pragma unroll
@@ -140,20 +160,164 @@ struct LdsSink {
     __device__ __forceinline__ void operator()(int pos, C v) const { buf[pos] = v; }
 };
 
+// The radix-4 butterfly of the Stockham stages (outputs in natural order of
+// the 4-point DFT; forward (v1 - v3)(-i), inverse (v1 - v3)(+i)).
+template <bool INV, typename C>
+__device__ __forceinline__ void dft4(C &v0, C &v1, C &v2, C &v3)
+{
+    const C a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), d = csub(v1, v3);
+    const C a3 = INV ? cmk<C>(-d.y, d.x) : cmk<C>(d.y, -d.x);
+    v0 = cadd(a0, a2);
+    v1 = cadd(a1, a3);
+    v2 = csub(a0, a2);
+    v3 = csub(a1, a3);
+}
+
+// 8-point DFT in place: the radix-4 DFTs of the even and the odd samples,
+// combined with W8^m (forward W8 = (1 - i)/sqrt 2, inverse its conjugate).
+template <bool INV, typename C>
+__device__ __forceinline__ void dft8(C (&v)[8])
+{
+    using R = typename C::value_type;
+    const R h = R(0.70710678118654752440);
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    o1 = INV ? cmk<C>((o1.x - o1.y) * h, (o1.x + o1.y) * h) : cmk<C>((o1.x + o1.y) * h, (o1.y - o1.x) * h);
+    o2 = INV ? cmk<C>(-o2.y, o2.x) : cmk<C>(o2.y, -o2.x);
+    o3 = INV ? cmk<C>(-(o3.x + o3.y) * h, (o3.x - o3.y) * h) : cmk<C>((o3.y - o3.x) * h, -(o3.x + o3.y) * h);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
+// One radix-8 Stockham stage of the n = 1024 plan: sub-transforms of length
+// NS (4 or 32) combined into 8 NS; butterfly j takes elements j + 128 r,
+// twiddled by W_1024^(r k 1024 / (8 NS)), k = j mod NS (= threadIdx.x mod NS
+// for every butterfly of the thread, so the thread's twiddles are preloaded:
+// flat index OFF + r - 1 of FftTw), and stores output m at 8 (j - k) + k + m NS.
+template <bool INV, int NS, int OFF, typename C>
+__device__ __forceinline__ void stage8_1024(C *buf, int nb, const FftTw<INV, C> &pt)
+{
+    constexpr int nq = N8 / 8;
+    const int total = nb * nq;
+    C v[2][8];
+    int dst[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int bq = threadIdx.x + q * NT;
+        dst[q] = -1;
+        if (bq < total) {
+            const int b = bq >> 7, j = bq & (nq - 1), k = j & (NS - 1);
+            const C *src = buf + b * N8;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[q][r] = src[j + r * nq];
+#pragma unroll
+            for (int r = 1; r < 8; ++r) v[q][r] = cmul(v[q][r], pt.w[(OFF + r - 1) / 3][(OFF + r - 1) % 3]);
+            dft8<INV>(v[q]);
+            dst[q] = b * N8 + ((j - k) << 3) + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) buf[dst[q] + m * NS] = v[q][m];
+        }
+    __syncthreads();
+}
+
+// n = 1024 as 4 x 8 x 8 x 4: the first radix-4 stage (from LDS when s0 == 0;
+// s0 == 1: the caller ran it in registers), two radix-8 stages and a last
+// radix-4 stage whose outputs go to `last` -- four passes over LDS instead of
+// the five of the radix-4 plan.  nb <= 4 transforms (MAXE elements).
+template <bool INV, typename C, typename Sink>
+__device__ __forceinline__ void lds_fft1024(C *buf, int nb, const FftTw<INV, C> &pt, int s0, const Sink &last)
+{
+    constexpr int nq = N8 / 4;
+    const int total = nb * nq;
+    if (s0 == 0) {
+        C v[4][4];
+        int dst[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int bq = threadIdx.x + q * NT;
+            dst[q] = -1;
+            if (bq < total) {
+                const int b = bq >> 8, j = bq & (nq - 1);
+                const C *src = buf + b * N8;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[q][r] = src[j + r * nq];
+                dft4<INV>(v[q][0], v[q][1], v[q][2], v[q][3]);
+                dst[q] = b * N8 + (j << 2);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (dst[q] >= 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) buf[dst[q] + r] = v[q][r];
+            }
+        __syncthreads();
+    }
+    stage8_1024<INV, 4, 0>(buf, nb, pt);
+    stage8_1024<INV, 32, 7>(buf, nb, pt);
+    // last stage: sub-transforms of 256, k = j = threadIdx.x, outputs j + 256 m
+    C v[4][4];
+    int dst[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int bq = threadIdx.x + q * NT;
+        dst[q] = -1;
+        if (bq < total) {
+            const int b = bq >> 8, j = bq & (nq - 1);
+            const C *src = buf + b * N8;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[q][r] = src[j + r * nq];
+            v[q][1] = cmul(v[q][1], pt.w[4][2]);
+            v[q][2] = cmul(v[q][2], pt.w2);
+            v[q][3] = cmul(v[q][3], pt.w8);
+            dft4<INV>(v[q][0], v[q][1], v[q][2], v[q][3]);
+            dst[q] = b * N8 + j;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) last(dst[q] + m * nq, v[q][m]);
+        }
+    __syncthreads();
+}
+
 // Batched in-LDS Stockham FFT of `nb` transforms of length n = 2^logn stored
 // back to back in buf.  Forward uses exp(-2 pi i m/n), inverse the conjugate
 // (unnormalised).  Each stage loads every butterfly of the thread into
 // registers, synchronises, then stores, so one buffer suffices.  C is
 // double2 (FP64 path) or float2 (FP32 preconditioner of the mixed solver).
-// PRE: twiddles from pt (n <= 4 NT), else from the global table tw.
+// PRE: twiddles from pt (n <= 4 NT), else from the global table tw.  P8:
+// n = 1024 by the 4 x 8 x 8 x 4 plan (pt loaded with load<true>).
 // s0: first radix-4 stage to run (1 when the caller did stage 0 in
 // registers); the last stage's outputs go to `last`.
 // Inlined so that buf is known to be LDS (ds_read/ds_write, not flat
 // accesses); index arithmetic by shifts (n is a power of two).
-template <bool INV, bool PRE, typename C, typename Sink>
+template <bool INV, bool PRE, bool P8 = false, typename C, typename Sink>
 __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw,
                                         const FftTw<INV, C> &pt, int s0, const Sink &last)
 {
+    if (P8) {                              // n = 1024, twiddles loaded for the plan
+        lds_fft1024<INV>(buf, nb, pt, s0, last);
+        return;
+    }
     const int nq = n >> 2, lq = logn - 2, ns = logn >> 1;
     const int total4 = nb * nq;
 #pragma unroll
@@ -284,8 +448,9 @@ __device__ __forceinline__ void store_real_spectrum(const C *buf, int nx, int R,
     }
 }
 
-// PRE (nx <= 4 NT): the thread's twiddles are loaded with its tile.
-template <typename T, bool PRE>
+// PRE (nx <= 4 NT): the thread's twiddles are loaded with its tile; P8:
+// nx = 1024 by the radix-8 plan.
+template <typename T, bool PRE, bool P8>
 __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
@@ -321,7 +486,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         }
     }
     FftTw<false, C> pt;
-    if (PRE) pt.load(a.tw, nx, a.logn);
+    if (PRE) pt.template load<P8>(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -340,7 +505,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         }
     }
     __syncthreads();
-    lds_fft<false, PRE>(buf, nx, a.logn, R >> 1, a.tw, pt, 1, LdsSink<C>{buf});
+    lds_fft<false, PRE, P8>(buf, nx, a.logn, R >> 1, a.tw, pt, 1, LdsSink<C>{buf});
     store_real_spectrum(buf, nx, R, j0, a.nv, spec);
 }
 
@@ -501,7 +666,7 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
     }
     // twiddles in flight during the first stage (not held through the walks)
     FftTw<false, C> pt;
-    pt.load(a.tw, nx, a.logn);
+    pt.template load<NS == 4>(a.tw, nx, a.logn);
     if (NS == 4) {
         // stage 0 of the transform in registers, as rfft_kernel: butterfly
         // (p, t) of complex row p (real rows 2p, 2p+1) takes columns t + m NT
@@ -525,7 +690,7 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
             for (int p = 0; p < (R >> 1); ++p) buf[p * nx + s * NT + t] = cmk<C>(res[s][2 * p], res[s][2 * p + 1]);
     }
     __syncthreads();
-    lds_fft<false, true>(buf, nx, a.logn, R >> 1, a.tw, pt, NS == 4 ? 1 : 0, LdsSink<C>{buf});
+    lds_fft<false, true, NS == 4>(buf, nx, a.logn, R >> 1, a.tw, pt, NS == 4 ? 1 : 0, LdsSink<C>{buf});
     store_real_spectrum(buf, nx, R, j0, nv, spec);
 }
 
@@ -555,7 +720,7 @@ struct IrfftSink {
     }
 };
 
-template <typename T, bool PRE>
+template <typename T, bool PRE, bool P8>
 __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
 {
     using C = cx_t<T>;
@@ -582,7 +747,7 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         vb[q] = load_or_zero(spec, (int64_t)k * a.nv + ja + 1, idx < total && ja + 1 < a.nv);
     }
     FftTw<true, C> pt;
-    if (PRE) pt.load(a.tw, nx, a.logn);
+    if (PRE) pt.template load<P8>(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -602,7 +767,7 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
     }
     __syncthreads();
     // the last stage's outputs (natural order) go straight to the output rows
-    lds_fft<true, PRE>(buf, nx, a.logn, npair, a.tw, pt, 0, IrfftSink<T>{y, j0, nx, a.logn, a.nv});
+    lds_fft<true, PRE, P8>(buf, nx, a.logn, npair, a.tw, pt, 0, IrfftSink<T>{y, j0, nx, a.logn, a.nv});
 }
 
 // Real rows of the irFFT left in the workgroup's LDS buffer itself, natural
@@ -632,7 +797,7 @@ struct RealRowsSink {
 // bit-identical to irfft_kernel followed by mdot_kernel.  What the fusion
 // saves: the second read of w, one launch, and the wait for the first basis
 // vectors, whose loads are in flight while the FFT stages run.
-template <typename T, int EPT, int PV>
+template <typename T, int EPT, int PV, bool P8>
 __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec,
                                                         CgsArgs<T> g, int col)
 {
@@ -666,7 +831,7 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
     }
     const double sc = (int)threadIdx.x < nvec ? gs->scale[threadIdx.x] : 0.0;
     FftTw<true, C> pt;
-    pt.load(a.tw, nx, a.logn);
+    pt.template load<P8>(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
     // the first pair of basis vectors is in flight while the FFT stages run
     const int64_t nvv = g.N / V16<T>::L;
@@ -700,7 +865,7 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
     }
     __syncthreads();
     T *wr = reinterpret_cast<T *>(buf);
-    lds_fft<true, true>(buf, nx, a.logn, npair, a.tw, pt, 0, RealRowsSink<T>{wr, nx, a.logn});
+    lds_fft<true, true, P8>(buf, nx, a.logn, npair, a.tw, pt, 0, RealRowsSink<T>{wr, nx, a.logn});
 
     // w of the slice: to the basis, and into registers for the dots
     const V *wl = reinterpret_cast<const V *>(wr);
@@ -1323,8 +1488,10 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    if (a.nx <= 4 * NT) launch_timed(c, rfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
-    else launch_timed(c, rfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    if (a.nx == N8) launch_timed(c, rfft_kernel<T, true, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    else if (a.nx <= 4 * NT)
+        launch_timed(c, rfft_kernel<T, true, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    else launch_timed(c, rfft_kernel<T, false, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
     return hipGetLastError();
 }
 
@@ -1380,8 +1547,10 @@ hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, cons
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    if (a.nx <= 4 * NT) launch_timed(c, irfft_kernel<T, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
-    else launch_timed(c, irfft_kernel<T, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    if (a.nx == N8) launch_timed(c, irfft_kernel<T, true, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    else if (a.nx <= 4 * NT)
+        launch_timed(c, irfft_kernel<T, true, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    else launch_timed(c, irfft_kernel<T, false, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
     return hipGetLastError();
 }
 
@@ -1406,8 +1575,14 @@ template <typename T, int EPT>
 void irfft_mdot_launch(vv_ctx *c, const FftArgs<T> &a, dim3 grid, size_t lds, const cx_t<T> *spec,
                        const CgsArgs<T> &g, int col)
 {
-    if (c->p.restart < 32) launch_timed(c, irfft_mdot_kernel<T, EPT, 32>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
-    else launch_timed(c, irfft_mdot_kernel<T, EPT, 64>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+    const bool p8 = a.nx == N8;
+    if (c->p.restart < 32) {
+        if (p8) launch_timed(c, irfft_mdot_kernel<T, EPT, 32, true>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+        else launch_timed(c, irfft_mdot_kernel<T, EPT, 32, false>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+    } else {
+        if (p8) launch_timed(c, irfft_mdot_kernel<T, EPT, 64, true>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+        else launch_timed(c, irfft_mdot_kernel<T, EPT, 64, false>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+    }
 }
 
 template <typename T>
