@@ -65,6 +65,18 @@ def test_create_rejects_bad_parameters(lib, field, value, msg):
 def test_null_context_is_an_error(lib):
     assert lib.vv_residual(None, None, None, None) == -1
     assert lib.vv_synchronize(None) == -1
+    assert lib.vv_batched_timestep(None, None, None, 1e-10, 20, None, None, 0, None) == -1
+
+
+def test_batched_timestep_layouts():
+    """vv_batch_fields: twelve pointers; vv_instance_step: int32 picard,
+    int32 reverted, int64 gmres, double initial, double final."""
+    from vivlasov1d_amd import _lib
+    F, S = _lib.BatchFields, _lib.InstanceStep
+    assert ctypes.sizeof(F) == 12 * ctypes.sizeof(ctypes.c_void_p)
+    assert [f[0] for f in F._fields_] == ["f", "fh", "fl", "fy", "h0", "h1", "h1h", "h2", "h2h", "n", "phi", "phih"]
+    assert ctypes.sizeof(S) == 32
+    assert (S.picard.offset, S.reverted.offset, S.gmres.offset, S.initial.offset, S.final.offset) == (0, 4, 8, 16, 24)
 
 
 def test_mixed_params_layout():
