@@ -3,8 +3,8 @@
 //
 //  rfft_kernel    real rows x[j][0..nx) -> spectrum S[k][j], k = 0..nx/2.
 //                 Two real rows are packed as one complex row and transformed
-//                 by an in-LDS Stockham FFT (radix-4/2; lengths 256, 512,
-//                 1024 by radix-8 plans); the output is written
+//                 by an in-LDS Stockham FFT (radix-4/2; length 1024 as
+//                 4 x 8 x 8 x 4); the output is written
 //                 mode-major (the transpose the reference does with a
 //                 VecScatter is folded into the store).
 //  tridiag_kernel one workgroup per mode k solves T_k c = S[k][.] along v:
