@@ -108,33 +108,30 @@ constexpr int MAXS = 6;     // radix-4 stages of the longest transform (MAXE = 4
 // in the radix-2 stage of odd logn, where n <= 2 NT), so 3 twiddles per stage
 // cover them; their loads are in flight with the tile's own loads instead of
 // being waited on after every barrier of the transform.
-// n = 256, 512, 1024 run the plans 4 x 8 x 8 (x 2 | x 4) (see lds_fft8):
-// their twiddles take the same registers, flat index i (always a
-// compile-time index) at w[i / 3][i % 3] for i < 15, then w2 and w8 (0..6 and
-// 7..13 for the radix-8 stages with sub-transforms of 4 and 32, 14..16 for
-// the last radix-2 or radix-4 stage; the last row of w, used only by
-// n = 4096, stays out of it).
+// n = 1024 runs the plan 4 x 8 x 8 x 4 (see lds_fft1024): its 17 twiddles
+// take the same registers, flat index i (always a compile-time index) at
+// w[i / 3][i % 3] for i < 15, then w2 and w8 (0..6 and 7..13 for the radix-8
+// stages with sub-transforms of 4 and 32, 14..16 for the last radix-4 stage;
+// the last row of w, used only by n = 4096, stays out of it).
+constexpr int N8 = 1024;
 template <bool INV, typename C>
 struct FftTw {
     C w[MAXS][3];
     C w2, w8;
-    // LEN (256, 512, 1024; n must equal it): that radix-8 plan's twiddles,
-    // else (0) the radix-4 ones
-    template <int LEN = 0>
+    // P8: the n = 1024 plan's twiddles (n must be 1024), else the radix-4 ones
+    template <bool P8 = false>
     __device__ __forceinline__ void load(const C *__restrict__ tw, int n, int logn)
     {
-        if constexpr (LEN != 0) {
+        if (P8) {
             const int t = threadIdx.x, ka = t & 3, kb = t & 31, kc = t & 255;
 #pragma unroll
             for (int r = 1; r < 8; ++r) {
-                w[(r - 1) / 3][(r - 1) % 3] = twiddle<INV>(tw, r * ka * (LEN / 32));
-                w[(6 + r) / 3][(6 + r) % 3] = twiddle<INV>(tw, r * kb * (LEN / 256));
+                w[(r - 1) / 3][(r - 1) % 3] = twiddle<INV>(tw, r * ka * 32);
+                w[(6 + r) / 3][(6 + r) % 3] = twiddle<INV>(tw, r * kb * 4);
             }
-            if constexpr (LEN >= 512) w[4][2] = twiddle<INV>(tw, kc);
-            if constexpr (LEN == 1024) {
-                w2 = twiddle<INV>(tw, 2 * kc);
-                w8 = twiddle<INV>(tw, 3 * kc);
-            }
+            w[4][2] = twiddle<INV>(tw, kc);
+            w2 = twiddle<INV>(tw, 2 * kc);
+            w8 = twiddle<INV>(tw, 3 * kc);
             return;
         }
         const int ns = logn >> 1;
@@ -201,16 +198,15 @@ __device__ __forceinline__ void dft8(C (&v)[8])
     v[7] = csub(e3, o3);
 }
 
-// One radix-8 Stockham stage of the length-LEN plan: sub-transforms of length
-// NS (4 or 32) combined into 8 NS; butterfly j takes elements j + r LEN/8,
-// twiddled by W_LEN^(r k LEN / (8 NS)), k = j mod NS (= threadIdx.x mod NS
-// for every butterfly of the thread, as LEN/8 divides NT, so the thread's
-// twiddles are preloaded: flat index OFF + r - 1 of FftTw), and stores output
-// m at 8 (j - k) + k + m NS (to `last` when the stage is the plan's last).
-template <bool INV, int LEN, int NS, int OFF, bool LAST, typename C, typename Sink>
-__device__ __forceinline__ void stage8(C *buf, int nb, const FftTw<INV, C> &pt, const Sink &last)
+// One radix-8 Stockham stage of the n = 1024 plan: sub-transforms of length
+// NS (4 or 32) combined into 8 NS; butterfly j takes elements j + 128 r,
+// twiddled by W_1024^(r k 1024 / (8 NS)), k = j mod NS (= threadIdx.x mod NS
+// for every butterfly of the thread, so the thread's twiddles are preloaded:
+// flat index OFF + r - 1 of FftTw), and stores output m at 8 (j - k) + k + m NS.
+template <bool INV, int NS, int OFF, typename C>
+__device__ __forceinline__ void stage8_1024(C *buf, int nb, const FftTw<INV, C> &pt)
 {
-    constexpr int nq = LEN / 8;
+    constexpr int nq = N8 / 8;
     const int total = nb * nq;
     C v[2][8];
     int dst[2];
@@ -219,14 +215,14 @@ __device__ __forceinline__ void stage8(C *buf, int nb, const FftTw<INV, C> &pt, 
         const int bq = threadIdx.x + q * NT;
         dst[q] = -1;
         if (bq < total) {
-            const int b = bq / nq, j = bq & (nq - 1), k = j & (NS - 1);
-            const C *src = buf + b * LEN;
+            const int b = bq >> 7, j = bq & (nq - 1), k = j & (NS - 1);
+            const C *src = buf + b * N8;
 #pragma unroll
             for (int r = 0; r < 8; ++r) v[q][r] = src[j + r * nq];
 #pragma unroll
             for (int r = 1; r < 8; ++r) v[q][r] = cmul(v[q][r], pt.w[(OFF + r - 1) / 3][(OFF + r - 1) % 3]);
             dft8<INV>(v[q]);
-            dst[q] = b * LEN + ((j - k) << 3) + k;
+            dst[q] = b * N8 + ((j - k) << 3) + k;
         }
     }
     __syncthreads();
@@ -234,26 +230,21 @@ __device__ __forceinline__ void stage8(C *buf, int nb, const FftTw<INV, C> &pt, 
     for (int q = 0; q < 2; ++q)
         if (dst[q] >= 0) {
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                if constexpr (LAST) last(dst[q] + m * NS, v[q][m]);
-                else buf[dst[q] + m * NS] = v[q][m];
-            }
+            for (int m = 0; m < 8; ++m) buf[dst[q] + m * NS] = v[q][m];
         }
     __syncthreads();
 }
 
-// LEN = 256, 512, 1024 as 4 x 8 x 8 (x 2 | x 4): the first radix-4 stage
-// (from LDS when s0 == 0; s0 == 1: the caller ran it in registers), two
-// radix-8 stages and for 512 / 1024 a last radix-2 / radix-4 stage; the last
-// stage's outputs go to `last`.  One LDS pass fewer than the radix-4 plan
-// (two fewer for 512).  nb LEN <= MAXE.
-template <bool INV, int LEN, typename C, typename Sink>
-__device__ __forceinline__ void lds_fft8(C *buf, int nb, const FftTw<INV, C> &pt, int s0, const Sink &last)
+// n = 1024 as 4 x 8 x 8 x 4: the first radix-4 stage (from LDS when s0 == 0;
+// s0 == 1: the caller ran it in registers), two radix-8 stages and a last
+// radix-4 stage whose outputs go to `last` -- four passes over LDS instead of
+// the five of the radix-4 plan.  nb <= 4 transforms (MAXE elements).
+template <bool INV, typename C, typename Sink>
+__device__ __forceinline__ void lds_fft1024(C *buf, int nb, const FftTw<INV, C> &pt, int s0, const Sink &last)
 {
-    constexpr int nq = LEN / 4;
-    const LdsSink<C> in_lds{buf};
+    constexpr int nq = N8 / 4;
+    const int total = nb * nq;
     if (s0 == 0) {
-        const int total = nb * nq;
         C v[4][4];
         int dst[4];
 #pragma unroll
@@ -261,12 +252,12 @@ __device__ __forceinline__ void lds_fft8(C *buf, int nb, const FftTw<INV, C> &pt
             const int bq = threadIdx.x + q * NT;
             dst[q] = -1;
             if (bq < total) {
-                const int b = bq / nq, j = bq & (nq - 1);
-                const C *src = buf + b * LEN;
+                const int b = bq >> 8, j = bq & (nq - 1);
+                const C *src = buf + b * N8;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) v[q][r] = src[j + r * nq];
                 dft4<INV>(v[q][0], v[q][1], v[q][2], v[q][3]);
-                dst[q] = b * LEN + (j << 2);
+                dst[q] = b * N8 + (j << 2);
             }
         }
         __syncthreads();
@@ -278,48 +269,33 @@ __device__ __forceinline__ void lds_fft8(C *buf, int nb, const FftTw<INV, C> &pt
             }
         __syncthreads();
     }
-    stage8<INV, LEN, 4, 0, false>(buf, nb, pt, in_lds);
-    if constexpr (LEN == 256) {
-        stage8<INV, LEN, 32, 7, true>(buf, nb, pt, last);
-        return;
-    }
-    stage8<INV, LEN, 32, 7, false>(buf, nb, pt, in_lds);
-    // last stage over sub-transforms of 256: k = j = threadIdx.x (mod 256),
-    // outputs j + 256 m
-    constexpr int P = LEN / 256;                 // its radix: 2 or 4
-    constexpr int QL = (MAXE / P + NT - 1) / NT;
-    const int total = nb * 256;
-    C v[QL][P];
-    int dst[QL];
+    stage8_1024<INV, 4, 0>(buf, nb, pt);
+    stage8_1024<INV, 32, 7>(buf, nb, pt);
+    // last stage: sub-transforms of 256, k = j = threadIdx.x, outputs j + 256 m
+    C v[4][4];
+    int dst[4];
 #pragma unroll
-    for (int q = 0; q < QL; ++q) {
+    for (int q = 0; q < 4; ++q) {
         const int bq = threadIdx.x + q * NT;
         dst[q] = -1;
         if (bq < total) {
-            const int b = bq >> 8, j = bq & 255;
-            const C *src = buf + b * LEN;
+            const int b = bq >> 8, j = bq & (nq - 1);
+            const C *src = buf + b * N8;
 #pragma unroll
-            for (int r = 0; r < P; ++r) v[q][r] = src[j + r * 256];
-            if constexpr (P == 4) {
-                v[q][1] = cmul(v[q][1], pt.w[4][2]);
-                v[q][2] = cmul(v[q][2], pt.w2);
-                v[q][3] = cmul(v[q][3], pt.w8);
-                dft4<INV>(v[q][0], v[q][1], v[q][2], v[q][3]);
-            } else {
-                const C x1 = cmul(v[q][1], pt.w[4][2]);
-                const C x0 = v[q][0];
-                v[q][0] = cadd(x0, x1);
-                v[q][1] = csub(x0, x1);
-            }
-            dst[q] = b * LEN + j;
+            for (int r = 0; r < 4; ++r) v[q][r] = src[j + r * nq];
+            v[q][1] = cmul(v[q][1], pt.w[4][2]);
+            v[q][2] = cmul(v[q][2], pt.w2);
+            v[q][3] = cmul(v[q][3], pt.w8);
+            dft4<INV>(v[q][0], v[q][1], v[q][2], v[q][3]);
+            dst[q] = b * N8 + j;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < QL; ++q)
+    for (int q = 0; q < 4; ++q)
         if (dst[q] >= 0) {
 #pragma unroll
-            for (int m = 0; m < P; ++m) last(dst[q] + m * 256, v[q][m]);
+            for (int m = 0; m < 4; ++m) last(dst[q] + m * nq, v[q][m]);
         }
     __syncthreads();
 }
@@ -329,18 +305,18 @@ __device__ __forceinline__ void lds_fft8(C *buf, int nb, const FftTw<INV, C> &pt
 // (unnormalised).  Each stage loads every butterfly of the thread into
 // registers, synchronises, then stores, so one buffer suffices.  C is
 // double2 (FP64 path) or float2 (FP32 preconditioner of the mixed solver).
-// PRE: twiddles from pt (n <= 4 NT), else from the global table tw.  LEN
-// (256, 512, 1024): n = LEN by its radix-8 plan (pt loaded with load<LEN>).
+// PRE: twiddles from pt (n <= 4 NT), else from the global table tw.  P8:
+// n = 1024 by the 4 x 8 x 8 x 4 plan (pt loaded with load<true>).
 // s0: first radix-4 stage to run (1 when the caller did stage 0 in
 // registers); the last stage's outputs go to `last`.
 // Inlined so that buf is known to be LDS (ds_read/ds_write, not flat
 // accesses); index arithmetic by shifts (n is a power of two).
-template <bool INV, bool PRE, int LEN = 0, typename C, typename Sink>
+template <bool INV, bool PRE, bool P8 = false, typename C, typename Sink>
 __device__ __forceinline__ void lds_fft(C *buf, int n, int logn, int nb, const C *__restrict__ tw,
                                         const FftTw<INV, C> &pt, int s0, const Sink &last)
 {
-    if constexpr (LEN != 0) {              // n = LEN, twiddles loaded for its plan
-        lds_fft8<INV, LEN>(buf, nb, pt, s0, last);
+    if (P8) {                              // n = 1024, twiddles loaded for the plan
+        lds_fft1024<INV>(buf, nb, pt, s0, last);
         return;
     }
     const int nq = n >> 2, lq = logn - 2, ns = logn >> 1;
@@ -473,9 +449,9 @@ __device__ __forceinline__ void store_real_spectrum(const C *buf, int nx, int R,
     }
 }
 
-// PRE (nx <= 4 NT): the thread's twiddles are loaded with its tile; LEN:
-// nx = LEN (256, 512, 1024) by its radix-8 plan.
-template <typename T, bool PRE, int LEN>
+// PRE (nx <= 4 NT): the thread's twiddles are loaded with its tile; P8:
+// nx = 1024 by the radix-8 plan.
+template <typename T, bool PRE, bool P8>
 __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restrict__ x, cx_t<T> *__restrict__ spec)
 {
     using C = cx_t<T>;
@@ -511,7 +487,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         }
     }
     FftTw<false, C> pt;
-    if (PRE) pt.template load<LEN>(a.tw, nx, a.logn);
+    if (PRE) pt.template load<P8>(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -530,7 +506,7 @@ __global__ __launch_bounds__(NT) void rfft_kernel(FftArgs<T> a, const T *__restr
         }
     }
     __syncthreads();
-    lds_fft<false, PRE, LEN>(buf, nx, a.logn, R >> 1, a.tw, pt, 1, LdsSink<C>{buf});
+    lds_fft<false, PRE, P8>(buf, nx, a.logn, R >> 1, a.tw, pt, 1, LdsSink<C>{buf});
     store_real_spectrum(buf, nx, R, j0, a.nv, spec);
 }
 
@@ -691,7 +667,7 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
     }
     // twiddles in flight during the first stage (not held through the walks)
     FftTw<false, C> pt;
-    pt.template load<nx>(a.tw, nx, a.logn);
+    pt.template load<NS == 4>(a.tw, nx, a.logn);
     if (NS == 4) {
         // stage 0 of the transform in registers, as rfft_kernel: butterfly
         // (p, t) of complex row p (real rows 2p, 2p+1) takes columns t + m NT
@@ -715,7 +691,7 @@ __device__ __forceinline__ void jac_rfft_body(FftArgs<T> a, JacRows jr, const T 
             for (int p = 0; p < (R >> 1); ++p) buf[p * nx + s * NT + t] = cmk<C>(res[s][2 * p], res[s][2 * p + 1]);
     }
     __syncthreads();
-    lds_fft<false, true, nx>(buf, nx, a.logn, R >> 1, a.tw, pt, NS == 4 ? 1 : 0, LdsSink<C>{buf});
+    lds_fft<false, true, NS == 4>(buf, nx, a.logn, R >> 1, a.tw, pt, NS == 4 ? 1 : 0, LdsSink<C>{buf});
     store_real_spectrum(buf, nx, R, j0, nv, spec);
 }
 
@@ -745,7 +721,7 @@ struct IrfftSink {
     }
 };
 
-template <typename T, bool PRE, int LEN>
+template <typename T, bool PRE, bool P8>
 __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec, T *__restrict__ y)
 {
     using C = cx_t<T>;
@@ -772,7 +748,7 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
         vb[q] = load_or_zero(spec, (int64_t)k * a.nv + ja + 1, idx < total && ja + 1 < a.nv);
     }
     FftTw<true, C> pt;
-    if (PRE) pt.template load<LEN>(a.tw, nx, a.logn);
+    if (PRE) pt.template load<P8>(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -792,7 +768,7 @@ __global__ __launch_bounds__(NT) void irfft_kernel(FftArgs<T> a, const cx_t<T> *
     }
     __syncthreads();
     // the last stage's outputs (natural order) go straight to the output rows
-    lds_fft<true, PRE, LEN>(buf, nx, a.logn, npair, a.tw, pt, 0, IrfftSink<T>{y, j0, nx, a.logn, a.nv});
+    lds_fft<true, PRE, P8>(buf, nx, a.logn, npair, a.tw, pt, 0, IrfftSink<T>{y, j0, nx, a.logn, a.nv});
 }
 
 // Real rows of the irFFT left in the workgroup's LDS buffer itself, natural
@@ -822,7 +798,7 @@ struct RealRowsSink {
 // bit-identical to irfft_kernel followed by mdot_kernel.  What the fusion
 // saves: the second read of w, one launch, and the wait for the first basis
 // vectors, whose loads are in flight while the FFT stages run.
-template <typename T, int EPT, int PV, int LEN>
+template <typename T, int EPT, int PV, bool P8>
 __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t<T> *__restrict__ spec,
                                                         CgsArgs<T> g, int col)
 {
@@ -856,7 +832,7 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
     }
     const double sc = (int)threadIdx.x < nvec ? gs->scale[threadIdx.x] : 0.0;
     FftTw<true, C> pt;
-    pt.template load<LEN>(a.tw, nx, a.logn);
+    pt.template load<P8>(a.tw, nx, a.logn);
     if (dflag) return;                 // uniform over the block, before any barrier
     // the first pair of basis vectors is in flight while the FFT stages run
     const int64_t nvv = g.N / V16<T>::L;
@@ -890,7 +866,7 @@ __global__ __launch_bounds__(NT) void irfft_mdot_kernel(FftArgs<T> a, const cx_t
     }
     __syncthreads();
     T *wr = reinterpret_cast<T *>(buf);
-    lds_fft<true, true, LEN>(buf, nx, a.logn, npair, a.tw, pt, 0, RealRowsSink<T>{wr, nx, a.logn});
+    lds_fft<true, true, P8>(buf, nx, a.logn, npair, a.tw, pt, 0, RealRowsSink<T>{wr, nx, a.logn});
 
     // w of the slice: to the basis, and into registers for the dots
     const V *wl = reinterpret_cast<const V *>(wr);
@@ -1513,12 +1489,10 @@ hipError_t rfft_t(vv_ctx *c, const cx_t<T> *tw, const T *x, cx_t<T> *spec, const
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    const dim3 grid(nb, c->B);
-    if (a.nx == 1024) launch_timed(c, rfft_kernel<T, true, 1024>, grid, dim3(NT), (uint32_t)lds, a, x, spec);
-    else if (a.nx == 512) launch_timed(c, rfft_kernel<T, true, 512>, grid, dim3(NT), (uint32_t)lds, a, x, spec);
-    else if (a.nx == 256) launch_timed(c, rfft_kernel<T, true, 256>, grid, dim3(NT), (uint32_t)lds, a, x, spec);
-    else if (a.nx <= 4 * NT) launch_timed(c, rfft_kernel<T, true, 0>, grid, dim3(NT), (uint32_t)lds, a, x, spec);
-    else launch_timed(c, rfft_kernel<T, false, 0>, grid, dim3(NT), (uint32_t)lds, a, x, spec);
+    if (a.nx == N8) launch_timed(c, rfft_kernel<T, true, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    else if (a.nx <= 4 * NT)
+        launch_timed(c, rfft_kernel<T, true, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
+    else launch_timed(c, rfft_kernel<T, false, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, x, spec);
     return hipGetLastError();
 }
 
@@ -1574,12 +1548,10 @@ hipError_t irfft_t(vv_ctx *c, const cx_t<T> *tw, const cx_t<T> *spec, T *y, cons
                  (int64_t)sizeof(GmresState), tw, done};
     const int nb = (a.nv + a.rows - 1) / a.rows;
     const size_t lds = (size_t)(a.rows >> 1) * a.nx * sizeof(cx_t<T>);
-    const dim3 grid(nb, c->B);
-    if (a.nx == 1024) launch_timed(c, irfft_kernel<T, true, 1024>, grid, dim3(NT), (uint32_t)lds, a, spec, y);
-    else if (a.nx == 512) launch_timed(c, irfft_kernel<T, true, 512>, grid, dim3(NT), (uint32_t)lds, a, spec, y);
-    else if (a.nx == 256) launch_timed(c, irfft_kernel<T, true, 256>, grid, dim3(NT), (uint32_t)lds, a, spec, y);
-    else if (a.nx <= 4 * NT) launch_timed(c, irfft_kernel<T, true, 0>, grid, dim3(NT), (uint32_t)lds, a, spec, y);
-    else launch_timed(c, irfft_kernel<T, false, 0>, grid, dim3(NT), (uint32_t)lds, a, spec, y);
+    if (a.nx == N8) launch_timed(c, irfft_kernel<T, true, true>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    else if (a.nx <= 4 * NT)
+        launch_timed(c, irfft_kernel<T, true, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
+    else launch_timed(c, irfft_kernel<T, false, false>, dim3(nb, c->B), dim3(NT), (uint32_t)lds, a, spec, y);
     return hipGetLastError();
 }
 
@@ -1600,23 +1572,18 @@ int irfft_mdot_ept(const vv_ctx *c)
     return (ept == 1 || ept == 2 || ept == 4 || ept == 8) ? (int)ept : 0;
 }
 
-template <typename T, int EPT, int PV>
-void irfft_mdot_len(vv_ctx *c, const FftArgs<T> &a, dim3 grid, size_t lds, const cx_t<T> *spec, const CgsArgs<T> &g,
-                    int col)
-{
-    const uint32_t L = (uint32_t)lds;
-    if (a.nx == 1024) launch_timed(c, irfft_mdot_kernel<T, EPT, PV, 1024>, grid, dim3(NT), L, a, spec, g, col);
-    else if (a.nx == 512) launch_timed(c, irfft_mdot_kernel<T, EPT, PV, 512>, grid, dim3(NT), L, a, spec, g, col);
-    else if (a.nx == 256) launch_timed(c, irfft_mdot_kernel<T, EPT, PV, 256>, grid, dim3(NT), L, a, spec, g, col);
-    else launch_timed(c, irfft_mdot_kernel<T, EPT, PV, 0>, grid, dim3(NT), L, a, spec, g, col);
-}
-
 template <typename T, int EPT>
 void irfft_mdot_launch(vv_ctx *c, const FftArgs<T> &a, dim3 grid, size_t lds, const cx_t<T> *spec,
                        const CgsArgs<T> &g, int col)
 {
-    if (c->p.restart < 32) irfft_mdot_len<T, EPT, 32>(c, a, grid, lds, spec, g, col);
-    else irfft_mdot_len<T, EPT, 64>(c, a, grid, lds, spec, g, col);
+    const bool p8 = a.nx == N8;
+    if (c->p.restart < 32) {
+        if (p8) launch_timed(c, irfft_mdot_kernel<T, EPT, 32, true>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+        else launch_timed(c, irfft_mdot_kernel<T, EPT, 32, false>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+    } else {
+        if (p8) launch_timed(c, irfft_mdot_kernel<T, EPT, 64, true>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+        else launch_timed(c, irfft_mdot_kernel<T, EPT, 64, false>, grid, dim3(NT), (uint32_t)lds, a, spec, g, col);
+    }
 }
 
 template <typename T>
