@@ -84,10 +84,26 @@ def test_increase_reverts_the_last_iterate():
     assert run.moments_calls == 3           # after each solve, and after the revert
 
 
-def test_first_iteration_never_reverts():
+def test_first_iteration_increase_does_not_stop():
     run = _run([1.0, 2.0, 1e-11])
     rec = run.step()
     assert len(rec['picard']) == 2 and 'reverted' not in rec
+
+
+def test_first_iteration_increase_reverts_when_stopping_below_atol():
+    # the stop test's "i > 1" guards only the rise; a first iterate that
+    # rose and lies below snes_atol still stops the loop, and is undone
+    run = _run([2e-11, 5e-11, 1e-12])
+    rec = run.step()
+    assert len(rec['picard']) == 1 and rec['reverted'] and rec['final'] == 5e-11
+    assert float(run.fc[0]) == 0.0 and run.moments_calls == 2
+
+
+def test_first_iteration_increase_reverts_at_the_cap():
+    run = _run([1.0, 2.0, 1e-3], snes_max_iter=1)
+    rec = run.step()
+    assert len(rec['picard']) == 1 and rec['reverted'] and run.solves == 1
+    assert float(run.fc[0]) == 0.0
 
 
 def test_iteration_cap():
