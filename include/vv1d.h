@@ -142,6 +142,11 @@ int vv_autotune(vv_ctx *ctx, int32_t *fft_rows, int32_t *cgs_ept);
  * sign of exact zeros); 0 runs the separate kernels (for timing). */
 int vv_set_fusion(vv_ctx *ctx, int enable);
 
+/* Context lifetime and stream plumbing.  vv_ctx_destroy frees what the
+ * context allocated, as PETScVlasovSolverBase.__dealloc__ destroys its local
+ * Vecs (PETScVlasovSolver.pyx:105-111); never the caller's buffers.  The
+ * others have no counterpart in the reference (PETSc keeps its own error
+ * stack and runs on the host). */
 int vv_ctx_destroy(vv_ctx *ctx);
 const char *vv_last_error(const vv_ctx *ctx);
 int vv_set_stream(vv_ctx *ctx, void *stream);
@@ -243,6 +248,9 @@ typedef struct {
     double  coll_drag;     /* weight of the drag term (default 1)              */
 } vv_dissipation;
 
+/* CollisionOperator.create / DoubleBracket.create as PETScVlasovSolverBase
+ * calls them (PETScVlasovSolver.pyx:101-102): the term every later residual
+ * and Jacobian apply adds (kind VV_DISSIPATION_NONE: none). */
 int vv_set_dissipation(vv_ctx *ctx, const vv_dissipation *d);
 
 /* PETScVlasovSolverBase.set_moments (PETScVlasovSolver.pyx:123-132): device
@@ -480,7 +488,7 @@ int vv_invariants(vv_ctx *ctx, const double *f, const double *h0,
                   const double *h1, const double *h2, const double *phi,
                   double *out);
 
-/* Per-kernel timing with HIP events attached to each timed kernel's own dispatch (for the
+/* Per-kernel timing (no counterpart in the reference) with HIP events attached to each timed kernel's own dispatch (for the
  * roofline report).  kernel: "jacobian", "residual", "rfft", "tridiag",
  * "irfft", "mdot", "maxpy_norm", the FP32 kernels of the mixed solver
  * "jacobian32", "rfft32", "tridiag32", "irfft32", "mdot32", "maxpy_norm32",
