@@ -45,3 +45,18 @@ def test_c_timesteps_match_oracle(cb, nx, nv):
         assert ec['picard'] == len(eo['picard'])
         assert abs(ec['gmres'] - sum(p['gmres_its'] for p in eo['picard'])) <= 2
         assert np.abs(crun.f().T - orun.fc).max() <= 1e-10 * np.abs(orun.fc).max()
+
+
+@pytest.mark.parametrize("nx", [64, 128, 256, 512, 1024])
+def test_c_precond_at_workload_lengths(cb, nx):
+    # the x-lengths of the workloads (64 .. 1024; 1024 is the benchmark's),
+    # each through the C restatement's own FFT, against numpy's rfft / irfft
+    # in the oracle's preconditioner; few v-rows keep it fast
+    g = O.Grid(nx, 16, 10 * np.pi, 10.0, ht=0.05)
+    rng = np.random.default_rng(nx)
+    f0 = O.normalise(O.initial_distribution(g, lambda x, v: (1 + 0.1 * np.cos(0.2 * x)) * O.maxwellian(v)), g)
+    run = cb.CRun(g, f0.T)
+    u = rng.standard_normal((g.nx, g.nv))
+    ref = O.Preconditioner(g).apply(u)
+    got = run.precond(u.T).T
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
