@@ -60,3 +60,31 @@ def test_c_precond_at_workload_lengths(cb, nx):
     ref = O.Preconditioner(g).apply(u)
     got = run.precond(u.T).T
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _c_against_numpy(cb, g, prof, kind, steps):
+    # every step: the same Picard count (or a tie at the stop decision, as
+    # the GPU long runs allow) and f within 1e-12 of the NumPy oracle's
+    from picard_ties import parting_kind
+    orun = O.OracleRun(g, O.initial_distribution(g, prof), bracket=kind)
+    crun = cb.CRun(g, O.normalise(O.initial_distribution(g, prof), g).T, bracket=kind)
+    for _ in range(steps):
+        eo, ec = orun.step(), crun.step()
+        if parting_kind(ec['residuals'], [p['residual'] for p in eo['picard']], 1e-10) is not None:
+            crun.set_f(orun.fc.T)
+        assert np.abs(crun.f().T - orun.fc).max() <= 1e-12 * np.abs(orun.fc).max()
+
+
+def test_c_linear_landau_full_run_matches_oracle(cb):
+    # linear Landau 64x128, L = 4 pi, v in [-10, 10], amplitude 0.01, J4,
+    # dt = 0.1: all 100 steps of the CPU-runnable workload
+    g = O.Grid(64, 128, 4 * np.pi, 10.0, ht=0.1)
+    _c_against_numpy(cb, g, lambda x, v: (1 + 0.01 * np.cos(0.5 * x)) * O.maxwellian(v), 'arakawaj4', 100)
+
+
+@pytest.mark.parametrize("kind", ['arakawaj1', 'arakawaj2', 'arakawaj4'])
+def test_c_nonlinear_landau_matches_oracle(cb, kind):
+    # nonlinear Landau 256x512, v in [-8, 8], amplitude 0.5, dt = 0.05: the
+    # first two steps at full size
+    g = O.Grid(256, 512, 4 * np.pi, 8.0, ht=0.05)
+    _c_against_numpy(cb, g, lambda x, v: (1 + 0.5 * np.cos(0.5 * x)) * O.maxwellian(v), kind, 2)
