@@ -1,0 +1,199 @@
+"""The oracle's pins catch plausible mistakes (CPU only).
+
+Each case plants one mistake in the oracle (a dropped term, a wrong sign, a
+wrong stencil index, transposed operands, a wrong weight or normalisation)
+in the Arakawa brackets, the residual and Jacobian, the kinetic
+preconditioner's eigenvalues and bands, or the Poisson stencil, solve and
+right-hand side; runs the independent pins of test_oracle_brackets.py and
+test_oracle_solvers.py against the planted oracle; and requires that at
+least two of them fail (every bracket case fails three or more).  Without
+the mistake every pin passes."""
+
+import numpy as np
+import pytest
+
+import test_oracle_brackets as P
+import test_oracle_solvers as SV
+from oracle import vlasov_oracle as O
+
+S = O._s
+_J1, _J2 = O._j1_terms, O._j2_terms      # the unplanted terms
+
+
+def _j1_without_jcp(x, h):
+    jpp = (S(x, 1, 0) - S(x, -1, 0)) * (S(h, 0, 1) - S(h, 0, -1)) \
+        - (S(x, 0, 1) - S(x, 0, -1)) * (S(h, 1, 0) - S(h, -1, 0))
+    jpc = S(x, 1, 0) * (S(h, 1, 1) - S(h, 1, -1)) \
+        - S(x, -1, 0) * (S(h, -1, 1) - S(h, -1, -1)) \
+        - S(x, 0, 1) * (S(h, 1, 1) - S(h, -1, 1)) \
+        + S(x, 0, -1) * (S(h, 1, -1) - S(h, -1, -1))
+    return 1.5 * (jpp + jpc)          # dropped term, rescaled so sums still look right
+
+
+def _j1_sign(x, h):
+    # the last jpc term with the wrong sign
+    return _J1(x, h) - 2 * S(x, 0, -1) * (S(h, 1, -1) - S(h, -1, -1))
+
+
+def _j1_index(x, h):
+    # S(h, 1, 1) read as S(h, 1, 0) in jpc's first term
+    return _J1(x, h) - S(x, 1, 0) * (S(h, 1, 1) - S(h, 1, 0))
+
+
+def _j2_index(x, h):
+    # jcp's first term with (0, 2) read as (0, 1)
+    return _J2(x, h) - S(x, 1, 1) * (S(h, 0, 2) - S(h, 0, 1))
+
+
+def _j1_transposed(x, h):
+    return _J1(h, x)
+
+
+def _j4_weight(kind, x, h, y, factor, grid):
+    af1 = grid.hx_inv * grid.hv_inv / 12.
+    af2 = grid.hx_inv * grid.hv_inv / 24.
+    inner = y[:, O.STENCIL:grid.nv - O.STENCIL]
+    inner += factor * (O._j1_terms(x, h) * af1 - 0.5 * O._j2_terms(x, h) * af2)   # 2 rJ1 - rJ2 halved wrongly
+    return y
+
+
+def _j2_factor(kind, x, h, y, factor, grid):
+    af = grid.hx_inv * grid.hv_inv / 12.                                        # 1/12 in place of 1/24
+    y[:, O.STENCIL:grid.nv - O.STENCIL] += factor * O._j2_terms(x, h) * af
+    return y
+
+
+# (kinds the mistake reaches, what is replaced, the replacement)
+CASES = {
+    'j1 dropped jcp': (['arakawaj1', 'arakawaj4'], '_j1_terms', _j1_without_jcp),
+    'j1 wrong sign': (['arakawaj1', 'arakawaj4'], '_j1_terms', _j1_sign),
+    'j1 wrong index': (['arakawaj1', 'arakawaj4'], '_j1_terms', _j1_index),
+    'j2 wrong index': (['arakawaj2', 'arakawaj4'], '_j2_terms', _j2_index),
+    'j1 transposed operands': (['arakawaj1', 'arakawaj4'], '_j1_terms', _j1_transposed),
+    'j4 wrong weight': (['arakawaj4'], 'bracket_add', _j4_weight),
+    'j2 wrong normalisation': (['arakawaj2'], 'bracket_add', _j2_factor),
+}
+
+
+def _pins(kind):
+    checks = [lambda: P.test_matrix_free_equals_stencil_table(kind), lambda: P.test_antisymmetry(kind),
+              lambda: P.test_conservation(kind), lambda: P.test_closed_form_symbol(kind)]
+    order = {'arakawaj1': 2, 'arakawaj2': 2, 'arakawaj4': 4}[kind]
+    checks.append(lambda: P.test_consistency_order(kind, order))
+    failed = 0
+    for c in checks:
+        try:
+            c()
+        except AssertionError:
+            failed += 1
+    return failed
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_planted_mistake_fails_a_pin(name, monkeypatch):
+    kinds, attr, repl = CASES[name]
+    monkeypatch.setattr(O, attr, repl)
+    for kind in kinds:
+        assert _pins(kind) >= 2, (name, kind)
+
+
+def test_unplanted_oracle_passes_every_pin():
+    for kind in P.KINDS:
+        assert _pins(kind) == 0, kind
+
+
+# ------------------------------------------------ residual, P^-1 and Poisson
+_RES, _JAC, _BANDS, _EIG = O.residual, O.jacobian, O.kinetic_bands, O.kinetic_eigenvalues
+_PMM, _PEIG, _PRHS = O.poisson_matrix_mult, O.poisson_eigenvalues, O.poisson_rhs
+
+
+def _res_fave(kind, f, fh, H, grid):
+    y = grid.ht_inv * (f - fh)
+    return O.bracket_add(kind, fh + 0.5 * f, H, y, 1.0, grid)          # Fave missing a 1/2
+
+
+def _res_sign(kind, f, fh, H, grid):
+    y = grid.ht_inv * (fh - f)                                          # Fder with the wrong sign
+    return O.bracket_add(kind, 0.5 * fh + 0.5 * f, H, y, 1.0, grid)
+
+
+def _jac_factor(kind, u, H, grid):
+    return O.bracket_add(kind, u, H, grid.ht_inv * u, 1.0, grid)        # 1 in place of 1/2
+
+
+def _bands_factor(grid, eigen_k):
+    dm, d, dp = _BANDS(grid, eigen_k)                                   # a without its 1/2
+    return 2 * dm, 2 * d - grid.ht_inv, 2 * dp
+
+
+def _bands_swapped(grid, eigen_k):
+    dm, d, dp = _BANDS(grid, eigen_k)                                   # sub / super-diagonal swapped
+    return dp, d, dm
+
+
+def _eig_conj(n):
+    return np.conj(_EIG(n))                                             # exp(-2 pi i ...) for exp(+...)
+
+
+def _pmm_coef(x, hx2_inv, laplace='cfd4'):
+    y = _PMM(x, hx2_inv, laplace)
+    return y - 2.0 * x * hx2_inv / 12. if laplace == 'cfd4' else y    # 54 read as 52
+
+
+def _peig_coef(nx, hx, laplace='cfd4'):
+    lam = _PEIG(nx, hx, laplace)
+    th = 2.0 * np.pi * np.arange(nx) / nx
+    if laplace == 'cfd4':
+        lam = lam + 2. * np.cos(2. * th) / (12. * hx ** 2)              # 10 cos 2th read as 12 cos 2th
+        lam[0] = 0.0
+    return lam
+
+
+def _prhs_sign(n, charge, nx):
+    return -_PRHS(n, charge, nx)
+
+
+_KINDS3 = ['arakawaj1', 'arakawaj2', 'arakawaj4']
+_RES_PINS = [lambda k=k: P.test_residual_affine_and_jacobian(k) for k in _KINDS3] + \
+            [lambda k=k: P.test_residual_structure(k) for k in _KINDS3]
+_PC_PINS = [lambda: SV.test_eigenvalues_are_centered_difference_symbol(),
+            lambda: SV.test_p_times_pinv_is_identity(8, 16),
+            lambda: SV.test_preconditioner_is_half_j1_of_h0_plus_identity_over_dt(8),
+            lambda: SV.test_preconditioner_is_half_j1_of_h0_plus_identity_over_dt(10),
+            lambda: SV.test_odd_length_preconditioner_is_a_modewise_dense_solve()]
+_POISSON_PINS = [lambda: SV.test_poisson_solve_vs_dense_lstsq('cfd4', 16),
+                 lambda: SV.test_poisson_solve_vs_dense_lstsq('cfd4', 9),
+                 lambda: SV.test_poisson_eigenvalues()]
+
+SOLVER_CASES = {
+    'residual Fave weight': ('residual', _res_fave, _RES_PINS),
+    'residual time-derivative sign': ('residual', _res_sign, _RES_PINS),
+    'jacobian bracket factor': ('jacobian', _jac_factor, _RES_PINS),
+    'kinetic bands missing 1/2': ('kinetic_bands', _bands_factor, _PC_PINS),
+    'kinetic bands swapped': ('kinetic_bands', _bands_swapped, _PC_PINS),
+    'kinetic eigenvalues conjugated': ('kinetic_eigenvalues', _eig_conj, _PC_PINS),
+    'poisson stencil coefficient': ('poisson_matrix_mult', _pmm_coef, _POISSON_PINS),
+    'poisson eigenvalue coefficient': ('poisson_eigenvalues', _peig_coef, _POISSON_PINS),
+    'poisson rhs sign': ('poisson_rhs', _prhs_sign, _POISSON_PINS),
+}
+
+
+def _failures(pins):
+    failed = 0
+    for c in pins:
+        try:
+            c()
+        except AssertionError:
+            failed += 1
+    return failed
+
+
+@pytest.mark.parametrize("name", sorted(SOLVER_CASES))
+def test_planted_solver_mistake_fails_a_pin(name, monkeypatch):
+    attr, repl, pins = SOLVER_CASES[name]
+    monkeypatch.setattr(O, attr, repl)
+    assert _failures(pins) >= 2, name
+
+
+def test_unplanted_solver_oracle_passes_every_pin():
+    assert _failures(_RES_PINS + _PC_PINS + _POISSON_PINS) == 0
