@@ -3,11 +3,13 @@
 Each case plants one mistake in the oracle (a dropped term, a wrong sign, a
 wrong stencil index, transposed operands, a wrong weight or normalisation)
 in the Arakawa brackets, the residual and Jacobian, the kinetic
-preconditioner's eigenvalues and bands, or the Poisson stencil, solve and
-right-hand side; runs the independent pins of test_oracle_brackets.py and
+preconditioner's eigenvalues and bands, the Poisson stencil, solve and
+right-hand side, or GMRES (edited into its source text); runs the independent pins of test_oracle_brackets.py and
 test_oracle_solvers.py against the planted oracle; and requires that at
 least two of them fail (every bracket case fails three or more).  Without
 the mistake every pin passes."""
+
+import inspect
 
 import numpy as np
 import pytest
@@ -197,3 +199,34 @@ def test_planted_solver_mistake_fails_a_pin(name, monkeypatch):
 
 def test_unplanted_solver_oracle_passes_every_pin():
     assert _failures(_RES_PINS + _PC_PINS + _POISSON_PINS) == 0
+
+
+# ------------------------------------------------------------------- GMRES
+# (text in O.gmres, its replacement)
+GMRES_CASES = {
+    'previous rotation sign': ('H[jj + 1, it] = cs[jj] * H[jj + 1, it] - sn[jj] * t',
+                               'H[jj + 1, it] = cs[jj] * H[jj + 1, it] + sn[jj] * t'),
+    'right-hand-side rotation sign': ('g[it + 1] = -(sn[it] * g[it])', 'g[it + 1] = (sn[it] * g[it])'),
+    'back substitution drops a term': ('for jj in range(k + 1, it):', 'for jj in range(k + 2, it):'),
+    'CGS skips the newest vector': ('for jj in range(it + 1):', 'for jj in range(it):'),
+    'restart residual sign': ('r = b.copy() if first else b - apply_A(x)', 'r = b.copy() if first else b + apply_A(x)'),
+    'update from the wrong basis vector': ('upd = upd + y[jj] * V[jj]', 'upd = upd + y[jj] * V[jj + 1]'),
+}
+_GMRES_PINS = [lambda r=r: SV.test_gmres_vs_dense_solve(r) for r in (50, 10, 3)] + \
+              [SV.test_gmres_full_restart_equivalence, SV.test_gmres_edge_cases]
+
+
+@pytest.mark.parametrize("name", sorted(GMRES_CASES))
+def test_planted_gmres_mistake_fails_a_pin(name, monkeypatch):
+    old, new = GMRES_CASES[name]
+    src = inspect.getsource(O.gmres)
+    assert src.count(old) == 1, name
+    ns = dict(vars(O))
+    exec(src.replace(old, new), ns)
+    monkeypatch.setattr(O, 'gmres', ns['gmres'])
+    with np.errstate(all='ignore'):
+        assert _failures(_GMRES_PINS) >= 2, name
+
+
+def test_unplanted_gmres_passes_every_pin():
+    assert _failures(_GMRES_PINS) == 0
