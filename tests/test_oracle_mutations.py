@@ -4,7 +4,8 @@ Each case plants one mistake in the oracle (a dropped term, a wrong sign, a
 wrong stencil index, transposed operands, a wrong weight or normalisation)
 in the Arakawa brackets, the residual and Jacobian, the kinetic
 preconditioner's eigenvalues and bands, the Poisson stencil, solve and
-right-hand side, or GMRES (edited into its source text); runs the independent pins of test_oracle_brackets.py and
+right-hand side, the collision operators and factor, or GMRES (the last
+two edited into their source text); runs the independent pins of test_oracle_brackets.py and
 test_oracle_solvers.py against the planted oracle; and requires that at
 least two of them fail (every bracket case fails three or more).  Without
 the mistake every pin passes."""
@@ -15,6 +16,7 @@ import numpy as np
 import pytest
 
 import test_oracle_brackets as P
+import test_oracle_dissipation as D
 import test_oracle_solvers as SV
 from oracle import vlasov_oracle as O
 
@@ -230,3 +232,48 @@ def test_planted_gmres_mistake_fails_a_pin(name, monkeypatch):
 
 def test_unplanted_gmres_passes_every_pin():
     assert _failures(_GMRES_PINS) == 0
+
+
+# ------------------------------------------------------ collision operators
+# (text in O.collision_add, its replacement, the operators it reaches)
+COLLISION_CASES = {
+    'drag without its 1/2': ('grid.hv_inv * 0.5', 'grid.hv_inv', ['collt', 'colle']),
+    'diffusion over hv, not hv^2': ('coll_diff * grid.hv2_inv', 'coll_diff * grid.hv_inv', ['collt', 'colle']),
+    'collT drag sign': ('drag = ((vp - u) * fp - (vm - u) * fm) * aa',
+                        'drag = ((vp - u) * fp + (vm - u) * fm) * aa', ['collt']),
+    'velocity density not divided by n': ('u = (U / n)[:, None]', 'u = U[:, None]', ['collt', 'colle']),
+    'rows j+1 and j-1 swapped': ('fp, f0, fm = f[', 'fm, f0, fp = f[', ['collt', 'colle']),
+    'collE diffusion times a': ('diff = ((fp - 2. * f0) + fm) / aa', 'diff = ((fp - 2. * f0) + fm) * aa', ['colle']),
+}
+
+
+def _collision_pins(kind):
+    return [lambda: D.test_collision_equals_assembled_stencil(kind),
+            lambda: D.test_collision_conserves_mass_momentum_energy(kind),
+            lambda: D.test_drifting_maxwellian_is_second_order_null(kind),
+            lambda: D.test_diffusion_sign_is_dissipative(kind)]
+
+
+@pytest.mark.parametrize("name", sorted(COLLISION_CASES))
+def test_planted_collision_mistake_fails_a_pin(name, monkeypatch):
+    old, new, kinds = COLLISION_CASES[name]
+    src = inspect.getsource(O.collision_add)
+    assert src.count(old) == 1, name
+    ns = dict(vars(O))
+    exec(src.replace(old, new), ns)
+    monkeypatch.setattr(O, 'collision_add', ns['collision_add'])
+    for kind in kinds:
+        with np.errstate(all='ignore'):
+            assert _failures(_collision_pins(kind)) >= 2, (name, kind)
+
+
+def test_planted_collision_factor_mistake_fails_a_pin(monkeypatch):
+    # n / (n e - u u): the density in place of n^2
+    monkeypatch.setattr(O, 'collision_factor', lambda n, u, e: n / (n * e - u * u))
+    pins = [D.test_collision_factor_is_inverse_temperature] + _collision_pins('collt') + _collision_pins('colle')
+    assert _failures(pins) >= 2
+
+
+def test_unplanted_collisions_pass_every_pin():
+    assert _failures([D.test_collision_factor_is_inverse_temperature] + _collision_pins('collt')
+                     + _collision_pins('colle')) == 0
