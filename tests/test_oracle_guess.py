@@ -7,6 +7,10 @@
     exp(dt A) of df/dt = A f = -{f, H} (halving dt divides the one-step error
     by 32); the symplectic splittings, whose substeps are forward-Euler steps,
     make an O(dt^2) one-step error (their coefficients sum to one);
+  * a call with niter substeps moves by dt / niter (the symplectic substep
+    matrix), and an RK4 step of size s = dt / niter is exactly the degree-4
+    Taylor polynomial of exp(s A) (the linear tendency makes every four-stage
+    fourth-order method equal to it);
   * every scheme conserves sum f (sum {f, h} = 0 for fields vanishing near
     the v-boundaries);
   * driver: the guessed Picard run converges to the no-guess solution with no
@@ -88,6 +92,41 @@ def test_symplectic_is_product_of_euler_substeps(scheme):
     for A, w in seq:
         M = (I + w * dt * A) @ M
     np.testing.assert_allclose(_vec(got), M @ _vec(f), rtol=0, atol=1e-13 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("niter", [1, 2, 3])
+def test_symplectic_substep_divides_the_step(niter):
+    """One call with niter substeps moves by factor * dt / niter:
+    x -> (I + factor dt / niter A) x, for the kinetic and potential parts."""
+    g = _grid()
+    f, h0, h1, h2 = _fields(g)
+    H0 = np.broadcast_to(h0[None, :], (g.nx, g.nv))
+    H12 = np.broadcast_to((h1 + h2)[:, None], (g.nx, g.nv))
+    I = np.eye(g.nx * g.nv)
+    for step, H in ((O.symplectic_kinetic, H0), (O.symplectic_potential, H12)):
+        got = step(KIND, f, H, 0.7, g, niter)
+        ref = (I + 0.7 * g.ht / niter * _matrix(g, H)) @ _vec(f)
+        np.testing.assert_allclose(_vec(got), ref, rtol=0, atol=1e-13 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("niter", [1, 3])
+@pytest.mark.parametrize("scheme", ['rk4_16', 'rk4_18'])
+def test_rk4_is_the_fourth_order_taylor_polynomial(scheme, niter):
+    """On df/dt = A f (H fixed, so the tendency is linear) every four-stage
+    fourth-order Runge-Kutta step of size s = dt / niter is exactly
+    sum_{k<=4} (s A)^k / k!: a wrong stage coefficient or weight changes the
+    polynomial."""
+    g = _grid()
+    f, h0, h1, h2 = _fields(g)
+    H = (np.broadcast_to(h0[None, :], (g.nx, g.nv)) + h1[:, None]) + h2[:, None]
+    sA = g.ht / niter * _matrix(g, H)
+    x, term = _vec(f), _vec(f)
+    ref = x.copy()
+    for k in range(1, 5):
+        term = sA @ term / k
+        ref = ref + term
+    got = (O.rk4_16 if scheme == 'rk4_16' else O.rk4_18)(KIND, f, H, g, niter)
+    np.testing.assert_allclose(_vec(got), ref, rtol=0, atol=1e-13 * np.abs(f).max())
 
 
 def _one_step_errors(scheme, dts):

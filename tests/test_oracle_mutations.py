@@ -4,8 +4,9 @@ Each case plants one mistake in the oracle (a dropped term, a wrong sign, a
 wrong stencil index, transposed operands, a wrong weight or normalisation)
 in the Arakawa brackets, the residual and Jacobian, the kinetic
 preconditioner's eigenvalues and bands, the Poisson stencil, solve and
-right-hand side, the collision operators and factor, or GMRES (the last
-two edited into their source text); runs the independent pins of test_oracle_brackets.py and
+right-hand side, the collision operators and factor, the double brackets,
+the explicit steppers of the initial guess, or GMRES (the last five edited
+into their source text); runs the independent pins of test_oracle_brackets.py and
 test_oracle_solvers.py against the planted oracle; and requires that at
 least two of them fail (every bracket case fails three or more).  Without
 the mistake every pin passes."""
@@ -17,6 +18,7 @@ import pytest
 
 import test_oracle_brackets as P
 import test_oracle_dissipation as D
+import test_oracle_guess as G
 import test_oracle_solvers as SV
 from oracle import vlasov_oracle as O
 
@@ -277,3 +279,67 @@ def test_planted_collision_factor_mistake_fails_a_pin(monkeypatch):
 def test_unplanted_collisions_pass_every_pin():
     assert _failures([D.test_collision_factor_is_inverse_temperature] + _collision_pins('collt')
                      + _collision_pins('colle')) == 0
+
+
+# ---------------------------------------- double brackets and explicit steppers
+def _planted(fn, old, new):
+    src = inspect.getsource(getattr(O, fn))
+    assert src.count(old) == 1, (fn, old)
+    ns = dict(vars(O))
+    exec(src.replace(old, new), ns)
+    return ns[fn]
+
+
+DB = '-coll_freq * factor, grid)'
+DOUBLE_BRACKET_CASES = {
+    'FFH arguments swapped': ('double_bracket_function', 'return bracket_add(bkind, fave, b, y, ' + DB,
+                              'return bracket_add(bkind, b, fave, y, ' + DB),
+    'FHH sign': ('double_bracket_function', 'return bracket_add(bkind, b, H, y, ' + DB,
+                 'return bracket_add(bkind, b, H, y, coll_freq * factor, grid)'),
+    'FFH Jacobian drops a term': ('double_bracket_jacobian', 'return bracket_add(bkind, fave, b, y, ' + DB, 'return y'),
+    'FHH Jacobian arguments swapped': ('double_bracket_jacobian', 'return bracket_add(bkind, b, H, y, ' + DB,
+                                       'return bracket_add(bkind, H, b, y, ' + DB),
+}
+_DB_PINS = [lambda b=b: D.test_double_bracket_identities(b) for b in D.BRACKETS] + \
+           [lambda b=b, k=k: D.test_double_bracket_jacobian_is_derivative(k, b)
+            for b in D.BRACKETS for k in ('ffh', 'fhh')] + \
+           [lambda k=k: D.test_residual_with_dissipation_has_jacobian(k) for k in ('ffh', 'fhh')]
+
+STEPPER_CASES = {
+    'RK4 (1/6) weights over 5': ('rk4_16', '(t1 + 2. * t2 + 2. * t3 + t4) / 6.', '(t1 + 2. * t2 + 2. * t3 + t4) / 5.'),
+    'RK4 (1/6) third stage from t1': ('rk4_16', 't3 = tendency(kind, x + 0.5 * grid.ht / float(niter) * t2, h, grid)',
+                                      't3 = tendency(kind, x + 0.5 * grid.ht / float(niter) * t1, h, grid)'),
+    'RK4 (3/8) third stage sign': ('rk4_18', 't3 = tendency(kind, x - 1. / 3. * grid.ht',
+                                   't3 = tendency(kind, x + 1. / 3. * grid.ht'),
+    'RK4 (3/8) weight': ('rk4_18', '(t1 + 3. * t2 + 3. * t3 + t4) / 8.', '(t1 + 2. * t2 + 3. * t3 + t4) / 8.'),
+    'Forest-Ruth c2 sign': ('symplectic4_coefficients', 'c2 = c1 * (1. - fac2)', 'c2 = c1 * (fac2 - 1.)'),
+    'kinetic substep without 1/niter': ('symplectic_kinetic', 'factor * grid.ht / float(niter)', 'factor * grid.ht'),
+    'potential substep sign': ('symplectic_potential', 'x + factor * grid.ht / float(niter) * tendency',
+                               'x - factor * grid.ht / float(niter) * tendency'),
+}
+_STEPPER_PINS = [G.test_symplectic4_coefficients_are_forest_ruth] + \
+                [lambda s=s: G.test_symplectic_is_product_of_euler_substeps(s) for s in ('symplectic2', 'symplectic4')] + \
+                [lambda n=n: G.test_symplectic_substep_divides_the_step(n) for n in (1, 2, 3)] + \
+                [lambda s=s, n=n: G.test_rk4_is_the_fourth_order_taylor_polynomial(s, n)
+                 for s in ('rk4_16', 'rk4_18') for n in (1, 3)] + \
+                [lambda s=s, r=r: G.test_one_step_error_order(s, r)
+                 for s, r in (('rk4_16', 32.0), ('rk4_18', 32.0), ('symplectic2', 4.0), ('symplectic4', 4.0))] + \
+                [lambda s=s: G.test_mass_is_conserved(s) for s in ('rk4_16', 'rk4_18', 'symplectic2', 'symplectic4')]
+
+
+@pytest.mark.parametrize("name", sorted(DOUBLE_BRACKET_CASES))
+def test_planted_double_bracket_mistake_fails_a_pin(name, monkeypatch):
+    fn, old, new = DOUBLE_BRACKET_CASES[name]
+    monkeypatch.setattr(O, fn, _planted(fn, old, new))
+    assert _failures(_DB_PINS) >= 2, name
+
+
+@pytest.mark.parametrize("name", sorted(STEPPER_CASES))
+def test_planted_stepper_mistake_fails_a_pin(name, monkeypatch):
+    fn, old, new = STEPPER_CASES[name]
+    monkeypatch.setattr(O, fn, _planted(fn, old, new))
+    assert _failures(_STEPPER_PINS) >= 2, name
+
+
+def test_unplanted_double_brackets_and_steppers_pass_every_pin():
+    assert _failures(_DB_PINS + _STEPPER_PINS) == 0
