@@ -5,13 +5,14 @@ wrong stencil index, transposed operands, a wrong weight or normalisation)
 in the Arakawa brackets, the residual and Jacobian, the kinetic
 preconditioner's eigenvalues and bands, the Poisson stencil, solve and
 right-hand side, the collision operators and factor, the double brackets,
-the explicit steppers of the initial guess, or GMRES (the last five edited
-into their source text); runs the independent pins of test_oracle_brackets.py and
+the explicit steppers of the initial guess, GMRES, or the Picard driver's
+stop / revert rule (the last six edited into their source text); runs the independent pins of test_oracle_brackets.py and
 test_oracle_solvers.py against the planted oracle; and requires that at
 least two of them fail (every bracket case fails three or more).  Without
 the mistake every pin passes."""
 
 import inspect
+import textwrap
 
 import numpy as np
 import pytest
@@ -343,3 +344,30 @@ def test_planted_stepper_mistake_fails_a_pin(name, monkeypatch):
 
 def test_unplanted_double_brackets_and_steppers_pass_every_pin():
     assert _failures(_DB_PINS + _STEPPER_PINS) == 0
+
+
+# ----------------------------------------------------------- Picard driver
+DRIVER_CASES = {
+    'a first-iteration rise stops the loop': ('(pred > prev and i > 1) or', '(pred > prev) or'),
+    'snes_atol itself stops the loop': ('pred < self.snes_atol', 'pred <= self.snes_atol'),
+    'iteration cap off by one': ('i >= self.snes_max_iter', 'i > self.snes_max_iter'),
+    'no revert at the first iteration': ('            if pred > prev:\n', '            if pred > prev and i > 1:\n'),
+    'the revert keeps the iterate': ('self.fc = fl\n', 'pass\n'),
+}
+_DRIVER_PINS = [lambda c=c, e=e: SV.test_driver_stop_and_revert_rule(c, e) for c, e in SV.PRESCRIBED] + \
+               [SV.test_driver_max_iter_branch, SV.test_driver_revert_branch]
+
+
+@pytest.mark.parametrize("name", sorted(DRIVER_CASES))
+def test_planted_driver_mistake_fails_a_pin(name, monkeypatch):
+    old, new = DRIVER_CASES[name]
+    src = textwrap.dedent(inspect.getsource(O.OracleRun.step))
+    assert src.count(old) == 1, name
+    ns = dict(vars(O))
+    exec(src.replace(old, new), ns)
+    monkeypatch.setattr(O.OracleRun, 'step', ns['step'])
+    assert _failures(_DRIVER_PINS) >= 2, name
+
+
+def test_unplanted_driver_passes_every_pin():
+    assert _failures(_DRIVER_PINS) == 0

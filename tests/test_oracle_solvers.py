@@ -237,6 +237,48 @@ def test_driver_revert_branch():
     assert last >= e['picard'][-2]['residual']
 
 
+def _prescribed_run(residuals, **kw):
+    """The oracle's Picard loop with the solve replaced by f += 1 and the
+    residual sequence prescribed (the pre-loop residual first)."""
+    run = _landau_run(**kw)
+    seq = iter(residuals)
+    run.calculate_residual = lambda: next(seq)
+    run.calculate_moments = lambda: None
+    run.update_previous = lambda: None
+
+    def solve():
+        run.fc = run.fc + 1.0
+        return 7, 2
+    run.snes_ksponly = solve
+    return run
+
+
+# (residuals, snes_atol, snes_max_iter) -> (Picard iterations, iterates kept)
+# by run_nonlinear_matrixfree_split_mp.py:158-163: stop when the residual
+# rose after the first iteration, fell below snes_atol or hit the cap; a rise
+# at the stop (the first iteration included) undoes the last iterate
+PRESCRIBED = [
+    (([1.0, 1e-3, 1e-7, 5e-11], 1e-10, 20), (3, 3)),      # converges
+    (([1.0, 1e-3, 2e-3, 1e-4], 1e-10, 20), (2, 1)),       # rise: reverted
+    (([1.0, 2.0, 1e-11], 1e-10, 20), (2, 2)),             # a first-iteration rise does not stop
+    (([2e-11, 5e-11, 1e-12], 1e-10, 20), (1, 0)),         # ... but is undone when below snes_atol
+    (([1.0, 2.0, 1e-3], 1e-10, 1), (1, 0)),               # ... or at the cap
+    (([1.0] + [0.5 ** k for k in range(1, 30)], 1e-10, 5), (5, 5)),   # the cap
+    (([1.0, 1e-10, 1e-12], 1e-10, 20), (2, 2)),           # snes_atol itself does not stop
+    (([1.0, 1e-3, 1e-10, 1e-12], 1e-10, 20), (3, 3)),
+    (([1.0, 3.0, 0.5, 1e-12], 1e-10, 20), (3, 3)),        # a first-iteration rise, then convergence
+]
+
+
+@pytest.mark.parametrize("case,expect", PRESCRIBED)
+def test_driver_stop_and_revert_rule(case, expect):
+    residuals, atol, cap = case
+    run = _prescribed_run(residuals, snes_atol=atol, snes_max_iter=cap)
+    f0 = run.fc.copy()
+    e = run.step()
+    assert (len(e['picard']), int(round((run.fc - f0).max()))) == expect
+
+
 def test_driver_abort_branch(monkeypatch):
     run = _landau_run()
     monkeypatch.setattr(run, 'calculate_residual', lambda: 100.0)
