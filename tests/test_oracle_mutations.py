@@ -8,8 +8,10 @@ right-hand side, the collision operators and factor, the double brackets,
 the explicit steppers of the initial guess, GMRES, or the Picard driver's
 stop / revert rule (the last six edited into their source text); runs the independent pins of test_oracle_brackets.py and
 test_oracle_solvers.py against the planted oracle; and requires that at
-least two of them fail (every bracket case fails three or more).  Without
-the mistake every pin passes."""
+least two of them fail (every bracket case fails three or more; the
+potential preconditioner's bands have one independent pin, the assembled
+stencil tables, which each of their cases must fail).  Without the mistake
+every pin passes."""
 
 import inspect
 import textwrap
@@ -371,3 +373,33 @@ def test_planted_driver_mistake_fails_a_pin(name, monkeypatch):
 
 def test_unplanted_driver_passes_every_pin():
     assert _failures(_DRIVER_PINS) == 0
+
+
+# ------------------------------------------------ potential preconditioner
+_PBANDS = O.potential_bands
+
+
+def _pb(change):
+    return lambda grid, phi, e: change(grid, *_PBANDS(grid, phi, e))
+
+
+POTENTIAL_CASES = {
+    'bands without the 1/2': _pb(lambda g, dm, d, dp: (2 * dm, 2 * d - g.ht_inv, 2 * dp)),
+    'sub- and super-diagonal swapped': _pb(lambda g, dm, d, dp: (dp, d, dm)),
+    'phi terms with the wrong sign': _pb(lambda g, dm, d, dp: (-dm, 2 * g.ht_inv - d, -dp)),
+    'main-diagonal weight 1 for 2': _pb(lambda g, dm, d, dp: (dm, 0.5 * (d - g.ht_inv) + g.ht_inv, dp)),
+}
+_POTENTIAL_PINS = [SV.test_potential_preconditioner_is_minus_half_j1_of_phi_plus_identity_over_dt,
+                   SV.test_potential_preconditioner_variants_and_inverse,
+                   SV.test_driver_with_potential_preconditioner]
+
+
+@pytest.mark.parametrize("name", sorted(POTENTIAL_CASES))
+def test_planted_potential_band_mistake_fails_the_stencil_pin(name, monkeypatch):
+    monkeypatch.setattr(O, 'potential_bands', POTENTIAL_CASES[name])
+    with pytest.raises(AssertionError):
+        SV.test_potential_preconditioner_is_minus_half_j1_of_phi_plus_identity_over_dt()
+
+
+def test_unplanted_potential_preconditioner_passes_every_pin():
+    assert _failures(_POTENTIAL_PINS) == 0
