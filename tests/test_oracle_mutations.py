@@ -5,8 +5,9 @@ wrong stencil index, transposed operands, a wrong weight or normalisation)
 in the Arakawa brackets, the residual and Jacobian, the kinetic
 preconditioner's eigenvalues and bands, the Poisson stencil, solve and
 right-hand side, the collision operators and factor, the double brackets,
-the explicit steppers of the initial guess, GMRES, or the Picard driver's
-stop / revert rule (the last six edited into their source text); runs the independent pins of test_oracle_brackets.py and
+the explicit steppers of the initial guess, GMRES, the Picard driver's
+stop / revert rule, or the tangent-linear model of the adjoint (the last
+seven edited into their source text); runs the independent pins of test_oracle_brackets.py and
 test_oracle_solvers.py against the planted oracle; and requires that at
 least two of them fail (every bracket case fails three or more; the
 potential preconditioner's bands have one independent pin, the assembled
@@ -22,6 +23,8 @@ import pytest
 import test_oracle_brackets as P
 import test_oracle_dissipation as D
 import test_oracle_guess as G
+import test_oracle_tangent as T
+from oracle import tangent_linear as TL
 import test_oracle_solvers as SV
 from oracle import vlasov_oracle as O
 
@@ -403,3 +406,29 @@ def test_planted_potential_band_mistake_fails_the_stencil_pin(name, monkeypatch)
 
 def test_unplanted_potential_preconditioner_passes_every_pin():
     assert _failures(_POTENTIAL_PINS) == 0
+
+
+# ------------------------------------------------------- tangent-linear model
+TANGENT_CASES = {
+    'field coupling with 1 for 1/2': ('O.bracket_add(kind, fbar, hu, y, 0.5, grid)', 'O.bracket_add(kind, fbar, hu, y, 1.0, grid)'),
+    'field coupling dropped': ('O.bracket_add(kind, fbar, hu, y, 0.5, grid)', 'pass'),
+    'A0 with the sign of A': ("sign = 1.0 if which == 'A' else -1.0", 'sign = 1.0'),
+    'midpoint taken at f1': ('fbar = 0.5 * f0 + 0.5 * f1', 'fbar = f1'),
+}
+_TANGENT_PINS = [lambda k=k: T.test_linearisation_is_the_central_difference(k) for k in T.KINDS] + \
+                [T.test_tangent_linear_matches_picard_differences]
+
+
+@pytest.mark.parametrize("name", sorted(TANGENT_CASES))
+def test_planted_tangent_mistake_fails_a_pin(name, monkeypatch):
+    old, new = TANGENT_CASES[name]
+    src = inspect.getsource(TL.linear_apply)
+    assert src.count(old) == 1, name
+    ns = dict(vars(TL))
+    exec(src.replace(old, new), ns)
+    monkeypatch.setattr(TL, 'linear_apply', ns['linear_apply'])
+    assert _failures(_TANGENT_PINS) >= 2, name
+
+
+def test_unplanted_tangent_passes_every_pin():
+    assert _failures(_TANGENT_PINS) == 0
