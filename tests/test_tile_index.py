@@ -11,6 +11,7 @@ test; this one fails on it (a single periodic wrap, the fault of an earlier
 version on nx < 66, reports thousands of violations)."""
 
 import os
+import re
 import shutil
 import subprocess
 
@@ -51,3 +52,34 @@ def test_fused_jac_rfft_indices_in_bounds(tmp_path):
     print(out.stdout)
     assert out.returncode == 0, out.stdout
     assert "0 failures" in out.stdout
+
+
+# (text in tile_index.h, its replacement, checker arguments that must report it)
+PLANTED = {
+    'a single periodic wrap': ('    if (gi >= nx) gi %= nx;\n', '', ["8", "130", "8", "70"]),
+    'bracket halo column off by one': ('return wrap_x(t.i0 - 2 + c, nx);\n}', 'return wrap_x(t.i0 - 1 + c, nx);\n}',
+                                       ["8", "130", "8", "70"]),
+    'fused strip column off by one': ('return wrap_x(cb - 2 + c, nx); }', 'return wrap_x(cb - 1 + c, nx); }',
+                                      ["jr", "8", "80", "1024", "2048"]),
+    'fused right halo columns shifted': ('*col = hc < 2 ? hc : kJrSW + hc;', '*col = hc < 2 ? hc : kJrSW + hc - 1;',
+                                         ["jr", "8", "80", "1024", "2048"]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PLANTED))
+def test_planted_index_mistake_is_reported(name, tmp_path):
+    # the checker built against a copy of the header with one mistake
+    old, new, args = PLANTED[name]
+    text = open(os.path.join(ROOT, "vivlasov1d_amd", "csrc", "tile_index.h")).read()
+    assert text.count(old) == 1, name
+    (tmp_path / "vivlasov1d_amd" / "csrc").mkdir(parents=True)
+    (tmp_path / "tests" / "native").mkdir(parents=True)
+    (tmp_path / "vivlasov1d_amd" / "csrc" / "tile_index.h").write_text(text.replace(old, new))
+    src = tmp_path / "tests" / "native" / "tile_index_check.cpp"
+    src.write_text(open(SRC).read())
+    exe = str(tmp_path / "planted_check")
+    subprocess.run([_hipcc(), "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", str(src), "-o", exe],
+                   check=True, cwd=str(tmp_path))
+    out = subprocess.run([exe] + args, capture_output=True, text=True)
+    m = re.search(r"checked \d+ grids, (\d+) failures", out.stdout)
+    assert out.returncode != 0 and m and int(m.group(1)) > 0, out.stdout[-300:]
